@@ -1,0 +1,387 @@
+/*
+ * frost_amd: C ABI of the MI355X (gfx950) per-gulp single-pulse search path.
+ *
+ * One gulp of channelised intensity data (time-major rows of packed
+ * samples) goes through
+ *   RFI excision + calibration  -> channel-major float32, low frequency first
+ *   streaming FDMT              -> DM-time rows of every searched plane
+ *   boxcar / trapezoid filters  -> S/N with the exact per-DM/width/mask noise
+ *   threshold + peak finding    -> candidate rows
+ * on one GPU. Each entry point names the frost interface it replaces
+ * (paths relative to the frost source tree).
+ *
+ * Conventions
+ *  - Every function returns FA_OK (0) or an error code; the message of the
+ *    last error on the calling thread is fa_last_error(). An error leaves the
+ *    object unchanged (arguments are validated before any state changes).
+ *  - FA_INVALID_ARGUMENT mirrors std::invalid_argument in frost,
+ *    FA_RUNTIME_ERROR mirrors std::runtime_error (including HIP failures),
+ *    FA_OUT_OF_MEMORY is a refused device allocation / budget.
+ *  - Pointers marked "device" are HIP device pointers; "host" pointers are
+ *    ordinary memory. Streams are hipStream_t passed as void*.
+ *  - Objects are single-threaded: one caller thread per object.
+ */
+#ifndef FROST_AMD_H
+#define FROST_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FA_OK 0
+#define FA_INVALID_ARGUMENT 1
+#define FA_RUNTIME_ERROR 2
+#define FA_OUT_OF_MEMORY 3
+
+/* Sample storage types, in frost's DataType order
+ * (include/frost/common/policies.hpp:51-59). */
+enum fa_dtype {
+    FA_UINT1   = 0,
+    FA_UINT2   = 1,
+    FA_UINT4   = 2,
+    FA_UINT8   = 3,
+    FA_UINT16  = 4,
+    FA_UINT32  = 5,
+    FA_FLOAT32 = 6
+};
+
+/* rfi::FillMode (include/frost/prepare/rfi.hpp:50-58). */
+enum fa_fill_mode { FA_FILL_NEIGHBOUR = 0, FA_FILL_MEAN = 1, FA_FILL_NOISE = 2 };
+
+/* rfi::ZeroDMMode (include/frost/prepare/rfi.hpp:61-70). */
+enum fa_zero_dm { FA_ZERO_DM_OFF = 0, FA_ZERO_DM_SUBTRACT = 1, FA_ZERO_DM_ZDOT = 2 };
+
+/* dmt::PeakFindingMode as frost documents it (docs/user_guide/search.md:38-47). */
+enum fa_peak_mode {
+    FA_PEAK_PER_WIDTH     = 0, /* one peak per above-threshold run, per width */
+    FA_PEAK_ACROSS_WIDTHS = 1, /* one peak per run over all widths together   */
+    FA_PEAK_ALL           = 2  /* every above-threshold sample                */
+};
+
+/* rfi::RfiConfig (include/frost/prepare/rfi.hpp:159-191), flattened. Channel
+ * zaps are inclusive [start, end] pairs; frequency zaps inclusive [lo, hi]
+ * MHz pairs. Defaults: fa_rfi_config_default(). */
+typedef struct fa_rfi_config {
+    const uint64_t* zap_chans; /* 2*n_zap_chans values, may be NULL */
+    uint64_t n_zap_chans;
+    const double* zap_freqs; /* 2*n_zap_freqs values, may be NULL */
+    uint64_t n_zap_freqs;
+    int32_t narrow_enabled;
+    float narrow_threshold;
+    int32_t broad_enabled;
+    float broad_threshold;
+    uint64_t broad_boxcar_max;
+    int32_t dropout_enabled;
+    float dropout_min_std_fraction;
+    int32_t moments_enabled;
+    float moments_threshold;
+    int32_t moments_mean;
+    int32_t moments_std;
+    int32_t moments_skewness;
+    int32_t moments_kurtosis;
+    int32_t moments_autocorr;
+    float moments_kurtosis_max;
+    int32_t iqrm_enabled;
+    float iqrm_threshold;
+    uint64_t iqrm_radius; /* 0: nchans / 10 */
+    uint64_t mask_grow;
+    int32_t kadane_channel_enabled;
+    float kadane_channel_threshold;
+    float kadane_channel_min_duration; /* seconds */
+    int32_t kadane_spectrum_enabled;
+    float kadane_spectrum_threshold;
+    float kadane_spectrum_min_bandwidth; /* MHz */
+    float kadane_spectrum_whole_band_fraction;
+    int32_t zero_dm;  /* enum fa_zero_dm */
+    int32_t equalize; /* bool */
+    int32_t fill;     /* enum fa_fill_mode */
+    uint64_t seed;
+} fa_rfi_config;
+
+/* rfi::BlockLayout (include/frost/prepare/rfi.hpp:193-201). */
+typedef struct fa_block_layout {
+    uint64_t nchans;
+    uint64_t max_nsamps;
+    int32_t dtype; /* enum fa_dtype */
+    double tsamp;  /* seconds */
+    double f0;     /* centre frequency of channel 0, MHz */
+    double df;     /* channel spacing, MHz (may be negative) */
+} fa_block_layout;
+
+/* rfi::RfiReport (include/frost/prepare/rfi.hpp:204-227); times are device
+ * stage times in seconds measured with HIP events. */
+typedef struct fa_rfi_report {
+    int32_t calibration_reused;
+    uint64_t nsamps;
+    uint64_t chans_zapped;
+    uint64_t chans_dead;
+    uint64_t chans_moments;
+    uint64_t chans_iqrm;
+    uint64_t chans_grown;
+    uint64_t chans_masked;
+    uint64_t samps_broad;
+    uint64_t samps_dropout;
+    uint64_t samps_kadane;
+    uint64_t samps_flagged;
+    uint64_t cells_narrow;
+    uint64_t cells_kadane_spectrum;
+    uint64_t cells_kadane_channel;
+    uint64_t fill_fallbacks;
+    double t_bandpass;
+    double t_moments;
+    double t_scan;
+    double t_flags;
+    double t_apply;
+} fa_rfi_report;
+
+/* dedisp::DedispPlanConfig (include/frost/dedisp/dedisp.hpp:12-27). */
+typedef struct fa_dedisp_config {
+    uint64_t nchans;
+    float dt;             /* seconds */
+    float f0;             /* MHz, channel 0 */
+    float df;             /* MHz, may be negative */
+    float dm_min;
+    float dm_max;
+    float dm_pulse_width; /* microseconds */
+    float dm_tol;
+    uint64_t block_nsamps;
+    uint64_t subband_depth; /* 0, 1 or 2 */
+} fa_dedisp_config;
+
+/* dmt::BoxcarConfig as frost fills it (lib/pipeline/pipeline.cpp:316-330). */
+typedef struct fa_boxcar_config {
+    uint64_t max_boxcar;
+    float wtsp;
+    int32_t use_trapezoid;
+    float detect_thresh;
+    int32_t peak_mode; /* enum fa_peak_mode */
+    uint64_t max_candidates;
+    uint64_t max_cands_per_dm; /* 0: unlimited */
+} fa_boxcar_config;
+
+/* One searched plane (dmt search plan fields frost reads,
+ * lib/search/postprocess.cpp:79-115, tests/cpp/api/dedisp_test.cpp:30-54). */
+typedef struct fa_plane_info {
+    uint32_t level;         /* FDMT level of the plane's node (0 = channels) */
+    uint32_t subband_level; /* search depth: 0 full band, 1 halves, 2 quarters */
+    uint32_t subband_idx;   /* tree nodes first (ascending f), then staggered  */
+    uint32_t staggered;     /* 1 if the plane straddles two tree nodes         */
+    uint32_t chan_begin;    /* half-open channel range, FDMT (ascending) order */
+    uint32_t chan_end;
+    uint32_t ndm;           /* rows (DM trials) of the plane                   */
+    uint32_t pad_;
+    float f_low_mhz;
+    float f_high_mhz;
+} fa_plane_info;
+
+/* dmt::RawCandidate fields frost consumes (lib/search/postprocess.cpp:79-115). */
+typedef struct fa_raw_candidate {
+    uint64_t peak_sample;  /* absolute sample index: window end at the plane's
+                              bottom-edge arrival                             */
+    uint64_t begin_sample; /* first above-threshold sample of the run        */
+    uint64_t end_sample;   /* last above-threshold sample of the run         */
+    float snr;
+    float dm;
+    uint32_t dm_idx;
+    uint32_t filter_idx;
+    uint32_t width;
+    uint32_t chan_begin;
+    uint32_t chan_end;
+    float f_low_mhz;
+    float f_high_mhz;
+    uint16_t subband_level;
+    uint16_t subband_idx;
+    uint8_t staggered;
+    uint8_t pad_[3];
+    uint16_t beam_idx;
+    uint16_t pad2_;
+} fa_raw_candidate;
+
+typedef struct fa_plan fa_plan;
+typedef struct fa_engine fa_engine;
+typedef struct fa_rfi fa_rfi;
+typedef struct fa_beam fa_beam;
+
+/* ---- errors and defaults ------------------------------------------------ */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fa_last_error(void);
+
+/* RfiConfig{} defaults (include/frost/prepare/rfi.hpp:80-191). */
+void fa_rfi_config_default(fa_rfi_config* cfg);
+
+/* BoxcarConfig as frost's CLI defaults fill it
+ * (include/frost/config/config.hpp:57-71, lib/pipeline/pipeline.cpp:316-330). */
+void fa_boxcar_config_default(fa_boxcar_config* cfg);
+
+/* ---- device helpers (streams and buffers for callers without a runtime) -- */
+
+int fa_device_count(int* n);
+int fa_stream_create(int device, void** stream);
+int fa_stream_destroy(void* stream);
+int fa_stream_synchronize(void* stream);
+int fa_device_malloc(int device, uint64_t bytes, void** ptr);
+int fa_device_free(void* ptr);
+int fa_memcpy_h2d(void* dst, const void* src, uint64_t bytes, void* stream);
+int fa_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream);
+
+/* ---- dedispersion plan -------------------------------------------------- */
+
+/* dedisp::generate_dm_list (lib/dedisp/dedisp.cpp:16-69). Writes up to cap
+ * trials to out (may be NULL to query) and the full count to *n. */
+int fa_generate_dm_list(const fa_dedisp_config* cfg, float* out, uint64_t cap,
+                        uint64_t* n);
+
+/* dedisp::make_dmt_fdmt_plan (lib/dedisp/dedisp.cpp:71-89): ascending band
+ * edges, DM list, the FDMT row program of every level and the searched
+ * planes (1, 4 or 11). Host object. */
+int fa_plan_create(const fa_dedisp_config* cfg, fa_plan** plan);
+int fa_plan_destroy(fa_plan* plan);
+
+/* FDMTPlan queries (get_f_min/get_f_max/get_dt_max/get_dm_grid_final,
+ * lib/pipeline/memory.cpp:320-373). nrows_total = sum of rows over levels. */
+int fa_plan_summary(const fa_plan* plan, float* f_min, float* f_max,
+                    uint64_t* dt_max, uint64_t* ndm_final, uint32_t* nlevels,
+                    uint32_t* nplanes, uint64_t* nrows_total);
+int fa_plan_plane(const fa_plan* plan, uint32_t p, fa_plane_info* out);
+/* DM value and total delay (samples across the plane's band) per row;
+ * the delay is the plane's per-DM warm-up (warmup_samples). */
+int fa_plan_plane_rows(const fa_plan* plan, uint32_t p, float* dm,
+                       int32_t* delay, uint64_t cap);
+/* Row program of level l (1..nlevels-1): for each output row, the input
+ * rows and the shift of the upper sub-band. Level 0 reports, per row, the
+ * channel in lo, -1 in hi and the number of summed samples in shift. */
+int fa_plan_level(const fa_plan* plan, uint32_t level, uint64_t* nrows,
+                  int32_t* lo, int32_t* hi, int32_t* shift, uint64_t cap);
+
+/* dmt::plans::generate_box_width_trials as frost pins it
+ * (tests/cpp/api/robustness_test.cpp:229-242): 1 then max(ceil(wtsp*w), w+1)
+ * while <= max_boxcar. */
+int fa_box_width_trials(uint64_t max_boxcar, float wtsp, uint32_t* out,
+                        uint64_t cap, uint64_t* n);
+
+/* ---- search engine (dmt::algorithms::FDMTBoxcar) -------------------------- */
+
+/* FDMTBoxcar(FDMTPlan, BoxcarConfig, ..., kill_mask)
+ * (lib/pipeline/pipeline.cpp:346-367). kill_mask: host, nchans bytes in FDMT
+ * (ascending) order, 1 = good; NULL = all good. Copies the plan. */
+int fa_engine_create(const fa_plan* plan, const fa_boxcar_config* cfg,
+                     const uint8_t* kill_mask, int device, fa_engine** engine);
+int fa_engine_destroy(fa_engine* engine);
+
+/* FDMTBoxcar::execute(DeviceSpan<const float>, span<const uint8_t>, Stream)
+ * (lib/pipeline/pipeline.cpp:603). waterfall: device, [nchans][stride]
+ * channel-major float32, ascending frequency, nsamps valid columns starting
+ * at column `offset`. kill_mask: host or NULL (= constructor mask). The
+ * call is asynchronous on `stream`; fa_engine_copy_result synchronises. */
+int fa_engine_execute(fa_engine* engine, const float* waterfall,
+                      uint64_t stride, uint64_t offset, uint64_t nsamps,
+                      const uint8_t* kill_mask, void* stream);
+
+/* FDMTBoxcar::copy_result(Stream) -> CandidateResultView
+ * (lib/pipeline/pipeline.cpp:605). Candidates are sorted by (plane, dm_idx,
+ * filter_idx, peak_sample). *n is the number available; at most cap are
+ * written. */
+int fa_engine_copy_result(fa_engine* engine, void* stream,
+                          fa_raw_candidate* out, uint64_t cap, uint64_t* n,
+                          uint64_t* dropped_global, uint64_t* dropped_per_dm);
+
+/* FDMTBoxcar::drain() (lib/pipeline/pipeline.cpp:517): closes every open
+ * excursion; results as fa_engine_copy_result. */
+int fa_engine_drain(fa_engine* engine, void* stream, fa_raw_candidate* out,
+                    uint64_t cap, uint64_t* n, uint64_t* dropped_global,
+                    uint64_t* dropped_per_dm);
+
+/* FDMTBoxcar::get_candidate_frontier() (lib/pipeline/pipeline.cpp:493). */
+int fa_engine_candidate_frontier(const fa_engine* engine, uint64_t* frontier);
+
+/* Search-plan fields widths and inv_sigma_lut (get_search_plans(),
+ * tests/cpp/api/calibration_stress_test.cpp:161-194): 1/sqrt(variance) of
+ * the window sum of plane p, row d, width index w under the current mask,
+ * laid out [ndm][nwidths]. */
+int fa_engine_widths(const fa_engine* engine, uint32_t* out, uint64_t cap,
+                     uint64_t* n);
+int fa_engine_inv_sigma(const fa_engine* engine, uint32_t p, float* out,
+                        uint64_t cap);
+
+/* FDMTBoxcar::get_memory_usage().total() (lib/pipeline/pipeline.cpp:426). */
+int fa_engine_memory_bytes(const fa_engine* engine, uint64_t* bytes);
+
+/* Debug / parity observer (dmt::algorithms::FDMT view_subband,
+ * tests/cpp/api/calibration_test.cpp:41-89): copies the DM-time rows of
+ * plane p from the last execute, [ndm][nsamps] float32, host. Only
+ * available when the engine was created with keep_planes set through
+ * fa_engine_keep_planes before the first execute. */
+int fa_engine_keep_planes(fa_engine* engine, int keep);
+int fa_engine_plane_rows(fa_engine* engine, uint32_t p, float* out,
+                         uint64_t cap);
+
+/* Stage timings of the last execute (HIP events on the engine's stream):
+ * FDMT merge kernels, the init kernel and the search kernels, seconds. */
+int fa_engine_timings(const fa_engine* engine, double* t_init,
+                      double* t_merge, double* t_search, double* t_total);
+
+/* ---- device RFI (RFICleaner::execute_normalized / DeviceRfi) ------------- */
+
+/* RFICleaner(cfg, layout, baseline_length) (include/frost/prepare/rfi.hpp:260)
+ * and DeviceRfi (lib/prepare/device_rfi.hpp:49-144). */
+int fa_rfi_create(const fa_rfi_config* cfg, const fa_block_layout* layout,
+                  float baseline_length, int device, fa_rfi** rfi);
+int fa_rfi_destroy(fa_rfi* rfi);
+
+/* RFICleaner::execute_normalized(block, nsamps)
+ * (include/frost/prepare/rfi.hpp:272-287): block is device memory holding
+ * `capacity` time-major rows, `nsamps` of them real (0 = all). Writes
+ * channel-major float32, low frequency first, into out (device) as
+ * out[c * out_stride + out_offset + t] for t < capacity; masked channels and
+ * padding are zero. Synchronises the stream (the channel mask decision is
+ * made on the host, as in DeviceRfi::execute). */
+int fa_rfi_execute_normalized(fa_rfi* rfi, const void* block,
+                              uint64_t capacity, uint64_t nsamps, float* out,
+                              uint64_t out_stride, uint64_t out_offset,
+                              void* stream);
+
+/* RFICleaner::channel_mask() (input channel order, 1 = good) and report(). */
+int fa_rfi_channel_mask(const fa_rfi* rfi, uint8_t* out, uint64_t cap);
+int fa_rfi_get_report(const fa_rfi* rfi, fa_rfi_report* out);
+
+/* Per-window calibration of the last block, [nwin][nchans] each, and the
+ * time flags [nsamps] (parity observer). */
+int fa_rfi_calibration(const fa_rfi* rfi, uint64_t* nwin, float* loc,
+                       float* scale, float* inv_scale, uint64_t cap);
+int fa_rfi_time_flags(const fa_rfi* rfi, uint8_t* out, uint64_t cap);
+
+/* ---- one beam: the per-gulp driver (Pipeline::Impl::execute_block) ------- */
+
+/* Pipeline::Impl (lib/pipeline/pipeline.cpp:378-438): RFI cleaner + FDMT
+ * engine with device-resident calibration tail, FDMT carry and pending
+ * excursions. dedisp->block_nsamps is the gulp capacity. */
+int fa_beam_create(const fa_rfi_config* rfi_cfg, const fa_block_layout* layout,
+                   float baseline_length, const fa_dedisp_config* dedisp,
+                   const fa_boxcar_config* boxcar, int device, fa_beam** beam);
+int fa_beam_destroy(fa_beam* beam);
+
+/* Pipeline::Impl::execute_block (lib/pipeline/pipeline.cpp:529-619): one
+ * gulp of `nsamps` rows (<= capacity) at `gulp` (device memory if
+ * is_device, else host). Returns after the candidates are on the host. */
+int fa_beam_execute(fa_beam* beam, const void* gulp, uint64_t nsamps,
+                    int is_device, void* stream);
+
+/* Results of the last fa_beam_execute / fa_beam_drain. */
+int fa_beam_result(fa_beam* beam, fa_raw_candidate* out, uint64_t cap,
+                   uint64_t* n, uint64_t* dropped_global,
+                   uint64_t* dropped_per_dm, uint64_t* frontier);
+int fa_beam_drain(fa_beam* beam, void* stream);
+int fa_beam_channel_mask(const fa_beam* beam, uint8_t* out, uint64_t cap);
+int fa_beam_rfi_report(const fa_beam* beam, fa_rfi_report* out);
+int fa_beam_engine(fa_beam* beam, fa_engine** engine);
+int fa_beam_rfi(fa_beam* beam, fa_rfi** rfi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FROST_AMD_H */

@@ -1,0 +1,289 @@
+"""ctypes loader of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker. Never imported by the
+frost_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from frost_amd import abi
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+P = C.c_void_p
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        sig = {
+            "oracle_last_error": (C.c_char_p, []),
+            "oracle_unpack": (C.c_int, [P, C.c_uint64, C.c_int, P]),
+            "oracle_largest_pow5": (C.c_uint64, [C.c_uint64]),
+            "oracle_median5": (C.c_float, [C.c_float] * 5),
+            "oracle_iqrm_mask": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_float, P, P]),
+            "oracle_kadane": (C.c_int, [P, P, C.c_uint64, P, C.c_uint64, P, P, P, P]),
+            "oracle_mad_scale": (C.c_float, [P, C.c_uint64]),
+            "oracle_baseline_filter": (C.c_int, [P, C.c_uint64, C.c_uint64]),
+            "oracle_broadband_flags": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_float,
+                                                 C.c_uint64, P, P]),
+            "oracle_measure_bandpass": (C.c_int, [P, C.c_uint64, C.c_uint64, P, P, P, P]),
+            "oracle_rfi_create": (C.c_int, [C.POINTER(abi.RfiConfig),
+                                            C.POINTER(abi.BlockLayout), C.c_float,
+                                            C.POINTER(P)]),
+            "oracle_rfi_destroy": (None, [P]),
+            "oracle_rfi_execute_normalized": (C.c_int, [P, P, C.c_uint64, C.c_uint64, P]),
+            "oracle_rfi_mask": (C.c_int, [P, P]),
+            "oracle_rfi_report": (C.c_int, [P, C.POINTER(abi.RfiReport)]),
+            "oracle_rfi_calibration": (C.c_int, [P, P, P, P, P, C.c_uint64]),
+            "oracle_rfi_time_flags": (C.c_int, [P, P, C.c_uint64]),
+            "oracle_rfi_series": (C.c_int, [P, P, P, C.c_uint64]),
+            "oracle_dm_list": (C.c_int, [C.POINTER(abi.DedispConfig), P, C.c_uint64, P]),
+            "oracle_width_trials": (C.c_int, [C.c_uint64, C.c_float, P, C.c_uint64, P]),
+            "oracle_window_variance": (C.c_double, [C.c_int32, C.c_uint32, C.c_int]),
+            "oracle_plan_create": (C.c_int, [C.POINTER(abi.DedispConfig), C.POINTER(P)]),
+            "oracle_plan_destroy": (None, [P]),
+            "oracle_plan_summary": (C.c_int, [P, P, P, P, P, P, P, P]),
+            "oracle_plan_plane": (C.c_int, [P, C.c_uint32, C.POINTER(abi.PlaneInfo)]),
+            "oracle_plan_plane_rows": (C.c_int, [P, C.c_uint32, P, P, C.c_uint64]),
+            "oracle_plan_level_rows": (C.c_int, [P, P]),
+            "oracle_plan_channel_terms": (C.c_int, [P, C.c_uint32, C.c_uint32, P, P]),
+            "oracle_brute_force_row": (C.c_int, [P, C.c_uint32, C.c_uint32, P, C.c_uint64,
+                                                 C.c_uint64, C.c_uint64, P]),
+            "oracle_engine_create": (C.c_int, [C.POINTER(abi.DedispConfig),
+                                               C.POINTER(abi.BoxcarConfig), P,
+                                               C.POINTER(P)]),
+            "oracle_engine_destroy": (None, [P]),
+            "oracle_engine_keep_planes": (None, [P, C.c_int]),
+            "oracle_engine_execute": (C.c_int, [P, P, C.c_uint64, P]),
+            "oracle_engine_result": (C.c_int, [P, P, C.c_uint64, P, P, P]),
+            "oracle_engine_drain": (C.c_int, [P]),
+            "oracle_engine_frontier": (C.c_uint64, [P]),
+            "oracle_engine_plane_rows": (C.c_int, [P, C.c_uint32, P, C.c_uint64]),
+            "oracle_engine_inv_sigma": (C.c_int, [P, C.c_uint32, P, C.c_uint64]),
+            "oracle_engine_hist": (C.c_uint64, [P]),
+            "oracle_sizeof_candidate": (C.c_uint64, []),
+            "oracle_sizeof_rfi_config": (C.c_uint64, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().oracle_last_error().decode()
+        if rc == abi.FA_INVALID_ARGUMENT:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---- convenience wrappers ------------------------------------------------
+
+def iqrm_mask(x, radius, threshold, ignore=None) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(x.size, np.uint8)
+    ig = None if ignore is None else np.ascontiguousarray(ignore, dtype=np.uint8)
+    check(lib().oracle_iqrm_mask(ptr(x), x.size, radius, threshold,
+                                 None if ig is None else ptr(ig), ptr(out)))
+    return out
+
+
+def dm_list(cfg: abi.DedispConfig) -> np.ndarray:
+    n = C.c_uint64()
+    check(lib().oracle_dm_list(C.byref(cfg), None, 0, C.byref(n)))
+    out = np.zeros(n.value, np.float32)
+    check(lib().oracle_dm_list(C.byref(cfg), ptr(out), out.size, C.byref(n)))
+    return out
+
+
+def width_trials(max_boxcar, wtsp) -> list:
+    n = C.c_uint64()
+    check(lib().oracle_width_trials(max_boxcar, wtsp, None, 0, C.byref(n)))
+    out = np.zeros(n.value, np.uint32)
+    check(lib().oracle_width_trials(max_boxcar, wtsp, ptr(out), out.size, C.byref(n)))
+    return out.tolist()
+
+
+class Cleaner:
+    """RFICleaner::execute_normalized restated on the CPU."""
+
+    def __init__(self, cfg: abi.RfiConfig, layout: abi.BlockLayout, baseline: float):
+        self.layout = layout
+        h = P()
+        check(lib().oracle_rfi_create(C.byref(cfg), C.byref(layout), baseline, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_rfi_destroy(self.h)
+
+    def execute_normalized(self, block: np.ndarray, capacity: int, nsamps: int = 0):
+        block = np.ascontiguousarray(block)
+        out = np.zeros((self.layout.nchans, capacity), np.float32)
+        check(lib().oracle_rfi_execute_normalized(self.h, ptr(block), capacity, nsamps,
+                                                  ptr(out)))
+        return out
+
+    def mask(self) -> np.ndarray:
+        out = np.zeros(self.layout.nchans, np.uint8)
+        check(lib().oracle_rfi_mask(self.h, ptr(out)))
+        return out
+
+    def report(self) -> abi.RfiReport:
+        r = abi.RfiReport()
+        check(lib().oracle_rfi_report(self.h, C.byref(r)))
+        return r
+
+    def calibration(self, max_windows=4096):
+        n = self.layout.nchans
+        loc = np.zeros(max_windows * n, np.float32)
+        sc = np.zeros_like(loc)
+        inv = np.zeros_like(loc)
+        nwin = C.c_uint64()
+        check(lib().oracle_rfi_calibration(self.h, C.byref(nwin), ptr(loc), ptr(sc), ptr(inv),
+                                           loc.size))
+        k = nwin.value * n
+        return (loc[:k].reshape(-1, n), sc[:k].reshape(-1, n), inv[:k].reshape(-1, n))
+
+    def time_flags(self, nsamps):
+        out = np.zeros(nsamps, np.uint8)
+        check(lib().oracle_rfi_time_flags(self.h, ptr(out), nsamps))
+        return out
+
+
+class Plan:
+    def __init__(self, cfg: abi.DedispConfig):
+        h = P()
+        check(lib().oracle_plan_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.cfg = cfg
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_plan_destroy(self.h)
+
+    def summary(self) -> dict:
+        fmn, fmx = C.c_float(), C.c_float()
+        dtm, ndm, nrow = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        nl, npl = C.c_uint32(), C.c_uint32()
+        check(lib().oracle_plan_summary(self.h, C.byref(fmn), C.byref(fmx), C.byref(dtm),
+                                        C.byref(ndm), C.byref(nl), C.byref(npl),
+                                        C.byref(nrow)))
+        return dict(f_min=fmn.value, f_max=fmx.value, dt_max=dtm.value, ndm=ndm.value,
+                    nlevels=nl.value, nplanes=npl.value, nrows_total=nrow.value)
+
+    def plane(self, p) -> abi.PlaneInfo:
+        info = abi.PlaneInfo()
+        check(lib().oracle_plan_plane(self.h, p, C.byref(info)))
+        return info
+
+    def plane_rows(self, p):
+        info = self.plane(p)
+        dm = np.zeros(info.ndm, np.float32)
+        delay = np.zeros(info.ndm, np.int32)
+        check(lib().oracle_plan_plane_rows(self.h, p, ptr(dm), ptr(delay), info.ndm))
+        return dm, delay
+
+    def level_rows(self):
+        nl = self.summary()["nlevels"]
+        out = np.zeros(nl, np.uint64)
+        check(lib().oracle_plan_level_rows(self.h, ptr(out)))
+        return out
+
+    def channel_terms(self, p, row):
+        info = self.plane(p)
+        n = info.chan_end - info.chan_begin
+        off = np.zeros(n, np.int32)
+        ln = np.zeros(n, np.int32)
+        check(lib().oracle_plan_channel_terms(self.h, p, row, ptr(off), ptr(ln)))
+        return off, ln
+
+    def brute_force_row(self, p, row, x_ext: np.ndarray, hist: int, nsamps: int):
+        x_ext = np.ascontiguousarray(x_ext, dtype=np.float32)
+        out = np.zeros(nsamps, np.float32)
+        check(lib().oracle_brute_force_row(self.h, p, row, ptr(x_ext), x_ext.shape[1], hist,
+                                           nsamps, ptr(out)))
+        return out
+
+
+class Engine:
+    """FDMTBoxcar rules of this project, on the CPU."""
+
+    def __init__(self, dcfg: abi.DedispConfig, bcfg: abi.BoxcarConfig, kill_mask=None,
+                 keep_planes=False):
+        h = P()
+        km = None if kill_mask is None else np.ascontiguousarray(kill_mask, np.uint8)
+        check(lib().oracle_engine_create(C.byref(dcfg), C.byref(bcfg),
+                                         None if km is None else ptr(km), C.byref(h)))
+        self.h = h
+        self.dcfg = dcfg
+        self.plan = Plan(dcfg)
+        lib().oracle_engine_keep_planes(h, int(keep_planes))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_engine_destroy(self.h)
+
+    @property
+    def hist(self) -> int:
+        return int(lib().oracle_engine_hist(self.h))
+
+    def execute(self, waterfall: np.ndarray, kill_mask=None):
+        wf = np.ascontiguousarray(waterfall, dtype=np.float32)
+        km = None if kill_mask is None else np.ascontiguousarray(kill_mask, np.uint8)
+        check(lib().oracle_engine_execute(self.h, ptr(wf), wf.shape[1],
+                                          None if km is None else ptr(km)))
+        return self.result()
+
+    def drain(self):
+        check(lib().oracle_engine_drain(self.h))
+        return self.result()
+
+    def result(self):
+        n, dg, dd = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().oracle_engine_result(self.h, None, 0, C.byref(n), C.byref(dg),
+                                         C.byref(dd)))
+        arr = (abi.RawCandidate * max(n.value, 1))()
+        check(lib().oracle_engine_result(self.h, C.cast(arr, P), n.value, C.byref(n),
+                                         C.byref(dg), C.byref(dd)))
+        return abi.candidate_tuples(arr, n.value), dg.value, dd.value
+
+    def frontier(self) -> int:
+        return int(lib().oracle_engine_frontier(self.h))
+
+    def plane_rows(self, p, nsamps):
+        info = self.plan.plane(p)
+        out = np.zeros(info.ndm * nsamps, np.float32)
+        check(lib().oracle_engine_plane_rows(self.h, p, ptr(out), out.size))
+        return out.reshape(info.ndm, nsamps)
+
+    def inv_sigma(self, p, nwidths):
+        info = self.plan.plane(p)
+        out = np.zeros(info.ndm * nwidths, np.float32)
+        check(lib().oracle_engine_inv_sigma(self.h, p, ptr(out), out.size))
+        return out.reshape(info.ndm, nwidths)
