@@ -286,6 +286,7 @@ EXPORTS = {
     "fa_engine_keep_planes": (C.c_int, [P, C.c_int]),
     "fa_engine_plane_rows": (C.c_int, [P, U32, PF32, U64]),
     "fa_engine_timings": (C.c_int, [P, PF64, PF64, PF64, PF64]),
+    "fa_engine_level_times": (C.c_int, [P, PF64, PU32, U64, PU64, PU64]),
     "fa_rfi_create": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32, C.c_int,
                                 C.POINTER(P)]),
     "fa_rfi_destroy": (C.c_int, [P]),
@@ -308,9 +309,16 @@ EXPORTS = {
 }
 
 
+MISSING: list = []
+
+
 def _declare(L: C.CDLL) -> None:
     for name, (res, args) in EXPORTS.items():
-        fn = getattr(L, name)
+        try:
+            fn = getattr(L, name)
+        except AttributeError:
+            MISSING.append(name)
+            continue
         fn.restype = res
         fn.argtypes = args
 

@@ -1,0 +1,860 @@
+// Streaming FDMT + matched-filter search on gfx950. See engine.hpp for the
+// rules and the device layout.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <tuple>
+
+namespace fa {
+
+namespace {
+
+constexpr int kBlock       = 256;
+constexpr int kMergeEpt    = 16;   // elements per thread per merge block
+constexpr int kInitPer     = 4;    // outputs per thread per init block
+constexpr int kChunk       = 1024; // search samples per LDS pass
+constexpr uint32_t kMaxBox = 1024; // ladder fits LDS up to this width
+
+// ---- kernels ---------------------------------------------------------------
+
+// Level 0: per channel, the left-fold sums x(t) + x(t-1) + ... + x(t-D) for
+// each of its row delays D (ascending), over [t0 - hc, t0 + tile).
+__global__ void __launch_bounds__(kBlock)
+fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
+                 const int32_t* __restrict__ chan, const int64_t* __restrict__ rows,
+                 float* __restrict__ out, int tile_len, int nchans) {
+    extern __shared__ float sx[];
+    for (int c = blockIdx.y; c < nchans; c += gridDim.y) {
+        const int rf = chan[4 * c], nr = chan[4 * c + 1], hc = chan[4 * c + 2],
+                  dm = chan[4 * c + 3];
+        const int total = tile_len + hc;
+        const int i0    = blockIdx.x * (kBlock * kInitPer);
+        if (i0 >= total) continue;
+        // xc[i] is the sample at t0 - hc + i.
+        const float* xc = x + (uint64_t)c * xstride + (xcol0 - hc);
+        const int span  = kBlock * kInitPer + dm;
+        __syncthreads();
+        for (int j = threadIdx.x; j < span; j += kBlock) {
+            const int gi = i0 - dm + j;
+            sx[j]        = gi < total ? xc[gi] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kInitPer; ++q) {
+            const int i = i0 + threadIdx.x + q * kBlock;
+            if (i >= total) continue;
+            const int li = dm + (i - i0);
+            float acc    = sx[li];
+            int k        = 0;
+            for (int jr = 0; jr < nr; ++jr) {
+                const int64_t* rw = rows + 3 * (int64_t)(rf + jr);
+                const int D       = (int)rw[0];
+                while (k < D) {
+                    ++k;
+                    acc = acc + sx[li - k];
+                }
+                const int h = (int)rw[2];
+                if (i >= hc - h) out[rw[1] + (i - (hc - h))] = acc;
+            }
+        }
+    }
+}
+
+// Level l >= 1: out[r][i] = in[lo_r + i] + in[hi_r + i], i < tile + h_r.
+__global__ void __launch_bounds__(kBlock)
+fdmt_merge_kernel(const float* __restrict__ in, float* __restrict__ out,
+                  const MergeRow* __restrict__ rows, int nrows, int tile_len) {
+    for (int r = blockIdx.y; r < nrows; r += gridDim.y) {
+        const MergeRow m = rows[r];
+        const int len    = tile_len + m.hist;
+        const int base   = blockIdx.x * (kBlock * kMergeEpt);
+        if (base >= len) continue;
+        const float* a = in + m.lo;
+        const float* b = in + m.hi;
+        float* o       = out + m.out;
+        float va[kMergeEpt], vb[kMergeEpt];
+#pragma unroll
+        for (int q = 0; q < kMergeEpt; ++q) {
+            const int i = base + threadIdx.x + q * kBlock;
+            if (i < len) {
+                va[q] = a[i];
+                vb[q] = b[i];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kMergeEpt; ++q) {
+            const int i = base + threadIdx.x + q * kBlock;
+            if (i < len) o[i] = va[q] + vb[q];
+        }
+    }
+}
+
+__device__ __forceinline__ float ladder_sum(const float* L, int S, int j, uint32_t W) {
+    int k         = 31 - __clz(W);
+    float acc     = L[k * S + j];
+    int off       = 1 << k;
+    for (--k; k >= 0; --k) {
+        if ((W >> k) & 1U) {
+            acc = acc + L[k * S + j - off];
+            off += 1 << k;
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ float window_value(const float* L, int S, int j, uint32_t W,
+                                              int trap) {
+    float s = ladder_sum(L, S, j, W);
+    if (trap && W >= 3) {
+        const uint32_t e = (W + 1) / 4;
+        s = (s - 0.5f * ladder_sum(L, S, j, e)) - 0.5f * ladder_sum(L, S, j - (int)(W - e), e);
+    }
+    return s;
+}
+
+struct SearchArgs {
+    const float* buf;
+    const SearchRow* rows;
+    int nrows;
+    const float* inv_cur;
+    const float* inv_or;
+    const uint32_t* widths;
+    int nw;
+    int kmax;
+    int wm1;
+    int tile_len;
+    int64_t tile_t0;   // gulp-local start of the tile
+    uint64_t gulp_abs; // absolute sample of gulp-local 0
+    int first;
+    float thr;
+    int mode;
+    int trap;
+    OpenRun* open;
+    CandRecord* cands;
+    unsigned long long* counter;
+    uint64_t cap;
+    float* keep;       // optional plane-row store [prow][keep_stride]
+    uint64_t keep_stride;
+};
+
+__device__ void emit(const SearchArgs& a, uint32_t prow, uint32_t w, uint64_t start,
+                     uint64_t peak, uint64_t end, float snr) {
+    const unsigned long long idx = atomicAdd(a.counter, 1ULL);
+    if (idx < a.cap) {
+        CandRecord r;
+        r.peak  = peak;
+        r.begin = start;
+        r.end   = end;
+        r.snr   = snr;
+        r.prow  = prow;
+        r.w     = w;
+        r.pad   = 0;
+        a.cands[idx] = r;
+    }
+}
+
+// One block per plane row: ladder in LDS per chunk of kChunk samples, S/N per
+// width, per-width run tracking (thread w walks width w when needed).
+__global__ void __launch_bounds__(kBlock) search_kernel(SearchArgs a) {
+    extern __shared__ float lds[];
+    __shared__ OpenRun st[32];
+    __shared__ uint32_t flags;
+    const int S = kChunk + a.wm1;
+    float* L    = lds;
+    const SearchRow row = a.rows[blockIdx.x];
+    const uint32_t prow = row.prow;
+    if (threadIdx.x < a.nw) st[threadIdx.x] = a.open[(uint64_t)prow * a.nw + threadIdx.x];
+    if (threadIdx.x == 0) flags = 0;
+    const float* src = a.buf + row.in;
+    for (int u = 0; u < a.tile_len; u += kChunk) {
+        const int clen = min(kChunk, a.tile_len - u);
+        const int span = clen + a.wm1;
+        __syncthreads();
+        for (int j = threadIdx.x; j < span; j += kBlock) L[j] = src[u + j];
+        for (int k = 1; k <= a.kmax; ++k) {
+            __syncthreads();
+            const int s = 1 << (k - 1);
+            float* Lk       = L + k * S;
+            const float* Lp = L + (k - 1) * S;
+            for (int j = threadIdx.x; j < span; j += kBlock)
+                if (j >= s) Lk[j] = Lp[j] + Lp[j - s];
+        }
+        __syncthreads();
+        if (a.keep) {
+            for (int i = threadIdx.x; i < clen; i += kBlock)
+                a.keep[(uint64_t)prow * a.keep_stride + a.tile_t0 + u + i] = L[a.wm1 + i];
+        }
+        uint32_t bits = 0;
+        for (int i = threadIdx.x; i < clen; i += kBlock) {
+            const int64_t t = a.tile_t0 + u + i;
+            const int j     = a.wm1 + i;
+            for (int w = 0; w < a.nw; ++w) {
+                const uint32_t W  = a.widths[w];
+                const int64_t warm = (int64_t)row.delay + W - 1;
+                bool searched = true;
+                float inv     = a.inv_cur[(uint64_t)prow * a.nw + w];
+                if (t < warm) {
+                    if (a.first) searched = false;
+                    else inv = a.inv_or[(uint64_t)prow * a.nw + w];
+                }
+                if (!searched) continue;
+                const float snr = window_value(L, S, j, W, a.trap) * inv;
+                if (snr > a.thr) bits |= 1U << w;
+            }
+        }
+        if (bits) atomicOr(&flags, bits);
+        __syncthreads();
+        const int w = threadIdx.x;
+        if (w < a.nw && (((flags >> w) & 1U) || st[w].open)) {
+            const uint32_t W   = a.widths[w];
+            const int64_t warm = (int64_t)row.delay + W - 1;
+            OpenRun o          = st[w];
+            for (int i = 0; i < clen; ++i) {
+                const int64_t t      = a.tile_t0 + u + i;
+                const uint64_t abs_t = a.gulp_abs + (uint64_t)t;
+                bool searched        = true;
+                float inv            = a.inv_cur[(uint64_t)prow * a.nw + w];
+                if (t < warm) {
+                    if (a.first) searched = false;
+                    else inv = a.inv_or[(uint64_t)prow * a.nw + w];
+                }
+                float snr  = 0.0f;
+                bool above = false;
+                if (searched) {
+                    snr   = window_value(L, S, a.wm1 + i, W, a.trap) * inv;
+                    above = snr > a.thr;
+                }
+                if (a.mode == FA_PEAK_ALL) {
+                    if (above) emit(a, prow, w, abs_t, abs_t, abs_t, snr);
+                    continue;
+                }
+                if (above) {
+                    if (!o.open) {
+                        o.open  = 1;
+                        o.start = o.peak = o.end = abs_t;
+                        o.snr   = snr;
+                    } else {
+                        if (snr > o.snr) {
+                            o.snr  = snr;
+                            o.peak = abs_t;
+                        }
+                        o.end = abs_t;
+                    }
+                } else if (o.open) {
+                    emit(a, prow, w, o.start, o.peak, o.end, o.snr);
+                    o.open = 0;
+                }
+            }
+            st[w] = o;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) flags = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < a.nw) a.open[(uint64_t)prow * a.nw + threadIdx.x] = st[threadIdx.x];
+}
+
+__global__ void set_u64_kernel(unsigned long long* p, unsigned long long v0,
+                               unsigned long long v1) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        p[0] = v0;
+        p[1] = v1;
+    }
+}
+
+__global__ void frontier_kernel(const OpenRun* open, uint64_t n, unsigned long long* out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && open[i].open) atomicMin(out, (unsigned long long)open[i].start);
+}
+
+__global__ void zero_open_kernel(OpenRun* open, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) open[i] = OpenRun{0, 0, 0, 0.0f, 0};
+}
+
+__global__ void copy_cols_kernel(const float* __restrict__ src, uint64_t sstride,
+                                 float* __restrict__ dst, uint64_t dstride, uint64_t ncols,
+                                 uint64_t nrows) {
+    for (uint64_t r = blockIdx.y; r < nrows; r += gridDim.y)
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncols;
+             i += (uint64_t)gridDim.x * blockDim.x)
+            dst[r * dstride + i] = src[r * sstride + i];
+}
+
+// ---- host helpers ------------------------------------------------------------
+
+double boxcar_window_variance(int64_t a, int64_t W) {
+    const int64_t lo = std::min(a, W);
+    const double l   = static_cast<double>(lo);
+    return 2.0 * ((l - 1.0) * l * (2.0 * l - 1.0) / 6.0) +
+           static_cast<double>(std::llabs(a - W) + 1) * l * l;
+}
+
+double trapezoid_window_variance(int64_t a, int64_t W) {
+    if (W < 3) return boxcar_window_variance(a, W);
+    const int64_t e = (W + 1) / 4;
+    std::vector<double> pre(W + 1, 0.0);
+    for (int64_t j = 0; j < W; ++j)
+        pre[j + 1] = pre[j] + ((j < e || j >= W - e) ? 0.5 : 1.0);
+    double total = 0.0;
+    for (int64_t m = 0; m < W + a - 1; ++m) {
+        const int64_t hi = std::min(W - 1, m);
+        const int64_t lo = std::max<int64_t>(0, m - a + 1);
+        const double s   = pre[hi + 1] - pre[lo];
+        total += s * s;
+    }
+    return total;
+}
+
+uint64_t env_tile() {
+    if (const char* v = std::getenv("FROST_AMD_TILE")) {
+        const long long t = std::atoll(v);
+        if (t > 0) return static_cast<uint64_t>(t);
+    }
+    return uint64_t{1} << 18;
+}
+
+uint64_t round64(uint64_t n) { return (n + 63) & ~uint64_t{63}; }
+
+} // namespace
+
+// ---- Engine ------------------------------------------------------------------
+
+Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t* kill_mask,
+               int device, uint64_t tile)
+    : plan_(plan), cfg_(cfg), device_(device) {
+    require(cfg.max_boxcar >= 1, "max_boxcar must be >= 1");
+    require(cfg.max_boxcar <= kMaxBox,
+            "max_boxcar above 1024 is not supported by the device search");
+    require(std::isfinite(cfg.detect_thresh) && cfg.detect_thresh > 0.0f,
+            "detect_thresh must be finite and > 0");
+    require(cfg.peak_mode == FA_PEAK_PER_WIDTH || cfg.peak_mode == FA_PEAK_ALL,
+            "peak_mode across_widths is not supported");
+    require(cfg.max_candidates >= 1, "max_candidates must be >= 1");
+    require(plan.cfg.block_nsamps >= 1, "block_nsamps (gulp capacity) must be >= 1");
+    widths_ = box_width_trials(cfg.max_boxcar, cfg.wtsp);
+    require(widths_.size() <= 32, "too many width trials (at most 32)");
+    kmax_ = 0;
+    while ((1U << (kmax_ + 1)) <= widths_.back()) ++kmax_;
+    wm1_ = widths_.back() - 1;
+    cap_ = plan.cfg.block_nsamps;
+    tile_ = std::min<uint64_t>(cap_, tile ? tile : env_tile());
+    const uint64_t nch = plan.cfg.nchans;
+    ctor_mask_.assign(nch, 1);
+    if (kill_mask) ctor_mask_.assign(kill_mask, kill_mask + nch);
+    prev_mask_ = ctor_mask_;
+
+    DeviceGuard dg(device_);
+    build_programs();
+
+    // Extended input, zero history.
+    xstride_ = round64(hist_in_ + cap_);
+    x_.resize(nch * xstride_);
+    FA_HIP(hipMemset(x_.get(), 0, x_.bytes()));
+    uint64_t len0 = 0, len1 = 0;
+    for (uint32_t l = 0; l < plan_.nlevels; ++l)
+        (l & 1 ? len1 : len0) = std::max(l & 1 ? len1 : len0, level_len_[l]);
+    buf_[0].resize(len0);
+    buf_[1].resize(std::max<uint64_t>(len1, 1));
+    const uint64_t nw = widths_.size();
+    inv_cur_.resize((uint64_t)nprows_ * nw);
+    inv_or_.resize((uint64_t)nprows_ * nw);
+    widths_dev_.resize(nw);
+    FA_HIP(hipMemcpy(widths_dev_.get(), widths_.data(), nw * 4, hipMemcpyHostToDevice));
+    open_.resize((uint64_t)nprows_ * nw);
+    FA_HIP(hipMemset(open_.get(), 0, open_.bytes()));
+    cand_cap_ = std::max<uint64_t>(4 * cfg.max_candidates, uint64_t{1} << 20);
+    cands_.resize(cand_cap_);
+    counters_.resize(2);
+    FA_HIP(hipMemset(counters_.get(), 0, counters_.bytes()));
+    level_seconds.assign(plan_.nlevels, 0.0);
+    level_launches.assign(plan_.nlevels, 0.0);
+    int lds_max = 0;
+    FA_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
+    const uint64_t lds_need = (uint64_t)(kmax_ + 1) * (kChunk + wm1_) * sizeof(float) +
+                              32 * sizeof(OpenRun) + 64;
+    if (lds_need > (uint64_t)lds_max)
+        throw std::invalid_argument("max_boxcar too large for the search kernel's LDS");
+    FA_HIP(hipEventCreate(&ev_begin_));
+    FA_HIP(hipEventCreate(&ev_end_));
+}
+
+Engine::~Engine() {
+    for (auto e : events_) (void)hipEventDestroy(e);
+    if (ev_begin_) (void)hipEventDestroy(ev_begin_);
+    if (ev_end_) (void)hipEventDestroy(ev_end_);
+}
+
+void Engine::build_programs() {
+    const uint32_t L = plan_.nlevels - 1;
+    hist_.assign(plan_.nlevels, {});
+    for (uint32_t l = 0; l <= L; ++l) hist_[l].assign(plan_.levels[l].size(), -1);
+    // Plane bookkeeping.
+    plane_first_.clear();
+    nprows_ = 0;
+    for (const auto& p : plan_.planes) {
+        plane_first_.push_back(nprows_);
+        nprows_ += static_cast<uint32_t>(p.delay.size());
+    }
+    // Look-back per row: planes need W_max - 1, upper halves their shift.
+    for (int l = static_cast<int>(L); l >= 0; --l) {
+        for (const auto& p : plan_.planes)
+            if (p.level == static_cast<uint32_t>(l))
+                for (const int32_t r : p.level_row)
+                    hist_[l][r] = std::max<int32_t>(hist_[l][r], static_cast<int32_t>(wm1_));
+        if (l == 0) break;
+        const auto& lv = plan_.levels[l];
+        for (std::size_t r = 0; r < lv.size(); ++r) {
+            const int32_t h = hist_[l][r];
+            if (h < 0) throw std::logic_error("FDMT plan has an unused row");
+            auto& lo = hist_[l - 1][lv.chan_or_lo[r]];
+            auto& hi = hist_[l - 1][lv.hi[r]];
+            lo       = std::max(lo, h);
+            hi       = std::max(hi, h + lv.shift[r]);
+        }
+    }
+    // Buffer layout per level.
+    base_.assign(plan_.nlevels, {});
+    level_len_.assign(plan_.nlevels, 0);
+    for (uint32_t l = 0; l <= L; ++l) {
+        uint64_t off = 0;
+        for (const int32_t h : hist_[l]) {
+            base_[l].push_back(static_cast<int64_t>(off));
+            off += round64(tile_ + static_cast<uint64_t>(h));
+        }
+        level_len_[l] = off;
+    }
+    // Level 0 program.
+    const auto& l0    = plan_.levels[0];
+    const uint64_t nch = plan_.cfg.nchans;
+    std::vector<int32_t> chan(4 * nch, 0);
+    std::vector<int64_t> rows(3 * l0.size());
+    hist_in_ = 0;
+    dmax0_   = 0;
+    for (std::size_t r = 0; r < l0.size(); ++r) {
+        const int32_t c = l0.chan_or_lo[r];
+        if (r == 0 || l0.chan_or_lo[r - 1] != c) chan[4 * c] = static_cast<int32_t>(r);
+        chan[4 * c + 1] += 1;
+        chan[4 * c + 2] = std::max(chan[4 * c + 2], hist_[0][r]);
+        chan[4 * c + 3] = std::max(chan[4 * c + 3], l0.delay[r]);
+        rows[3 * r]     = l0.delay[r];
+        rows[3 * r + 1] = base_[0][r];
+        rows[3 * r + 2] = hist_[0][r];
+    }
+    for (uint64_t c = 0; c < nch; ++c) {
+        hist_in_ = std::max<uint64_t>(hist_in_, static_cast<uint64_t>(chan[4 * c + 2]) +
+                                                    static_cast<uint64_t>(chan[4 * c + 3]));
+        dmax0_   = std::max(dmax0_, chan[4 * c + 3]);
+    }
+    init_chan_.resize(chan.size());
+    FA_HIP(hipMemcpy(init_chan_.get(), chan.data(), chan.size() * 4, hipMemcpyHostToDevice));
+    init_rows_.resize(rows.size());
+    FA_HIP(hipMemcpy(init_rows_.get(), rows.data(), rows.size() * 8, hipMemcpyHostToDevice));
+    // Merge programs.
+    merge_rows_.clear();
+    merge_rows_.resize(plan_.nlevels);
+    for (uint32_t l = 1; l <= L; ++l) {
+        const auto& lv = plan_.levels[l];
+        std::vector<MergeRow> mr(lv.size());
+        for (std::size_t r = 0; r < lv.size(); ++r) {
+            const int32_t h  = hist_[l][r];
+            const int32_t lo = lv.chan_or_lo[r], hi = lv.hi[r];
+            mr[r].out  = base_[l][r];
+            mr[r].lo   = base_[l - 1][lo] + (hist_[l - 1][lo] - h);
+            mr[r].hi   = base_[l - 1][hi] + (hist_[l - 1][hi] - h - lv.shift[r]);
+            mr[r].hist = h;
+            mr[r].pad  = 0;
+        }
+        merge_rows_[l].resize(mr.size());
+        FA_HIP(hipMemcpy(merge_rows_[l].get(), mr.data(), mr.size() * sizeof(MergeRow),
+                         hipMemcpyHostToDevice));
+    }
+    // Search programs per level.
+    search_rows_.clear();
+    search_rows_.resize(plan_.nlevels);
+    search_count_.assign(plan_.nlevels, 0);
+    for (uint32_t l = 0; l <= L; ++l) {
+        std::vector<SearchRow> sr;
+        for (std::size_t p = 0; p < plan_.planes.size(); ++p) {
+            const auto& pd = plan_.planes[p];
+            if (pd.level != l) continue;
+            for (std::size_t i = 0; i < pd.delay.size(); ++i) {
+                const int32_t r = pd.level_row[i];
+                SearchRow s;
+                s.in    = base_[l][r] + (hist_[l][r] - static_cast<int32_t>(wm1_));
+                s.delay = pd.delay[i];
+                s.prow  = plane_first_[p] + static_cast<uint32_t>(i);
+                sr.push_back(s);
+            }
+        }
+        search_count_[l] = static_cast<uint32_t>(sr.size());
+        if (sr.empty()) continue;
+        search_rows_[l].resize(sr.size());
+        FA_HIP(hipMemcpy(search_rows_[l].get(), sr.data(), sr.size() * sizeof(SearchRow),
+                         hipMemcpyHostToDevice));
+    }
+}
+
+void Engine::compute_inv(const std::vector<uint8_t>& mask, std::vector<float>& out) const {
+    const std::size_t nw = widths_.size();
+    // Distinct init lengths -> per-width window variance.
+    std::vector<double> vtab((static_cast<std::size_t>(dmax0_) + 2) * nw, 0.0);
+    for (int32_t a = 1; a <= dmax0_ + 1; ++a)
+        for (std::size_t w = 0; w < nw; ++w)
+            vtab[a * nw + w] = cfg_.use_trapezoid
+                                   ? trapezoid_window_variance(a, widths_[w])
+                                   : boxcar_window_variance(a, widths_[w]);
+    const auto& l0 = plan_.levels[0];
+    std::vector<double> prev(l0.size() * nw), cur;
+    for (std::size_t r = 0; r < l0.size(); ++r)
+        for (std::size_t w = 0; w < nw; ++w)
+            prev[r * nw + w] = mask[l0.chan_or_lo[r]] ? vtab[(l0.delay[r] + 1) * nw + w] : 0.0;
+    out.assign(static_cast<std::size_t>(nprows_) * nw, 0.0f);
+    auto emit_level = [&](uint32_t l, const std::vector<double>& var) {
+        for (std::size_t p = 0; p < plan_.planes.size(); ++p) {
+            const auto& pd = plan_.planes[p];
+            if (pd.level != l) continue;
+            for (std::size_t i = 0; i < pd.delay.size(); ++i) {
+                const std::size_t r = static_cast<std::size_t>(pd.level_row[i]);
+                for (std::size_t w = 0; w < nw; ++w) {
+                    const double v = var[r * nw + w];
+                    out[(plane_first_[p] + i) * nw + w] =
+                        v > 0.0 ? static_cast<float>(1.0 / std::sqrt(v)) : 0.0f;
+                }
+            }
+        }
+    };
+    emit_level(0, prev);
+    for (uint32_t l = 1; l < plan_.nlevels; ++l) {
+        const auto& lv = plan_.levels[l];
+        cur.assign(lv.size() * nw, 0.0);
+        for (std::size_t r = 0; r < lv.size(); ++r) {
+            const std::size_t lo = static_cast<std::size_t>(lv.chan_or_lo[r]);
+            const std::size_t hi = static_cast<std::size_t>(lv.hi[r]);
+            for (std::size_t w = 0; w < nw; ++w)
+                cur[r * nw + w] = prev[lo * nw + w] + prev[hi * nw + w];
+        }
+        emit_level(l, cur);
+        prev.swap(cur);
+    }
+}
+
+std::vector<float> Engine::inv_sigma(uint32_t plane) const {
+    if (plane >= plan_.planes.size()) throw std::invalid_argument("plane out of range");
+    const std::size_t nw = widths_.size();
+    const std::size_t b  = plane_first_[plane] * nw;
+    const std::size_t n  = plan_.planes[plane].delay.size() * nw;
+    if (inv_cur_host_.empty()) {
+        std::vector<float> tmp;
+        compute_inv(ctor_mask_, tmp);
+        return std::vector<float>(tmp.begin() + b, tmp.begin() + b + n);
+    }
+    return std::vector<float>(inv_cur_host_.begin() + b, inv_cur_host_.begin() + b + n);
+}
+
+void Engine::record_pair(int level, hipStream_t s, bool start) {
+    if (start) {
+        while (events_.size() < events_used_ + 2) {
+            hipEvent_t e;
+            FA_HIP(hipEventCreate(&e));
+            events_.push_back(e);
+            event_level_.push_back(0);
+        }
+        event_level_[events_used_] = level;
+        FA_HIP(hipEventRecord(events_[events_used_], s));
+    } else {
+        FA_HIP(hipEventRecord(events_[events_used_ + 1], s));
+        events_used_ += 2;
+    }
+}
+
+void Engine::execute(const float* wf, uint64_t stride, uint64_t offset, uint64_t nsamps,
+                     const uint8_t* kill_mask, hipStream_t s) {
+    require(wf != nullptr, "waterfall must not be null");
+    require(nsamps >= 1 && nsamps <= cap_, "nsamps must be in [1, block_nsamps]");
+    require(stride >= offset + nsamps, "waterfall stride too small");
+    DeviceGuard dg(device_);
+    FA_HIP(hipMemcpy2DAsync(input_columns(), xstride_ * 4, wf + offset, stride * 4, nsamps * 4,
+                            plan_.cfg.nchans, hipMemcpyDeviceToDevice, s));
+    execute_prepared(nsamps, kill_mask, s);
+}
+
+void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t s) {
+    require(N >= 1 && N <= cap_, "nsamps must be in [1, block_nsamps]");
+    require(s != nullptr, "an explicit non-null stream is required");
+    DeviceGuard dg(device_);
+    const uint64_t nch = plan_.cfg.nchans;
+    const std::vector<uint8_t> m =
+        kill_mask ? std::vector<uint8_t>(kill_mask, kill_mask + nch) : ctor_mask_;
+    const bool first   = processed_ == 0;
+    const std::size_t nw = widths_.size();
+    // Noise tables for this gulp's mask (and the OR mask across a change).
+    if (inv_cur_host_.empty() || m != tables_mask_) {
+        compute_inv(m, inv_cur_host_);
+        tables_mask_ = m;
+        FA_HIP(hipMemcpyAsync(inv_cur_.get(), inv_cur_host_.data(), inv_cur_host_.size() * 4,
+                              hipMemcpyHostToDevice, s));
+    }
+    const bool use_or = !first && m != prev_mask_;
+    if (use_or) {
+        std::vector<uint8_t> mor(nch);
+        for (uint64_t c = 0; c < nch; ++c) mor[c] = static_cast<uint8_t>(m[c] | prev_mask_[c]);
+        compute_inv(mor, inv_or_host_);
+        FA_HIP(hipMemcpyAsync(inv_or_.get(), inv_or_host_.data(), inv_or_host_.size() * 4,
+                              hipMemcpyHostToDevice, s));
+    }
+    if (keep_planes_) {
+        keep_.reserve(static_cast<uint64_t>(nprows_) * cap_);
+    }
+    events_used_ = 0;
+    FA_HIP(hipEventRecord(ev_begin_, s));
+    set_u64_kernel<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(counters_.get()), 0,
+                                    processed_ + N);
+    const int lds_search =
+        static_cast<int>((kmax_ + 1) * (kChunk + wm1_) * sizeof(float));
+    const uint32_t L = plan_.nlevels - 1;
+    for (uint64_t t0 = 0; t0 < N; t0 += tile_) {
+        const int tl = static_cast<int>(std::min<uint64_t>(tile_, N - t0));
+        for (uint32_t l = 0; l <= L; ++l) {
+            float* out = buf_[l & 1].get();
+            record_pair(static_cast<int>(l), s, true);
+            if (l == 0) {
+                int hcmax = 0;
+                for (const int32_t h : hist_[0]) hcmax = std::max(hcmax, h);
+                const unsigned gx = ceil_div(tl + hcmax, kBlock * kInitPer);
+                const unsigned gy = static_cast<unsigned>(std::min<uint64_t>(nch, 65535));
+                const size_t lds  = (kBlock * kInitPer + dmax0_) * sizeof(float);
+                fdmt_init_kernel<<<dim3(gx, gy), kBlock, lds, s>>>(
+                    x_.get(), xstride_, static_cast<int64_t>(hist_in_ + t0), init_chan_.get(),
+                    init_rows_.get(), out, tl, static_cast<int>(nch));
+            } else {
+                int hmax = 0;
+                for (const int32_t h : hist_[l]) hmax = std::max(hmax, h);
+                unsigned gx = ceil_div(tl + hmax, kBlock * kMergeEpt);
+                gx          = (gx + 7) & ~7U; // same chunk of adjacent rows -> same XCD
+                const unsigned nrows = static_cast<unsigned>(plan_.levels[l].size());
+                const unsigned gy    = std::min<unsigned>(nrows, 65535);
+                fdmt_merge_kernel<<<dim3(gx, gy), kBlock, 0, s>>>(
+                    buf_[(l - 1) & 1].get(), out, merge_rows_[l].get(), static_cast<int>(nrows),
+                    tl);
+            }
+            record_pair(static_cast<int>(l), s, false);
+            if (search_count_[l]) {
+                SearchArgs a{};
+                a.buf      = out;
+                a.rows     = search_rows_[l].get();
+                a.nrows    = static_cast<int>(search_count_[l]);
+                a.inv_cur  = inv_cur_.get();
+                a.inv_or   = use_or ? inv_or_.get() : inv_cur_.get();
+                a.widths   = widths_dev_.get();
+                a.nw       = static_cast<int>(nw);
+                a.kmax     = static_cast<int>(kmax_);
+                a.wm1      = static_cast<int>(wm1_);
+                a.tile_len = tl;
+                a.tile_t0  = static_cast<int64_t>(t0);
+                a.gulp_abs = processed_;
+                a.first    = first ? 1 : 0;
+                a.thr      = cfg_.detect_thresh;
+                a.mode     = cfg_.peak_mode;
+                a.trap     = cfg_.use_trapezoid ? 1 : 0;
+                a.open     = open_.get();
+                a.cands    = cands_.get();
+                a.counter  = reinterpret_cast<unsigned long long*>(counters_.get());
+                a.cap      = cand_cap_;
+                a.keep     = keep_planes_ ? keep_.get() : nullptr;
+                a.keep_stride = cap_;
+                record_pair(-1, s, true);
+                search_kernel<<<search_count_[l], kBlock, lds_search, s>>>(a);
+                record_pair(-1, s, false);
+            }
+        }
+    }
+    // Frontier: earliest start of an open run, else the next sample.
+    const uint64_t nopen = static_cast<uint64_t>(nprows_) * nw;
+    frontier_kernel<<<ceil_div(nopen, 256), 256, 0, s>>>(
+        open_.get(), nopen, reinterpret_cast<unsigned long long*>(counters_.get()) + 1);
+    // Carry the last H input samples of each channel to the front.
+    if (N >= hist_in_) {
+        if (hist_in_ > 0)
+            FA_HIP(hipMemcpy2DAsync(x_.get(), xstride_ * 4, x_.get() + N, xstride_ * 4,
+                                    hist_in_ * 4, nch, hipMemcpyDeviceToDevice, s));
+    } else {
+        scratch_.reserve(nch * hist_in_);
+        const unsigned gy = static_cast<unsigned>(std::min<uint64_t>(nch, 65535));
+        copy_cols_kernel<<<dim3(ceil_div(hist_in_, 256), gy), 256, 0, s>>>(
+            x_.get() + N, xstride_, scratch_.get(), hist_in_, hist_in_, nch);
+        copy_cols_kernel<<<dim3(ceil_div(hist_in_, 256), gy), 256, 0, s>>>(
+            scratch_.get(), hist_in_, x_.get(), xstride_, hist_in_, nch);
+    }
+    FA_HIP(hipGetLastError());
+    FA_HIP(hipEventRecord(ev_end_, s));
+    prev_mask_ = m;
+    processed_ += N;
+    last_n_          = N;
+    pending_collect_ = true;
+}
+
+void Engine::timing_collect() {
+    std::fill(level_seconds.begin(), level_seconds.end(), 0.0);
+    std::fill(level_launches.begin(), level_launches.end(), 0.0);
+    search_seconds = 0;
+    for (size_t i = 0; i + 1 < events_used_; i += 2) {
+        const double t = event_seconds(events_[i], events_[i + 1]);
+        const int l    = event_level_[i];
+        if (l < 0) {
+            search_seconds += t;
+        } else {
+            level_seconds[l] += t;
+            level_launches[l] += 1;
+        }
+    }
+    total_seconds = event_seconds(ev_begin_, ev_end_);
+}
+
+namespace {
+void finalize(std::vector<fa_raw_candidate>& res, const fa_boxcar_config& cfg, uint64_t& dg,
+              uint64_t& dd) {
+    auto key = [](const fa_raw_candidate& c) {
+        return std::make_tuple(c.subband_level, c.subband_idx, c.dm_idx, c.filter_idx,
+                               c.peak_sample);
+    };
+    std::sort(res.begin(), res.end(),
+              [&](const fa_raw_candidate& a, const fa_raw_candidate& b) { return key(a) < key(b); });
+    std::vector<fa_raw_candidate> kept;
+    kept.reserve(std::min<std::size_t>(res.size(), cfg.max_candidates));
+    uint64_t run = 0;
+    for (std::size_t i = 0; i < res.size(); ++i) {
+        const bool same = i > 0 && res[i].subband_level == res[i - 1].subband_level &&
+                          res[i].subband_idx == res[i - 1].subband_idx &&
+                          res[i].dm_idx == res[i - 1].dm_idx;
+        run = same ? run + 1 : 0;
+        if (cfg.max_cands_per_dm > 0 && run >= cfg.max_cands_per_dm) {
+            ++dd;
+            continue;
+        }
+        if (kept.size() >= cfg.max_candidates) {
+            ++dg;
+            continue;
+        }
+        kept.push_back(res[i]);
+    }
+    res.swap(kept);
+}
+} // namespace
+
+void Engine::collect(hipStream_t s) {
+    DeviceGuard dg(device_);
+    FA_HIP(hipStreamSynchronize(s));
+    if (!pending_collect_) return; // the last call's view stays valid
+    pending_collect_ = false;
+    result_.clear();
+    dropped_global_ = dropped_per_dm_ = 0;
+    uint64_t ctr[2];
+    FA_HIP(hipMemcpy(ctr, counters_.get(), sizeof(ctr), hipMemcpyDeviceToHost));
+    frontier_             = ctr[1];
+    const uint64_t stored = std::min(ctr[0], cand_cap_);
+    dropped_global_       = ctr[0] - stored;
+    std::vector<CandRecord> recs(stored);
+    if (stored)
+        FA_HIP(hipMemcpy(recs.data(), cands_.get(), stored * sizeof(CandRecord),
+                         hipMemcpyDeviceToHost));
+    result_.reserve(stored);
+    for (const auto& r : recs) {
+        const auto it = std::upper_bound(plane_first_.begin(), plane_first_.end(), r.prow);
+        const auto p  = static_cast<std::size_t>(it - plane_first_.begin()) - 1;
+        const auto& pd = plan_.planes[p];
+        const uint32_t i = r.prow - plane_first_[p];
+        fa_raw_candidate c{};
+        c.peak_sample   = r.peak;
+        c.begin_sample  = r.begin;
+        c.end_sample    = r.end;
+        c.snr           = r.snr;
+        c.dm            = pd.dm[i];
+        c.dm_idx        = i;
+        c.filter_idx    = r.w;
+        c.width         = widths_[r.w];
+        c.chan_begin    = pd.c0;
+        c.chan_end      = pd.c1;
+        c.f_low_mhz     = pd.f_low;
+        c.f_high_mhz    = pd.f_high;
+        c.subband_level = static_cast<uint16_t>(pd.depth);
+        c.subband_idx   = static_cast<uint16_t>(pd.idx);
+        c.staggered     = static_cast<uint8_t>(pd.staggered);
+        c.beam_idx      = 0;
+        result_.push_back(c);
+    }
+    uint64_t dg2 = 0, dd2 = 0;
+    finalize(result_, cfg_, dg2, dd2);
+    dropped_global_ += dg2;
+    dropped_per_dm_ = dd2;
+    timing_collect();
+}
+
+void Engine::drain(hipStream_t s) {
+    DeviceGuard dg(device_);
+    if (pending_collect_) collect(s);
+    FA_HIP(hipStreamSynchronize(s));
+    const std::size_t nw = widths_.size();
+    std::vector<OpenRun> st(static_cast<std::size_t>(nprows_) * nw);
+    FA_HIP(hipMemcpy(st.data(), open_.get(), st.size() * sizeof(OpenRun),
+                     hipMemcpyDeviceToHost));
+    result_.clear();
+    for (std::size_t g = 0; g < st.size(); ++g) {
+        if (!st[g].open) continue;
+        const uint32_t prow = static_cast<uint32_t>(g / nw);
+        const uint32_t w    = static_cast<uint32_t>(g % nw);
+        const auto it = std::upper_bound(plane_first_.begin(), plane_first_.end(), prow);
+        const auto p  = static_cast<std::size_t>(it - plane_first_.begin()) - 1;
+        const auto& pd = plan_.planes[p];
+        const uint32_t i = prow - plane_first_[p];
+        fa_raw_candidate c{};
+        c.peak_sample   = st[g].peak;
+        c.begin_sample  = st[g].start;
+        c.end_sample    = st[g].end;
+        c.snr           = st[g].snr;
+        c.dm            = pd.dm[i];
+        c.dm_idx        = i;
+        c.filter_idx    = w;
+        c.width         = widths_[w];
+        c.chan_begin    = pd.c0;
+        c.chan_end      = pd.c1;
+        c.f_low_mhz     = pd.f_low;
+        c.f_high_mhz    = pd.f_high;
+        c.subband_level = static_cast<uint16_t>(pd.depth);
+        c.subband_idx   = static_cast<uint16_t>(pd.idx);
+        c.staggered     = static_cast<uint8_t>(pd.staggered);
+        result_.push_back(c);
+    }
+    zero_open_kernel<<<ceil_div(st.size(), 256), 256, 0, s>>>(open_.get(), st.size());
+    FA_HIP(hipGetLastError());
+    FA_HIP(hipStreamSynchronize(s));
+    dropped_global_ = dropped_per_dm_ = 0;
+    finalize(result_, cfg_, dropped_global_, dropped_per_dm_);
+    frontier_ = processed_;
+}
+
+uint64_t Engine::memory_bytes() const {
+    return x_.bytes() + buf_[0].bytes() + buf_[1].bytes() + inv_cur_.bytes() + inv_or_.bytes() +
+           open_.bytes() + cands_.bytes() + keep_.bytes() + scratch_.bytes() +
+           init_chan_.bytes() + init_rows_.bytes();
+}
+
+void Engine::keep_planes(bool keep) { keep_planes_ = keep; }
+
+void Engine::plane_rows(uint32_t plane, float* out, uint64_t cap, hipStream_t s) const {
+    require(keep_planes_ && keep_.size() > 0, "plane rows were not kept");
+    require(plane < plan_.planes.size(), "plane out of range");
+    const uint64_t n = plan_.planes[plane].delay.size();
+    require(cap >= n * last_n_, "output too small");
+    DeviceGuard dg(device_);
+    if (s) FA_HIP(hipStreamSynchronize(s));
+    else FA_HIP(hipDeviceSynchronize());
+    FA_HIP(hipMemcpy2D(out, last_n_ * 4, keep_.get() + static_cast<uint64_t>(plane_first_[plane]) * cap_,
+                       cap_ * 4, last_n_ * 4, n, hipMemcpyDeviceToHost));
+}
+
+} // namespace fa

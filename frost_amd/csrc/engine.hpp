@@ -1,0 +1,164 @@
+// Streaming FDMT + matched-filter search engine on one GPU: the replacement
+// of dmt::algorithms::FDMTBoxcar as frost drives it
+// (lib/pipeline/pipeline.cpp:346-367,478-527,603-608).
+//
+// Rules (the CPU oracle states the same ones):
+//  * Rows of a level are (node, delay) pairs from FdmtPlan. Level 0 (one
+//    channel, delay D) is x(t) + x(t-1) + ... + x(t-D) summed in that order;
+//    a merged row is lower(t) + upper(t - shift).
+//  * Window sums: ladder D_0 = row, D_k(t) = D_{k-1}(t) + D_{k-1}(t-2^{k-1});
+//    width W sums its set bits highest first. Trapezoid (W >= 3): e=(W+1)/4,
+//    T = (B_W(t) - 0.5*B_e(t)) - 0.5*B_e(t-W+e). S/N = sum * inv_sigma.
+//  * inv_sigma = float(1/sqrt(var)), var = sum over active channels of the
+//    squared weights of (init window) * (filter), exact in double.
+//  * Windows ending at gulp-local t < delay + W - 1 are not searched in the
+//    first gulp and use the OR-mask noise in later gulps.
+//  * per_width peaks: maximal above-threshold runs per (plane, row, width),
+//    first maximum, continued across tiles and gulps.
+//
+// Device layout: the input is an extended channel-major buffer
+// x[nchans][H + capacity] whose first H columns carry the previous gulp's
+// last H samples (H = dt_max + W_max - 1 at most). Each gulp is processed in
+// time tiles of T samples; a level's rows live in one buffer, row r holding
+// the T + h_r samples [t0 - h_r, t0 + T), where h_r is the look-back its
+// consumers need.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "plan.hpp"
+
+namespace fa {
+
+struct CandRecord {
+    uint64_t peak, begin, end;
+    float snr;
+    uint32_t prow; // global plane row
+    uint32_t w;    // width index
+    uint32_t pad;
+};
+
+struct OpenRun {
+    uint64_t start, peak, end;
+    float snr;
+    uint32_t open;
+};
+
+struct MergeRow {
+    int64_t out;  // output offset (sample t0 - h)
+    int64_t lo;   // lower-half input offset aligned to the same sample
+    int64_t hi;   // upper-half input offset aligned to the same sample
+    int32_t hist; // h of this row
+    int32_t pad;
+};
+
+struct SearchRow {
+    int64_t in;     // offset of sample t0 - (W_max - 1) in the level buffer
+    int32_t delay;  // plane row delay (cold prefix / OR-mask window)
+    uint32_t prow;  // global plane row
+};
+
+class Engine {
+public:
+    Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t* kill_mask,
+           int device, uint64_t tile = 0);
+    ~Engine();
+    Engine(const Engine&)            = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Device columns where a gulp's samples go: x + H, row stride xstride().
+    float* input_columns() const { return x_.get() + hist_in_; }
+    uint64_t xstride() const { return xstride_; }
+    uint64_t capacity() const { return cap_; }
+
+    // Searches the `nsamps` samples already written to input_columns().
+    void execute_prepared(uint64_t nsamps, const uint8_t* kill_mask, hipStream_t s);
+    // Copies a channel-major waterfall in first.
+    void execute(const float* wf, uint64_t stride, uint64_t offset, uint64_t nsamps,
+                 const uint8_t* kill_mask, hipStream_t s);
+    // Synchronises `s`, then sorts and caps the last call's candidates.
+    void collect(hipStream_t s);
+    void drain(hipStream_t s);
+
+    const std::vector<fa_raw_candidate>& result() const { return result_; }
+    uint64_t dropped_global() const { return dropped_global_; }
+    uint64_t dropped_per_dm() const { return dropped_per_dm_; }
+    uint64_t frontier() const { return frontier_; }
+    uint64_t memory_bytes() const;
+    const std::vector<uint32_t>& widths() const { return widths_; }
+    std::vector<float> inv_sigma(uint32_t plane) const;
+    const FdmtPlan& plan() const { return plan_; }
+    int device() const { return device_; }
+
+    void keep_planes(bool keep);
+    void plane_rows(uint32_t plane, float* out, uint64_t cap, hipStream_t s) const;
+
+    // Per level: kernel seconds and launches in the last execute (level 0 is
+    // the init kernel); search seconds; total.
+    std::vector<double> level_seconds, level_launches;
+    double search_seconds = 0, total_seconds = 0;
+    uint64_t tile() const { return tile_; }
+    uint64_t hist_in() const { return hist_in_; }
+
+private:
+    void compute_inv(const std::vector<uint8_t>& mask, std::vector<float>& out) const;
+    void build_programs();
+    void timing_collect();
+
+    FdmtPlan plan_;
+    fa_boxcar_config cfg_{};
+    int device_ = 0;
+    std::vector<uint32_t> widths_;
+    uint32_t kmax_ = 0, wm1_ = 0;
+    uint64_t cap_ = 0, tile_ = 0, hist_in_ = 0, xstride_ = 0;
+
+    // Per-level geometry.
+    std::vector<std::vector<int32_t>> hist_;   // h per row
+    std::vector<std::vector<int64_t>> base_;   // buffer offset per row
+    std::vector<uint64_t> level_len_;          // floats per level buffer
+    std::vector<DeviceBuffer<MergeRow>> merge_rows_;
+    // Level 0 program: per channel first row, row count, hc, Dmax; per row D,
+    // out base, h.
+    DeviceBuffer<int32_t> init_chan_;  // [nch][4]
+    DeviceBuffer<int64_t> init_rows_;  // [R0][3]
+    int32_t dmax0_ = 0;
+    // Search program per level (empty if no plane at the level).
+    std::vector<DeviceBuffer<SearchRow>> search_rows_;
+    std::vector<uint32_t> search_count_;
+    std::vector<uint32_t> plane_first_; // first global plane row of each plane
+    uint32_t nprows_ = 0;
+
+    DeviceBuffer<float> x_;
+    DeviceBuffer<float> buf_[2];
+    DeviceBuffer<float> inv_cur_, inv_or_;
+    DeviceBuffer<uint32_t> widths_dev_;
+    DeviceBuffer<OpenRun> open_;
+    DeviceBuffer<CandRecord> cands_;
+    DeviceBuffer<uint64_t> counters_; // [0] candidate count, [1] frontier
+    DeviceBuffer<float> keep_;
+    DeviceBuffer<float> scratch_;
+    uint64_t cand_cap_ = 0;
+    bool keep_planes_  = false;
+
+    std::vector<uint8_t> ctor_mask_, prev_mask_, tables_mask_;
+    std::vector<float> inv_cur_host_, inv_or_host_;
+    uint64_t processed_ = 0;
+    uint64_t last_n_    = 0;
+    bool pending_collect_ = false;
+
+    std::vector<fa_raw_candidate> result_;
+    uint64_t dropped_global_ = 0, dropped_per_dm_ = 0, frontier_ = 0;
+
+    // Timing events: pairs per launch, tagged with a level (-1 = search).
+    std::vector<hipEvent_t> events_;
+    std::vector<int> event_level_;
+    size_t events_used_ = 0;
+    hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
+    void record_pair(int level, hipStream_t s, bool start);
+};
+
+} // namespace fa

@@ -324,6 +324,12 @@ int fa_engine_plane_rows(fa_engine* engine, uint32_t p, float* out,
 int fa_engine_timings(const fa_engine* engine, double* t_init,
                       double* t_merge, double* t_search, double* t_total);
 
+/* Per FDMT level (0 = init kernel): summed kernel seconds and launches in the
+ * last execute, the time-tile length and the number of levels. */
+int fa_engine_level_times(const fa_engine* engine, double* seconds,
+                          uint32_t* launches, uint64_t cap, uint64_t* tile,
+                          uint64_t* nlevels);
+
 /* ---- device RFI (RFICleaner::execute_normalized / DeviceRfi) ------------- */
 
 /* RFICleaner(cfg, layout, baseline_length) (include/frost/prepare/rfi.hpp:260)

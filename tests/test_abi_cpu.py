@@ -1,0 +1,86 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, and its host-side plan agrees with the oracle's."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from frost_amd import abi, search
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "frost_amd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(fa_\w+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = abi.lib()
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(abi.EXPORTS), set(names) ^ set(abi.EXPORTS)
+
+
+def test_struct_layouts():
+    assert C.sizeof(abi.RawCandidate) == 72
+    assert C.sizeof(abi.PlaneInfo) == 40
+
+
+CONFIGS = [
+    dict(nchans=16, dt=1e-3, f0=1400.0, df=-1.0, dm_max=60.0, dm_pulse_width=1000.0,
+         subband_depth=2, block_nsamps=1024),
+    dict(nchans=256, dt=64e-6, f0=1500.0, df=-300 / 256, dm_max=1000.0, block_nsamps=65536),
+    dict(nchans=1024, dt=128e-6, f0=1500.0, df=-300 / 1024, dm_max=500.0, subband_depth=1,
+         block_nsamps=1 << 18),
+]
+
+
+@pytest.mark.parametrize("kw", CONFIGS)
+def test_plan_matches_oracle(oracle, kw):
+    cfg = abi.dedisp_config(**kw)
+    p = search.make_fdmt_plan(cfg)
+    o = oracle.Plan(cfg)
+    s, so = p.summary(), o.summary()
+    assert s == so
+    assert (p.level_rows() == o.level_rows()).all()
+    for i in range(s["nplanes"]):
+        a, b = p.plane(i), o.plane(i)
+        for f, _ in abi.PlaneInfo._fields_:
+            assert getattr(a, f) == getattr(b, f), f
+        dm_a, d_a = p.plane_rows(i)
+        dm_b, d_b = o.plane_rows(i)
+        assert (dm_a == dm_b).all() and (d_a == d_b).all()
+    assert (search.generate_dm_list(cfg) == oracle.dm_list(cfg)).all()
+
+
+def test_plan_production_shape_counts():
+    """4096 ch, 1200-1500 MHz, 64 us, DM 0-2000, 11 planes."""
+    cfg = abi.dedisp_config(4096, 64e-6, 1500.0, -300 / 4096, 2000.0, block_nsamps=1 << 20,
+                            subband_depth=2)
+    p = search.make_fdmt_plan(cfg)
+    s = p.summary()
+    assert s["ndm"] == 3526 and s["nplanes"] == 11 and s["nlevels"] == 13
+    assert 32000 < s["dt_max"] < 33000
+    rows = p.level_rows()
+    assert rows[-1] == 3526 and rows[0] > 30000
+
+
+def test_width_trials_match_oracle(oracle):
+    for mb, w in [(32, 2.0), (256, 2.0), (64, 1.3), (100, 1.0), (1024, 1.5)]:
+        assert search.box_width_trials(mb, w) == oracle.width_trials(mb, w)
+
+
+def test_invalid_arguments_raise():
+    with pytest.raises(abi.InvalidArgument):
+        search.make_fdmt_plan(abi.dedisp_config(16, 1e-3, 1400.0, 0.0, 50.0))
+    with pytest.raises(abi.InvalidArgument):
+        search.make_fdmt_plan(abi.dedisp_config(12, 1e-3, 1400.0, -1.0, 50.0))
+    with pytest.raises(abi.InvalidArgument):
+        search.make_fdmt_plan(abi.dedisp_config(16, 1e-3, 1400.0, -1.0, 50.0, dm_tol=1.0))
+    with pytest.raises(abi.InvalidArgument):
+        search.box_width_trials(0, 2.0)

@@ -1,0 +1,191 @@
+"""GPU parity of the FDMT + search engine against the CPU oracle: dedispersed
+plane rows and candidate rows bit-exact on the same seeded inputs, across
+gulps, mask changes, widths, trapezoids, peak modes and time tiles."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from frost_amd import abi, search
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def stream():
+    return search.Stream(0)
+
+
+def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None):
+    """Runs the GPU engine and the oracle on the same gulps; returns both
+    result lists (per gulp + drain) and, if keep, the plane rows."""
+    if tile:
+        os.environ["FROST_AMD_TILE"] = str(tile)
+    try:
+        plan = search.make_fdmt_plan(dcfg)
+        gpu = search.FDMTBoxcar(plan, bcfg)
+    finally:
+        os.environ.pop("FROST_AMD_TILE", None)
+    if keep:
+        gpu.keep_planes(True)
+    ora = oracle.Engine(dcfg, bcfg, keep_planes=keep)
+    out_g, out_o = [], []
+    nplanes = plan.summary()["nplanes"]
+    for g, wf in enumerate(gulps):
+        m = None if masks is None else masks[g]
+        say(f"  gulp {g} shape={wf.shape}")
+        rg = gpu.execute(wf, stream, kill_mask=m)
+        ro = ora.execute(wf, kill_mask=m)
+        out_g.append(rg)
+        out_o.append(ro)
+        if keep:
+            N = wf.shape[1]
+            for p in range(nplanes):
+                a = gpu.plane_rows(p, N)
+                b = ora.plane_rows(p, N)
+                bad = np.argwhere(a != b)
+                assert bad.size == 0, (g, p, bad[:5], a[tuple(bad[0])], b[tuple(bad[0])])
+        assert gpu.frontier() == ora.frontier(), g
+    out_g.append(gpu.drain(stream))
+    out_o.append(ora.drain())
+    return out_g, out_o
+
+
+def assert_same(out_g, out_o):
+    for k, (a, b) in enumerate(zip(out_g, out_o)):
+        ca, dga, dda = a
+        cb, dgb, ddb = b
+        assert (dga, dda) == (dgb, ddb), k
+        assert len(ca) == len(cb), (k, len(ca), len(cb))
+        for x, y in zip(ca, cb):
+            assert x == y, (k, x, y)
+
+
+def pulse_waterfall(rng, nch, N, pulses, plan_cfg, sigma=1.0, integer=False):
+    """Unit-noise channel-major waterfall with dispersed boxcar pulses."""
+    if integer:
+        wf = rng.integers(-4, 5, (nch, N)).astype(np.float32)
+    else:
+        wf = rng.standard_normal((nch, N)).astype(np.float32) * sigma
+    f_lo = min(plan_cfg.f0, plan_cfg.f0 + (nch - 1) * plan_cfg.df) - abs(plan_cfg.df) / 2
+    bw = abs(plan_cfg.df)
+    for (dm, t0, width, amp) in pulses:
+        for c in range(nch):
+            f = f_lo + (c + 0.5) * bw
+            d = int(round(4.148808e3 * dm * (f ** -2 - (f_lo + nch * bw) ** -2) / plan_cfg.dt))
+            a, b = t0 + d, min(t0 + d + width, N)
+            if a < N:
+                wf[c, a:b] += amp
+    return wf
+
+
+def test_engine_single_gulp_small(oracle, stream):
+    say("test_engine_single_gulp_small")
+    dcfg = abi.dedisp_config(16, 1e-3, 1400.0, -1.0, 60.0, dm_pulse_width=1000.0,
+                             block_nsamps=1024)
+    bcfg = abi.boxcar_config(max_boxcar=16, detect_thresh=4.0)
+    rng = np.random.default_rng(1)
+    wf = pulse_waterfall(rng, 16, 1024, [(30.0, 300, 4, 2.0)], dcfg)
+    g, o = run_pair(oracle, stream, dcfg, bcfg, [wf])
+    assert_same(g, o)
+    assert len(o[0][0]) > 0
+
+
+@pytest.mark.parametrize("depth", [0, 1, 2])
+def test_engine_gulps_and_planes(oracle, stream, depth):
+    say(f"test_engine_gulps_and_planes depth={depth}")
+    dcfg = abi.dedisp_config(64, 256e-6, 1500.0, -300 / 64, 500.0, block_nsamps=4096,
+                             subband_depth=depth)
+    bcfg = abi.boxcar_config(max_boxcar=64, detect_thresh=5.0)
+    rng = np.random.default_rng(10 + depth)
+    gulps = [pulse_waterfall(rng, 64, 4096, [(100.0 * (k + 1), 1000 + 500 * k, 8, 1.5),
+                                             (320.0, 3900, 3, 3.0)], dcfg)
+             for k in range(3)]
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps)
+    assert_same(g, o)
+
+
+def test_engine_mask_change_or_variance(oracle, stream):
+    say("test_engine_mask_change_or_variance")
+    dcfg = abi.dedisp_config(32, 512e-6, 1400.0, -2.0, 300.0, block_nsamps=2048,
+                             subband_depth=1)
+    bcfg = abi.boxcar_config(max_boxcar=32, detect_thresh=4.5)
+    rng = np.random.default_rng(7)
+    m0 = np.ones(32, np.uint8)
+    m1 = m0.copy()
+    m1[3:9] = 0
+    m2 = m1.copy()
+    m2[20] = 0
+    gulps = [pulse_waterfall(rng, 32, 2048, [(150.0, 100 + 600 * k, 4, 2.5)], dcfg)
+             for k in range(3)]
+    for wf, m in zip(gulps, (m0, m1, m2)):
+        wf[m == 0] = 0.0
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, masks=[m0, m1, m2])
+    assert_same(g, o)
+
+
+@pytest.mark.parametrize("trap,wtsp,mode", [(True, 2.0, abi.PEAK_PER_WIDTH),
+                                            (False, 1.5, abi.PEAK_PER_WIDTH),
+                                            (True, 1.3, abi.PEAK_ALL)])
+def test_engine_widths_and_modes(oracle, stream, trap, wtsp, mode):
+    say(f"test_engine_widths_and_modes trap={trap} wtsp={wtsp} mode={mode}")
+    dcfg = abi.dedisp_config(32, 256e-6, 1500.0, -4.0, 400.0, block_nsamps=3000)
+    bcfg = abi.boxcar_config(max_boxcar=100, wtsp=wtsp, use_trapezoid=trap,
+                             detect_thresh=4.0 if mode == abi.PEAK_ALL else 4.5,
+                             peak_mode=mode, max_candidates=200000, max_cands_per_dm=5000)
+    rng = np.random.default_rng(3)
+    gulps = [pulse_waterfall(rng, 32, 3000, [(200.0, 1200, 20, 1.0)], dcfg) for _ in range(2)]
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps)
+    assert_same(g, o)
+
+
+def test_engine_tiles_and_short_gulp(oracle, stream):
+    """Time tiles smaller than the gulp, and a final gulp shorter than the
+    carried history, give the same rows and candidates."""
+    say("test_engine_tiles_and_short_gulp")
+    dcfg = abi.dedisp_config(64, 256e-6, 1500.0, -300 / 64, 800.0, block_nsamps=5000,
+                             subband_depth=2)
+    bcfg = abi.boxcar_config(max_boxcar=32, detect_thresh=5.0)
+    rng = np.random.default_rng(5)
+    gulps = [pulse_waterfall(rng, 64, 5000, [(400.0, 2500, 4, 2.0)], dcfg),
+             pulse_waterfall(rng, 64, 5000, [(600.0, 100, 2, 3.0)], dcfg),
+             pulse_waterfall(rng, 64, 700, [], dcfg)]
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, tile=1111)
+    assert_same(g, o)
+
+
+def test_engine_integer_rows_exact_vs_brute_force(oracle, stream):
+    """Integer inputs: GPU plane rows equal the oracle's brute-force sums."""
+    say("test_engine_integer_rows_exact_vs_brute_force")
+    dcfg = abi.dedisp_config(32, 1e-3, 1400.0, -1.0, 100.0, dm_pulse_width=1000.0,
+                             block_nsamps=1024, subband_depth=1)
+    bcfg = abi.boxcar_config(max_boxcar=8, detect_thresh=100.0)
+    plan = search.make_fdmt_plan(dcfg)
+    gpu = search.FDMTBoxcar(plan, bcfg)
+    gpu.keep_planes(True)
+    o = oracle.Plan(dcfg)
+    hist = oracle.Engine(dcfg, bcfg).hist
+    rng = np.random.default_rng(0)
+    wf = rng.integers(-6, 7, (32, 1024)).astype(np.float32)
+    gpu.execute(wf, stream)
+    x_ext = np.concatenate([np.zeros((32, hist), np.float32), wf], axis=1)
+    for p in range(plan.summary()["nplanes"]):
+        rows = gpu.plane_rows(p, 1024)
+        for r in range(plan.plane(p).ndm):
+            assert (rows[r] == o.brute_force_row(p, r, x_ext, hist, 1024)).all(), (p, r)
+
+
+def test_engine_rejects_bad_config():
+    dcfg = abi.dedisp_config(16, 1e-3, 1400.0, -1.0, 60.0, block_nsamps=1024)
+    plan = search.make_fdmt_plan(dcfg)
+    with pytest.raises(abi.InvalidArgument):
+        search.FDMTBoxcar(plan, abi.boxcar_config(max_boxcar=16, detect_thresh=-1.0))
+    with pytest.raises(abi.InvalidArgument):
+        search.FDMTBoxcar(plan, abi.boxcar_config(max_boxcar=16,
+                                                  peak_mode=abi.PEAK_ACROSS_WIDTHS))
