@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Per-gulp single-pulse search throughput on one MI355X.
+
+One step = one gulp of the production-shape beam (4096 ch uint8, 1200-1500
+MHz, 64 us, gulp 2^20, DM 0-2000, 11 planes, moments + IQRM + Kadane RFI,
+widths to 256) through device RFI excision + calibration, streaming FDMT,
+boxcar search and candidate compaction, ending with the candidate rows on
+the host. The gulp is generated on the device before timing (synthetic,
+seeded) and stays resident in HBM.
+
+Prints one JSON line: DM-trial*samples/s, p50/p99 gulp latency as a fraction
+of the gulp's real-time duration, the FDMT merge kernel against the HBM
+roofline (live HIP-event timings), and the CPU oracle timed on a bounded
+sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def merge_traffic_from_profile():
+    """HBM bytes per merge launch measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (committed summary), if present."""
+    p = ROOT / "profiles" / "fdmt_merge_4096ch_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(W, dev_gulp, stream, sample_rows):
+    """The CPU oracle (RFI restatement + FDMT/search rules) on the first
+    `sample_rows` rows of the same gulp, on this host's cores."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_ffi
+    from frost_amd import abi
+    host = np.zeros(W.gulp_bytes(), np.uint8)
+    dev_gulp.to_host(host, stream)
+    blk = host.reshape(W.gulp, W.nchans)[:sample_rows].copy()
+    lay = abi.BlockLayout(W.nchans, sample_rows, W.dtype, W.tsamp, W.f0, W.df)
+    dcfg = abi.dedisp_config(W.nchans, W.tsamp, W.f0, W.df, W.dm_max, block_nsamps=sample_rows,
+                             subband_depth=W.subband_levels)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    cl = oracle_ffi.Cleaner(W.rfi_config(), lay, W.baseline)
+    wf = cl.execute_normalized(blk, sample_rows)
+    mask = cl.mask()[::-1].copy()
+    eng = oracle_ffi.Engine(dcfg, W.boxcar_config(), kill_mask=mask)
+    eng.execute(wf)
+    el = time.perf_counter() - t0
+    ndm = eng.plan.summary()["ndm"]
+    return {
+        "value": ndm * sample_rows / el,
+        "unit": "DM-trial*samples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"first {sample_rows} rows of the same gulp (one cold gulp: RFI + FDMT over "
+                   f"{sample_rows} samples plus the carried {eng.hist}-sample history + search "
+                   f"of 11 planes), {el:.1f} s wall on {threads} OpenMP threads"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=8192,
+                    help="rows of the gulp timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--workload", default="4096ch_64us_gulp1M_dm2000_11planes")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(beams shard across GPUs as independent processes, see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+    from frost_amd import abi, pipeline, search, workloads
+    W = workloads.WORKLOADS[a.workload]
+    stream = search.Stream(0)
+    gulp = search.DeviceArray(W.gulp_bytes())
+    sc = W.synth_config(0)
+    abi.check(abi.lib().fa_synth_filterbank(C.byref(sc), gulp.ptr, W.gulp, stream.h))
+    log(f"workload {W.name}: gulp {W.gulp_bytes() / 2**30:.2f} GiB generated on device")
+    beam = pipeline.Beam(W.rfi_config(), W.layout(), W.baseline, W.dedisp_config(),
+                         W.boxcar_config())
+    plan = search.make_fdmt_plan(W.dedisp_config())
+    summ = plan.summary()
+    rows = plan.level_rows().astype(np.float64)
+    ndm = summ["ndm"]
+    for i in range(a.warmup):
+        beam.execute_device(gulp, W.gulp, stream)
+        log(f"warmup {i}: {len(beam.result()[0])} candidates")
+    lat, merge_s, merge_launches, ncand = [], 0.0, 0, 0
+    stage = {"rfi": 0.0, "init": 0.0, "merge": 0.0, "search": 0.0}
+    t_start = time.perf_counter()
+    for i in range(a.steps):
+        t0 = time.perf_counter()
+        beam.execute_device(gulp, W.gulp, stream)
+        lat.append(time.perf_counter() - t0)
+        res = beam.result()
+        ncand += len(res[0])
+        sec, cnt, tile = beam.level_times()
+        merge_s += float(sec[1:].sum())
+        merge_launches += int(cnt[1:].sum())
+        tm = beam.engine_timings()
+        rep = beam.rfi_report()
+        stage["rfi"] += rep.t_bandpass + rep.t_moments + rep.t_scan + rep.t_flags + rep.t_apply
+        for k in ("init", "merge", "search"):
+            stage[k] += tm[k]
+        log(f"step {i}: {lat[-1]:.3f} s, {len(res[0])} candidates, masked "
+            f"{rep.chans_masked} ch, flagged {rep.samps_flagged} samples")
+    stream.synchronize()
+    total = time.perf_counter() - t_start
+    value = ndm * W.gulp * a.steps / total
+    # Dominant kernel: the FDMT merge. Algorithmic bytes of one full gulp pass:
+    # read the input rows and write the output rows of every level.
+    alg_bytes_gulp = 4.0 * W.gulp * float((rows[:-1] + rows[1:]).sum())
+    achieved = alg_bytes_gulp * a.steps / merge_s / 1e9
+    launches_per_gulp = merge_launches / a.steps
+    traffic = merge_traffic_from_profile()
+    out = {
+        "metric": "DM-trial·samples/s at 4096 ch, 1 GPU; p99 gulp latency / real-time",
+        "value": value,
+        "unit": "DM-trial*samples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * total / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8 in, f32 compute",
+        "data": "synthetic (seeded device generator: Gaussian noise, 50 dispersed pulses, "
+                "5% narrow-band RFI channels, 20 DM-0 impulses)",
+        "config": {
+            "workload": W.name,
+            "nchans": W.nchans, "tsamp_us": W.tsamp * 1e6, "gulp": W.gulp,
+            "dm_max": W.dm_max, "dm_trials": ndm, "planes": summ["nplanes"],
+            "fdmt_rows": summ["nrows_total"], "boxcar_max": W.boxcar_max,
+            "rfi": "narrow+broad+moments+iqrm+kadane_channel+kadane_spectrum",
+            "candidates_per_gulp": ncand / a.steps,
+        },
+        "latency": {
+            "gulp_seconds": W.gulp_seconds(),
+            "p50_fraction_of_realtime": float(np.percentile(lat, 50)) / W.gulp_seconds(),
+            "p99_fraction_of_realtime": float(np.percentile(lat, 99)) / W.gulp_seconds(),
+            "realtime_beams": W.gulp_seconds() / float(np.mean(lat)),
+        },
+        "stage_seconds_per_gulp": {k: v / a.steps for k, v in stage.items()},
+        "roofline": {
+            "kernel": "fdmt_merge_kernel",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": alg_bytes_gulp / launches_per_gulp,
+            "avg_launch_ms": 1e3 * merge_s / merge_launches,
+        },
+    }
+    if a.cpu_sample > 0:
+        log(f"CPU baseline on {a.cpu_sample} rows ...")
+        out["cpu_baseline"] = cpu_baseline(W, gulp, stream, a.cpu_sample)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

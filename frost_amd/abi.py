@@ -217,6 +217,33 @@ CANDIDATE_FIELDS = ("peak_sample", "begin_sample", "end_sample", "snr", "dm", "d
                     "beam_idx")
 
 
+PULSE_BOXCAR, PULSE_GAUSSIAN, PULSE_SCATTERED = range(3)
+LAYOUT_TIME_MAJOR, LAYOUT_CHANNEL_MAJOR = range(2)
+
+
+class SynthPulse(C.Structure):
+    _fields_ = [("dm", C.c_float), ("t0", C.c_float), ("width", C.c_float),
+                ("amplitude", C.c_float), ("tau", C.c_float), ("shape", C.c_int32)]
+
+
+class SynthRfi(C.Structure):
+    _fields_ = [("chan", C.c_uint32), ("amplitude", C.c_float), ("period", C.c_float),
+                ("pad_", C.c_float), ("t_begin", C.c_uint64), ("t_end", C.c_uint64)]
+
+
+class SynthConfig(C.Structure):
+    _fields_ = [
+        ("nchans", C.c_uint64), ("nsamps", C.c_uint64), ("dtype", C.c_int32),
+        ("layout", C.c_int32), ("tsamp", C.c_double), ("f0", C.c_double),
+        ("df", C.c_double), ("mean", C.c_float), ("sigma", C.c_float),
+        ("seed", C.c_uint64), ("sample_offset", C.c_uint64),
+        ("chan_gain", C.POINTER(C.c_float)), ("pulses", C.POINTER(SynthPulse)),
+        ("npulses", C.c_uint64), ("rfi", C.POINTER(SynthRfi)), ("nrfi", C.c_uint64),
+        ("impulses", C.POINTER(C.c_uint64)), ("nimpulses", C.c_uint64),
+        ("impulse_amplitude", C.c_float), ("pad_", C.c_float),
+    ]
+
+
 def candidate_tuples(arr, n):
     """Candidate rows as plain tuples (CANDIDATE_FIELDS order)."""
     return [tuple(getattr(arr[i], f) for f in CANDIDATE_FIELDS) for i in range(n)]
@@ -295,6 +322,7 @@ EXPORTS = {
     "fa_rfi_get_report": (C.c_int, [P, C.POINTER(RfiReport)]),
     "fa_rfi_calibration": (C.c_int, [P, PU64, PF32, PF32, PF32, U64]),
     "fa_rfi_time_flags": (C.c_int, [P, PU8, U64]),
+    "fa_synth_filterbank": (C.c_int, [C.POINTER(SynthConfig), P, U64, P]),
     "fa_beam_create": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32,
                                  C.POINTER(DedispConfig), C.POINTER(BoxcarConfig), C.c_int,
                                  C.POINTER(P)]),

@@ -10,13 +10,11 @@
 
 #include "common.hpp"
 #include "engine.hpp"
+#include "handles.hpp"
 #include "plan.hpp"
 
 struct fa_plan {
     fa::FdmtPlan plan;
-};
-struct fa_engine {
-    std::unique_ptr<fa::Engine> eng;
 };
 
 namespace fa {
@@ -248,15 +246,19 @@ int fa_engine_create(const fa_plan* plan, const fa_boxcar_config* cfg,
                      const uint8_t* kill_mask, int device, fa_engine** engine) {
     return guard([&] {
         require(plan && cfg && engine, "plan, cfg and engine must not be null");
-        auto e  = std::make_unique<fa_engine>();
-        e->eng  = std::make_unique<fa::Engine>(plan->plan, *cfg, kill_mask, device);
-        *engine = e.release();
+        auto e   = std::make_unique<fa_engine>();
+        e->owned = std::make_unique<fa::Engine>(plan->plan, *cfg, kill_mask, device);
+        e->eng   = e->owned.get();
+        *engine  = e.release();
     });
 }
 
 int fa_engine_destroy(fa_engine* engine) {
-    delete engine;
-    return FA_OK;
+    return guard([&] {
+        if (!engine) return;
+        require(engine->owned != nullptr, "a beam's borrowed engine handle is not destroyed");
+        delete engine;
+    });
 }
 
 int fa_engine_execute(fa_engine* engine, const float* waterfall, uint64_t stride,

@@ -157,18 +157,51 @@ __device__ void emit(const SearchArgs& a, uint32_t prow, uint32_t w, uint64_t st
     }
 }
 
+// S/N of window ending at ladder index j for width slot w. POW2: every width
+// is a power of two (window = one ladder level, trapezoid edges another).
+template <bool POW2>
+__device__ __forceinline__ float width_sum(const float* L, int S, int j, uint32_t W, int kw,
+                                           int trap) {
+    if (POW2) {
+        float s = L[kw * S + j];
+        if (trap && W >= 3) {
+            // e = (W + 1) / 4 = W / 4 = 2^(kw - 2)
+            const float* Le = L + (kw - 2) * S;
+            s = (s - 0.5f * Le[j]) - 0.5f * Le[j - (int)(W - (W >> 2))];
+        }
+        return s;
+    }
+    return window_value(L, S, j, W, trap);
+}
+
 // One block per plane row: ladder in LDS per chunk of kChunk samples, S/N per
 // width, per-width run tracking (thread w walks width w when needed).
+template <bool POW2>
 __global__ void __launch_bounds__(kBlock) search_kernel(SearchArgs a) {
     extern __shared__ float lds[];
     __shared__ OpenRun st[32];
+    __shared__ float s_inv[32], s_invor[32];
+    __shared__ uint32_t s_w[32];
+    __shared__ int s_kw[32];
+    __shared__ int64_t s_warm[32];
     __shared__ uint32_t flags;
     const int S = kChunk + a.wm1;
     float* L    = lds;
     const SearchRow row = a.rows[blockIdx.x];
     const uint32_t prow = row.prow;
-    if (threadIdx.x < a.nw) st[threadIdx.x] = a.open[(uint64_t)prow * a.nw + threadIdx.x];
+    const int nw        = a.nw;
+    if (threadIdx.x < nw) {
+        const int w  = threadIdx.x;
+        st[w]        = a.open[(uint64_t)prow * nw + w];
+        s_inv[w]     = a.inv_cur[(uint64_t)prow * nw + w];
+        s_invor[w]   = a.inv_or[(uint64_t)prow * nw + w];
+        s_w[w]       = a.widths[w];
+        s_kw[w]      = 31 - __clz(a.widths[w]);
+        s_warm[w]    = (int64_t)row.delay + a.widths[w] - 1;
+    }
     if (threadIdx.x == 0) flags = 0;
+    __syncthreads();
+    const int64_t warm_max = s_warm[nw - 1] > s_warm[0] ? s_warm[nw - 1] : s_warm[0];
     const float* src = a.buf + row.in;
     for (int u = 0; u < a.tile_len; u += kChunk) {
         const int clen = min(kChunk, a.tile_len - u);
@@ -180,52 +213,80 @@ __global__ void __launch_bounds__(kBlock) search_kernel(SearchArgs a) {
             const int s = 1 << (k - 1);
             float* Lk       = L + k * S;
             const float* Lp = L + (k - 1) * S;
-            for (int j = threadIdx.x; j < span; j += kBlock)
-                if (j >= s) Lk[j] = Lp[j] + Lp[j - s];
+            for (int j = threadIdx.x + s; j < span; j += kBlock) Lk[j] = Lp[j] + Lp[j - s];
         }
         __syncthreads();
         if (a.keep) {
             for (int i = threadIdx.x; i < clen; i += kBlock)
                 a.keep[(uint64_t)prow * a.keep_stride + a.tile_t0 + u + i] = L[a.wm1 + i];
         }
-        uint32_t bits = 0;
-        for (int i = threadIdx.x; i < clen; i += kBlock) {
-            const int64_t t = a.tile_t0 + u + i;
-            const int j     = a.wm1 + i;
-            for (int w = 0; w < a.nw; ++w) {
-                const uint32_t W  = a.widths[w];
-                const int64_t warm = (int64_t)row.delay + W - 1;
-                bool searched = true;
-                float inv     = a.inv_cur[(uint64_t)prow * a.nw + w];
-                if (t < warm) {
-                    if (a.first) searched = false;
-                    else inv = a.inv_or[(uint64_t)prow * a.nw + w];
+        const int64_t tc0 = a.tile_t0 + u;
+        uint32_t bits     = 0;
+        if (tc0 >= warm_max) {
+            // Whole chunk past the cold / OR-mask window of every width.
+            for (int w = 0; w < nw; ++w) {
+                const uint32_t W = s_w[w];
+                const int kw     = s_kw[w];
+                const float inv  = s_inv[w];
+                bool any         = false;
+#pragma unroll
+                for (int q = 0; q < kChunk / kBlock; ++q) {
+                    const int i = threadIdx.x + q * kBlock;
+                    if (i < clen) {
+                        const float snr = width_sum<POW2>(L, S, a.wm1 + i, W, kw, a.trap) * inv;
+                        any |= snr > a.thr;
+                    }
                 }
-                if (!searched) continue;
-                const float snr = window_value(L, S, j, W, a.trap) * inv;
-                if (snr > a.thr) bits |= 1U << w;
+                if (any) bits |= 1U << w;
+            }
+        } else {
+            for (int w = 0; w < nw; ++w) {
+                const uint32_t W    = s_w[w];
+                const int kw        = s_kw[w];
+                const int64_t warm  = s_warm[w];
+                bool any            = false;
+#pragma unroll
+                for (int q = 0; q < kChunk / kBlock; ++q) {
+                    const int i = threadIdx.x + q * kBlock;
+                    if (i < clen) {
+                        const int64_t t = tc0 + i;
+                        float inv       = s_inv[w];
+                        bool searched   = true;
+                        if (t < warm) {
+                            if (a.first) searched = false;
+                            else inv = s_invor[w];
+                        }
+                        if (searched) {
+                            const float snr =
+                                width_sum<POW2>(L, S, a.wm1 + i, W, kw, a.trap) * inv;
+                            any |= snr > a.thr;
+                        }
+                    }
+                }
+                if (any) bits |= 1U << w;
             }
         }
         if (bits) atomicOr(&flags, bits);
         __syncthreads();
         const int w = threadIdx.x;
-        if (w < a.nw && (((flags >> w) & 1U) || st[w].open)) {
-            const uint32_t W   = a.widths[w];
-            const int64_t warm = (int64_t)row.delay + W - 1;
+        if (w < nw && (((flags >> w) & 1U) || st[w].open)) {
+            const uint32_t W   = s_w[w];
+            const int kw       = s_kw[w];
+            const int64_t warm = s_warm[w];
             OpenRun o          = st[w];
             for (int i = 0; i < clen; ++i) {
-                const int64_t t      = a.tile_t0 + u + i;
+                const int64_t t      = tc0 + i;
                 const uint64_t abs_t = a.gulp_abs + (uint64_t)t;
                 bool searched        = true;
-                float inv            = a.inv_cur[(uint64_t)prow * a.nw + w];
+                float inv            = s_inv[w];
                 if (t < warm) {
                     if (a.first) searched = false;
-                    else inv = a.inv_or[(uint64_t)prow * a.nw + w];
+                    else inv = s_invor[w];
                 }
                 float snr  = 0.0f;
                 bool above = false;
                 if (searched) {
-                    snr   = window_value(L, S, a.wm1 + i, W, a.trap) * inv;
+                    snr   = width_sum<POW2>(L, S, a.wm1 + i, W, kw, a.trap) * inv;
                     above = snr > a.thr;
                 }
                 if (a.mode == FA_PEAK_ALL) {
@@ -255,7 +316,7 @@ __global__ void __launch_bounds__(kBlock) search_kernel(SearchArgs a) {
         if (threadIdx.x == 0) flags = 0;
     }
     __syncthreads();
-    if (threadIdx.x < a.nw) a.open[(uint64_t)prow * a.nw + threadIdx.x] = st[threadIdx.x];
+    if (threadIdx.x < nw) a.open[(uint64_t)prow * nw + threadIdx.x] = st[threadIdx.x];
 }
 
 __global__ void set_u64_kernel(unsigned long long* p, unsigned long long v0,
@@ -341,6 +402,8 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     kmax_ = 0;
     while ((1U << (kmax_ + 1)) <= widths_.back()) ++kmax_;
     wm1_ = widths_.back() - 1;
+    pow2_widths_ = true;
+    for (const uint32_t w : widths_) pow2_widths_ = pow2_widths_ && (w & (w - 1)) == 0;
     cap_ = plan.cfg.block_nsamps;
     tile_ = std::min<uint64_t>(cap_, tile ? tile : env_tile());
     const uint64_t nch = plan.cfg.nchans;
@@ -668,7 +731,10 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                 a.keep     = keep_planes_ ? keep_.get() : nullptr;
                 a.keep_stride = cap_;
                 record_pair(-1, s, true);
-                search_kernel<<<search_count_[l], kBlock, lds_search, s>>>(a);
+                if (pow2_widths_)
+                    search_kernel<true><<<search_count_[l], kBlock, lds_search, s>>>(a);
+                else
+                    search_kernel<false><<<search_count_[l], kBlock, lds_search, s>>>(a);
                 record_pair(-1, s, false);
             }
         }

@@ -1,0 +1,1348 @@
+// Device RFI excision + calibration on gfx950. See rfi.hpp for the stage map;
+// every kernel follows the CPU cleaner's operation order
+// (lib/cpu/rfi_cpu.cpp, lib/cpu/rfi_cpu_measure.cpp,
+// lib/cpu/bandpass_kernels.hpp, lib/cpu/spectra_cpu.cpp) so that masks,
+// flags and fills are bit-exact.
+#include "rfi.hpp"
+#include "handles.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+namespace fa {
+
+namespace {
+
+constexpr uint64_t kMaxBandpassRows = 3125;
+constexpr uint64_t kMinWindow       = 5;
+constexpr uint64_t kMinBlockSamps   = 64;
+constexpr int kMaxDraws             = 10;
+constexpr uint64_t kMaxChunks       = 64;
+constexpr uint64_t kMinChunkRows    = 1024;
+constexpr uint64_t kStreamCell = 1, kStreamTime = 2, kStreamNoise = 3;
+
+int dtype_bits(int dtype) {
+    switch (dtype) {
+    case FA_UINT1: return 1;
+    case FA_UINT2: return 2;
+    case FA_UINT4: return 4;
+    case FA_UINT8: return 8;
+    case FA_UINT16: return 16;
+    case FA_UINT32:
+    case FA_FLOAT32: return 32;
+    default: throw std::invalid_argument("RFI: unknown dtype");
+    }
+}
+
+// ---- device helpers ------------------------------------------------------------
+
+__device__ __forceinline__ float unpack(const uint8_t* row, uint64_t c, int dtype) {
+    switch (dtype) {
+    case FA_UINT8: return (float)row[c];
+    case FA_UINT16: return (float)reinterpret_cast<const uint16_t*>(row)[c];
+    case FA_UINT32: return (float)reinterpret_cast<const uint32_t*>(row)[c];
+    case FA_FLOAT32: return reinterpret_cast<const float*>(row)[c];
+    default: {
+        const int bits     = dtype == FA_UINT1 ? 1 : (dtype == FA_UINT2 ? 2 : 4);
+        const uint64_t bit = c * (uint64_t)bits;
+        return (float)((row[bit >> 3] >> (bit & 7)) & ((1u << bits) - 1u));
+    }
+    }
+}
+
+__device__ __forceinline__ float fmin_(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float fmax_(float a, float b) { return a < b ? b : a; }
+
+__device__ __forceinline__ float median5_dev(float a, float b, float c, float d, float e) {
+    const float f = fmax_(fmin_(a, b), fmin_(c, d));
+    const float g = fmin_(fmax_(a, b), fmax_(c, d));
+    return e < f ? (f < g ? f : (e < g ? g : e)) : (e < g ? e : (f < g ? g : f));
+}
+
+__device__ __forceinline__ uint64_t splitmix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ uint64_t hash5(uint64_t a, uint64_t b, uint64_t c, uint64_t d,
+                                          uint64_t e) {
+    return splitmix(splitmix(splitmix(splitmix(splitmix(a) ^ b) ^ c) ^ d) ^ e);
+}
+
+__device__ __forceinline__ float normal_draw(uint64_t h) {
+    const double u1 = ((double)(h >> 11) + 0.5) * 0x1.0p-53;
+    const double u2 = ((double)(splitmix(h) >> 11) + 0.5) * 0x1.0p-53;
+    return (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+struct Geometry {
+    const uint8_t* block;
+    uint64_t row_bytes, nch, nsamps, window_len, nwin;
+    int dtype;
+    __device__ uint64_t window_of(uint64_t t) const {
+        const uint64_t w = t / window_len;
+        return w < nwin - 1 ? w : nwin - 1;
+    }
+    __device__ uint64_t window_begin(uint64_t w) const { return w * window_len; }
+    __device__ uint64_t window_end(uint64_t w) const {
+        return w + 1 == nwin ? nsamps : (w + 1) * window_len;
+    }
+    __device__ float x(uint64_t t, uint64_t c) const {
+        return unpack(block + t * row_bytes, c, dtype);
+    }
+};
+
+// ---- bandpass ------------------------------------------------------------------
+
+// One lane per (window, channel): remedian of the 5^k sampled rows, remedian of
+// |x - median| (x 1.4826), then two 4.5-sigma clipped FP64 refinements
+// (lib/cpu/rfi_cpu_measure.cpp:19-50, lib/cpu/spectra_cpu.cpp:44-130,
+// lib/cpu/bandpass_kernels.hpp:17-76).
+__global__ void __launch_bounds__(64)
+bandpass_kernel(Geometry g, float* __restrict__ loc_out, float* __restrict__ scale_out,
+                float* __restrict__ inv_out, uint8_t* __restrict__ measured) {
+    __shared__ float buf[5][5][64];
+    __shared__ int cnt[6][64];
+    const uint64_t w = blockIdx.x;
+    const uint64_t c = (uint64_t)blockIdx.y * 64 + threadIdx.x;
+    const int lane   = threadIdx.x;
+    if (c >= g.nch) return;
+    const uint64_t b   = g.window_begin(w);
+    const uint64_t len = g.window_end(w) - b;
+    uint64_t m         = 1;
+    const uint64_t lim = len < kMaxBandpassRows ? len : kMaxBandpassRows;
+    while (m <= lim / 5) m *= 5;
+    auto sample = [&](uint64_t i) { return g.x(b + (i * len) / m, c); };
+    // Streaming remedian: level buffers of five per lane.
+    auto remedian = [&](bool absdiff, float centre, float& lo, float& hi) {
+        for (int k = 0; k < 6; ++k) cnt[k][lane] = 0;
+        float result = 0.0f;
+        lo = hi = sample(0);
+        for (uint64_t i = 0; i < m; ++i) {
+            float v = sample(i);
+            if (!absdiff) {
+                lo = fmin_(lo, v);
+                hi = fmax_(hi, v);
+            } else {
+                v = fabsf(v - centre);
+            }
+            int lev = 0;
+            for (;;) {
+                if (m == 1) {
+                    result = v;
+                    break;
+                }
+                const int n = cnt[lev][lane];
+                buf[lev][n][lane] = v;
+                if (n + 1 < 5) {
+                    cnt[lev][lane] = n + 1;
+                    break;
+                }
+                cnt[lev][lane] = 0;
+                v = median5_dev(buf[lev][0][lane], buf[lev][1][lane], buf[lev][2][lane],
+                                buf[lev][3][lane], buf[lev][4][lane]);
+                uint64_t p = 5;
+                for (int q = 0; q < lev; ++q) p *= 5;
+                if (p == m) {
+                    result = v;
+                    break;
+                }
+                ++lev;
+            }
+        }
+        return result;
+    };
+    float lo, hi, dummy0, dummy1;
+    const float bp  = remedian(false, 0.0f, lo, hi);
+    const float mad = remedian(true, bp, dummy0, dummy1);
+    float loc       = bp;
+    float scale     = mad * 1.4826f;
+    const uint64_t o = w * g.nch + c;
+    if (!(hi > lo)) {
+        loc_out[o]   = lo;
+        scale_out[o] = 0.0f;
+        inv_out[o]   = 0.0f;
+        measured[o]  = 0;
+        return;
+    }
+    double centre = loc;
+    double s      = scale;
+    if (!(s > 0.0)) {
+        double mean = 0.0;
+        for (uint64_t i = 0; i < m; ++i) mean += sample(i);
+        mean /= (double)m;
+        double ss = 0.0;
+        for (uint64_t i = 0; i < m; ++i) {
+            const double d = sample(i) - mean;
+            ss += d * d;
+        }
+        s = sqrt(ss / (double)m);
+    }
+    for (int it = 0; it < 2; ++it) {
+        const double cut = 4.5 * s;
+        double count = 0.0, sum = 0.0;
+        for (uint64_t i = 0; i < m; ++i) {
+            const double d = sample(i) - centre;
+            if (fabs(d) <= cut) {
+                count += 1.0;
+                sum += d;
+            }
+        }
+        if (count < 2.0) break;
+        const double shift = sum / count;
+        double ss          = 0.0;
+        for (uint64_t i = 0; i < m; ++i) {
+            const double d = sample(i) - centre;
+            if (fabs(d) <= cut) ss += (d - shift) * (d - shift);
+        }
+        const double clipped = sqrt(ss / count);
+        if (!(clipped > 0.0)) break;
+        centre += shift;
+        s = clipped;
+    }
+    loc          = (float)centre;
+    scale        = (float)s;
+    loc_out[o]   = loc;
+    scale_out[o] = scale;
+    inv_out[o]   = 1.0f / scale;
+    measured[o]  = 1;
+}
+
+// ---- moments -------------------------------------------------------------------
+
+// One lane per (window, channel): FP64 raw and clipped moments of
+// z = (x - loc) * inv in time order (lib/cpu/rfi_cpu_measure.cpp:91-138).
+__global__ void __launch_bounds__(64)
+moments_kernel(Geometry g, const float* __restrict__ loc, const float* __restrict__ inv,
+               const uint8_t* __restrict__ static_mask, float clip,
+               MomentSumsDev* __restrict__ out) {
+    const uint64_t w = blockIdx.x;
+    const uint64_t c = (uint64_t)blockIdx.y * 64 + threadIdx.x;
+    if (c >= g.nch) return;
+    MomentSumsDev s{};
+    const float lc = loc[w * g.nch + c], ic = inv[w * g.nch + c];
+    if (static_mask[c] != 0 && ic != 0.0f) {
+        const uint64_t b = g.window_begin(w), e = g.window_end(w);
+        double prev      = 0.0;
+        for (uint64_t t = b; t < e; ++t) {
+            const float zf  = (g.x(t, c) - lc) * ic;
+            const double z  = zf;
+            const double z2 = z * z;
+            s.n += 1.0;
+            s.s1 += z;
+            s.s2 += z2;
+            s.s3 += z2 * z;
+            s.s4 += z2 * z2;
+            if (t > b) s.lag += z * prev;
+            prev = z;
+            if (fabs(z) <= (double)clip) {
+                s.nc += 1.0;
+                s.c1 += z;
+                s.c2 += z2;
+            }
+        }
+    }
+    out[w * g.nch + c] = s;
+}
+
+// ---- Kadane (lib/cpu/rfi_kernels.hpp:65-133) ------------------------------------
+
+__device__ __forceinline__ void kadane_init(KadaneDev& k) {
+    k.total = 0.0;
+    k.first = 0;
+    k.last  = -1;
+    k.empty = 1;
+    k.pad   = 0;
+    k.prefix = {0.0, 0, -1};
+    k.suffix = {0.0, 0, -1};
+    k.best   = {0.0, 0, -1};
+}
+
+__device__ __forceinline__ void kadane_append(KadaneDev& k, int64_t i, double u) {
+    if (k.empty) {
+        k.empty  = 0;
+        k.first  = i;
+        k.prefix = {0.0, i, i - 1};
+        k.suffix = {0.0, i, i - 1};
+        k.best   = k.suffix;
+    }
+    k.last = i;
+    k.total += u;
+    if (k.total > k.prefix.sum) k.prefix = {k.total, k.first, i};
+    if (k.suffix.sum + u > 0.0) {
+        if (k.suffix.end - k.suffix.start + 1 <= 0) k.suffix.start = i;
+        k.suffix.sum += u;
+        k.suffix.end = i;
+    } else {
+        k.suffix = {0.0, i + 1, i};
+    }
+    if (k.suffix.sum > k.best.sum) k.best = k.suffix;
+}
+
+__device__ KadaneDev kadane_combine(const KadaneDev& a, const KadaneDev& b) {
+    if (a.empty) return b;
+    if (b.empty) return a;
+    KadaneDev r;
+    r.empty  = 0;
+    r.pad    = 0;
+    r.first  = a.first;
+    r.last   = b.last;
+    r.total  = a.total + b.total;
+    r.prefix = a.prefix;
+    if (a.total + b.prefix.sum > r.prefix.sum)
+        r.prefix = {a.total + b.prefix.sum, a.first, b.prefix.end};
+    const int64_t join =
+        (a.suffix.end - a.suffix.start + 1) > 0 ? a.suffix.start : b.first;
+    r.suffix = b.suffix;
+    if (b.total + a.suffix.sum > r.suffix.sum) r.suffix = {b.total + a.suffix.sum, join, b.last};
+    r.best = a.best;
+    if (b.best.sum > r.best.sum) r.best = b.best;
+    const double cross = a.suffix.sum + b.prefix.sum;
+    if (cross > r.best.sum) r.best = {cross, join, b.prefix.end};
+    return r;
+}
+
+// ---- scan: rows ------------------------------------------------------------------
+
+struct ScanArgs {
+    Geometry g;
+    const float* loc;
+    const float* inv;
+    const uint8_t* mask;
+    const int32_t* good_prefix;
+    float clip;
+    int kspec;
+    double min_bw, whole, kthr;
+    float* s_det;
+    float* s_keep;
+    float* row_std;
+    int32_t* kspec_span; // [nsamps][4]: pos lo, hi, neg lo, hi
+    uint8_t* kwhole;
+};
+
+constexpr int kRowTileCh = 256;
+
+// 64 rows per block; channel tiles transposed through LDS so each lane walks
+// its own row in channel order (lib/cpu/rfi_cpu_measure.cpp:365-415).
+__global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
+    extern __shared__ uint32_t tile[]; // [64][words + 1]
+    const Geometry& g   = a.g;
+    const uint64_t t0   = (uint64_t)blockIdx.x * 64;
+    const int lane      = threadIdx.x;
+    const uint64_t t    = t0 + lane;
+    const bool active   = t < g.nsamps;
+    const int bits      = g.dtype == FA_UINT1 ? 1 : g.dtype == FA_UINT2 ? 2 : g.dtype == FA_UINT4 ? 4
+                         : g.dtype == FA_UINT8 ? 8 : g.dtype == FA_UINT16 ? 16 : 32;
+    const int words     = kRowTileCh * bits / 32; // 32-bit words per row per tile
+    const int pitch     = words + 1;
+    const uint64_t w    = active ? g.window_of(t) : 0;
+    const float* lc     = a.loc + w * g.nch;
+    const float* ic     = a.inv + w * g.nch;
+    double sum_det = 0, sum_keep = 0, n_keep = 0, sx = 0, sxx = 0, n_meas = 0;
+    KadaneDev sp, sn;
+    kadane_init(sp);
+    kadane_init(sn);
+    for (uint64_t c0 = 0; c0 < g.nch; c0 += kRowTileCh) {
+        const uint64_t nc   = g.nch - c0 < kRowTileCh ? g.nch - c0 : kRowTileCh;
+        const uint64_t nbyt = nc * bits / 8;
+        __syncthreads();
+        for (int r = 0; r < 64; ++r) {
+            const uint64_t tr = t0 + r;
+            if (tr >= g.nsamps) break;
+            const uint8_t* rowp = g.block + tr * g.row_bytes + c0 * bits / 8;
+            for (int k = lane; k < words; k += 64) {
+                uint32_t v = 0;
+                const uint64_t byte = (uint64_t)k * 4;
+                if (byte + 4 <= nbyt) {
+                    if ((reinterpret_cast<uintptr_t>(rowp + byte) & 3) == 0)
+                        v = *reinterpret_cast<const uint32_t*>(rowp + byte);
+                    else
+                        for (int q = 0; q < 4; ++q) v |= (uint32_t)rowp[byte + q] << (8 * q);
+                } else {
+                    for (uint64_t q = 0; byte + q < nbyt && q < 4; ++q)
+                        v |= (uint32_t)rowp[byte + q] << (8 * q);
+                }
+                tile[r * pitch + k] = v;
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        const uint8_t* myrow = reinterpret_cast<const uint8_t*>(tile + lane * pitch);
+        for (uint64_t j = 0; j < nc; ++j) {
+            const uint64_t c = c0 + j;
+            if (a.mask[c] == 0) continue;
+            const float x = unpack(myrow, j, g.dtype);
+            if (ic[c] == 0.0f) continue;
+            n_meas += 1.0;
+            const float zc = (x - lc[c]) * ic[c];
+            const float cl = zc < -a.clip ? -a.clip : (a.clip < zc ? a.clip : zc);
+            sum_det += cl;
+            if (fabsf(zc) <= a.clip) {
+                sum_keep += zc;
+                n_keep += 1.0;
+            }
+            sx += x;
+            sxx += (double)x * x;
+            if (a.kspec) {
+                kadane_append(sp, (int64_t)c, cl);
+                kadane_append(sn, (int64_t)c, -cl);
+            }
+        }
+    }
+    if (!active) return;
+    const double dn = n_meas > 1.0 ? n_meas : 1.0;
+    a.s_det[t]      = (float)(sum_det / dn);
+    a.s_keep[t]     = (float)(n_keep > 0.0 ? sum_keep / n_keep : 0.0);
+    const double mx = sx / dn;
+    const double vr = (sxx / dn) - (mx * mx);
+    a.row_std[t]    = (float)sqrt(vr > 0.0 ? vr : 0.0);
+    if (a.kspec) {
+        int32_t span[4] = {-1, -1, -1, -1};
+        bool whole      = false;
+        const RunDev runs[2] = {sp.best, sn.best};
+        for (int k = 0; k < 2; ++k) {
+            const RunDev& r = runs[k];
+            if (r.end - r.start + 1 <= 0) continue;
+            const int64_t lo = r.start, hi = r.end;
+            const int32_t count = a.good_prefix[hi + 1] - a.good_prefix[lo];
+            const double nspan  = (double)(hi - lo + 1);
+            const double snr    = r.sum / sqrt((double)count);
+            if (snr > a.kthr && nspan >= a.min_bw) {
+                span[2 * k]     = (int32_t)lo;
+                span[2 * k + 1] = (int32_t)hi;
+                whole           = whole || nspan > a.whole;
+            }
+        }
+        for (int k = 0; k < 4; ++k) a.kspec_span[4 * t + k] = span[k];
+        a.kwhole[t] = whole ? 1 : 0;
+    }
+}
+
+// ---- scan: columns -----------------------------------------------------------------
+
+struct ColArgs {
+    Geometry g;
+    const float* loc;
+    const float* inv;
+    const uint8_t* mask;
+    const float* clip_mean;
+    const float* clip_inv;
+    float clip;
+    int kchan;
+    uint64_t chunk, nchunks;
+    float* cmin;
+    float* cmax;
+    KadaneDev* kpos;
+    KadaneDev* kneg;
+};
+
+// One lane per (chunk, channel), rows in order: min/max of unmeasured cells
+// and the Kadane-along-time states (lib/cpu/rfi_cpu_measure.cpp:356-444).
+__global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
+    const Geometry& g = a.g;
+    const uint64_t k  = blockIdx.x;
+    const uint64_t c  = (uint64_t)blockIdx.y * 64 + threadIdx.x;
+    if (c >= g.nch) return;
+    const uint64_t o  = k * g.nch + c;
+    float lo          = 3.402823466e+38f;
+    float hi          = -3.402823466e+38f;
+    KadaneDev kp, kn;
+    kadane_init(kp);
+    kadane_init(kn);
+    if (a.mask[c] != 0) {
+        const uint64_t t0 = k * a.chunk;
+        const uint64_t t1 = t0 + a.chunk < g.nsamps ? t0 + a.chunk : g.nsamps;
+        for (uint64_t t = t0; t < t1; ++t) {
+            const uint64_t w = g.window_of(t);
+            const float x    = g.x(t, c);
+            const float ic   = a.inv[w * g.nch + c];
+            float z          = 0.0f;
+            if (ic == 0.0f) {
+                lo = fmin_(lo, x);
+                hi = fmax_(hi, x);
+            } else {
+                z = (x - a.loc[w * g.nch + c]) * ic;
+            }
+            if (a.kchan) {
+                const bool outlier = fabsf(z) > a.clip;
+                const double u =
+                    outlier ? 0.0
+                            : (double)((z - a.clip_mean[w * g.nch + c]) * a.clip_inv[w * g.nch + c]);
+                kadane_append(kp, (int64_t)t, u);
+                kadane_append(kn, (int64_t)t, -u);
+            }
+        }
+    }
+    a.cmin[o] = lo;
+    a.cmax[o] = hi;
+    if (a.kchan) {
+        a.kpos[o] = kp;
+        a.kneg[o] = kn;
+    }
+}
+
+// Per channel: dead-channel test, then Kadane-along-time acceptance
+// (lib/cpu/rfi_cpu.cpp:352-366,438-467).
+__global__ void combine_cols_kernel(uint64_t nch, uint64_t nchunks, uint64_t nwin,
+                                    const uint64_t* __restrict__ unmeasured,
+                                    uint8_t* __restrict__ mask, const float* __restrict__ cmin,
+                                    const float* __restrict__ cmax, int kchan,
+                                    const KadaneDev* __restrict__ kpos,
+                                    const KadaneDev* __restrict__ kneg, double min_len,
+                                    float thr, int64_t* __restrict__ seg,
+                                    unsigned long long* __restrict__ dead) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    seg[4 * c] = 0;
+    seg[4 * c + 1] = -1;
+    seg[4 * c + 2] = 0;
+    seg[4 * c + 3] = -1;
+    if (mask[c] == 0) return;
+    if (unmeasured[c] >= nwin) {
+        float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
+        for (uint64_t k = 0; k < nchunks; ++k) {
+            lo = fmin_(lo, cmin[k * nch + c]);
+            hi = fmax_(hi, cmax[k * nch + c]);
+        }
+        if (!(hi > lo)) {
+            mask[c] = 0;
+            atomicAdd(dead, 1ULL);
+            return;
+        }
+    }
+    if (!kchan) return;
+    KadaneDev p, n;
+    kadane_init(p);
+    kadane_init(n);
+    for (uint64_t k = 0; k < nchunks; ++k) {
+        p = kadane_combine(p, kpos[k * nch + c]);
+        n = kadane_combine(n, kneg[k * nch + c]);
+    }
+    const RunDev runs[2] = {p.best, n.best};
+    for (int q = 0; q < 2; ++q) {
+        const double len = (double)(runs[q].end - runs[q].start + 1);
+        if (len > 0.0 && len >= min_len && runs[q].sum / sqrt(len) > (double)thr) {
+            seg[4 * c + 2 * q]     = runs[q].start;
+            seg[4 * c + 2 * q + 1] = runs[q].end;
+        }
+    }
+}
+
+// ---- apply + normalise ------------------------------------------------------------------
+
+struct ApplyArgs {
+    Geometry g;
+    const float* loc;
+    const float* scale;
+    const float* inv;
+    const uint8_t* mask;      // final channel mask
+    const uint8_t* flags;     // time flags
+    const int32_t* kspec;     // [nsamps][4] or null
+    const int64_t* kseg;      // [nch][4] or null
+    float clip;
+    int narrow;
+    int fill;
+    uint64_t seed, block_index;
+    uint64_t chan_r, time_r;
+    uint64_t capacity;
+    int flip;
+    float* out;
+    uint64_t stride, offset;
+    unsigned long long* counters; // narrow, kspec, kchan, fallbacks
+};
+
+enum : int { kClean = 0, kMasked = 1, kUnmeasured = 2, kNarrow = 3, kKSpec = 4, kKChan = 5 };
+
+__device__ __forceinline__ int cell_state(const ApplyArgs& a, uint64_t t, uint64_t c,
+                                          uint64_t w, float& z) {
+    if (a.mask[c] == 0) return kMasked;
+    const float ic = a.inv[w * a.g.nch + c];
+    if (ic == 0.0f) return kUnmeasured;
+    z = (a.g.x(t, c) - a.loc[w * a.g.nch + c]) * ic;
+    if (a.narrow && fabsf(z) > a.clip) return kNarrow;
+    if (a.kspec) {
+        const int32_t* sp = a.kspec + 4 * t;
+        if ((sp[0] >= 0 && (int32_t)c >= sp[0] && (int32_t)c <= sp[1]) ||
+            (sp[2] >= 0 && (int32_t)c >= sp[2] && (int32_t)c <= sp[3]))
+            return kKSpec;
+    }
+    if (a.kseg) {
+        const int64_t* sg = a.kseg + 4 * c;
+        const int64_t it  = (int64_t)t;
+        if ((it >= sg[0] && it <= sg[1]) || (it >= sg[2] && it <= sg[3])) return kKChan;
+    }
+    return kClean;
+}
+
+// Final value of an unmasked cell in an unflagged row
+// (lib/cpu/rfi_cpu.cpp:537-624, float samples). `counts` only for the cell's
+// own row pass.
+__device__ float cleaned_unflagged(const ApplyArgs& a, uint64_t t, uint64_t c, int* state_out,
+                                   bool* fallback) {
+    const uint64_t w = a.g.window_of(t);
+    float z          = 0.0f;
+    const int st     = cell_state(a, t, c, w, z);
+    if (state_out) *state_out = st;
+    if (fallback) *fallback = false;
+    const float x = a.g.x(t, c);
+    if (st == kClean || st == kUnmeasured || st == kMasked) return x;
+    const uint64_t i = w * a.g.nch + c;
+    if (a.fill != FA_FILL_NEIGHBOUR) {
+        const float v = a.fill == FA_FILL_NOISE
+                            ? normal_draw(hash5(a.seed, a.block_index, (kStreamNoise << 56) ^ t, c, 0))
+                            : 0.0f;
+        return a.loc[i] + (a.scale[i] * v);
+    }
+    const uint64_t lo = c > a.chan_r ? c - a.chan_r : 0;
+    const uint64_t hi = c + a.chan_r < a.g.nch - 1 ? c + a.chan_r : a.g.nch - 1;
+    float value       = 0.0f;
+    bool found        = false;
+    for (int k = 0; k < kMaxDraws && !found; ++k) {
+        const uint64_t h   = hash5(a.seed, a.block_index, (kStreamCell << 56) ^ t, c, (uint64_t)k);
+        const uint64_t src = lo + h % (hi - lo + 1);
+        float zs           = 0.0f;
+        if (cell_state(a, t, src, w, zs) == kClean) {
+            value = zs - 0.0f;
+            found = true;
+        }
+    }
+    if (fallback) *fallback = !found;
+    return a.loc[i] + (a.scale[i] * value);
+}
+
+// 64 x 64 tile of (time, channel): computes each cell's cleaned value, then
+// writes (cleaned - loc) / scale channel-major through LDS
+// (lib/cpu/rfi_cpu.cpp:245-257).
+__global__ void __launch_bounds__(256) apply_kernel(ApplyArgs a) {
+    __shared__ float tile[64][65];
+    __shared__ unsigned long long cnt[4];
+    const uint64_t t0 = (uint64_t)blockIdx.x * 64;
+    const uint64_t c0 = (uint64_t)blockIdx.y * 64;
+    if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long local[4] = {0, 0, 0, 0};
+    // Each thread: channel c0 + (tid % 64), rows t0 + (tid / 64) + 4k.
+    const int cl    = threadIdx.x % 64;
+    const uint64_t c = c0 + cl;
+    for (int r = threadIdx.x / 64; r < 64; r += 4) {
+        const uint64_t t = t0 + r;
+        float v          = 0.0f;
+        if (c < a.g.nch && t < a.g.nsamps && a.mask[c] != 0) {
+            const uint64_t w = a.g.window_of(t);
+            const uint64_t i = w * a.g.nch + c;
+            float cleaned;
+            if (a.flags[t] == 0) {
+                int st   = kClean;
+                bool fb  = false;
+                cleaned  = cleaned_unflagged(a, t, c, &st, &fb);
+                if (st == kNarrow) ++local[0];
+                else if (st == kKSpec) ++local[1];
+                else if (st == kKChan) ++local[2];
+                if (fb) ++local[3];
+            } else if (a.fill != FA_FILL_NEIGHBOUR) {
+                const float vv = a.fill == FA_FILL_NOISE
+                                     ? normal_draw(hash5(a.seed, a.block_index,
+                                                         (kStreamNoise << 56) ^ t, c, 0))
+                                     : 0.0f;
+                cleaned = a.loc[i] + (a.scale[i] * vv);
+            } else {
+                const uint64_t lo = t > a.time_r ? t - a.time_r : 0;
+                const uint64_t hi =
+                    t + a.time_r < a.g.nsamps - 1 ? t + a.time_r : a.g.nsamps - 1;
+                bool found = false;
+                cleaned    = 0.0f;
+                for (int k = 0; k < kMaxDraws && !found; ++k) {
+                    const uint64_t h =
+                        hash5(a.seed, a.block_index, (kStreamTime << 56) ^ t, c, (uint64_t)k);
+                    const uint64_t src = lo + h % (hi - lo + 1);
+                    if (a.flags[src] != 0) continue;
+                    found   = true;
+                    cleaned = cleaned_unflagged(a, src, c, nullptr, nullptr);
+                }
+                if (!found) {
+                    ++local[3];
+                    cleaned = a.loc[i] + (a.scale[i] * 0.0f);
+                }
+            }
+            const float sc = a.scale[i] > 0.0f ? 1.0f / a.scale[i] : 1.0f;
+            v              = (cleaned - a.loc[i]) * sc;
+        }
+        tile[cl][r] = v;
+    }
+    for (int q = 0; q < 4; ++q)
+        if (local[q]) atomicAdd(&cnt[q], local[q]);
+    __syncthreads();
+    // Write: 64 channels x 64 samples, time contiguous per channel.
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+        const int cc     = idx / 64, rr = idx % 64;
+        const uint64_t c2 = c0 + cc, t2 = t0 + rr;
+        if (c2 >= a.g.nch || t2 >= a.capacity) continue;
+        const uint64_t dst = a.flip ? a.g.nch - 1 - c2 : c2;
+        a.out[dst * a.stride + a.offset + t2] = tile[cc][rr];
+    }
+    if (threadIdx.x < 4 && cnt[threadIdx.x]) atomicAdd(&a.counters[threadIdx.x], cnt[threadIdx.x]);
+}
+
+// Short blocks: out = (x - loc[c]) * (scale[c] > 0 ? 1/scale[c] : 1) for good
+// channels (lib/cpu/rfi_cpu.cpp:209-257,262-315 without transforms).
+__global__ void short_normalize_kernel(Geometry g, const float* __restrict__ loc,
+                                       const float* __restrict__ scale,
+                                       const uint8_t* __restrict__ mask, uint64_t capacity,
+                                       int flip, float* __restrict__ out, uint64_t stride,
+                                       uint64_t offset) {
+    const uint64_t c = blockIdx.y;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < capacity;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        float v = 0.0f;
+        if (t < g.nsamps && mask[c] != 0) {
+            const float sc = scale[c] > 0.0f ? 1.0f / scale[c] : 1.0f;
+            v              = (g.x(t, c) - loc[c]) * sc;
+        }
+        const uint64_t dst = flip ? g.nch - 1 - c : c;
+        out[dst * stride + offset + t] = v;
+    }
+}
+
+// Welford mean / population std per channel over a short first block.
+__global__ void welford_kernel(Geometry g, float* __restrict__ loc, float* __restrict__ scale) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.nch) return;
+    double mean = 0.0, var = 0.0;
+    for (uint64_t t = 0; t < g.nsamps; ++t) {
+        const double x     = g.x(t, c);
+        const double delta = x - mean;
+        mean += delta / (double)(t + 1);
+        var += delta * (x - mean);
+    }
+    loc[c]   = (float)mean;
+    scale[c] = g.nsamps > 0 ? (float)sqrt(var / (double)g.nsamps) : 0.0f;
+}
+
+// Index of the first non-finite float32 sample (bit test, no FP loads).
+__global__ void first_nonfinite_kernel(const uint32_t* __restrict__ v, uint64_t n,
+                                       unsigned long long* __restrict__ first) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        if ((v[i] & 0x7F800000U) == 0x7F800000U) atomicMin(first, (unsigned long long)i);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+} // namespace
+
+// ---- host orchestration ---------------------------------------------------------------
+
+DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, float baseline,
+                     int device)
+    : cfg_(cfg), layout_(layout), device_(device) {
+    // Validation (lib/prepare/rfi_config.cpp:14-110, lib/cpu/rfi_cpu.cpp:317-332).
+    auto positive = [](float v, const char* n) {
+        if (!std::isfinite(v) || !(v > 0.0f))
+            throw std::invalid_argument(std::string("rfi.") + n + " must be finite and > 0");
+    };
+    auto nonneg = [](float v, const char* n) {
+        if (!std::isfinite(v) || v < 0.0f)
+            throw std::invalid_argument(std::string("rfi.") + n + " must be finite and >= 0");
+    };
+    auto fraction = [](float v, const char* n) {
+        if (!std::isfinite(v) || v < 0.0f || v > 1.0f)
+            throw std::invalid_argument(std::string("rfi.") + n + " must be finite and in [0, 1]");
+    };
+    positive(cfg.narrow_threshold, "narrow.threshold");
+    positive(cfg.broad_threshold, "broad.threshold");
+    positive(cfg.moments_threshold, "moments.threshold");
+    positive(cfg.iqrm_threshold, "iqrm.threshold");
+    positive(cfg.kadane_channel_threshold, "kadane_channel.threshold");
+    positive(cfg.kadane_spectrum_threshold, "kadane_spectrum.threshold");
+    nonneg(cfg.moments_kurtosis_max, "moments.kurtosis_max");
+    nonneg(cfg.kadane_channel_min_duration, "kadane_channel.min_duration");
+    nonneg(cfg.kadane_spectrum_min_bandwidth, "kadane_spectrum.min_bandwidth");
+    fraction(cfg.dropout_min_std_fraction, "dropout.min_std_fraction");
+    fraction(cfg.kadane_spectrum_whole_band_fraction, "kadane_spectrum.whole_band_fraction");
+    require(cfg.broad_boxcar_max >= 1, "rfi.broad.boxcar_max must be >= 1");
+    require(cfg.fill >= FA_FILL_NEIGHBOUR && cfg.fill <= FA_FILL_NOISE, "Invalid rfi.fill");
+    require(cfg.zero_dm >= FA_ZERO_DM_OFF && cfg.zero_dm <= FA_ZERO_DM_ZDOT, "Invalid rfi.zero_dm");
+    if (cfg.zero_dm != FA_ZERO_DM_OFF || cfg.equalize)
+        throw std::invalid_argument(
+            "device RFI: zero_dm and equalize are not implemented on the device");
+    if (cfg.zap_chans && cfg.n_zap_chans)
+        zap_chans_.assign(cfg.zap_chans, cfg.zap_chans + 2 * cfg.n_zap_chans);
+    if (cfg.zap_freqs && cfg.n_zap_freqs)
+        zap_freqs_.assign(cfg.zap_freqs, cfg.zap_freqs + 2 * cfg.n_zap_freqs);
+    for (std::size_t i = 0; i < cfg.n_zap_chans; ++i)
+        require(zap_chans_[2 * i] <= zap_chans_[2 * i + 1], "rfi.zap_chans ranges need start <= end");
+    for (std::size_t i = 0; i < cfg.n_zap_freqs; ++i)
+        require(std::isfinite(zap_freqs_[2 * i]) && std::isfinite(zap_freqs_[2 * i + 1]) &&
+                    zap_freqs_[2 * i] <= zap_freqs_[2 * i + 1],
+                "rfi.zap_freqs ranges must be finite with lo <= hi");
+    require(layout.nchans > 0 && layout.max_nsamps > 0, "RFI: nchans and max_nsamps must be > 0");
+    bits_ = dtype_bits(layout.dtype);
+    require((layout.nchans * static_cast<uint64_t>(bits_)) % 8 == 0,
+            "RFI: each time sample must be a whole number of bytes");
+    require(std::isfinite(layout.tsamp) && layout.tsamp > 0 && std::isfinite(baseline) &&
+                baseline > 0,
+            "RFI: tsamp and baseline_length must be finite and > 0");
+    require(std::isfinite(layout.f0) && std::isfinite(layout.df), "RFI: f0 and df must be finite");
+    if (cfg.kadane_spectrum_enabled || !zap_freqs_.empty())
+        require(layout.df != 0.0, "RFICleaner: zap_freqs and kadane_spectrum need the channel spacing (df)");
+    nch_       = layout.nchans;
+    row_bytes_ = nch_ * static_cast<uint64_t>(bits_) / 8;
+    // Derived geometry (lib/cpu/rfi_cpu.cpp:24-60).
+    const double ws   = static_cast<double>(baseline) / layout.tsamp;
+    window_len_       = std::max<uint64_t>(static_cast<uint64_t>(ws), kMinWindow);
+    smooth_radius_    = std::max<uint64_t>(static_cast<uint64_t>(ws / 2), 1);
+    time_fill_radius_ = std::max<uint64_t>(smooth_radius_ / 4, 1);
+    chan_fill_radius_ = std::max<uint64_t>(nch_ / 60, 1);
+    clip_             = cfg.narrow_threshold;
+    narrow_active_    = cfg.narrow_enabled && (layout.dtype == FA_FLOAT32 || bits_ > 4);
+    need_moments_ = cfg.moments_enabled || cfg.iqrm_enabled || cfg.kadane_channel_enabled;
+    static_mask_.assign(nch_, 1);
+    for (std::size_t i = 0; i < cfg.n_zap_chans; ++i) {
+        const uint64_t e = std::min<uint64_t>(zap_chans_[2 * i + 1], nch_ - 1);
+        for (uint64_t c = zap_chans_[2 * i]; c <= e; ++c) static_mask_[c] = 0;
+    }
+    for (uint64_t c = 0; c < nch_; ++c) {
+        const double f = layout.f0 + static_cast<double>(c) * layout.df;
+        for (std::size_t i = 0; i < cfg.n_zap_freqs; ++i)
+            if (f >= zap_freqs_[2 * i] && f <= zap_freqs_[2 * i + 1]) static_mask_[c] = 0;
+    }
+    mask_                = static_mask_;
+    report_              = fa_rfi_report{};
+    report_.chans_zapped = static_cast<uint64_t>(std::count(static_mask_.begin(), static_mask_.end(), 0));
+    report_.chans_masked = report_.chans_zapped;
+
+    DeviceGuard dg(device_);
+    const uint64_t maxwin = std::max<uint64_t>(layout.max_nsamps / window_len_, 1);
+    loc_.resize(maxwin * nch_);
+    scale_.resize(maxwin * nch_);
+    inv_.resize(maxwin * nch_);
+    measured_.resize(maxwin * nch_);
+    if (need_moments_) {
+        moments_.resize(maxwin * nch_);
+        clip_mean_.resize(maxwin * nch_);
+        clip_inv_.resize(maxwin * nch_);
+    }
+    mask_dev_.resize(nch_);
+    good_prefix_.resize(nch_ + 1);
+    flags_dev_.resize(layout.max_nsamps);
+    s_det_.resize(layout.max_nsamps);
+    s_keep_.resize(layout.max_nsamps);
+    row_std_.resize(layout.max_nsamps);
+    if (cfg.kadane_spectrum_enabled) {
+        kspec_.resize(4 * layout.max_nsamps);
+        kwhole_.resize(layout.max_nsamps);
+    }
+    cmin_.resize(kMaxChunks * nch_);
+    cmax_.resize(kMaxChunks * nch_);
+    if (cfg.kadane_channel_enabled) {
+        kpos_.resize(kMaxChunks * nch_);
+        kneg_.resize(kMaxChunks * nch_);
+    }
+    kseg_.resize(4 * nch_);
+    counters_.resize(8);
+    tail_loc_.resize(nch_);
+    tail_scale_.resize(nch_);
+    bad_index_.resize(1);
+    for (auto& e : ev_) FA_HIP(hipEventCreate(&e));
+}
+
+DeviceRfi::~DeviceRfi() {
+    for (auto e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
+
+uint64_t DeviceRfi::chunk_count() const {
+    const uint64_t by = (nsamps_ + kMinChunkRows - 1) / kMinChunkRows;
+    return std::clamp<uint64_t>(by, 1, kMaxChunks);
+}
+
+uint64_t DeviceRfi::memory_bytes() const {
+    return loc_.bytes() + scale_.bytes() + inv_.bytes() + measured_.bytes() + moments_.bytes() +
+           clip_mean_.bytes() + clip_inv_.bytes() + mask_dev_.bytes() + flags_dev_.bytes() +
+           kwhole_.bytes() + good_prefix_.bytes() + kspec_.bytes() + s_det_.bytes() +
+           s_keep_.bytes() + row_std_.bytes() + cmin_.bytes() + cmax_.bytes() + kpos_.bytes() +
+           kneg_.bytes() + kseg_.bytes() + counters_.bytes() + tail_loc_.bytes() +
+           tail_scale_.bytes();
+}
+
+void DeviceRfi::calibration(float* loc, float* scale, float* inv) const {
+    DeviceGuard dg(device_);
+    const uint64_t n = nwin_ * nch_;
+    if (loc) FA_HIP(hipMemcpy(loc, loc_.get(), n * 4, hipMemcpyDeviceToHost));
+    if (scale) FA_HIP(hipMemcpy(scale, scale_.get(), n * 4, hipMemcpyDeviceToHost));
+    if (inv) FA_HIP(hipMemcpy(inv, inv_.get(), n * 4, hipMemcpyDeviceToHost));
+}
+
+void DeviceRfi::execute_normalized(const uint8_t* block, uint64_t capacity, uint64_t nsamps,
+                                   float* out, uint64_t stride, uint64_t offset,
+                                   hipStream_t s) {
+    require(block != nullptr && out != nullptr, "block and out must not be null");
+    require(s != nullptr, "an explicit non-null stream is required");
+    require(capacity >= 1 && capacity <= layout_.max_nsamps, "RFICleaner: invalid normalized block size");
+    if (nsamps == 0) nsamps = capacity;
+    require(nsamps <= capacity, "RFICleaner: nsamps exceeds block");
+    require(stride >= offset + capacity, "output stride too small");
+    DeviceGuard dg(device_);
+    if (layout_.dtype == FA_FLOAT32) {
+        const unsigned long long none = ~0ULL;
+        FA_HIP(hipMemcpyAsync(bad_index_.get(), &none, 8, hipMemcpyHostToDevice, s));
+        first_nonfinite_kernel<<<1024, 256, 0, s>>>(reinterpret_cast<const uint32_t*>(block),
+                                                    nsamps * nch_, bad_index_.get());
+        unsigned long long bad = 0;
+        FA_HIP(hipMemcpyAsync(&bad, bad_index_.get(), 8, hipMemcpyDeviceToHost, s));
+        FA_HIP(hipStreamSynchronize(s));
+        if (bad != none)
+            throw std::invalid_argument(
+                "RFI: non-finite float32 input at sample " + std::to_string(bad / nch_) +
+                ", channel " + std::to_string(bad % nch_));
+    }
+    const bool short_block = nsamps < std::max(kMinWindow, kMinBlockSamps);
+    const int flip         = layout_.df < 0.0 ? 1 : 0;
+    Geometry g{block, row_bytes_, nch_, nsamps, window_len_, 1, layout_.dtype};
+    if (short_block) {
+        report_              = fa_rfi_report{};
+        report_.nsamps       = nsamps;
+        report_.chans_zapped = static_cast<uint64_t>(std::count(static_mask_.begin(), static_mask_.end(), 0));
+        if (tail_ready_) {
+            // Reuse the last full block's final window and mask.
+            report_.chans_masked       = static_cast<uint64_t>(std::count(mask_.begin(), mask_.end(), 0));
+            report_.calibration_reused = 1;
+            FA_HIP(hipMemcpyAsync(mask_dev_.get(), mask_.data(), nch_, hipMemcpyHostToDevice, s));
+            dim3 grid(ceil_div(capacity, 256 * 8), static_cast<unsigned>(nch_));
+            short_normalize_kernel<<<grid, 256, 0, s>>>(g, tail_loc_.get(), tail_scale_.get(),
+                                                        mask_dev_.get(), capacity, flip, out,
+                                                        stride, offset);
+        } else {
+            mask_                = static_mask_;
+            report_.chans_masked = report_.chans_zapped;
+            FA_HIP(hipMemcpyAsync(mask_dev_.get(), mask_.data(), nch_, hipMemcpyHostToDevice, s));
+            welford_kernel<<<ceil_div(nch_, 64), 64, 0, s>>>(g, loc_.get(), scale_.get());
+            dim3 grid(ceil_div(capacity, 256 * 8), static_cast<unsigned>(nch_));
+            short_normalize_kernel<<<grid, 256, 0, s>>>(g, loc_.get(), scale_.get(),
+                                                        mask_dev_.get(), capacity, flip, out,
+                                                        stride, offset);
+            nwin_ = 1;
+        }
+        FA_HIP(hipGetLastError());
+        FA_HIP(hipStreamSynchronize(s));
+        time_flags_host_.assign(nsamps, 0);
+        ++block_index_;
+        return;
+    }
+    full_block(block, capacity, nsamps, out, stride, offset, s);
+    ++block_index_;
+    tail_ready_ = true;
+}
+
+void DeviceRfi::build_channel_mask(hipStream_t s) {
+    // Moments and IQRM on the host from the per-window sums
+    // (lib/cpu/rfi_cpu_measure.cpp:140-300).
+    const uint64_t n = nch_, nw = nwin_;
+    std::vector<uint8_t> dyn(n, 0);
+    if (need_moments_) {
+        moments_host_.resize(nw * n);
+        FA_HIP(hipMemcpyAsync(moments_host_.data(), moments_.get(), nw * n * sizeof(MomentSumsDev),
+                              hipMemcpyDeviceToHost, s));
+        FA_HIP(hipStreamSynchronize(s));
+        std::vector<float> cm(nw * n, 0.0f), ci(nw * n, 0.0f);
+        for (uint64_t i = 0; i < nw * n; ++i) {
+            const auto& m = moments_host_[i];
+            if (m.nc > 1.0) {
+                const double mean = m.c1 / m.nc;
+                const double var  = (m.c2 / m.nc) - (mean * mean);
+                cm[i]             = static_cast<float>(mean);
+                ci[i]             = var > 0.0 ? static_cast<float>(1.0 / std::sqrt(var)) : 0.0f;
+            }
+        }
+        FA_HIP(hipMemcpyAsync(clip_mean_.get(), cm.data(), cm.size() * 4, hipMemcpyHostToDevice, s));
+        FA_HIP(hipMemcpyAsync(clip_inv_.get(), ci.data(), ci.size() * 4, hipMemcpyHostToDevice, s));
+        if (cfg_.moments_enabled) {
+            std::vector<double> mean(n, 0), sd(n, 0), sk(n, 0), ku(n, 0), ac(n, 0);
+            std::vector<uint8_t> good(n, 0);
+            for (uint64_t c = 0; c < n; ++c) {
+                if (mask_[c] == 0) continue;
+                double sn = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, sl = 0, pairs = 0;
+                for (uint64_t w = 0; w < nw; ++w) {
+                    const auto& m = moments_host_[w * n + c];
+                    sn += m.n;
+                    s1 += m.s1;
+                    s2 += m.s2;
+                    s3 += m.s3;
+                    s4 += m.s4;
+                    sl += m.lag;
+                    pairs += std::max(m.n - 1.0, 0.0);
+                }
+                if (sn < 2.0) continue;
+                const double m1 = s1 / sn, m2 = s2 / sn, m3 = s3 / sn, m4 = s4 / sn;
+                const double var = m2 - (m1 * m1);
+                if (!(var > 0.0)) continue;
+                good[c] = 1;
+                mean[c] = m1;
+                sd[c]   = std::sqrt(var);
+                sk[c]   = (m3 - (3.0 * m1 * m2) + (2.0 * m1 * m1 * m1)) / (var * sd[c]);
+                ku[c]   = ((m4 - (4.0 * m1 * m3) + (6.0 * m1 * m1 * m2) - (3.0 * m1 * m1 * m1 * m1)) /
+                         (var * var)) -
+                        3.0;
+                ac[c] = pairs > 0.0 ? ((sl / pairs) - (m1 * m1)) / var : 0.0;
+            }
+            auto outliers = [&](const std::vector<double>& st) {
+                std::vector<double> v;
+                for (uint64_t c = 0; c < n; ++c)
+                    if (good[c]) v.push_back(st[c]);
+                if (v.size() < 4) return;
+                std::sort(v.begin(), v.end());
+                auto q = [&](double p) {
+                    const double h  = static_cast<double>(v.size() - 1) * p;
+                    const auto lo   = static_cast<std::size_t>(h);
+                    const auto hi   = std::min(lo + 1, v.size() - 1);
+                    return v[lo] + ((h - static_cast<double>(lo)) * (v[hi] - v[lo]));
+                };
+                const double q1 = q(0.25), q3 = q(0.75), iqr = q3 - q1;
+                if (!(iqr > 0.0)) return;
+                const double k = cfg_.moments_threshold;
+                const double lo = q1 - (k * iqr), hi = q3 + (k * iqr);
+                for (uint64_t c = 0; c < n; ++c)
+                    if (good[c] && (st[c] < lo || st[c] > hi)) dyn[c] = 1;
+            };
+            if (cfg_.moments_mean) outliers(mean);
+            if (cfg_.moments_std) outliers(sd);
+            if (cfg_.moments_skewness) outliers(sk);
+            if (cfg_.moments_kurtosis) outliers(ku);
+            if (cfg_.moments_autocorr) outliers(ac);
+            if (cfg_.moments_kurtosis_max > 0.0f)
+                for (uint64_t c = 0; c < n; ++c)
+                    if (good[c] && std::abs(ku[c]) > cfg_.moments_kurtosis_max) dyn[c] = 1;
+            report_.chans_moments = static_cast<uint64_t>(std::count(dyn.begin(), dyn.end(), 1));
+        }
+        if (cfg_.iqrm_enabled) {
+            std::vector<float> sd(n, 0.0f);
+            std::vector<uint8_t> ignore(n, 0);
+            for (uint64_t c = 0; c < n; ++c) {
+                ignore[c]     = static_cast<uint8_t>(mask_[c] == 0 || unmeasured_[c] > 0);
+                double weight = 0.0, acc = 0.0;
+                for (uint64_t w = 0; w < nw; ++w) {
+                    const auto& m = moments_host_[w * n + c];
+                    if (m.n > 1.0) {
+                        const double mean = m.s1 / m.n;
+                        const double var  = (m.s2 / m.n) - (mean * mean);
+                        const double sc   = scale_host_[w * n + c];
+                        acc += m.n * std::max(var, 0.0) * sc * sc;
+                        weight += m.n;
+                    }
+                }
+                sd[c] = weight > 0.0 ? static_cast<float>(std::sqrt(acc / weight)) : 0.0f;
+            }
+            const uint64_t radius = cfg_.iqrm_radius > 0 ? cfg_.iqrm_radius : std::max<uint64_t>(n / 10, 1);
+            const auto flags      = iqrm_flags(sd, radius, cfg_.iqrm_threshold, ignore);
+            for (uint64_t c = 0; c < n; ++c)
+                if (flags[c] && !ignore[c] && !dyn[c]) {
+                    dyn[c] = 1;
+                    ++report_.chans_iqrm;
+                }
+        }
+    }
+    if (cfg_.mask_grow > 0) {
+        std::vector<uint8_t> grown = dyn;
+        const uint64_t g           = cfg_.mask_grow;
+        for (uint64_t c = 0; c < n; ++c) {
+            if (!dyn[c]) continue;
+            const uint64_t first = c > g ? c - g : 0;
+            const uint64_t last  = c + std::min(g, n - 1 - c);
+            for (uint64_t d = first; d <= last; ++d) grown[d] = 1;
+        }
+        for (uint64_t c = 0; c < n; ++c)
+            if (grown[c] && !dyn[c] && mask_[c]) ++report_.chans_grown;
+        dyn.swap(grown);
+    }
+    for (uint64_t c = 0; c < n; ++c)
+        if (dyn[c]) mask_[c] = 0;
+    report_.chans_masked = static_cast<uint64_t>(std::count(mask_.begin(), mask_.end(), 0));
+    std::vector<int32_t> gp(n + 1, 0);
+    for (uint64_t c = 0; c < n; ++c) gp[c + 1] = gp[c] + (mask_[c] ? 1 : 0);
+    FA_HIP(hipMemcpyAsync(mask_dev_.get(), mask_.data(), n, hipMemcpyHostToDevice, s));
+    FA_HIP(hipMemcpyAsync(good_prefix_.get(), gp.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
+    FA_HIP(hipStreamSynchronize(s));
+}
+
+void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsamps, float* out,
+                           uint64_t stride, uint64_t offset, hipStream_t s) {
+    const uint64_t n = nch_;
+    nsamps_          = nsamps;
+    nwin_            = std::max<uint64_t>(nsamps / window_len_, 1);
+    report_              = fa_rfi_report{};
+    report_.nsamps       = nsamps;
+    mask_                = static_mask_;
+    report_.chans_zapped = static_cast<uint64_t>(std::count(static_mask_.begin(), static_mask_.end(), 0));
+    report_.chans_masked = report_.chans_zapped;
+    Geometry g{block, row_bytes_, n, nsamps, window_len_, nwin_, layout_.dtype};
+    const unsigned cblk = ceil_div(n, 64);
+
+    FA_HIP(hipEventRecord(ev_[0], s));
+    bandpass_kernel<<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
+        g, loc_.get(), scale_.get(), inv_.get(), measured_.get());
+    FA_HIP(hipGetLastError());
+    scale_host_.resize(nwin_ * n);
+    measured_host_.resize(nwin_ * n);
+    FA_HIP(hipMemcpyAsync(scale_host_.data(), scale_.get(), nwin_ * n * 4, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(measured_host_.data(), measured_.get(), nwin_ * n, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(mask_dev_.get(), static_mask_.data(), n, hipMemcpyHostToDevice, s));
+    FA_HIP(hipEventRecord(ev_[1], s));
+    if (need_moments_) {
+        moments_kernel<<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
+            g, loc_.get(), inv_.get(), mask_dev_.get(), clip_, moments_.get());
+        FA_HIP(hipGetLastError());
+    }
+    FA_HIP(hipStreamSynchronize(s));
+    unmeasured_.assign(n, 0);
+    for (uint64_t w = 0; w < nwin_; ++w)
+        for (uint64_t c = 0; c < n; ++c) unmeasured_[c] += measured_host_[w * n + c] ? 0 : 1;
+    build_channel_mask(s);
+    FA_HIP(hipEventRecord(ev_[2], s));
+
+    // Row and column scans.
+    const uint64_t nchunks = chunk_count();
+    const uint64_t chunk   = (nsamps + nchunks - 1) / nchunks;
+    const bool good_any    = std::count(mask_.begin(), mask_.end(), 1) > 0;
+    const bool kspec = cfg_.kadane_spectrum_enabled, kchan = cfg_.kadane_channel_enabled;
+    if (good_any) {
+        ScanArgs a{};
+        a.g           = g;
+        a.loc         = loc_.get();
+        a.inv         = inv_.get();
+        a.mask        = mask_dev_.get();
+        a.good_prefix = good_prefix_.get();
+        a.clip        = clip_;
+        a.kspec       = kspec ? 1 : 0;
+        a.min_bw      = kspec ? static_cast<double>(cfg_.kadane_spectrum_min_bandwidth) / std::abs(layout_.df) : 0.0;
+        a.whole       = static_cast<double>(cfg_.kadane_spectrum_whole_band_fraction) * static_cast<double>(n);
+        a.kthr        = cfg_.kadane_spectrum_threshold;
+        a.s_det       = s_det_.get();
+        a.s_keep      = s_keep_.get();
+        a.row_std     = row_std_.get();
+        a.kspec_span  = kspec ? kspec_.get() : nullptr;
+        a.kwhole      = kspec ? kwhole_.get() : nullptr;
+        const size_t lds = 64 * (kRowTileCh * bits_ / 32 + 1) * 4;
+        scan_rows_kernel<<<ceil_div(nsamps, 64), 64, lds, s>>>(a);
+        FA_HIP(hipGetLastError());
+        ColArgs b{};
+        b.g         = g;
+        b.loc       = loc_.get();
+        b.inv       = inv_.get();
+        b.mask      = mask_dev_.get();
+        b.clip_mean = kchan ? clip_mean_.get() : nullptr;
+        b.clip_inv  = kchan ? clip_inv_.get() : nullptr;
+        b.clip      = clip_;
+        b.kchan     = kchan ? 1 : 0;
+        b.chunk     = chunk;
+        b.nchunks   = nchunks;
+        b.cmin      = cmin_.get();
+        b.cmax      = cmax_.get();
+        b.kpos      = kchan ? kpos_.get() : nullptr;
+        b.kneg      = kchan ? kneg_.get() : nullptr;
+        scan_cols_kernel<<<dim3(static_cast<unsigned>(nchunks), cblk), 64, 0, s>>>(b);
+        FA_HIP(hipGetLastError());
+    }
+    FA_HIP(hipEventRecord(ev_[3], s));
+    build_flags(s);
+    FA_HIP(hipEventRecord(ev_[4], s));
+
+    // Apply + normalise.
+    zero_u64_kernel<<<1, 64, 0, s>>>(counters_.get(), 8);
+    ApplyArgs ap{};
+    ap.g           = g;
+    ap.loc         = loc_.get();
+    ap.scale       = scale_.get();
+    ap.inv         = inv_.get();
+    ap.mask        = mask_dev_.get();
+    ap.flags       = flags_dev_.get();
+    ap.kspec       = kspec ? kspec_.get() : nullptr;
+    ap.kseg        = kchan ? kseg_.get() : nullptr;
+    ap.clip        = clip_;
+    ap.narrow      = narrow_active_ ? 1 : 0;
+    ap.fill        = cfg_.fill;
+    ap.seed        = cfg_.seed;
+    ap.block_index = block_index_;
+    ap.chan_r      = chan_fill_radius_;
+    ap.time_r      = time_fill_radius_;
+    ap.capacity    = capacity;
+    ap.flip        = layout_.df < 0.0 ? 1 : 0;
+    ap.out         = out;
+    ap.stride      = stride;
+    ap.offset      = offset;
+    ap.counters    = counters_.get();
+    apply_kernel<<<dim3(ceil_div(capacity, 64), cblk), 256, 0, s>>>(ap);
+    FA_HIP(hipGetLastError());
+    // Last window's calibration for short tails.
+    const uint64_t last = (nwin_ - 1) * n;
+    FA_HIP(hipMemcpyAsync(tail_loc_.get(), loc_.get() + last, n * 4, hipMemcpyDeviceToDevice, s));
+    FA_HIP(hipMemcpyAsync(tail_scale_.get(), scale_.get() + last, n * 4, hipMemcpyDeviceToDevice, s));
+    unsigned long long cnt[8];
+    FA_HIP(hipMemcpyAsync(cnt, counters_.get(), sizeof(cnt), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipEventRecord(ev_[5], s));
+    FA_HIP(hipStreamSynchronize(s));
+    report_.cells_narrow          = cnt[0];
+    report_.cells_kadane_spectrum = cnt[1];
+    report_.cells_kadane_channel  = cnt[2];
+    report_.fill_fallbacks        = cnt[3];
+    report_.t_bandpass = event_seconds(ev_[0], ev_[1]);
+    report_.t_moments  = event_seconds(ev_[1], ev_[2]);
+    report_.t_scan     = event_seconds(ev_[2], ev_[3]);
+    report_.t_flags    = event_seconds(ev_[3], ev_[4]);
+    report_.t_apply    = event_seconds(ev_[4], ev_[5]);
+}
+
+void DeviceRfi::build_flags(hipStream_t s) {
+    // Dead channels + Kadane-along-time on the device, time flags on the host
+    // (lib/cpu/rfi_cpu.cpp:347-468).
+    const uint64_t n = nch_, nsamps = nsamps_;
+    const uint64_t nchunks = chunk_count();
+    const bool good_any    = std::count(mask_.begin(), mask_.end(), 1) > 0;
+    DeviceBuffer<uint64_t> unm(n);
+    FA_HIP(hipMemcpyAsync(unm.get(), unmeasured_.data(), n * 8, hipMemcpyHostToDevice, s));
+    zero_u64_kernel<<<1, 64, 0, s>>>(counters_.get() + 4, 1);
+    if (good_any) {
+        const double min_len = static_cast<double>(cfg_.kadane_channel_min_duration) / layout_.tsamp;
+        combine_cols_kernel<<<ceil_div(n, 256), 256, 0, s>>>(
+            n, nchunks, nwin_, unm.get(), mask_dev_.get(), cmin_.get(), cmax_.get(),
+            cfg_.kadane_channel_enabled ? 1 : 0, kpos_.get(), kneg_.get(), min_len,
+            cfg_.kadane_channel_threshold, kseg_.get(), counters_.get() + 4);
+        FA_HIP(hipGetLastError());
+    }
+    time_flags_host_.assign(nsamps, 0);
+    std::vector<float> sdet, rstd;
+    std::vector<uint8_t> whole;
+    if (good_any && cfg_.broad_enabled) {
+        sdet.resize(nsamps);
+        FA_HIP(hipMemcpyAsync(sdet.data(), s_det_.get(), nsamps * 4, hipMemcpyDeviceToHost, s));
+    }
+    if (good_any && cfg_.dropout_enabled) {
+        rstd.resize(nsamps);
+        FA_HIP(hipMemcpyAsync(rstd.data(), row_std_.get(), nsamps * 4, hipMemcpyDeviceToHost, s));
+    }
+    if (good_any && cfg_.kadane_spectrum_enabled) {
+        whole.resize(nsamps);
+        FA_HIP(hipMemcpyAsync(whole.data(), kwhole_.get(), nsamps, hipMemcpyDeviceToHost, s));
+    }
+    unsigned long long dead = 0;
+    FA_HIP(hipMemcpyAsync(&dead, counters_.get() + 4, 8, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(mask_.data(), mask_dev_.get(), n, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    report_.chans_dead   = dead;
+    report_.chans_masked = static_cast<uint64_t>(std::count(mask_.begin(), mask_.end(), 0));
+    if (good_any) {
+        if (cfg_.broad_enabled)
+            report_.samps_broad = broadband_flags(sdet, smooth_radius_, cfg_.broad_threshold,
+                                                  cfg_.broad_boxcar_max, time_flags_host_.data());
+        if (cfg_.dropout_enabled) {
+            std::vector<float> spread = rstd;
+            std::nth_element(spread.begin(), spread.begin() + static_cast<std::ptrdiff_t>(nsamps / 2),
+                             spread.end());
+            const float cut = cfg_.dropout_min_std_fraction * spread[nsamps / 2];
+            std::vector<uint8_t> drop(nsamps, 0);
+            for (uint64_t t = 0; t < nsamps; ++t)
+                if (rstd[t] < cut) {
+                    drop[t] = 1;
+                    if (t > 0) drop[t - 1] = 1;
+                    if (t + 1 < nsamps) drop[t + 1] = 1;
+                }
+            for (uint64_t t = 0; t < nsamps; ++t)
+                if (drop[t] && !time_flags_host_[t]) {
+                    time_flags_host_[t] = 1;
+                    ++report_.samps_dropout;
+                }
+        }
+        if (cfg_.kadane_spectrum_enabled)
+            for (uint64_t t = 0; t < nsamps; ++t)
+                if (whole[t] && !time_flags_host_[t]) {
+                    time_flags_host_[t] = 1;
+                    ++report_.samps_kadane;
+                }
+    }
+    report_.samps_flagged =
+        static_cast<uint64_t>(std::count(time_flags_host_.begin(), time_flags_host_.end(), 1));
+    FA_HIP(hipMemcpyAsync(flags_dev_.get(), time_flags_host_.data(), nsamps, hipMemcpyHostToDevice, s));
+    FA_HIP(hipStreamSynchronize(s));
+}
+
+} // namespace fa
+
+// ---- C ABI --------------------------------------------------------------------------
+
+
+using fa::guard;
+using fa::require;
+
+extern "C" {
+
+int fa_rfi_create(const fa_rfi_config* cfg, const fa_block_layout* layout, float baseline,
+                  int device, fa_rfi** rfi) {
+    return guard([&] {
+        require(cfg && layout && rfi, "cfg, layout and rfi must not be null");
+        auto r   = std::make_unique<fa_rfi>();
+        r->owned = std::make_unique<fa::DeviceRfi>(*cfg, *layout, baseline, device);
+        r->rfi   = r->owned.get();
+        *rfi     = r.release();
+    });
+}
+
+int fa_rfi_destroy(fa_rfi* rfi) {
+    return guard([&] {
+        if (!rfi) return;
+        require(rfi->owned != nullptr, "a beam's borrowed RFI handle is not destroyed");
+        delete rfi;
+    });
+}
+
+int fa_rfi_execute_normalized(fa_rfi* rfi, const void* block, uint64_t capacity, uint64_t nsamps,
+                              float* out, uint64_t out_stride, uint64_t out_offset, void* stream) {
+    return guard([&] {
+        require(rfi != nullptr, "rfi must not be null");
+        rfi->rfi->execute_normalized(static_cast<const uint8_t*>(block), capacity, nsamps, out,
+                                     out_stride, out_offset, fa::as_stream(stream));
+    });
+}
+
+int fa_rfi_channel_mask(const fa_rfi* rfi, uint8_t* out, uint64_t cap) {
+    return guard([&] {
+        require(rfi && out, "rfi and out must not be null");
+        const auto& m = rfi->rfi->channel_mask();
+        require(cap >= m.size(), "output too small");
+        std::memcpy(out, m.data(), m.size());
+    });
+}
+
+int fa_rfi_get_report(const fa_rfi* rfi, fa_rfi_report* out) {
+    return guard([&] {
+        require(rfi && out, "rfi and out must not be null");
+        *out = rfi->rfi->report();
+    });
+}
+
+int fa_rfi_calibration(const fa_rfi* rfi, uint64_t* nwin, float* loc, float* scale, float* inv,
+                       uint64_t cap) {
+    return guard([&] {
+        require(rfi && nwin, "rfi and nwin must not be null");
+        *nwin = rfi->rfi->nwin();
+        require(cap >= rfi->rfi->nwin() * rfi->rfi->layout().nchans, "output too small");
+        rfi->rfi->calibration(loc, scale, inv);
+    });
+}
+
+int fa_rfi_time_flags(const fa_rfi* rfi, uint8_t* out, uint64_t cap) {
+    return guard([&] {
+        require(rfi && out, "rfi and out must not be null");
+        const auto& f = rfi->rfi->time_flags();
+        require(cap >= f.size(), "output too small");
+        std::memcpy(out, f.data(), f.size());
+    });
+}
+
+} // extern "C"

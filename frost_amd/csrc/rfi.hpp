@@ -1,0 +1,127 @@
+// Device RFI excision + calibration: RFICleaner::execute_normalized
+// (lib/cpu/rfi_cpu.cpp:161-260) on gfx950.
+//
+// Stages and where they run:
+//  bandpass   device: per (window, channel) thread, 5^k sampled rows, streaming
+//             median-of-5 remedian in LDS, FP64 clipped refinement in frost's
+//             sequential order (bit-exact with the CPU path).
+//  moments    device: per (window, channel) FP64 sums, sequential in time.
+//  channel    host: moment outliers, IQRM, mask growth on per-channel
+//   mask      statistics (nwin x nchans MomentSums come back, a few MB).
+//  scan       device: row statistics (one lane per row over an LDS-transposed
+//             tile, FP64 in channel order, Kadane across frequency) and
+//             column statistics (one lane per chunk x channel, Kadane along
+//             time, min/max of unmeasured cells); per-channel combine.
+//  flags      host: broad-band (baseline filter, exact MAD, boxcars),
+//             dropouts and whole-band Kadane rows on the [nsamps] series.
+//  apply      device: cell states, neighbour fills with frost's counter
+//             hash, time fills, and the normalising transpose to
+//             channel-major float32 (ascending frequency) in one pass.
+// zero_dm and equalize are not implemented on the device and are rejected.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+
+namespace fa {
+
+struct MomentSumsDev {
+    double n, s1, s2, s3, s4, lag, nc, c1, c2;
+};
+
+struct RunDev {
+    double sum;
+    int64_t start, end;
+};
+
+struct KadaneDev {
+    double total;
+    int64_t first, last;
+    int32_t empty, pad;
+    RunDev prefix, suffix, best;
+};
+
+class DeviceRfi {
+public:
+    DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, float baseline,
+              int device);
+    ~DeviceRfi();
+    DeviceRfi(const DeviceRfi&)            = delete;
+    DeviceRfi& operator=(const DeviceRfi&) = delete;
+
+    // block: device, capacity rows; writes out[c * stride + offset + t].
+    void execute_normalized(const uint8_t* block, uint64_t capacity, uint64_t nsamps,
+                            float* out, uint64_t stride, uint64_t offset, hipStream_t s);
+
+    const std::vector<uint8_t>& channel_mask() const { return mask_; }
+    const fa_rfi_report& report() const { return report_; }
+    uint64_t nwin() const { return nwin_; }
+    void calibration(float* loc, float* scale, float* inv) const;
+    const std::vector<uint8_t>& time_flags() const { return time_flags_host_; }
+    const fa_block_layout& layout() const { return layout_; }
+    int device() const { return device_; }
+    uint64_t memory_bytes() const;
+
+private:
+    void full_block(const uint8_t* block, uint64_t capacity, uint64_t nsamps, float* out,
+                    uint64_t stride, uint64_t offset, hipStream_t s);
+    void build_channel_mask(hipStream_t s);
+    void build_flags(hipStream_t s);
+    uint64_t window_of(uint64_t t) const {
+        const uint64_t w = t / window_len_;
+        return w < nwin_ - 1 ? w : nwin_ - 1;
+    }
+    uint64_t chunk_count() const;
+
+    fa_rfi_config cfg_{};
+    std::vector<uint64_t> zap_chans_;
+    std::vector<double> zap_freqs_;
+    fa_block_layout layout_{};
+    int device_   = 0;
+    int bits_     = 8;
+    uint64_t nch_ = 0, row_bytes_ = 0;
+    uint64_t window_len_ = 0, smooth_radius_ = 0, time_fill_radius_ = 0,
+             chan_fill_radius_ = 0;
+    float clip_ = 0;
+    bool narrow_active_ = false, need_moments_ = false;
+    uint64_t block_index_ = 0;
+    uint64_t nsamps_ = 0, nwin_ = 0;
+    bool tail_ready_ = false;
+
+    std::vector<uint8_t> static_mask_, mask_, time_flags_host_;
+    fa_rfi_report report_{};
+
+    // Device state.
+    DeviceBuffer<float> loc_, scale_, inv_;
+    DeviceBuffer<uint8_t> measured_;
+    DeviceBuffer<MomentSumsDev> moments_;
+    DeviceBuffer<float> clip_mean_, clip_inv_;
+    DeviceBuffer<uint8_t> mask_dev_, flags_dev_, kwhole_;
+    DeviceBuffer<int32_t> good_prefix_, kspec_;
+    DeviceBuffer<float> s_det_, s_keep_, row_std_;
+    DeviceBuffer<float> cmin_, cmax_;
+    DeviceBuffer<KadaneDev> kpos_, kneg_;
+    DeviceBuffer<int64_t> kseg_; // [nch][4]: pos start/end, neg start/end
+    DeviceBuffer<unsigned long long> counters_;
+    DeviceBuffer<float> tail_loc_, tail_scale_;
+    DeviceBuffer<unsigned long long> bad_index_;
+
+    // Host mirrors.
+    std::vector<float> scale_host_;
+    std::vector<uint8_t> measured_host_;
+    std::vector<uint64_t> unmeasured_;
+    std::vector<MomentSumsDev> moments_host_;
+    hipEvent_t ev_[6] = {};
+};
+
+// Host-side helpers shared with tests of the flag logic.
+std::vector<uint8_t> iqrm_flags(const std::vector<float>& x, uint64_t radius, float threshold,
+                                const std::vector<uint8_t>& ignore);
+uint64_t broadband_flags(std::vector<float> series, uint64_t smooth_radius, float threshold,
+                         uint64_t boxcar_max, uint8_t* flags);
+
+} // namespace fa

@@ -1,0 +1,206 @@
+// Host part of the device RFI stage: decisions on small per-channel and
+// per-sample statistics (IQRM, moment outliers, broad-band time flags).
+// frost runs these on the host in its device path too
+// (lib/cuda/device_rfi_cuda.cu:232-262); the algorithms follow
+// lib/cpu/rfi_masks_cpu.cpp:29-122, lib/cpu/rfi_cpu_measure.cpp:56-84,229-300
+// and lib/cpu/device_rfi_flags_cpu.cpp:17-75 with lib/cpu/stats_cpu.cpp.
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "rfi.hpp"
+
+namespace fa {
+
+namespace {
+
+double lin_quantile(const std::vector<double>& s, double q) {
+    // numpy "linear" quantile of ascending s (lib/cpu/rfi_kernels.hpp:136-142)
+    const double pos = static_cast<double>(s.size() - 1) * q;
+    const auto i     = static_cast<std::size_t>(pos);
+    const auto j     = std::min(i + 1, s.size() - 1);
+    return s[i] + ((pos - static_cast<double>(i)) * (s[j] - s[i]));
+}
+
+inline float minf_(float a, float b) { return b < a ? b : a; }
+inline float maxf_(float a, float b) { return a < b ? b : a; }
+
+float med5(float a, float b, float c, float d, float e) {
+    const float lo = maxf_(minf_(a, b), minf_(c, d));
+    const float hi = minf_(maxf_(a, b), maxf_(c, d));
+    // median of (e, lo, hi)
+    return e < lo ? (lo < hi ? lo : (e < hi ? hi : e)) : (e < hi ? e : (lo < hi ? hi : lo));
+}
+
+void stretch(const float* in, std::size_t n_in, float* out, std::size_t n_out) {
+    // Linear resampling in double (lib/cpu/stats_kernels.hpp:152-176).
+    if (n_out == 0) return;
+    if (n_in == 0) {
+        std::fill(out, out + n_out, 0.0f);
+        return;
+    }
+    if (n_out == 1 || n_in == 1) {
+        std::fill(out, out + n_out, in[0]);
+        return;
+    }
+    const double step = static_cast<double>(n_in - 1) / static_cast<double>(n_out - 1);
+    for (std::size_t i = 0; i < n_out; ++i) {
+        const double x       = static_cast<double>(i) * step;
+        const std::size_t lo = static_cast<std::size_t>(x);
+        const double f       = x - static_cast<double>(lo);
+        const std::size_t hi = std::min(lo + 1, n_in - 1);
+        out[i] = static_cast<float>(static_cast<double>(in[lo]) * (1.0 - f) +
+                                    static_cast<double>(in[hi]) * f);
+    }
+}
+
+// Median-of-5 knots stretched back: subtracts a robust slow baseline
+// (lib/cpu/stats_cpu.cpp:216-277).
+void subtract_baseline(std::vector<float>& x, std::size_t radius) {
+    const std::size_t n = x.size();
+    if (n == 0) return;
+    const std::size_t knots = n / radius + ((n % radius) >= (radius / 2 + radius % 2) ? 1 : 0);
+    if (knots <= 1) return;
+    std::size_t size = knots;
+    while (size <= n / 5) size *= 5;
+    std::vector<float> a(size), b(size);
+    stretch(x.data(), n, a.data(), size);
+    while (size > knots) {
+        const std::size_t m = size / 5;
+        if (size < 5) {
+            b[0] = a[0];
+        } else {
+            for (std::size_t i = 0; i < m; ++i)
+                b[i] = med5(a[5 * i], a[5 * i + 1], a[5 * i + 2], a[5 * i + 3], a[5 * i + 4]);
+        }
+        std::swap(a, b);
+        size = m;
+    }
+    std::vector<float> ext(2 * knots + 2);
+    stretch(a.data(), knots, ext.data() + 1, 2 * knots);
+    ext[0]             = 2 * ext[1] - ext[2];
+    ext[2 * knots + 1] = 2 * ext[2 * knots] - ext[2 * knots - 1];
+    std::vector<float> base(n);
+    stretch(ext.data(), 2 * knots + 2, base.data(), n);
+    for (std::size_t i = 0; i < n; ++i) x[i] -= base[i];
+}
+
+float exact_mad(const std::vector<float>& x) {
+    std::vector<float> a(x.size());
+    for (std::size_t i = 0; i < x.size(); ++i) a[i] = std::abs(x[i]);
+    const std::size_t mid = a.size() / 2;
+    std::nth_element(a.begin(), a.begin() + static_cast<std::ptrdiff_t>(mid), a.end());
+    float m = a[mid];
+    if (a.size() % 2 == 0) {
+        const float lower = *std::max_element(a.begin(), a.begin() + static_cast<std::ptrdiff_t>(mid));
+        m                 = 0.5f * lower + 0.5f * m;
+    }
+    return m / 0.6744897501960817f;
+}
+
+} // namespace
+
+std::vector<uint8_t> iqrm_flags(const std::vector<float>& x, uint64_t radius, float threshold,
+                                const std::vector<uint8_t>& ignore) {
+    // Morello et al. (2022) lagged-difference Tukey votes with edge clamping.
+    const std::size_t n = x.size();
+    if (radius == 0) throw std::invalid_argument("iqrm: radius must be > 0");
+    if (!(threshold > 0.0f) || !std::isfinite(threshold))
+        throw std::invalid_argument("iqrm: threshold must be > 0");
+    std::vector<uint8_t> out(n, 0);
+    if (n == 0) return out;
+    auto skip = [&](std::size_t i) { return !ignore.empty() && ignore[i] != 0; };
+    std::vector<std::vector<uint32_t>> casts(n), got(n);
+    std::vector<double> d(n), vals;
+    std::vector<uint8_t> ok(n);
+    const auto last = static_cast<int64_t>(n) - 1;
+    uint64_t lag    = 1;
+    while (lag <= radius) {
+        for (const int64_t k : {static_cast<int64_t>(lag), -static_cast<int64_t>(lag)}) {
+            vals.clear();
+            for (int64_t i = 0; i <= last; ++i) {
+                const auto j = static_cast<std::size_t>(std::min(std::max<int64_t>(i - k, 0), last));
+                ok[i] = static_cast<uint8_t>(!skip(i) && !skip(j));
+                d[i]  = static_cast<double>(x[i]) - static_cast<double>(x[j]);
+                if (ok[i]) vals.push_back(d[i]);
+            }
+            if (vals.empty()) continue;
+            std::sort(vals.begin(), vals.end());
+            const double q1 = lin_quantile(vals, 0.25), q2 = lin_quantile(vals, 0.5),
+                         q3 = lin_quantile(vals, 0.75);
+            const double cut = static_cast<double>(threshold) * (q3 - q1) / 1.349;
+            for (int64_t i = 0; i <= last; ++i) {
+                if (!ok[i] || !(d[i] - q2 > cut)) continue;
+                const auto j = static_cast<std::size_t>(std::min(std::max<int64_t>(i - k, 0), last));
+                casts[j].push_back(static_cast<uint32_t>(i));
+                got[i].push_back(static_cast<uint32_t>(j));
+            }
+        }
+        if (lag >= n - 1) break;
+        lag = std::max(lag + lag / 2, lag + 1);
+    }
+    for (auto& v : casts) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    for (std::size_t i = 0; i < n; ++i) {
+        auto& g = got[i];
+        std::sort(g.begin(), g.end());
+        g.erase(std::unique(g.begin(), g.end()), g.end());
+        for (const uint32_t j : g)
+            if (casts[j].size() < g.size()) {
+                out[i] = 1;
+                break;
+            }
+        if (skip(i)) out[i] = 1;
+    }
+    return out;
+}
+
+uint64_t broadband_flags(std::vector<float> s, uint64_t radius, float threshold,
+                         uint64_t boxcar_max, uint8_t* flags) {
+    const std::size_t n = s.size();
+    subtract_baseline(s, radius);
+    const float rms = exact_mad(s);
+    if (!(rms > 0.0f)) return 0;
+    for (auto& v : s) v /= rms;
+    std::vector<uint8_t> hit(n, 0);
+    for (std::size_t t = 0; t < n; ++t) hit[t] = std::abs(s[t]) > threshold;
+    if (n > boxcar_max && boxcar_max > 1) {
+        // Centred boxcars from one prefix sum (lib/cpu/stats_cpu.cpp:153-199).
+        auto prefix = [&](const std::vector<float>& in) {
+            std::vector<float> p(n + 1, 0.0f);
+            float acc = 0.0f;
+            for (std::size_t i = 0; i < n; ++i) {
+                acc      = i == 0 ? in[0] : acc + in[i];
+                p[i + 1] = acc;
+            }
+            return p;
+        };
+        const auto ps         = prefix(s);
+        const std::size_t len = n + 1 - boxcar_max, off = boxcar_max / 2;
+        std::vector<float> fm(n);
+        for (uint64_t w = 1; w <= boxcar_max; w *= 2) {
+            const std::size_t ahead = (w - 1) / 2 + 1, behind = w / 2;
+            const float norm        = 1.0f / std::sqrt(static_cast<float>(w));
+            std::fill(fm.begin(), fm.end(), 0.0f);
+            for (std::size_t i = 0; i < len; ++i) {
+                const float f = ps[off + i + ahead] - ps[off + i - behind];
+                fm[off + i]   = static_cast<float>(std::abs(f * norm) > threshold);
+            }
+            const auto pm = prefix(fm);
+            for (std::size_t i = 0; i < len; ++i)
+                if (pm[off + i + ahead] - pm[off + i - behind] > 0.5f) hit[off + i] = 1;
+        }
+    }
+    uint64_t count = 0;
+    for (std::size_t t = 0; t < n; ++t)
+        if (hit[t] && flags[t] == 0) {
+            flags[t] = 1;
+            ++count;
+        }
+    return count;
+}
+
+} // namespace fa

@@ -1,0 +1,82 @@
+"""Python mirror of one beam of frost's Pipeline (lib/pipeline/pipeline.cpp:
+378-619): device RFI -> FDMT -> matched filter -> candidate rows per gulp."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import check, lib
+from .prepare import RFICleaner
+from .search import DeviceArray, FDMTBoxcar, Stream
+
+
+class Beam:
+    def __init__(self, rfi_cfg: abi.RfiConfig, layout: abi.BlockLayout, baseline: float,
+                 dedisp: abi.DedispConfig, boxcar: abi.BoxcarConfig, device: int = 0):
+        h = C.c_void_p()
+        check(lib().fa_beam_create(C.byref(rfi_cfg), C.byref(layout), baseline,
+                                   C.byref(dedisp), C.byref(boxcar), device, C.byref(h)))
+        self.h = h
+        self.layout = layout
+        self.row_bytes = layout.nchans * abi.BITS[layout.dtype] // 8
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().fa_beam_destroy(self.h)
+            self.h = None
+
+    def execute_device(self, gulp: DeviceArray, nsamps: int, stream: Stream):
+        check(lib().fa_beam_execute(self.h, gulp.ptr, nsamps, 1, stream.h))
+
+    def execute(self, gulp: np.ndarray, stream: Stream, nsamps: int = 0):
+        g = np.ascontiguousarray(gulp).view(np.uint8).reshape(-1)
+        n = nsamps or g.size // self.row_bytes
+        check(lib().fa_beam_execute(self.h, g.ctypes.data_as(C.c_void_p), n, 0, stream.h))
+        return self.result()
+
+    def result(self):
+        n, dg, dd, fr = (C.c_uint64() for _ in range(4))
+        check(lib().fa_beam_result(self.h, None, 0, C.byref(n), C.byref(dg), C.byref(dd),
+                                   C.byref(fr)))
+        arr = (abi.RawCandidate * max(n.value, 1))()
+        check(lib().fa_beam_result(self.h, arr, n.value, C.byref(n), C.byref(dg), C.byref(dd),
+                                   C.byref(fr)))
+        return abi.candidate_tuples(arr, n.value), dg.value, dd.value, fr.value
+
+    def drain(self, stream: Stream):
+        check(lib().fa_beam_drain(self.h, stream.h))
+        return self.result()
+
+    def channel_mask(self) -> np.ndarray:
+        out = np.zeros(self.layout.nchans, np.uint8)
+        check(lib().fa_beam_channel_mask(self.h, out.ctypes.data_as(abi.PU8), out.size))
+        return out
+
+    def rfi_report(self) -> abi.RfiReport:
+        r = abi.RfiReport()
+        check(lib().fa_beam_rfi_report(self.h, C.byref(r)))
+        return r
+
+    def engine_handle(self):
+        e = C.c_void_p()
+        check(lib().fa_beam_engine(self.h, C.byref(e)))
+        return e
+
+    def engine_timings(self) -> dict:
+        a, b, c, d = (C.c_double() for _ in range(4))
+        check(lib().fa_engine_timings(self.engine_handle(), C.byref(a), C.byref(b), C.byref(c),
+                                      C.byref(d)))
+        return dict(init=a.value, merge=b.value, search=c.value, total=d.value)
+
+    def level_times(self):
+        e = self.engine_handle()
+        nl, tile = C.c_uint64(), C.c_uint64()
+        check(lib().fa_engine_level_times(e, None, None, 0, C.byref(tile), C.byref(nl)))
+        sec = np.zeros(nl.value, np.float64)
+        cnt = np.zeros(nl.value, np.uint32)
+        check(lib().fa_engine_level_times(e, sec.ctypes.data_as(abi.PF64),
+                                          cnt.ctypes.data_as(abi.PU32), nl.value, C.byref(tile),
+                                          C.byref(nl)))
+        return sec, cnt, tile.value

@@ -360,6 +360,65 @@ int fa_rfi_calibration(const fa_rfi* rfi, uint64_t* nwin, float* loc,
                        float* scale, float* inv_scale, uint64_t cap);
 int fa_rfi_time_flags(const fa_rfi* rfi, uint8_t* out, uint64_t cap);
 
+/* ---- synthetic filterbanks (counterpart of frost's frozen --benchmark
+ *      generator, lib/pipeline/benchmark.cpp:35-60) ------------------------- */
+
+enum fa_pulse_shape { FA_PULSE_BOXCAR = 0, FA_PULSE_GAUSSIAN = 1, FA_PULSE_SCATTERED = 2 };
+enum fa_synth_layout { FA_LAYOUT_TIME_MAJOR = 0, FA_LAYOUT_CHANNEL_MAJOR = 1 };
+
+/* A cold-plasma dispersed pulse: arrival t0 (absolute sample, at the highest
+ * channel centre), width (boxcar samples or Gaussian FWHM), amplitude per
+ * cell in units of sigma, exponential scattering tail tau (samples). */
+typedef struct fa_synth_pulse {
+    float dm;
+    float t0;
+    float width;
+    float amplitude;
+    float tau;
+    int32_t shape; /* enum fa_pulse_shape */
+} fa_synth_pulse;
+
+/* Narrow-band RFI on one channel: sinusoid (period > 0) or step (period 0)
+ * of `amplitude` sigma over absolute samples [t_begin, t_end). */
+typedef struct fa_synth_rfi {
+    uint32_t chan;
+    float amplitude;
+    float period;
+    float pad_;
+    uint64_t t_begin;
+    uint64_t t_end;
+} fa_synth_rfi;
+
+typedef struct fa_synth_config {
+    uint64_t nchans;
+    uint64_t nsamps;
+    int32_t dtype;  /* FA_UINT8 or FA_FLOAT32 */
+    int32_t layout; /* enum fa_synth_layout */
+    double tsamp;
+    double f0;
+    double df;
+    float mean;
+    float sigma;
+    uint64_t seed;
+    uint64_t sample_offset; /* absolute index of the first sample */
+    const float* chan_gain; /* per-channel bandpass gain, NULL = 1 */
+    const fa_synth_pulse* pulses;
+    uint64_t npulses;
+    const fa_synth_rfi* rfi;
+    uint64_t nrfi;
+    const uint64_t* impulses; /* absolute samples of DM-0 broadband impulses */
+    uint64_t nimpulses;
+    float impulse_amplitude;
+    float pad_;
+} fa_synth_config;
+
+/* Writes Gaussian noise (mean, sigma, per-channel gain) plus the listed
+ * pulses, RFI and impulses into `out` (device). uint8 samples are rounded to
+ * nearest and clipped to [0, 255]. Time-major: out[t * nchans + c]; channel
+ * major: out[c * out_stride + t]. Deterministic in (seed, sample, channel). */
+int fa_synth_filterbank(const fa_synth_config* cfg, void* out, uint64_t out_stride,
+                        void* stream);
+
 /* ---- one beam: the per-gulp driver (Pipeline::Impl::execute_block) ------- */
 
 /* Pipeline::Impl (lib/pipeline/pipeline.cpp:378-438): RFI cleaner + FDMT
@@ -383,6 +442,8 @@ int fa_beam_result(fa_beam* beam, fa_raw_candidate* out, uint64_t cap,
 int fa_beam_drain(fa_beam* beam, void* stream);
 int fa_beam_channel_mask(const fa_beam* beam, uint8_t* out, uint64_t cap);
 int fa_beam_rfi_report(const fa_beam* beam, fa_rfi_report* out);
+/* Borrowed handles of the beam's engine and RFI cleaner (valid while the
+ * beam lives; never passed to fa_engine_destroy / fa_rfi_destroy). */
 int fa_beam_engine(fa_beam* beam, fa_engine** engine);
 int fa_beam_rfi(fa_beam* beam, fa_rfi** rfi);
 

@@ -225,8 +225,10 @@ public:
     void search_plane(uint32_t p,
                       const std::vector<std::vector<float>>& node_rows,
                       SizeType N, bool first);
-    mutable std::map<std::pair<int32_t, uint32_t>, double> var_cache;
+    double cached_variance(int32_t len, uint32_t w) const;
+    mutable std::vector<double> var_table; // [len][width], -1 = not yet computed
     void emit(uint32_t p, uint32_t row, uint32_t w, const Open& o);
+    fa_raw_candidate make_candidate(uint32_t p, uint32_t row, uint32_t w, const Open& o) const;
 };
 
 // Direct (brute-force) dedispersion of one plane row from the input, using

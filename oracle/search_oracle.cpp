@@ -337,26 +337,36 @@ Engine::Engine(const fa_dedisp_config& dcfg, const fa_boxcar_config& b,
         open[p].assign(plan.planes[p].delay.size() * widths.size(), Open{});
 }
 
+double Engine::cached_variance(int32_t len, uint32_t w) const {
+    const std::size_t nw = widths.size();
+    if ((std::size_t)len >= var_table.size() / nw) var_table.resize((len + 1) * nw, -1.0);
+    double& v = var_table[(std::size_t)len * nw + w];
+    if (v < 0.0) v = window_variance(len, widths[w], bcfg.use_trapezoid);
+    return v;
+}
+
 std::vector<float> Engine::compute_inv(const std::vector<uint8_t>& m,
                                        uint32_t p) const {
     const PlanePlan& pp = plan.planes[p];
     const auto nw       = widths.size();
     std::vector<float> inv(pp.delay.size() * nw, 0.0F);
-    std::vector<int32_t> off, len;
+    {
+        int32_t maxlen = 1;
+        for (const auto& nd : plan.nodes)
+            if (nd.level == 0 && !nd.delays.empty())
+                maxlen = std::max(maxlen, nd.delays.back() + 1);
+        for (int32_t a = 1; a <= maxlen; ++a)
+            for (uint32_t w = 0; w < nw; ++w) (void)cached_variance(a, w);
+    }
+#pragma omp parallel for schedule(dynamic, 4)
     for (std::size_t r = 0; r < pp.delay.size(); ++r) {
+        std::vector<int32_t> off, len;
         plan.channel_terms(pp.node, pp.delay[r], off, len);
         for (std::size_t w = 0; w < nw; ++w) {
             double var = 0.0;
             for (std::size_t i = 0; i < off.size(); ++i) {
                 if (!m[pp.c0 + i]) continue;
-                const auto key = std::make_pair(len[i], widths[w]);
-                auto it        = var_cache.find(key);
-                if (it == var_cache.end())
-                    it = var_cache
-                             .emplace(key, window_variance(len[i], widths[w],
-                                                           bcfg.use_trapezoid))
-                             .first;
-                var += it->second;
+                var += cached_variance(len[i], (uint32_t)w);
             }
             inv[r * nw + w] = var > 0.0 ? (float)(1.0 / std::sqrt(var)) : 0.0F;
         }
@@ -381,6 +391,11 @@ float window_sum(const std::vector<std::vector<float>>& ladder, SizeType i,
 } // namespace
 
 void Engine::emit(uint32_t p, uint32_t row, uint32_t w, const Open& o) {
+    result.push_back(make_candidate(p, row, w, o));
+}
+
+fa_raw_candidate Engine::make_candidate(uint32_t p, uint32_t row, uint32_t w,
+                                        const Open& o) const {
     const PlanePlan& pp = plan.planes[p];
     fa_raw_candidate c{};
     c.peak_sample   = o.peak;
@@ -399,7 +414,7 @@ void Engine::emit(uint32_t p, uint32_t row, uint32_t w, const Open& o) {
     c.subband_idx   = (uint16_t)pp.idx;
     c.staggered     = (uint8_t)pp.staggered;
     c.beam_idx      = 0;
-    result.push_back(c);
+    return c;
 }
 
 static void finalize_result(std::vector<fa_raw_candidate>& res,
@@ -446,7 +461,12 @@ void Engine::search_plane(uint32_t p,
     int kmax            = 0;
     while ((1U << (kmax + 1)) <= widths.back()) ++kmax;
     if (keep_planes) plane_rows[p].assign(pp.delay.size() * N, 0.0F);
+    std::vector<fa_raw_candidate> found;
+#pragma omp parallel
+    {
     std::vector<std::vector<float>> ladder(kmax + 1, std::vector<float>(E, 0.0F));
+    std::vector<fa_raw_candidate> mine;
+#pragma omp for schedule(dynamic, 4)
     for (uint32_t r = 0; r < pp.delay.size(); ++r) {
         const auto ir =
             std::lower_bound(nd.delays.begin(), nd.delays.end(), pp.delay[r]) -
@@ -484,7 +504,7 @@ void Engine::search_plane(uint32_t p,
                 const float snr  = sum * inv;
                 const bool above = searched && snr > thr;
                 if (bcfg.peak_mode == FA_PEAK_ALL) {
-                    if (above) emit(p, r, w, Open{1, abs_t, abs_t, abs_t, snr});
+                    if (above) mine.push_back(make_candidate(p, r, w, Open{1, abs_t, abs_t, abs_t, snr}));
                     continue;
                 }
                 if (above) {
@@ -498,12 +518,16 @@ void Engine::search_plane(uint32_t p,
                         o.end = abs_t;
                     }
                 } else if (o.open) {
-                    emit(p, r, w, o);
+                    mine.push_back(make_candidate(p, r, w, o));
                     o = Open{};
                 }
             }
         }
     }
+#pragma omp critical
+    found.insert(found.end(), mine.begin(), mine.end());
+    }
+    result.insert(result.end(), found.begin(), found.end());
 }
 
 void Engine::execute(const float* wf, SizeType N, const uint8_t* kill_mask) {
@@ -533,42 +557,42 @@ void Engine::execute(const float* wf, SizeType N, const uint8_t* kill_mask) {
     std::vector<std::vector<std::vector<float>>> prev_rows(plan.nodes.size()),
         cur_rows(plan.nodes.size());
     for (uint32_t l = 0; l < plan.nlevels; ++l) {
+        std::vector<std::pair<std::size_t, std::size_t>> work; // (node, row)
         for (std::size_t id = 0; id < plan.nodes.size(); ++id) {
             const Node& nd = plan.nodes[id];
             if (nd.level != l) continue;
-            auto& lr = cur_rows[id];
-            lr.assign(nd.delays.size(), std::vector<float>(E, 0.0F));
+            cur_rows[id].resize(nd.delays.size());
+            for (std::size_t r = 0; r < nd.delays.size(); ++r) work.push_back({id, r});
+        }
+#pragma omp parallel for schedule(dynamic, 8)
+        for (std::size_t k = 0; k < work.size(); ++k) {
+            const std::size_t id = work[k].first, r = work[k].second;
+            const Node& nd       = plan.nodes[id];
+            auto& out            = cur_rows[id][r];
+            out.assign(E, 0.0F);
+            const int32_t D = nd.delays[r];
             if (l == 0) {
                 const float* x = &X[nd.c0 * E];
-                for (std::size_t r = 0; r < nd.delays.size(); ++r) {
-                    const int32_t D = nd.delays[r];
-                    for (SizeType t = 0; t < E; ++t) {
-                        float acc = x[t];
-                        for (int32_t k = 1; k <= D; ++k)
-                            acc = acc + ((int64_t)t - k >= 0 ? x[t - k] : 0.0F);
-                        lr[r][t] = acc;
-                    }
+                for (SizeType t = 0; t < E; ++t) {
+                    float acc = x[t];
+                    for (int32_t q = 1; q <= D; ++q)
+                        acc = acc + ((int64_t)t - q >= 0 ? x[t - q] : 0.0F);
+                    out[t] = acc;
                 }
                 continue;
             }
             const Node& lo    = plan.nodes[nd.lo];
             const Node& hi    = plan.nodes[nd.hi];
             const uint32_t cm = (nd.c0 + nd.c1) / 2;
-            const double frac = plan.lo_fraction(nd.c0, cm, nd.c1);
-            for (std::size_t r = 0; r < nd.delays.size(); ++r) {
-                const int32_t D  = nd.delays[r];
-                const int32_t dl = round_delay((double)D * frac);
-                const auto il =
-                    std::lower_bound(lo.delays.begin(), lo.delays.end(), dl) -
-                    lo.delays.begin();
-                const auto ih = std::lower_bound(hi.delays.begin(),
-                                                 hi.delays.end(), D - dl) -
-                                hi.delays.begin();
-                const auto& a = prev_rows[nd.lo][il];
-                const auto& b = prev_rows[nd.hi][ih];
-                for (SizeType t = 0; t < E; ++t)
-                    lr[r][t] = a[t] + ((int64_t)t - dl >= 0 ? b[t - dl] : 0.0F);
-            }
+            const int32_t dl  = round_delay((double)D * plan.lo_fraction(nd.c0, cm, nd.c1));
+            const auto il =
+                std::lower_bound(lo.delays.begin(), lo.delays.end(), dl) - lo.delays.begin();
+            const auto ih = std::lower_bound(hi.delays.begin(), hi.delays.end(), D - dl) -
+                            hi.delays.begin();
+            const auto& a = prev_rows[nd.lo][il];
+            const auto& b = prev_rows[nd.hi][ih];
+            for (SizeType t = 0; t < E; ++t)
+                out[t] = a[t] + ((int64_t)t - dl >= 0 ? b[t - dl] : 0.0F);
         }
         for (uint32_t p = 0; p < plan.planes.size(); ++p)
             if (plan.planes[p].level == l)
