@@ -109,16 +109,20 @@ def main() -> int:
     ndm = summ["ndm"]
     for i in range(a.warmup):
         beam.execute_device(gulp, W.gulp, stream)
-        log(f"warmup {i}: {len(beam.result()[0])} candidates")
+        log(f"warmup {i}: {beam.candidate_count()} candidates")
     lat, merge_s, merge_launches, ncand = [], 0.0, 0, 0
     stage = {"rfi": 0.0, "init": 0.0, "merge": 0.0, "search": 0.0}
-    t_start = time.perf_counter()
+    stream.synchronize()
     for i in range(a.steps):
+        # One step: RFI + FDMT + search + candidate rows on the host. The
+        # call returns after the candidates are copied back (it synchronises).
         t0 = time.perf_counter()
         beam.execute_device(gulp, W.gulp, stream)
+        stream.synchronize()
         lat.append(time.perf_counter() - t0)
-        res = beam.result()
-        ncand += len(res[0])
+        # Diagnostics between steps, outside the timed intervals.
+        nc = beam.candidate_count()
+        ncand += nc
         sec, cnt, tile = beam.level_times()
         merge_s += float(sec[1:].sum())
         merge_launches += int(cnt[1:].sum())
@@ -127,10 +131,9 @@ def main() -> int:
         stage["rfi"] += rep.t_bandpass + rep.t_moments + rep.t_scan + rep.t_flags + rep.t_apply
         for k in ("init", "merge", "search"):
             stage[k] += tm[k]
-        log(f"step {i}: {lat[-1]:.3f} s, {len(res[0])} candidates, masked "
+        log(f"step {i}: {lat[-1]:.3f} s, {nc} candidates, masked "
             f"{rep.chans_masked} ch, flagged {rep.samps_flagged} samples")
-    stream.synchronize()
-    total = time.perf_counter() - t_start
+    total = float(sum(lat))
     value = ndm * W.gulp * a.steps / total
     # Dominant kernel: the FDMT merge. Algorithmic bytes of one full gulp pass:
     # read the input rows and write the output rows of every level.

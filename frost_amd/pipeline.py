@@ -45,6 +45,11 @@ class Beam:
                                    C.byref(fr)))
         return abi.candidate_tuples(arr, n.value), dg.value, dd.value, fr.value
 
+    def candidate_count(self) -> int:
+        n = C.c_uint64()
+        check(lib().fa_beam_result(self.h, None, 0, C.byref(n), None, None, None))
+        return n.value
+
     def drain(self, stream: Stream):
         check(lib().fa_beam_drain(self.h, stream.h))
         return self.result()
