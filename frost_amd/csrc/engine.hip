@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 #include <tuple>
 
 namespace fa {
@@ -319,6 +320,277 @@ __global__ void __launch_bounds__(kBlock) search_kernel(SearchArgs a) {
     if (threadIdx.x < nw) a.open[(uint64_t)prow * nw + threadIdx.x] = st[threadIdx.x];
 }
 
+// ---- register-ladder search (power-of-two widths up to 256) -----------------
+//
+// One wave per plane row. A chunk is 256 samples: lane l holds samples
+// 4l..4l+3 of the chunk in registers. Ladder level k is built from level k-1
+// shifted by 2^(k-1) samples: in-lane for shifts below 4, otherwise a wave
+// shuffle from lane l - shift/4, taking the previous chunk's registers for
+// lanes that fall before the chunk. The sums are the same additions, in the
+// same order, as the LDS ladder (D_k(t) = D_{k-1}(t) + D_{k-1}(t - 2^(k-1))).
+
+constexpr int kRegChunk = 256;
+
+template <int S>
+__device__ __forceinline__ float back(const float (&cur)[4], const float (&prev)[4], int j,
+                                      int lane) {
+    // Value at sample (4*lane + j - S) of the chunk, S >= 1.
+    const int p  = j - S;                      // relative to 4*lane
+    const int dl = (p >= 0) ? 0 : -((-p + 3) / 4); // floor(p / 4)
+    const int jj = p - 4 * dl;                  // 0..3
+    if (dl == 0) return cur[jj];
+    const int src  = lane + dl;
+    const float a  = __shfl(cur[jj], src & 63);
+    const float b  = __shfl(prev[jj], src & 63);
+    return src >= 0 ? a : b;
+}
+
+template <int K, int KMAX, bool TRAP> struct LadderStep;
+
+struct RegSearchArgs {
+    SearchArgs a;
+    uint32_t present; // bit k set when width 2^k is searched
+    int widx[16];     // width index of level k
+};
+
+template <int K>
+__device__ __forceinline__ float ladder_back(const float (&D)[9][4], const float (&P)[9][4], int j,
+                                             int lane) {
+    // D_{K-1} at t - 2^(K-1)
+    switch (K) {
+    case 1: return back<1>(D[0], P[0], j, lane);
+    case 2: return back<2>(D[1], P[1], j, lane);
+    case 3: return back<4>(D[2], P[2], j, lane);
+    case 4: return back<8>(D[3], P[3], j, lane);
+    case 5: return back<16>(D[4], P[4], j, lane);
+    case 6: return back<32>(D[5], P[5], j, lane);
+    case 7: return back<64>(D[6], P[6], j, lane);
+    default: return back<128>(D[7], P[7], j, lane);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ float trap_back(const float (&D)[9][4], const float (&P)[9][4], int j,
+                                           int lane) {
+    // D_{K-2} at t - 3 * 2^(K-2)  (second trapezoid edge, e = 2^(K-2))
+    switch (K) {
+    case 2: return back<3>(D[0], P[0], j, lane);
+    case 3: return back<6>(D[1], P[1], j, lane);
+    case 4: return back<12>(D[2], P[2], j, lane);
+    case 5: return back<24>(D[3], P[3], j, lane);
+    case 6: return back<48>(D[4], P[4], j, lane);
+    case 7: return back<96>(D[5], P[5], j, lane);
+    default: return back<192>(D[6], P[6], j, lane);
+    }
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Serial run tracking of one width over one chunk (lane 0), the S/N values
+// staged in LDS. Returns the run's open flag afterwards.
+__device__ uint32_t walk_chunk(const SearchArgs& a, float* snr_lds, OpenRun* st, uint32_t prow,
+                               int w, int64_t tc0, int clen, int lane) {
+    if (lane == 0) {
+        OpenRun o = *st;
+        for (int i = 0; i < clen; ++i) {
+            const float sn       = snr_lds[i];
+            const uint64_t abs_t = a.gulp_abs + (uint64_t)(tc0 + i);
+            const bool above     = sn > a.thr;
+            if (a.mode == FA_PEAK_ALL) {
+                if (above) emit(a, prow, w, abs_t, abs_t, abs_t, sn);
+                continue;
+            }
+            if (above) {
+                if (!o.open) {
+                    o.open  = 1;
+                    o.start = o.peak = o.end = abs_t;
+                    o.snr   = sn;
+                } else {
+                    if (sn > o.snr) {
+                        o.snr  = sn;
+                        o.peak = abs_t;
+                    }
+                    o.end = abs_t;
+                }
+            } else if (o.open) {
+                emit(a, prow, w, o.start, o.peak, o.end, o.snr);
+                o.open = 0;
+            }
+        }
+        *st = o;
+    }
+    wave_sync();
+    return __builtin_amdgcn_readfirstlane(st->open);
+}
+
+template <int KMAX, bool TRAP>
+__global__ void __launch_bounds__(kBlock) search_reg_kernel(RegSearchArgs ra) {
+    const SearchArgs& a = ra.a;
+    __shared__ float s_snr[kBlock / 64][kRegChunk];
+    __shared__ OpenRun s_st[kBlock / 64][16];
+    const int lane = threadIdx.x % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int rowi = blockIdx.x * (kBlock / 64) + wave;
+    if (rowi >= a.nrows) return;
+    const SearchRow row = a.rows[rowi];
+    const uint32_t prow = row.prow;
+    const int nw        = a.nw;
+    float inv_c[KMAX + 1], inv_o[KMAX + 1];
+#pragma unroll
+    for (int k = 0; k <= KMAX; ++k) {
+        const int w = ra.widx[k];
+        inv_c[k]    = w >= 0 ? a.inv_cur[(uint64_t)prow * nw + w] : 0.0f;
+        inv_o[k]    = w >= 0 ? a.inv_or[(uint64_t)prow * nw + w] : 0.0f;
+    }
+    uint32_t open_mask = 0; // bit w: width index w has an open run
+    {
+        uint32_t mine = 0;
+        if (lane < nw) {
+            s_st[wave][lane] = a.open[(uint64_t)prow * nw + lane];
+            mine             = s_st[wave][lane].open ? (1U << lane) : 0U;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) mine |= __shfl_xor(mine, off);
+        open_mask = __builtin_amdgcn_readfirstlane(mine);
+    }
+    wave_sync();
+    const int64_t warm_max = (int64_t)row.delay + (1 << KMAX) - 1;
+    const float* src       = a.buf + row.in + a.wm1; // sample t0
+    const int tl           = a.tile_len;
+    float D[9][4], P[9][4], nx[4];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) P[k][j] = D[k][j] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int i = -kRegChunk + 4 * lane + j;
+        nx[j]       = (i >= -a.wm1 && i < tl) ? src[i] : 0.0f;
+    }
+    for (int u = -kRegChunk; u < tl; u += kRegChunk) {
+        const int base = u + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) D[0][j] = nx[j];
+        // Prefetch the next chunk.
+        if (u + 2 * kRegChunk <= tl) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) nx[j] = src[base + kRegChunk + j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int i = base + kRegChunk + j;
+                nx[j]       = i < tl ? src[i] : 0.0f;
+            }
+        }
+        if (a.keep && u >= 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (base + j < tl)
+                    a.keep[(uint64_t)prow * a.keep_stride + a.tile_t0 + base + j] = D[0][j];
+        }
+        const int64_t tc0 = a.tile_t0 + u;
+        const bool fast   = tc0 >= warm_max && u + kRegChunk <= tl;
+#pragma unroll
+        for (int k = 0; k <= KMAX; ++k) {
+            if (k > 0) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float bj;
+                    switch (k) {
+                    case 1: bj = ladder_back<1>(D, P, j, lane); break;
+                    case 2: bj = ladder_back<2>(D, P, j, lane); break;
+                    case 3: bj = ladder_back<3>(D, P, j, lane); break;
+                    case 4: bj = ladder_back<4>(D, P, j, lane); break;
+                    case 5: bj = ladder_back<5>(D, P, j, lane); break;
+                    case 6: bj = ladder_back<6>(D, P, j, lane); break;
+                    case 7: bj = ladder_back<7>(D, P, j, lane); break;
+                    default: bj = ladder_back<8>(D, P, j, lane); break;
+                    }
+                    D[k][j] = D[k - 1][j] + bj;
+                }
+            }
+            const int w = ra.widx[k];
+            if (u < 0 || w < 0) continue;
+            const uint32_t W = 1U << k;
+            float sv[4];
+            bool pred = false;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float sum = D[k][j];
+                if (TRAP && k >= 2) {
+                    float e2;
+                    switch (k) {
+                    case 2: e2 = trap_back<2>(D, P, j, lane); break;
+                    case 3: e2 = trap_back<3>(D, P, j, lane); break;
+                    case 4: e2 = trap_back<4>(D, P, j, lane); break;
+                    case 5: e2 = trap_back<5>(D, P, j, lane); break;
+                    case 6: e2 = trap_back<6>(D, P, j, lane); break;
+                    case 7: e2 = trap_back<7>(D, P, j, lane); break;
+                    default: e2 = trap_back<8>(D, P, j, lane); break;
+                    }
+                    sum = (sum - 0.5f * D[k - 2][j]) - 0.5f * e2;
+                }
+                float sn;
+                if (fast) {
+                    sn = sum * inv_c[k];
+                } else {
+                    const int64_t t = tc0 + 4 * lane + j;
+                    sn              = -INFINITY;
+                    if (base + j < tl) {
+                        if (t < (int64_t)row.delay + (int64_t)W - 1) {
+                            if (!a.first) sn = sum * inv_o[k];
+                        } else {
+                            sn = sum * inv_c[k];
+                        }
+                    }
+                }
+                sv[j] = sn;
+                pred  = pred || sn > a.thr;
+            }
+            if (__ballot(pred) != 0 || ((open_mask >> w) & 1U)) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) s_snr[wave][4 * lane + j] = sv[j];
+                wave_sync();
+                const int clen = min(kRegChunk, tl - u);
+                const uint32_t op =
+                    walk_chunk(a, s_snr[wave], &s_st[wave][w], prow, w, tc0, clen, lane);
+                open_mask = op ? (open_mask | (1U << w)) : (open_mask & ~(1U << w));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k <= KMAX; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) P[k][j] = D[k][j];
+    }
+    wave_sync();
+    if (lane < nw) a.open[(uint64_t)prow * nw + lane] = s_st[wave][lane];
+}
+
+template <int KMAX, bool TRAP>
+void launch_reg(const RegSearchArgs& ra, unsigned blocks, hipStream_t s) {
+    search_reg_kernel<KMAX, TRAP><<<blocks, kBlock, 0, s>>>(ra);
+}
+
+using RegLauncher = void (*)(const RegSearchArgs&, unsigned, hipStream_t);
+
+template <bool TRAP> RegLauncher reg_launcher(int kmax) {
+    switch (kmax) {
+    case 0: return launch_reg<0, TRAP>;
+    case 1: return launch_reg<1, TRAP>;
+    case 2: return launch_reg<2, TRAP>;
+    case 3: return launch_reg<3, TRAP>;
+    case 4: return launch_reg<4, TRAP>;
+    case 5: return launch_reg<5, TRAP>;
+    case 6: return launch_reg<6, TRAP>;
+    case 7: return launch_reg<7, TRAP>;
+    default: return launch_reg<8, TRAP>;
+    }
+}
+
 __global__ void set_u64_kernel(unsigned long long* p, unsigned long long v0,
                                unsigned long long v1) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -404,6 +676,10 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     wm1_ = widths_.back() - 1;
     pow2_widths_ = true;
     for (const uint32_t w : widths_) pow2_widths_ = pow2_widths_ && (w & (w - 1)) == 0;
+    // Register ladder: power-of-two widths up to 256 (shifts within one
+    // 256-sample chunk). FROST_AMD_SEARCH=lds forces the LDS ladder.
+    const char* sv = std::getenv("FROST_AMD_SEARCH");
+    reg_search_ = pow2_widths_ && widths_.back() <= 256 && !(sv && std::string(sv) == "lds");
     cap_ = plan.cfg.block_nsamps;
     tile_ = std::min<uint64_t>(cap_, tile ? tile : env_tile());
     const uint64_t nch = plan.cfg.nchans;
@@ -731,10 +1007,24 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                 a.keep     = keep_planes_ ? keep_.get() : nullptr;
                 a.keep_stride = cap_;
                 record_pair(-1, s, true);
-                if (pow2_widths_)
+                if (reg_search_) {
+                    RegSearchArgs ra{};
+                    ra.a       = a;
+                    ra.present = 0;
+                    for (int k = 0; k < 16; ++k) ra.widx[k] = -1;
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        const int k = 31 - __builtin_clz(widths_[w]);
+                        ra.present |= 1U << k;
+                        ra.widx[k] = static_cast<int>(w);
+                    }
+                    const unsigned blocks = ceil_div(search_count_[l], kBlock / 64);
+                    (cfg_.use_trapezoid ? reg_launcher<true>(static_cast<int>(kmax_))
+                                        : reg_launcher<false>(static_cast<int>(kmax_)))(ra, blocks, s);
+                } else if (pow2_widths_) {
                     search_kernel<true><<<search_count_[l], kBlock, lds_search, s>>>(a);
-                else
+                } else {
                     search_kernel<false><<<search_count_[l], kBlock, lds_search, s>>>(a);
+                }
                 record_pair(-1, s, false);
             }
         }

@@ -115,6 +115,7 @@ private:
     std::vector<uint32_t> widths_;
     uint32_t kmax_ = 0, wm1_ = 0;
     bool pow2_widths_ = true;
+    bool reg_search_  = false;
     uint64_t cap_ = 0, tile_ = 0, hist_in_ = 0, xstride_ = 0;
 
     // Per-level geometry.

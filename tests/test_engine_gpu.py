@@ -22,16 +22,20 @@ def stream():
     return search.Stream(0)
 
 
-def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None):
+def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None,
+             search_path=None):
     """Runs the GPU engine and the oracle on the same gulps; returns both
     result lists (per gulp + drain) and, if keep, the plane rows."""
     if tile:
         os.environ["FROST_AMD_TILE"] = str(tile)
+    if search_path:
+        os.environ["FROST_AMD_SEARCH"] = search_path
     try:
         plan = search.make_fdmt_plan(dcfg)
         gpu = search.FDMTBoxcar(plan, bcfg)
     finally:
         os.environ.pop("FROST_AMD_TILE", None)
+        os.environ.pop("FROST_AMD_SEARCH", None)
     if keep:
         gpu.keep_planes(True)
     ora = oracle.Engine(dcfg, bcfg, keep_planes=keep)
@@ -97,9 +101,9 @@ def test_engine_single_gulp_small(oracle, stream):
     assert len(o[0][0]) > 0
 
 
-@pytest.mark.parametrize("depth", [0, 1, 2])
-def test_engine_gulps_and_planes(oracle, stream, depth):
-    say(f"test_engine_gulps_and_planes depth={depth}")
+@pytest.mark.parametrize("depth,path", [(0, None), (1, None), (2, None), (2, "lds")])
+def test_engine_gulps_and_planes(oracle, stream, depth, path):
+    say(f"test_engine_gulps_and_planes depth={depth} search={path}")
     dcfg = abi.dedisp_config(64, 256e-6, 1500.0, -300 / 64, 500.0, block_nsamps=4096,
                              subband_depth=depth)
     bcfg = abi.boxcar_config(max_boxcar=64, detect_thresh=5.0)
@@ -107,7 +111,7 @@ def test_engine_gulps_and_planes(oracle, stream, depth):
     gulps = [pulse_waterfall(rng, 64, 4096, [(100.0 * (k + 1), 1000 + 500 * k, 8, 1.5),
                                              (320.0, 3900, 3, 3.0)], dcfg)
              for k in range(3)]
-    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps)
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, search_path=path)
     assert_same(g, o)
 
 
@@ -142,6 +146,20 @@ def test_engine_widths_and_modes(oracle, stream, trap, wtsp, mode):
     rng = np.random.default_rng(3)
     gulps = [pulse_waterfall(rng, 32, 3000, [(200.0, 1200, 20, 1.0)], dcfg) for _ in range(2)]
     g, o = run_pair(oracle, stream, dcfg, bcfg, gulps)
+    assert_same(g, o)
+
+
+@pytest.mark.parametrize("trap", [False, True])
+def test_engine_register_ladder_wide(oracle, stream, trap):
+    """Widths 1..256 (register ladder, all shifts up to 192 samples) with and
+    without trapezoids, two tiles per gulp, three gulps."""
+    say(f"test_engine_register_ladder_wide trap={trap}")
+    dcfg = abi.dedisp_config(32, 256e-6, 1500.0, -4.0, 300.0, block_nsamps=6000)
+    bcfg = abi.boxcar_config(max_boxcar=256, use_trapezoid=trap, detect_thresh=4.0)
+    rng = np.random.default_rng(17)
+    gulps = [pulse_waterfall(rng, 32, 6000, [(150.0, 700 + 1500 * k, 40 * (k + 1), 0.5),
+                                             (250.0, 5950, 2, 2.5)], dcfg) for k in range(3)]
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, tile=2500)
     assert_same(g, o)
 
 
