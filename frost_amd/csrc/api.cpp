@@ -377,4 +377,15 @@ int fa_engine_level_times(const fa_engine* engine, double* seconds, uint32_t* la
     });
 }
 
+int fa_engine_level_sources(const fa_engine* engine, int32_t* source, uint64_t cap,
+                            uint64_t* nlevels) {
+    return guard([&] {
+        require(engine != nullptr, "engine must not be null");
+        const auto& src = engine->eng->level_source();
+        if (nlevels) *nlevels = src.size();
+        for (std::size_t l = 0; l < std::min<uint64_t>(cap, src.size()); ++l)
+            if (source) source[l] = src[l];
+    });
+}
+
 } // extern "C"

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -15,7 +16,6 @@ namespace fa {
 namespace {
 
 constexpr int kBlock       = 256;
-constexpr int kMergeEpt    = 16;   // elements per thread per merge block
 constexpr int kInitPer     = 4;    // outputs per thread per init block
 constexpr int kChunk       = 1024; // search samples per LDS pass
 constexpr uint32_t kMaxBox = 1024; // ladder fits LDS up to this width
@@ -65,33 +65,282 @@ fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
     }
 }
 
-// Level l >= 1: out[r][i] = in[lo_r + i] + in[hi_r + i], i < tile + h_r.
+// Levels in+1 .. out in one pass: out[r][i] = tree sum of in[src_k + i] over
+// the row's K = 2^depth descendants, i < tile + h_r.
+// Tree-order sum of v[O .. O+K): ((v0 + v1) + (v2 + v3)) + ...
+template <int K, int O = 0, int N>
+__device__ __forceinline__ float tree_sum(const float (&v)[N]) {
+    if constexpr (K == 1) {
+        return v[O];
+    } else {
+        return tree_sum<K / 2, O>(v) + tree_sum<K / 2, O + K / 2>(v);
+    }
+}
+
+// One block: G consecutive rows of the launch over one time chunk of
+// kBlock * EPT samples. Blocks go round-robin to the 8 XCDs by linear id, so
+// the id is decoded as (xcd = id % 8, k = id / 8) and each XCD sweeps all
+// row groups of chunk xcd, then of chunk xcd + 8, ...: consecutive rows share
+// most of their input rows, which then stay in that XCD's L2.
+template <int K, int EPT, int G>
 __global__ void __launch_bounds__(kBlock)
 fdmt_merge_kernel(const float* __restrict__ in, float* __restrict__ out,
-                  const MergeRow* __restrict__ rows, int nrows, int tile_len) {
-    for (int r = blockIdx.y; r < nrows; r += gridDim.y) {
-        const MergeRow m = rows[r];
-        const int len    = tile_len + m.hist;
-        const int base   = blockIdx.x * (kBlock * kMergeEpt);
+                  const int64_t* __restrict__ prog, int nrows, int ngroups, int tile_len) {
+    const uint32_t id    = blockIdx.x;
+    const uint32_t k     = id >> 3;
+    const uint32_t group = k % (uint32_t)ngroups;
+    const uint32_t chunk = (id & 7U) + 8U * (k / (uint32_t)ngroups);
+    const int base       = (int)chunk * (kBlock * EPT);
+    const int r_end      = min((int)group * G + G, nrows);
+    for (int r = (int)group * G; r < r_end; ++r) {
+        const int64_t* p = prog + (int64_t)r * (K + 2);
+        const int len    = tile_len + (int)p[K + 1];
         if (base >= len) continue;
-        const float* a = in + m.lo;
-        const float* b = in + m.hi;
-        float* o       = out + m.out;
-        float va[kMergeEpt], vb[kMergeEpt];
+        const float* src[K];
 #pragma unroll
-        for (int q = 0; q < kMergeEpt; ++q) {
+        for (int j = 0; j < K; ++j) src[j] = in + p[1 + j];
+        float* o = out + p[0];
+        float v[EPT][K];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
             const int i = base + threadIdx.x + q * kBlock;
             if (i < len) {
-                va[q] = a[i];
-                vb[q] = b[i];
+#pragma unroll
+                for (int j = 0; j < K; ++j) v[q][j] = src[j][i];
             }
         }
 #pragma unroll
-        for (int q = 0; q < kMergeEpt; ++q) {
+        for (int q = 0; q < EPT; ++q) {
             const int i = base + threadIdx.x + q * kBlock;
-            if (i < len) o[i] = va[q] + vb[q];
+            if (i < len) o[i] = tree_sum<K>(v[q]);
         }
     }
+}
+
+// Staged gather. One block: one row block over a run of consecutive time
+// chunks of kChunkG samples (tile-relative tau). Waves 0-3 (producers) load
+// each distinct input row's window for chunk c + 1 into registers and stage
+// it in one half of a double-buffered LDS area while waves 4-7 (consumers)
+// sum each output row's K descendants for chunk c from the other half in
+// tree order and store them coalesced. Producers issue only loads and
+// consumers only stores, so neither waits on the other's memory traffic;
+// one barrier per chunk (LDS-only fences) hands the halves over.
+constexpr int kChunkG  = 256;
+constexpr int kSpreadG = 64; // max shift spread of one input row within a block
+constexpr int kGatherThreads = 512;
+
+struct GatherDims {
+    int nblocks;
+    int tile_len;
+    int tau_origin;
+    int half;   // floats per LDS half
+    int chunks; // consecutive chunks per block
+};
+
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int K, int UMAX>
+__global__ void __launch_bounds__(kGatherThreads)
+fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
+                   const GatherBlock* __restrict__ blocks, const GatherLeaf* __restrict__ leaves,
+                   const int32_t* __restrict__ rows, GatherDims g) {
+    extern __shared__ float lds[];
+    constexpr int M     = (kChunkG + kSpreadG) / 64;
+    const uint32_t id   = blockIdx.x;
+    const uint32_t k    = id >> 3;
+    const uint32_t blk  = k % (uint32_t)g.nblocks;
+    const uint32_t grp  = (id & 7U) + 8U * (k / (uint32_t)g.nblocks);
+    const int tau_first = g.tau_origin + (int)grp * (kChunkG * g.chunks);
+    const GatherBlock B = blocks[blk];
+    const int lo_c      = max(0, (-B.hmax - tau_first) / kChunkG); // first chunk in range
+    const int hi_c      = min(g.chunks, (g.tile_len - tau_first + kChunkG - 1) / kChunkG);
+    if (tau_first + kChunkG * g.chunks <= -B.hmax || lo_c >= hi_c) return; // uniform
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wave < 4) {
+        // Producer: two chunks of loads in flight (register sets A and B).
+        // Loads are unconditional at clamped in-range addresses (the value
+        // is replaced by 0 outside the input row's samples), so their count
+        // is static and waiting for set A leaves set B in flight.
+        float va[UMAX][M], vb[UMAX][M];
+        auto load = [&](float (&v)[UMAX][M], int c) {
+            const int tau0 = tau_first + c * kChunkG;
+#pragma unroll
+            for (int u = 0; u < UMAX; ++u) {
+                const int sidx      = min(wave + 4 * u, B.nleaves - 1);
+                const GatherLeaf lf = leaves[B.leaf_first + sidx];
+                const float* row0   = in + lf.src + lf.shmax; // sample t0 of the input row
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const int tp  = tau0 + lane + 64 * m - lf.shmax;
+                    const int tpc = min(max(tp, -lf.hc), g.tile_len - 1);
+                    v[u][m]       = row0[tpc];
+                }
+            }
+        };
+        // Zero outside the input row's samples is applied here, where the
+        // loaded values are consumed, so the loads stay in flight until then.
+        auto stage = [&](const float (&v)[UMAX][M], int c) {
+            const int tau0 = tau_first + c * kChunkG;
+            float* buf     = lds + (c & 1) * g.half;
+#pragma unroll
+            for (int u = 0; u < UMAX; ++u) {
+                const int sidx = wave + 4 * u;
+                if (sidx < B.nleaves) {
+                    const GatherLeaf lf = leaves[B.leaf_first + sidx];
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        const int j  = lane + 64 * m;
+                        const int tp = tau0 + j - lf.shmax;
+                        if (j < lf.len)
+                            buf[lf.slot + j] = (tp >= -lf.hc && tp < g.tile_len) ? v[u][m] : 0.0f;
+                    }
+                }
+            }
+        };
+        load(va, lo_c);
+        if (lo_c + 1 < hi_c) load(vb, lo_c + 1);
+        for (int c = lo_c; c < hi_c; c += 2) {
+            stage(va, c);
+            lds_barrier();
+            if (c + 2 < hi_c) load(va, c + 2);
+            if (c + 1 < hi_c) {
+                stage(vb, c + 1);
+                lds_barrier();
+                if (c + 3 < hi_c) load(vb, c + 3);
+            }
+        }
+    } else {
+        // Consumer: wave 4-7 covers samples 64 * (wave - 4) .. + 64 of a chunk.
+        // Row record: [out lo, out hi, h, pad, delta_0 .. delta_{K-1}].
+        constexpr int RS = 4 + K;
+        const int t      = threadIdx.x - 256;
+        const int r_end  = B.row_first + B.nrows;
+        for (int c = lo_c; c < hi_c; ++c) {
+            lds_barrier();
+            const float* buf   = lds + (c & 1) * g.half;
+            const int tau      = tau_first + c * kChunkG + t;
+            const bool in_tile = tau < g.tile_len;
+            int r = B.row_first;
+            for (; r + 4 <= r_end; r += 4) {
+                int32_t rec[4][RS];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int j = 0; j < RS; ++j) rec[q][j] = rows[(int64_t)(r + q) * RS + j];
+                // Every LDS offset stays inside the staged windows, so all four
+                // rows' sums are formed first and only the stores are guarded.
+                float sum[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    float w[K];
+#pragma unroll
+                    for (int j = 0; j < K; ++j) w[j] = buf[rec[q][4 + j] + t];
+                    sum[q] = tree_sum<K>(w);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (in_tile && tau >= -rec[q][2]) {
+                        const int64_t o = (int64_t)(uint32_t)rec[q][0] | ((int64_t)rec[q][1] << 32);
+                        out[o + tau]    = sum[q];
+                    }
+                }
+            }
+            for (; r < r_end; ++r) {
+                int32_t rec[RS];
+#pragma unroll
+                for (int j = 0; j < RS; ++j) rec[j] = rows[(int64_t)r * RS + j];
+                if (in_tile && tau >= -rec[2]) {
+                    float w[K];
+#pragma unroll
+                    for (int j = 0; j < K; ++j) w[j] = buf[rec[4 + j] + t];
+                    const int64_t o = (int64_t)(uint32_t)rec[0] | ((int64_t)rec[1] << 32);
+                    out[o + tau]    = tree_sum<K>(w);
+                }
+            }
+        }
+    }
+}
+
+template <int K, int UMAX>
+void launch_gather_u(const MergeLaunch& ml, const float* in, float* out, const GatherDims& g,
+                     int tl, hipStream_t s) {
+    const unsigned ngrp   = ceil_div(tl + ml.hmax, kChunkG * g.chunks);
+    const uint64_t blocks = (uint64_t)((ngrp + 7) / 8) * 8 * ml.nblocks;
+    if (blocks * kGatherThreads >= (uint64_t{1} << 32))
+        throw std::invalid_argument("FDMT merge grid too large; reduce the time tile");
+    fdmt_gather_kernel<K, UMAX><<<static_cast<unsigned>(blocks), kGatherThreads,
+                                  2 * g.half * sizeof(float), s>>>(
+        in, out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), g);
+}
+
+int gather_chunks() {
+    static const int n = [] {
+        const char* v = std::getenv("FROST_AMD_GATHER_CHUNKS");
+        const int c   = v ? std::atoi(v) : 8;
+        return (c >= 1 && c <= 64) ? c : 8;
+    }();
+    return n;
+}
+
+template <int K>
+void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, hipStream_t s) {
+    static_assert(kChunkG == 256, "four consumer waves cover one chunk");
+    GatherDims g{};
+    g.nblocks    = static_cast<int>(ml.nblocks);
+    g.tile_len   = tl;
+    g.tau_origin = -ml.hmax;
+    g.half       = static_cast<int>(ml.lds_floats);
+    g.chunks     = gather_chunks();
+    switch (ml.max_leaves) {
+    case 16: return launch_gather_u<K, 4>(ml, in, out, g, tl, s);
+    case 32: return launch_gather_u<K, 8>(ml, in, out, g, tl, s);
+    case 48: return launch_gather_u<K, 12>(ml, in, out, g, tl, s);
+    case 64: return launch_gather_u<K, 16>(ml, in, out, g, tl, s);
+    default: throw std::logic_error("staged FDMT gather: unsupported input rows per block");
+    }
+}
+
+struct MergeShape {
+    int ept = 4, rows = 4;
+};
+
+MergeShape merge_shape(int K) {
+    MergeShape m;
+    m.ept  = K == 2 ? 8 : 4;
+    m.rows = 4;
+    if (const char* v = std::getenv("FROST_AMD_MERGE_EPT")) m.ept = std::atoi(v);
+    if (const char* v = std::getenv("FROST_AMD_MERGE_ROWS")) m.rows = std::atoi(v);
+    return m;
+}
+
+template <int K, int EPT, int G>
+void launch_merge_shape(const float* in, float* out, const int64_t* prog, unsigned nrows, int tl,
+                        int hmax, hipStream_t s) {
+    const unsigned nchunks = ceil_div(tl + hmax, kBlock * EPT);
+    const unsigned ngroups = ceil_div(nrows, G);
+    const uint64_t blocks  = (uint64_t)((nchunks + 7) / 8) * 8 * ngroups;
+    if (blocks * kBlock >= (uint64_t{1} << 32))
+        throw std::invalid_argument("FDMT merge grid too large; reduce the time tile");
+    fdmt_merge_kernel<K, EPT, G><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
+        in, out, prog, static_cast<int>(nrows), static_cast<int>(ngroups), tl);
+}
+
+template <int K>
+void launch_merge(const float* in, float* out, const int64_t* prog, unsigned nrows, int tl,
+                  int hmax, hipStream_t s) {
+    const MergeShape m = merge_shape(K);
+#define FA_MERGE_CASE(E, G)                                                                  \
+    if (m.ept == E && m.rows == G) return launch_merge_shape<K, E, G>(in, out, prog, nrows, tl, hmax, s);
+    FA_MERGE_CASE(4, 1) FA_MERGE_CASE(4, 4) FA_MERGE_CASE(4, 8)
+    FA_MERGE_CASE(8, 1) FA_MERGE_CASE(8, 4) FA_MERGE_CASE(8, 8)
+    FA_MERGE_CASE(16, 1) FA_MERGE_CASE(16, 4)
+#undef FA_MERGE_CASE
+    throw std::invalid_argument("unsupported FDMT merge shape (elements per thread, rows)");
 }
 
 __device__ __forceinline__ float ladder_sum(const float* L, int S, int j, uint32_t W) {
@@ -687,6 +936,20 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     if (kill_mask) ctor_mask_.assign(kill_mask, kill_mask + nch);
     prev_mask_ = ctor_mask_;
 
+    // Tuning: FROST_AMD_FUSE (levels per launch, 1-4), FROST_AMD_MERGE=direct
+    // (gather from global memory instead of LDS staging), FROST_AMD_GATHER_LDS
+    // (LDS floats per staged block).
+    if (const char* fv = std::getenv("FROST_AMD_FUSE")) {
+        const long f = std::atol(fv);
+        if (f >= 1 && f <= 4) max_fuse_ = static_cast<uint32_t>(f);
+    }
+    if (const char* mv = std::getenv("FROST_AMD_MERGE")) staged_merge_ = std::string(mv) != "direct";
+    if (!staged_merge_) max_fuse_ = std::min<uint32_t>(max_fuse_, 3);
+    if (const char* gv = std::getenv("FROST_AMD_GATHER_LDS")) {
+        const long f = std::atol(gv);
+        if (f >= 2048 && f <= 16384) gather_lds_ = static_cast<uint32_t>(f);
+    }
+
     DeviceGuard dg(device_);
     build_programs();
 
@@ -695,8 +958,10 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     x_.resize(nch * xstride_);
     FA_HIP(hipMemset(x_.get(), 0, x_.bytes()));
     uint64_t len0 = 0, len1 = 0;
-    for (uint32_t l = 0; l < plan_.nlevels; ++l)
-        (l & 1 ? len1 : len0) = std::max(l & 1 ? len1 : len0, level_len_[l]);
+    for (uint32_t l = 0; l < plan_.nlevels; ++l) {
+        if (level_buf_[l] == 0) len0 = std::max(len0, level_len_[l]);
+        if (level_buf_[l] == 1) len1 = std::max(len1, level_len_[l]);
+    }
     buf_[0].resize(len0);
     buf_[1].resize(std::max<uint64_t>(len1, 1));
     const uint64_t nw = widths_.size();
@@ -793,24 +1058,82 @@ void Engine::build_programs() {
     FA_HIP(hipMemcpy(init_chan_.get(), chan.data(), chan.size() * 4, hipMemcpyHostToDevice));
     init_rows_.resize(rows.size());
     FA_HIP(hipMemcpy(init_rows_.get(), rows.data(), rows.size() * 8, hipMemcpyHostToDevice));
-    // Merge programs.
-    merge_rows_.clear();
-    merge_rows_.resize(plan_.nlevels);
+    // Stored levels: level 0, every level holding a plane, and enough others
+    // that no launch spans more than max_fuse_ levels; the fewest stored
+    // floats win (each stored level is written and read once per tile).
+    std::vector<bool> must(plan_.nlevels, false);
+    must[0] = must[L] = true;
+    for (const auto& p : plan_.planes) must[p.level] = true;
+    const uint32_t F = std::max<uint32_t>(1, std::min<uint32_t>(max_fuse_, 4));
+    constexpr uint64_t kInf = ~uint64_t{0};
+    std::vector<uint64_t> best(plan_.nlevels, kInf);
+    std::vector<int32_t> prev(plan_.nlevels, -1);
+    best[0] = level_len_[0];
     for (uint32_t l = 1; l <= L; ++l) {
-        const auto& lv = plan_.levels[l];
-        std::vector<MergeRow> mr(lv.size());
-        for (std::size_t r = 0; r < lv.size(); ++r) {
-            const int32_t h  = hist_[l][r];
-            const int32_t lo = lv.chan_or_lo[r], hi = lv.hi[r];
-            mr[r].out  = base_[l][r];
-            mr[r].lo   = base_[l - 1][lo] + (hist_[l - 1][lo] - h);
-            mr[r].hi   = base_[l - 1][hi] + (hist_[l - 1][hi] - h - lv.shift[r]);
-            mr[r].hist = h;
-            mr[r].pad  = 0;
+        for (uint32_t j = l; j-- > 0 && l - j <= F;) {
+            if (best[j] != kInf && best[j] + level_len_[l] < best[l]) {
+                best[l] = best[j] + level_len_[l];
+                prev[l] = static_cast<int32_t>(j);
+            }
+            if (must[j]) break; // a plane level cannot be skipped
         }
-        merge_rows_[l].resize(mr.size());
-        FA_HIP(hipMemcpy(merge_rows_[l].get(), mr.data(), mr.size() * sizeof(MergeRow),
+    }
+    std::vector<uint32_t> stored;
+    for (int32_t l = static_cast<int32_t>(L); l > 0; l = prev[l]) {
+        if (l < 0 || prev[l] < 0) throw std::logic_error("FDMT level fusion has no solution");
+        stored.push_back(static_cast<uint32_t>(l));
+    }
+    stored.push_back(0);
+    std::reverse(stored.begin(), stored.end());
+    level_source_.assign(plan_.nlevels, -2);
+    level_buf_.assign(plan_.nlevels, -1);
+    level_source_[0] = -1;
+    for (std::size_t i = 0; i < stored.size(); ++i) level_buf_[stored[i]] = static_cast<int>(i & 1);
+    launches_.clear();
+    for (std::size_t i = 1; i < stored.size(); ++i) {
+        const uint32_t a = stored[i - 1], b = stored[i];
+        level_source_[b] = static_cast<int32_t>(a);
+        MergeLaunch ml;
+        ml.in_level  = a;
+        ml.out_level = b;
+        ml.radix     = 1 << (b - a);
+        ml.in_buf    = level_buf_[a];
+        ml.out_buf   = level_buf_[b];
+        const auto& lv = plan_.levels[b];
+        ml.nrows       = static_cast<uint32_t>(lv.size());
+        const int K    = ml.radix;
+        std::vector<int64_t> prog(static_cast<std::size_t>(ml.nrows) * (K + 2));
+        std::vector<std::pair<int32_t, int64_t>> leaves;
+        // Descendants of (level, row) at level a in tree order, with the
+        // summed shift of the upper branches taken on the way down.
+        auto expand = [&](auto&& self, uint32_t l, int32_t r, int64_t sh) -> void {
+            if (l == a) {
+                leaves.emplace_back(r, sh);
+                return;
+            }
+            const auto& v = plan_.levels[l];
+            self(self, l - 1, v.chan_or_lo[r], sh);
+            self(self, l - 1, v.hi[r], sh + v.shift[r]);
+        };
+        for (uint32_t r = 0; r < ml.nrows; ++r) {
+            const int32_t h = hist_[b][r];
+            ml.hmax         = std::max(ml.hmax, h);
+            leaves.clear();
+            expand(expand, b, static_cast<int32_t>(r), 0);
+            int64_t* row = prog.data() + static_cast<std::size_t>(r) * (K + 2);
+            row[0]       = base_[b][r];
+            for (int k = 0; k < K; ++k) {
+                const auto [c, sh] = leaves[k];
+                if (hist_[a][c] < h + sh) throw std::logic_error("FDMT look-back too short");
+                row[1 + k] = base_[a][c] + (hist_[a][c] - h - sh);
+            }
+            row[K + 1] = h;
+        }
+        ml.prog.resize(prog.size());
+        FA_HIP(hipMemcpy(ml.prog.get(), prog.data(), prog.size() * sizeof(int64_t),
                          hipMemcpyHostToDevice));
+        build_gather(ml, prog);
+        launches_.push_back(std::move(ml));
     }
     // Search programs per level.
     search_rows_.clear();
@@ -836,6 +1159,144 @@ void Engine::build_programs() {
         FA_HIP(hipMemcpy(search_rows_[l].get(), sr.data(), sr.size() * sizeof(SearchRow),
                          hipMemcpyHostToDevice));
     }
+}
+
+// Splits a launch's output rows into runs whose distinct input rows (with
+// the spread of shifts their consumers apply) fit the LDS budget, and emits
+// the staged program. prog holds the direct program [out, src_0..K-1, h].
+void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) const {
+    const int K        = ml.radix;
+    const uint32_t a   = ml.in_level;
+    const auto& base_a = base_[a];
+    const auto& hist_a = hist_[a];
+    // Input row and shift of each source: src = base_a[c] + h_a[c] - h - sh.
+    // Recover (c, sh) by matching base offsets (rows' regions are disjoint).
+    std::vector<std::pair<int64_t, int32_t>> starts; // (base, row)
+    starts.reserve(base_a.size());
+    for (std::size_t c = 0; c < base_a.size(); ++c)
+        starts.emplace_back(base_a[c], static_cast<int32_t>(c));
+    std::sort(starts.begin(), starts.end());
+    auto row_of = [&](int64_t off) {
+        auto it = std::upper_bound(starts.begin(), starts.end(),
+                                   std::make_pair(off, std::numeric_limits<int32_t>::max()));
+        return (--it)->second;
+    };
+    struct Lf {
+        int32_t c;
+        int32_t shmin, shmax;
+    };
+    std::vector<GatherBlock> gb;
+    std::vector<GatherLeaf> gl;
+    const std::size_t RS = 4 + static_cast<std::size_t>(K);
+    std::vector<int32_t> rec(static_cast<std::size_t>(ml.nrows) * RS, 0);
+    std::vector<Lf> cur, tent;
+    // Input rows per block: the staging keeps each wave's rows in registers.
+    const std::size_t max_leaves = gather_lds_ <= 4096 ? 16 : gather_lds_ <= 8192 ? 32
+                                 : gather_lds_ <= 12288 ? 48 : 64;
+    if (2 * static_cast<uint64_t>(gather_lds_) * sizeof(float) > 160 * 1024)
+        throw std::invalid_argument("FROST_AMD_GATHER_LDS too large");
+    ml.max_leaves = static_cast<uint32_t>(max_leaves);
+    std::vector<std::pair<int32_t, int32_t>> rl(static_cast<std::size_t>(ml.nrows) * K);
+    uint32_t first = 0, lds_max = 0;
+    int32_t hmax = 0;
+    auto need = [&](const std::vector<Lf>& v, bool& ok) {
+        uint64_t n = 0;
+        ok         = true;
+        for (const auto& x : v) {
+            n += kChunkG + (x.shmax - x.shmin);
+            ok = ok && (x.shmax - x.shmin) <= kSpreadG;
+        }
+        return n;
+    };
+    auto close = [&](uint32_t end) {
+        GatherBlock b{};
+        b.leaf_first = static_cast<int32_t>(gl.size());
+        b.nleaves    = static_cast<int32_t>(cur.size());
+        b.row_first  = static_cast<int32_t>(first);
+        b.nrows      = static_cast<int32_t>(end - first);
+        b.hmax       = hmax;
+        std::vector<int32_t> slot(cur.size());
+        int32_t off = 0;
+        for (std::size_t i = 0; i < cur.size(); ++i) {
+            const auto& x = cur[i];
+            GatherLeaf f{};
+            f.src   = base_a[x.c] + hist_a[x.c] - x.shmax;
+            f.slot  = off;
+            f.len   = kChunkG + (x.shmax - x.shmin);
+            f.shmax = x.shmax;
+            f.hc    = hist_a[x.c];
+            slot[i] = off;
+            off += f.len;
+            gl.push_back(f);
+        }
+        lds_max = std::max<uint32_t>(lds_max, static_cast<uint32_t>(off));
+        for (uint32_t r = first; r < end; ++r)
+            for (int k = 0; k < K; ++k) {
+                const auto [c, sh] = rl[static_cast<std::size_t>(r) * K + k];
+                std::size_t i      = 0;
+                while (cur[i].c != c) ++i;
+                rec[r * RS + 4 + k] = slot[i] + (cur[i].shmax - sh);
+            }
+        gb.push_back(b);
+    };
+    for (uint32_t r = 0; r < ml.nrows; ++r) {
+        const int64_t* row = prog.data() + static_cast<std::size_t>(r) * (K + 2);
+        const int32_t h    = static_cast<int32_t>(row[K + 1]);
+        const int64_t o    = row[0] + h;
+        rec[r * RS + 0]    = static_cast<int32_t>(static_cast<uint32_t>(o & 0xffffffff));
+        rec[r * RS + 1]    = static_cast<int32_t>(o >> 32);
+        rec[r * RS + 2]    = h;
+        tent               = cur;
+        for (int k = 0; k < K; ++k) {
+            const int32_t c  = row_of(row[1 + k]);
+            const int64_t sh = base_a[c] + hist_a[c] - h - row[1 + k];
+            if (sh < 0 || sh > std::numeric_limits<int32_t>::max() / 2)
+                throw std::logic_error("FDMT gather shift out of range");
+            rl[static_cast<std::size_t>(r) * K + k] = {c, static_cast<int32_t>(sh)};
+            bool found = false;
+            for (auto& x : tent)
+                if (x.c == c) {
+                    x.shmin = std::min(x.shmin, static_cast<int32_t>(sh));
+                    x.shmax = std::max(x.shmax, static_cast<int32_t>(sh));
+                    found   = true;
+                }
+            if (!found) tent.push_back({c, static_cast<int32_t>(sh), static_cast<int32_t>(sh)});
+        }
+        bool ok          = true;
+        const uint64_t n = need(tent, ok);
+        if (r > first && (!ok || n > gather_lds_ || tent.size() > max_leaves ||
+                          r - first >= 128)) {
+            close(r);
+            first = r;
+            hmax  = 0;
+            cur.clear();
+            for (int k = 0; k < K; ++k) {
+                const auto [c, sh] = rl[static_cast<std::size_t>(r) * K + k];
+                bool found         = false;
+                for (auto& x : cur)
+                    if (x.c == c) {
+                        x.shmin = std::min(x.shmin, sh);
+                        x.shmax = std::max(x.shmax, sh);
+                        found   = true;
+                    }
+                if (!found) cur.push_back({c, sh, sh});
+            }
+        } else {
+            cur.swap(tent);
+        }
+        hmax = std::max(hmax, h);
+    }
+    if (ml.nrows > first) close(ml.nrows);
+    ml.nblocks    = static_cast<uint32_t>(gb.size());
+    ml.lds_floats = lds_max;
+    ml.blocks.resize(gb.size());
+    ml.leaves.resize(gl.size());
+    ml.rows.resize(rec.size());
+    FA_HIP(hipMemcpy(ml.blocks.get(), gb.data(), gb.size() * sizeof(GatherBlock),
+                     hipMemcpyHostToDevice));
+    FA_HIP(hipMemcpy(ml.leaves.get(), gl.data(), gl.size() * sizeof(GatherLeaf),
+                     hipMemcpyHostToDevice));
+    FA_HIP(hipMemcpy(ml.rows.get(), rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
 }
 
 void Engine::compute_inv(const std::vector<uint8_t>& mask, std::vector<float>& out) const {
@@ -959,7 +1420,8 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
     for (uint64_t t0 = 0; t0 < N; t0 += tile_) {
         const int tl = static_cast<int>(std::min<uint64_t>(tile_, N - t0));
         for (uint32_t l = 0; l <= L; ++l) {
-            float* out = buf_[l & 1].get();
+            if (level_source_[l] == -2) continue; // computed inside a fused launch
+            float* out = buf_[level_buf_[l]].get();
             record_pair(static_cast<int>(l), s, true);
             if (l == 0) {
                 int hcmax = 0;
@@ -971,15 +1433,27 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                     x_.get(), xstride_, static_cast<int64_t>(hist_in_ + t0), init_chan_.get(),
                     init_rows_.get(), out, tl, static_cast<int>(nch));
             } else {
-                int hmax = 0;
-                for (const int32_t h : hist_[l]) hmax = std::max(hmax, h);
-                unsigned gx = ceil_div(tl + hmax, kBlock * kMergeEpt);
-                gx          = (gx + 7) & ~7U; // same chunk of adjacent rows -> same XCD
-                const unsigned nrows = static_cast<unsigned>(plan_.levels[l].size());
-                const unsigned gy    = std::min<unsigned>(nrows, 65535);
-                fdmt_merge_kernel<<<dim3(gx, gy), kBlock, 0, s>>>(
-                    buf_[(l - 1) & 1].get(), out, merge_rows_[l].get(), static_cast<int>(nrows),
-                    tl);
+                const MergeLaunch* ml = nullptr;
+                for (const auto& m : launches_)
+                    if (m.out_level == l) ml = &m;
+                if (!ml) throw std::logic_error("FDMT level without a launch");
+                const float* in = buf_[ml->in_buf].get();
+                if (staged_merge_) {
+                    switch (ml->radix) {
+                    case 2: launch_gather<2>(*ml, in, out, tl, s); break;
+                    case 4: launch_gather<4>(*ml, in, out, tl, s); break;
+                    case 8: launch_gather<8>(*ml, in, out, tl, s); break;
+                    case 16: launch_gather<16>(*ml, in, out, tl, s); break;
+                    default: throw std::logic_error("FDMT launch radix must be 2 to 16");
+                    }
+                } else {
+                    switch (ml->radix) {
+                    case 2: launch_merge<2>(in, out, ml->prog.get(), ml->nrows, tl, ml->hmax, s); break;
+                    case 4: launch_merge<4>(in, out, ml->prog.get(), ml->nrows, tl, ml->hmax, s); break;
+                    case 8: launch_merge<8>(in, out, ml->prog.get(), ml->nrows, tl, ml->hmax, s); break;
+                    default: throw std::logic_error("direct FDMT merge supports radix 2 to 8");
+                    }
+                }
             }
             record_pair(static_cast<int>(l), s, false);
             if (search_count_[l]) {

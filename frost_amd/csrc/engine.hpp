@@ -48,12 +48,42 @@ struct OpenRun {
     uint32_t open;
 };
 
-struct MergeRow {
-    int64_t out;  // output offset (sample t0 - h)
-    int64_t lo;   // lower-half input offset aligned to the same sample
-    int64_t hi;   // upper-half input offset aligned to the same sample
-    int32_t hist; // h of this row
-    int32_t pad;
+struct GatherBlock {
+    int32_t leaf_first, nleaves;
+    int32_t row_first, nrows;
+    int32_t hmax, pad;
+};
+
+// An input row staged for one row block: LDS [slot, slot + len) holds the
+// row's samples t0 + tau0 - shmax + j, j < len (zero outside [t0 - hc, t0 + T)).
+struct GatherLeaf {
+    int64_t src; // offset of sample t0 - shmax in the input level buffer
+    int32_t slot, len, shmax, hc;
+};
+
+// One launch computes levels in_level+1 .. out_level (1 to 3 tree levels):
+// each output row is the sum of its 2^depth descendants at in_level, added
+// in tree order ((a + b) + (c + d)) so results equal level-by-level merging.
+// Program row: [out, src_0 .. src_{K-1}, h] as int64, offsets of sample
+// t0 - h in the output and input level buffers.
+struct MergeLaunch {
+    uint32_t in_level  = 0;
+    uint32_t out_level = 0;
+    int radix          = 2;
+    int hmax           = 0;
+    uint32_t nrows     = 0;
+    int in_buf = 0, out_buf = 0;
+    DeviceBuffer<int64_t> prog;
+    // Staged form: output rows in runs ("row blocks"); each run's distinct
+    // input rows are staged once per time chunk in LDS.
+    uint32_t nblocks    = 0;
+    uint32_t lds_floats = 0;
+    uint32_t max_leaves = 0; // input rows per block bound (16, 32, 48 or 64)
+    DeviceBuffer<GatherBlock> blocks;
+    DeviceBuffer<GatherLeaf> leaves;
+    // Per output row: [out offset of sample t0 (lo, hi), h, 0, K LDS offsets
+    // of the descendants at chunk sample 0].
+    DeviceBuffer<int32_t> rows;
 };
 
 struct SearchRow {
@@ -102,6 +132,7 @@ public:
     std::vector<double> level_seconds, level_launches;
     double search_seconds = 0, total_seconds = 0;
     uint64_t tile() const { return tile_; }
+    const std::vector<int32_t>& level_source() const { return level_source_; }
     uint64_t hist_in() const { return hist_in_; }
 
 private:
@@ -122,7 +153,13 @@ private:
     std::vector<std::vector<int32_t>> hist_;   // h per row
     std::vector<std::vector<int64_t>> base_;   // buffer offset per row
     std::vector<uint64_t> level_len_;          // floats per level buffer
-    std::vector<DeviceBuffer<MergeRow>> merge_rows_;
+    std::vector<MergeLaunch> launches_;
+    std::vector<int32_t> level_source_; // level read to make l; -1 input; -2 not stored
+    std::vector<int> level_buf_;        // buffer holding a stored level
+    uint32_t max_fuse_   = 4;
+    bool staged_merge_   = true;
+    uint32_t gather_lds_ = 8192; // LDS floats per staged row block (one half)
+    void build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) const;
     // Level 0 program: per channel first row, row count, hc, Dmax; per row D,
     // out base, h.
     DeviceBuffer<int32_t> init_chan_;  // [nch][4]

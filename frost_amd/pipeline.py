@@ -9,6 +9,7 @@ import numpy as np
 from . import abi
 from .abi import check, lib
 from .prepare import RFICleaner
+from . import search
 from .search import DeviceArray, FDMTBoxcar, Stream
 
 
@@ -74,6 +75,9 @@ class Beam:
         check(lib().fa_engine_timings(self.engine_handle(), C.byref(a), C.byref(b), C.byref(c),
                                       C.byref(d)))
         return dict(init=a.value, merge=b.value, search=c.value, total=d.value)
+
+    def level_sources(self):
+        return search.engine_level_sources(self.engine_handle())
 
     def level_times(self):
         e = self.engine_handle()

@@ -135,6 +135,17 @@ class DeviceArray:
             self.ptr = None
 
 
+def engine_level_sources(handle) -> np.ndarray:
+    """Per FDMT level, the level its launch reads (-1 the input, -2 summed
+    inside a later launch and not stored)."""
+    nl = C.c_uint64()
+    check(lib().fa_engine_level_sources(handle, None, 0, C.byref(nl)))
+    src = np.zeros(nl.value, np.int32)
+    check(lib().fa_engine_level_sources(handle, src.ctypes.data_as(abi.PI32), nl.value,
+                                        C.byref(nl)))
+    return src
+
+
 class FDMTBoxcar:
     def __init__(self, plan: FDMTPlan, cfg: abi.BoxcarConfig, kill_mask=None,
                  device: int = 0):
@@ -229,6 +240,9 @@ class FDMTBoxcar:
                                           cnt.ctypes.data_as(abi.PU32), nl.value,
                                           C.byref(tile), C.byref(nl)))
         return sec, cnt, tile.value
+
+    def level_sources(self) -> np.ndarray:
+        return engine_level_sources(self.h)
 
     def memory_bytes(self) -> int:
         b = C.c_uint64()

@@ -330,6 +330,13 @@ int fa_engine_level_times(const fa_engine* engine, double* seconds,
                           uint32_t* launches, uint64_t cap, uint64_t* tile,
                           uint64_t* nlevels);
 
+/* Per FDMT level: the level its launch reads (-1: the input waterfall, for
+ * level 0), or -2 when the level is summed inside a later launch and never
+ * stored. A launch reading level a and writing level b sums 2^(b-a) rows of
+ * level a per output row. No reference counterpart (diagnostics). */
+int fa_engine_level_sources(const fa_engine* engine, int32_t* source, uint64_t cap,
+                            uint64_t* nlevels);
+
 /* ---- device RFI (RFICleaner::execute_normalized / DeviceRfi) ------------- */
 
 /* RFICleaner(cfg, layout, baseline_length) (include/frost/prepare/rfi.hpp:260)

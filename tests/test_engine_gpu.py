@@ -23,19 +23,27 @@ def stream():
 
 
 def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None,
-             search_path=None):
+             search_path=None, fuse=None, merge=None):
     """Runs the GPU engine and the oracle on the same gulps; returns both
-    result lists (per gulp + drain) and, if keep, the plane rows."""
-    if tile:
-        os.environ["FROST_AMD_TILE"] = str(tile)
-    if search_path:
-        os.environ["FROST_AMD_SEARCH"] = search_path
+    result lists (per gulp + drain) and, if keep, the plane rows. `fuse` caps
+    the FDMT levels summed per launch (default 4); merge="direct" gathers
+    from global memory instead of staging through LDS."""
+    env = {"FROST_AMD_TILE": tile, "FROST_AMD_SEARCH": search_path, "FROST_AMD_FUSE": fuse,
+           "FROST_AMD_MERGE": merge}
+    for k, v in env.items():
+        if v:
+            os.environ[k] = str(v)
     try:
         plan = search.make_fdmt_plan(dcfg)
         gpu = search.FDMTBoxcar(plan, bcfg)
     finally:
-        os.environ.pop("FROST_AMD_TILE", None)
-        os.environ.pop("FROST_AMD_SEARCH", None)
+        for k in env:
+            os.environ.pop(k, None)
+    src = gpu.level_sources()
+    stored = [l for l in range(len(src)) if src[l] != -2]
+    gaps = [b - a for a, b in zip(stored, stored[1:])]
+    say(f"  stored levels {stored}, levels per launch {gaps}")
+    assert max(gaps) <= int(fuse or 4)
     if keep:
         gpu.keep_planes(True)
     ora = oracle.Engine(dcfg, bcfg, keep_planes=keep)
@@ -101,9 +109,12 @@ def test_engine_single_gulp_small(oracle, stream):
     assert len(o[0][0]) > 0
 
 
-@pytest.mark.parametrize("depth,path", [(0, None), (1, None), (2, None), (2, "lds")])
-def test_engine_gulps_and_planes(oracle, stream, depth, path):
-    say(f"test_engine_gulps_and_planes depth={depth} search={path}")
+@pytest.mark.parametrize("depth,path,fuse,merge", [
+    (0, None, None, None), (1, None, None, None), (2, None, None, None), (2, "lds", None, None),
+    (2, None, 1, None), (2, None, 2, None), (2, None, 3, None), (2, None, 3, "direct"),
+    (1, None, 1, "direct")])
+def test_engine_gulps_and_planes(oracle, stream, depth, path, fuse, merge):
+    say(f"test_engine_gulps_and_planes depth={depth} search={path} fuse={fuse} merge={merge}")
     dcfg = abi.dedisp_config(64, 256e-6, 1500.0, -300 / 64, 500.0, block_nsamps=4096,
                              subband_depth=depth)
     bcfg = abi.boxcar_config(max_boxcar=64, detect_thresh=5.0)
@@ -111,7 +122,7 @@ def test_engine_gulps_and_planes(oracle, stream, depth, path):
     gulps = [pulse_waterfall(rng, 64, 4096, [(100.0 * (k + 1), 1000 + 500 * k, 8, 1.5),
                                              (320.0, 3900, 3, 3.0)], dcfg)
              for k in range(3)]
-    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, search_path=path)
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, search_path=path, fuse=fuse, merge=merge)
     assert_same(g, o)
 
 
