@@ -98,12 +98,27 @@ struct Geometry {
     }
 };
 
+// Sample (t, c) of a row pointer; DT == FA_UINT8 skips the dtype switch.
+template <int DT>
+__device__ __forceinline__ float xat(const uint8_t* row, uint64_t c, int dtype) {
+    if constexpr (DT == FA_UINT8) {
+        return (float)row[c];
+    } else {
+        return unpack(row, c, dtype);
+    }
+}
+
+// Rows are read kRowBatch at a time, all loads issued before the sequential
+// arithmetic that consumes them (which keeps the CPU cleaner's order).
+constexpr int kRowBatch = 16;
+
 // ---- bandpass ------------------------------------------------------------------
 
 // One lane per (window, channel): remedian of the 5^k sampled rows, remedian of
 // |x - median| (x 1.4826), then two 4.5-sigma clipped FP64 refinements
 // (lib/cpu/rfi_cpu_measure.cpp:19-50, lib/cpu/spectra_cpu.cpp:44-130,
 // lib/cpu/bandpass_kernels.hpp:17-76).
+template <int DT>
 __global__ void __launch_bounds__(64)
 bandpass_kernel(Geometry g, float* __restrict__ loc_out, float* __restrict__ scale_out,
                 float* __restrict__ inv_out, uint8_t* __restrict__ measured) {
@@ -112,20 +127,48 @@ bandpass_kernel(Geometry g, float* __restrict__ loc_out, float* __restrict__ sca
     const uint64_t w = blockIdx.x;
     const uint64_t c = (uint64_t)blockIdx.y * 64 + threadIdx.x;
     const int lane   = threadIdx.x;
-    if (c >= g.nch) return;
     const uint64_t b   = g.window_begin(w);
     const uint64_t len = g.window_end(w) - b;
     uint64_t m         = 1;
     const uint64_t lim = len < kMaxBandpassRows ? len : kMaxBandpassRows;
     while (m <= lim / 5) m *= 5;
-    auto sample = [&](uint64_t i) { return g.x(b + (i * len) / m, c); };
+    if (c >= g.nch) return;
+    const uint8_t* wrow = g.block + b * g.row_bytes;
+    auto sample = [&](uint64_t i) {
+        return xat<DT>(wrow + ((i * len) / m) * g.row_bytes, c, g.dtype);
+    };
+    // f(x) over the sampled rows i = 0 .. m-1 in order (row (i * len) / m,
+    // stepped as quotient and remainder), loads issued kRowBatch ahead.
+    const uint64_t dq = len / m, dr = len % m;
+    auto each = [&](auto&& f) {
+        uint64_t qo = 0, ro = 0;
+        for (uint64_t i0 = 0; i0 < m; i0 += kRowBatch) {
+            const int nq = (int)min((uint64_t)kRowBatch, m - i0);
+            float xs[kRowBatch];
+#pragma unroll
+            for (int q = 0; q < kRowBatch; ++q) {
+                const uint64_t row = q < nq ? qo : 0; // in-window address for every q
+                xs[q]              = xat<DT>(wrow + row * g.row_bytes, c, g.dtype);
+                qo += dq;
+                ro += dr;
+                if (ro >= m) {
+                    ++qo;
+                    ro -= m;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kRowBatch; ++q) {
+                if (q >= nq) break;
+                f(xs[q]);
+            }
+        }
+    };
     // Streaming remedian: level buffers of five per lane.
     auto remedian = [&](bool absdiff, float centre, float& lo, float& hi) {
         for (int k = 0; k < 6; ++k) cnt[k][lane] = 0;
         float result = 0.0f;
         lo = hi = sample(0);
-        for (uint64_t i = 0; i < m; ++i) {
-            float v = sample(i);
+        each([&](float v) {
             if (!absdiff) {
                 lo = fmin_(lo, v);
                 hi = fmax_(hi, v);
@@ -155,7 +198,7 @@ bandpass_kernel(Geometry g, float* __restrict__ loc_out, float* __restrict__ sca
                 }
                 ++lev;
             }
-        }
+        });
         return result;
     };
     float lo, hi, dummy0, dummy1;
@@ -175,32 +218,32 @@ bandpass_kernel(Geometry g, float* __restrict__ loc_out, float* __restrict__ sca
     double s      = scale;
     if (!(s > 0.0)) {
         double mean = 0.0;
-        for (uint64_t i = 0; i < m; ++i) mean += sample(i);
+        each([&](float x) { mean += x; });
         mean /= (double)m;
         double ss = 0.0;
-        for (uint64_t i = 0; i < m; ++i) {
-            const double d = sample(i) - mean;
+        each([&](float x) {
+            const double d = x - mean;
             ss += d * d;
-        }
+        });
         s = sqrt(ss / (double)m);
     }
     for (int it = 0; it < 2; ++it) {
         const double cut = 4.5 * s;
         double count = 0.0, sum = 0.0;
-        for (uint64_t i = 0; i < m; ++i) {
-            const double d = sample(i) - centre;
+        each([&](float x) {
+            const double d = x - centre;
             if (fabs(d) <= cut) {
                 count += 1.0;
                 sum += d;
             }
-        }
+        });
         if (count < 2.0) break;
         const double shift = sum / count;
         double ss          = 0.0;
-        for (uint64_t i = 0; i < m; ++i) {
-            const double d = sample(i) - centre;
+        each([&](float x) {
+            const double d = x - centre;
             if (fabs(d) <= cut) ss += (d - shift) * (d - shift);
-        }
+        });
         const double clipped = sqrt(ss / count);
         if (!(clipped > 0.0)) break;
         centre += shift;
@@ -218,6 +261,7 @@ bandpass_kernel(Geometry g, float* __restrict__ loc_out, float* __restrict__ sca
 
 // One lane per (window, channel): FP64 raw and clipped moments of
 // z = (x - loc) * inv in time order (lib/cpu/rfi_cpu_measure.cpp:91-138).
+template <int DT>
 __global__ void __launch_bounds__(64)
 moments_kernel(Geometry g, const float* __restrict__ loc, const float* __restrict__ inv,
                const uint8_t* __restrict__ static_mask, float clip,
@@ -230,21 +274,32 @@ moments_kernel(Geometry g, const float* __restrict__ loc, const float* __restric
     if (static_mask[c] != 0 && ic != 0.0f) {
         const uint64_t b = g.window_begin(w), e = g.window_end(w);
         double prev      = 0.0;
-        for (uint64_t t = b; t < e; ++t) {
-            const float zf  = (g.x(t, c) - lc) * ic;
-            const double z  = zf;
-            const double z2 = z * z;
-            s.n += 1.0;
-            s.s1 += z;
-            s.s2 += z2;
-            s.s3 += z2 * z;
-            s.s4 += z2 * z2;
-            if (t > b) s.lag += z * prev;
-            prev = z;
-            if (fabs(z) <= (double)clip) {
-                s.nc += 1.0;
-                s.c1 += z;
-                s.c2 += z2;
+        const uint8_t* p = g.block + b * g.row_bytes;
+        for (uint64_t t0 = b; t0 < e; t0 += kRowBatch) {
+            const int nq = (int)min((uint64_t)kRowBatch, e - t0);
+            float xs[kRowBatch];
+#pragma unroll
+            for (int q = 0; q < kRowBatch; ++q)
+                xs[q] = xat<DT>(p + (uint64_t)min(q, nq - 1) * g.row_bytes, c, g.dtype);
+            p += kRowBatch * g.row_bytes;
+#pragma unroll
+            for (int q = 0; q < kRowBatch; ++q) {
+                if (q >= nq) break;
+                const float zf  = (xs[q] - lc) * ic;
+                const double z  = zf;
+                const double z2 = z * z;
+                s.n += 1.0;
+                s.s1 += z;
+                s.s2 += z2;
+                s.s3 += z2 * z;
+                s.s4 += z2 * z2;
+                if (t0 + q > b) s.lag += z * prev;
+                prev = z;
+                if (fabs(z) <= (double)clip) {
+                    s.nc += 1.0;
+                    s.c1 += z;
+                    s.c2 += z2;
+                }
             }
         }
     }
@@ -330,6 +385,7 @@ constexpr int kRowTileCh = 256;
 
 // 64 rows per block; channel tiles transposed through LDS so each lane walks
 // its own row in channel order (lib/cpu/rfi_cpu_measure.cpp:365-415).
+template <int DT>
 __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
     extern __shared__ uint32_t tile[]; // [64][words + 1]
     const Geometry& g   = a.g;
@@ -344,6 +400,9 @@ __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
     const uint64_t w    = active ? g.window_of(t) : 0;
     const float* lc     = a.loc + w * g.nch;
     const float* ic     = a.inv + w * g.nch;
+    const uint64_t t_last   = min(t0 + 63, g.nsamps - 1);
+    const uint64_t wu       = g.window_of(t0);
+    const bool w_uniform    = g.window_of(t_last) == wu;
     double sum_det = 0, sum_keep = 0, n_keep = 0, sx = 0, sxx = 0, n_meas = 0;
     KadaneDev sp, sn;
     kadane_init(sp);
@@ -374,25 +433,34 @@ __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
         __syncthreads();
         if (!active) continue;
         const uint8_t* myrow = reinterpret_cast<const uint8_t*>(tile + lane * pitch);
-        for (uint64_t j = 0; j < nc; ++j) {
-            const uint64_t c = c0 + j;
-            if (a.mask[c] == 0) continue;
-            const float x = unpack(myrow, j, g.dtype);
-            if (ic[c] == 0.0f) continue;
-            n_meas += 1.0;
-            const float zc = (x - lc[c]) * ic[c];
-            const float cl = zc < -a.clip ? -a.clip : (a.clip < zc ? a.clip : zc);
-            sum_det += cl;
-            if (fabsf(zc) <= a.clip) {
-                sum_keep += zc;
-                n_keep += 1.0;
+        // Channel walk in order; lcp/icp are the window's calibration rows.
+        auto walk = [&](const float* lcp, const float* icp) {
+            for (uint64_t j = 0; j < nc; ++j) {
+                const uint64_t c = c0 + j;
+                if (a.mask[c] == 0) continue;
+                const float x = xat<DT>(myrow, j, g.dtype);
+                if (icp[c] == 0.0f) continue;
+                n_meas += 1.0;
+                const float zc = (x - lcp[c]) * icp[c];
+                const float cl = zc < -a.clip ? -a.clip : (a.clip < zc ? a.clip : zc);
+                sum_det += cl;
+                if (fabsf(zc) <= a.clip) {
+                    sum_keep += zc;
+                    n_keep += 1.0;
+                }
+                sx += x;
+                sxx += (double)x * x;
+                if (a.kspec) {
+                    kadane_append(sp, (int64_t)c, cl);
+                    kadane_append(sn, (int64_t)c, -cl);
+                }
             }
-            sx += x;
-            sxx += (double)x * x;
-            if (a.kspec) {
-                kadane_append(sp, (int64_t)c, cl);
-                kadane_append(sn, (int64_t)c, -cl);
-            }
+        };
+        if (w_uniform) {
+            // All 64 rows in one window: block-uniform calibration rows.
+            walk(a.loc + wu * g.nch, a.inv + wu * g.nch);
+        } else {
+            walk(lc, ic);
         }
     }
     if (!active) return;
@@ -444,6 +512,7 @@ struct ColArgs {
 
 // One lane per (chunk, channel), rows in order: min/max of unmeasured cells
 // and the Kadane-along-time states (lib/cpu/rfi_cpu_measure.cpp:356-444).
+template <int DT>
 __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
     const Geometry& g = a.g;
     const uint64_t k  = blockIdx.x;
@@ -458,24 +527,52 @@ __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
     if (a.mask[c] != 0) {
         const uint64_t t0 = k * a.chunk;
         const uint64_t t1 = t0 + a.chunk < g.nsamps ? t0 + a.chunk : g.nsamps;
-        for (uint64_t t = t0; t < t1; ++t) {
-            const uint64_t w = g.window_of(t);
-            const float x    = g.x(t, c);
-            const float ic   = a.inv[w * g.nch + c];
-            float z          = 0.0f;
-            if (ic == 0.0f) {
-                lo = fmin_(lo, x);
-                hi = fmax_(hi, x);
-            } else {
-                z = (x - a.loc[w * g.nch + c]) * ic;
-            }
+        // Window of the current row, advanced at window ends (the last
+        // window runs to nsamps).
+        uint64_t w    = g.window_of(t0);
+        uint64_t wend = g.window_end(w);
+        float ic = 0.0f, lc = 0.0f, cm = 0.0f, ci = 0.0f;
+        auto load_w = [&]() {
+            const uint64_t i = w * g.nch + c;
+            ic               = a.inv[i];
+            lc               = a.loc[i];
             if (a.kchan) {
-                const bool outlier = fabsf(z) > a.clip;
-                const double u =
-                    outlier ? 0.0
-                            : (double)((z - a.clip_mean[w * g.nch + c]) * a.clip_inv[w * g.nch + c]);
-                kadane_append(kp, (int64_t)t, u);
-                kadane_append(kn, (int64_t)t, -u);
+                cm = a.clip_mean[i];
+                ci = a.clip_inv[i];
+            }
+        };
+        load_w();
+        const uint8_t* p = g.block + t0 * g.row_bytes;
+        for (uint64_t tb = t0; tb < t1; tb += kRowBatch) {
+            const int nq = (int)min((uint64_t)kRowBatch, t1 - tb);
+            float xs[kRowBatch];
+#pragma unroll
+            for (int q = 0; q < kRowBatch; ++q)
+                xs[q] = xat<DT>(p + (uint64_t)min(q, nq - 1) * g.row_bytes, c, g.dtype);
+            p += kRowBatch * g.row_bytes;
+#pragma unroll
+            for (int q = 0; q < kRowBatch; ++q) {
+                if (q >= nq) break;
+                const uint64_t t = tb + q;
+                if (t >= wend) {
+                    ++w;
+                    wend = g.window_end(w);
+                    load_w();
+                }
+                const float x = xs[q];
+                float z       = 0.0f;
+                if (ic == 0.0f) {
+                    lo = fmin_(lo, x);
+                    hi = fmax_(hi, x);
+                } else {
+                    z = (x - lc) * ic;
+                }
+                if (a.kchan) {
+                    const bool outlier = fabsf(z) > a.clip;
+                    const double u     = outlier ? 0.0 : (double)((z - cm) * ci);
+                    kadane_append(kp, (int64_t)t, u);
+                    kadane_append(kn, (int64_t)t, -u);
+                }
             }
         }
     }
@@ -616,77 +713,190 @@ __device__ float cleaned_unflagged(const ApplyArgs& a, uint64_t t, uint64_t c, i
     return a.loc[i] + (a.scale[i] * value);
 }
 
-// 64 x 64 tile of (time, channel): computes each cell's cleaned value, then
-// writes (cleaned - loc) / scale channel-major through LDS
-// (lib/cpu/rfi_cpu.cpp:245-257).
-__global__ void __launch_bounds__(256) apply_kernel(ApplyArgs a) {
+// Cells that need a fill (flagged rows, narrow-band / Kadane cells): the
+// generic path, identical to the CPU cleaner's order of draws.
+__device__ float apply_slow(const ApplyArgs& a, uint64_t t, uint64_t c,
+                                         unsigned long long (&local)[4]) {
+    const uint64_t w = a.g.window_of(t);
+    const uint64_t i = w * a.g.nch + c;
+    float cleaned;
+    if (a.flags[t] == 0) {
+        int st  = kClean;
+        bool fb = false;
+        cleaned = cleaned_unflagged(a, t, c, &st, &fb);
+        if (st == kNarrow) ++local[0];
+        else if (st == kKSpec) ++local[1];
+        else if (st == kKChan) ++local[2];
+        if (fb) ++local[3];
+    } else if (a.fill != FA_FILL_NEIGHBOUR) {
+        const float vv = a.fill == FA_FILL_NOISE
+                             ? normal_draw(hash5(a.seed, a.block_index,
+                                                 (kStreamNoise << 56) ^ t, c, 0))
+                             : 0.0f;
+        cleaned = a.loc[i] + (a.scale[i] * vv);
+    } else {
+        const uint64_t lo = t > a.time_r ? t - a.time_r : 0;
+        const uint64_t hi = t + a.time_r < a.g.nsamps - 1 ? t + a.time_r : a.g.nsamps - 1;
+        bool found        = false;
+        cleaned           = 0.0f;
+        for (int k = 0; k < kMaxDraws && !found; ++k) {
+            const uint64_t h =
+                hash5(a.seed, a.block_index, (kStreamTime << 56) ^ t, c, (uint64_t)k);
+            const uint64_t src = lo + h % (hi - lo + 1);
+            if (a.flags[src] != 0) continue;
+            found   = true;
+            cleaned = cleaned_unflagged(a, src, c, nullptr, nullptr);
+        }
+        if (!found) {
+            ++local[3];
+            cleaned = a.loc[i] + (a.scale[i] * 0.0f);
+        }
+    }
+    const float sc = a.scale[i] > 0.0f ? 1.0f / a.scale[i] : 1.0f;
+    return (cleaned - a.loc[i]) * sc;
+}
+
+// 64 x 64 tile of (time, channel): each cell's cleaned value, written as
+// (cleaned - loc) / scale channel-major through LDS
+// (lib/cpu/rfi_cpu.cpp:245-257). The fast kernel handles tiles whose cells
+// are all clean cells of unflagged rows (or unmeasured), with the channel's
+// and window's values in registers; a tile with any cell that needs a fill
+// is appended to a list and written by apply_slow_kernel instead.
+template <int DT>
+__global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* __restrict__ slow,
+                                                         unsigned int* __restrict__ nslow) {
     __shared__ float tile[64][65];
-    __shared__ unsigned long long cnt[4];
-    const uint64_t t0 = (uint64_t)blockIdx.x * 64;
-    const uint64_t c0 = (uint64_t)blockIdx.y * 64;
-    if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+    __shared__ int32_t rowinfo[64][5]; // [row]: flagged, Kadane-spectrum spans lo0 hi0 lo1 hi1
+    __shared__ int any_slow;
+    const uint64_t t0  = (uint64_t)blockIdx.x * 64;
+    const uint64_t c0  = (uint64_t)blockIdx.y * 64;
+    const int wave     = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int cl       = threadIdx.x % 64;
+    const uint64_t c   = c0 + cl;
+    const uint64_t ns  = a.g.nsamps;
+    if (threadIdx.x < 64) {
+        const uint64_t t = t0 + threadIdx.x;
+        int32_t info[5]  = {1, -1, -1, -1, -1};
+        if (t < ns) {
+            info[0] = a.flags[t] != 0;
+            if (a.kspec)
+                for (int q = 0; q < 4; ++q) info[1 + q] = a.kspec[4 * t + q];
+        }
+        for (int q = 0; q < 5; ++q) rowinfo[threadIdx.x][q] = info[q];
+    }
+    if (threadIdx.x == 0) any_slow = 0;
+    // The tile's rows lie in window wa, or in wa then wb from row tb on.
+    const uint64_t tl  = min(t0 + 63, ns - 1);
+    const uint64_t wa  = a.g.window_of(t0);
+    const uint64_t wb  = a.g.window_of(tl);
+    const uint64_t tb  = a.g.window_begin(wb);
+    const bool good    = c < a.g.nch && a.mask[c] != 0 && wb - wa <= 1;
+    int64_t ks[4]      = {0, -1, 0, -1};
+    float lca = 0.0f, ica = 0.0f, sca = 1.0f, lcb = 0.0f, icb = 0.0f, scb = 1.0f;
+    if (good) {
+        if (a.kseg)
+            for (int q = 0; q < 4; ++q) ks[q] = a.kseg[4 * c + q];
+        const uint64_t ia = wa * a.g.nch + c, ib = wb * a.g.nch + c;
+        lca               = a.loc[ia];
+        ica               = a.inv[ia];
+        sca               = a.scale[ia] > 0.0f ? 1.0f / a.scale[ia] : 1.0f;
+        lcb               = a.loc[ib];
+        icb               = a.inv[ib];
+        scb               = a.scale[ib] > 0.0f ? 1.0f / a.scale[ib] : 1.0f;
+    }
+    // Samples of this thread's rows, loaded up front at clamped in-range
+    // addresses (out-of-range cells are discarded below).
+    const uint64_t cc = c < a.g.nch ? c : a.g.nch - 1;
+    float xs[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint64_t t  = t0 + wave + 4 * k;
+        const uint64_t tc = t < ns ? t : ns - 1;
+        xs[k]             = xat<DT>(a.g.block + tc * a.g.row_bytes, cc, a.g.dtype);
+    }
     __syncthreads();
-    unsigned long long local[4] = {0, 0, 0, 0};
-    // Each thread: channel c0 + (tid % 64), rows t0 + (tid / 64) + 4k.
-    const int cl    = threadIdx.x % 64;
-    const uint64_t c = c0 + cl;
-    for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    bool slow_cell = c < a.g.nch && a.mask[c] != 0 && wb - wa > 1;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int r      = wave + 4 * k;
         const uint64_t t = t0 + r;
         float v          = 0.0f;
-        if (c < a.g.nch && t < a.g.nsamps && a.mask[c] != 0) {
-            const uint64_t w = a.g.window_of(t);
-            const uint64_t i = w * a.g.nch + c;
-            float cleaned;
-            if (a.flags[t] == 0) {
-                int st   = kClean;
-                bool fb  = false;
-                cleaned  = cleaned_unflagged(a, t, c, &st, &fb);
-                if (st == kNarrow) ++local[0];
-                else if (st == kKSpec) ++local[1];
-                else if (st == kKChan) ++local[2];
-                if (fb) ++local[3];
-            } else if (a.fill != FA_FILL_NEIGHBOUR) {
-                const float vv = a.fill == FA_FILL_NOISE
-                                     ? normal_draw(hash5(a.seed, a.block_index,
-                                                         (kStreamNoise << 56) ^ t, c, 0))
-                                     : 0.0f;
-                cleaned = a.loc[i] + (a.scale[i] * vv);
-            } else {
-                const uint64_t lo = t > a.time_r ? t - a.time_r : 0;
-                const uint64_t hi =
-                    t + a.time_r < a.g.nsamps - 1 ? t + a.time_r : a.g.nsamps - 1;
-                bool found = false;
-                cleaned    = 0.0f;
-                for (int k = 0; k < kMaxDraws && !found; ++k) {
-                    const uint64_t h =
-                        hash5(a.seed, a.block_index, (kStreamTime << 56) ^ t, c, (uint64_t)k);
-                    const uint64_t src = lo + h % (hi - lo + 1);
-                    if (a.flags[src] != 0) continue;
-                    found   = true;
-                    cleaned = cleaned_unflagged(a, src, c, nullptr, nullptr);
-                }
-                if (!found) {
-                    ++local[3];
-                    cleaned = a.loc[i] + (a.scale[i] * 0.0f);
-                }
+        if (good && t < ns) {
+            const bool inb = t >= tb;
+            const float lc = inb ? lcb : lca, ic = inb ? icb : ica, sc = inb ? scb : sca;
+            const float x  = xs[k];
+            bool clean     = rowinfo[r][0] == 0;
+            if (clean && ic != 0.0f) {
+                const float z = (x - lc) * ic;
+                if (a.narrow && fabsf(z) > a.clip) clean = false;
+                const int32_t ci = (int32_t)c;
+                if ((rowinfo[r][1] >= 0 && ci >= rowinfo[r][1] && ci <= rowinfo[r][2]) ||
+                    (rowinfo[r][3] >= 0 && ci >= rowinfo[r][3] && ci <= rowinfo[r][4]))
+                    clean = false;
+                const int64_t it = (int64_t)t;
+                if ((it >= ks[0] && it <= ks[1]) || (it >= ks[2] && it <= ks[3])) clean = false;
             }
-            const float sc = a.scale[i] > 0.0f ? 1.0f / a.scale[i] : 1.0f;
-            v              = (cleaned - a.loc[i]) * sc;
+            slow_cell = slow_cell || !clean;
+            v         = (x - lc) * sc;
         }
         tile[cl][r] = v;
     }
-    for (int q = 0; q < 4; ++q)
-        if (local[q]) atomicAdd(&cnt[q], local[q]);
+    if (slow_cell) any_slow = 1;
     __syncthreads();
+    if (any_slow) {
+        if (threadIdx.x == 0) {
+            const unsigned int k = atomicAdd(nslow, 1U);
+            slow[2 * k]          = blockIdx.x;
+            slow[2 * k + 1]      = blockIdx.y;
+        }
+        return;
+    }
     // Write: 64 channels x 64 samples, time contiguous per channel.
     for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-        const int cc     = idx / 64, rr = idx % 64;
-        const uint64_t c2 = c0 + cc, t2 = t0 + rr;
+        const int cc2     = idx / 64, rr = idx % 64;
+        const uint64_t c2 = c0 + cc2, t2 = t0 + rr;
         if (c2 >= a.g.nch || t2 >= a.capacity) continue;
         const uint64_t dst = a.flip ? a.g.nch - 1 - c2 : c2;
-        a.out[dst * a.stride + a.offset + t2] = tile[cc][rr];
+        a.out[dst * a.stride + a.offset + t2] = tile[cc2][rr];
     }
-    if (threadIdx.x < 4 && cnt[threadIdx.x]) atomicAdd(&a.counters[threadIdx.x], cnt[threadIdx.x]);
+}
+
+// Tiles listed by apply_fast_kernel, every cell through the CPU cleaner's
+// fill rules; grid-stride over the list.
+__global__ void __launch_bounds__(256) apply_slow_kernel(ApplyArgs a,
+                                                         const uint32_t* __restrict__ slow,
+                                                         const unsigned int* __restrict__ nslow) {
+    __shared__ float tile[64][65];
+    __shared__ unsigned long long cnt[4];
+    const unsigned int n = *nslow;
+    for (unsigned int k = blockIdx.x; k < n; k += gridDim.x) {
+        const uint64_t t0 = (uint64_t)slow[2 * k] * 64;
+        const uint64_t c0 = (uint64_t)slow[2 * k + 1] * 64;
+        if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        unsigned long long local[4] = {0, 0, 0, 0};
+        const int cl      = threadIdx.x % 64;
+        const uint64_t c  = c0 + cl;
+        for (int r = threadIdx.x / 64; r < 64; r += 4) {
+            const uint64_t t = t0 + r;
+            float v          = 0.0f;
+            if (c < a.g.nch && t < a.g.nsamps && a.mask[c] != 0) v = apply_slow(a, t, c, local);
+            tile[cl][r] = v;
+        }
+        for (int q = 0; q < 4; ++q)
+            if (local[q]) atomicAdd(&cnt[q], local[q]);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+            const int cc      = idx / 64, rr = idx % 64;
+            const uint64_t c2 = c0 + cc, t2 = t0 + rr;
+            if (c2 >= a.g.nch || t2 >= a.capacity) continue;
+            const uint64_t dst = a.flip ? a.g.nch - 1 - c2 : c2;
+            a.out[dst * a.stride + a.offset + t2] = tile[cc][rr];
+        }
+        if (threadIdx.x < 4 && cnt[threadIdx.x])
+            atomicAdd(&a.counters[threadIdx.x], cnt[threadIdx.x]);
+        __syncthreads();
+    }
 }
 
 // Short blocks: out = (x - loc[c]) * (scale[c] > 0 ? 1/scale[c] : 1) for good
@@ -784,6 +994,9 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
                     zap_freqs_[2 * i] <= zap_freqs_[2 * i + 1],
                 "rfi.zap_freqs ranges must be finite with lo <= hi");
     require(layout.nchans > 0 && layout.max_nsamps > 0, "RFI: nchans and max_nsamps must be > 0");
+    // Device index math: samples and (window, channel) cells fit 31 bits.
+    require(layout.max_nsamps < (uint64_t{1} << 31), "RFI: max_nsamps must be below 2^31");
+
     bits_ = dtype_bits(layout.dtype);
     require((layout.nchans * static_cast<uint64_t>(bits_)) % 8 == 0,
             "RFI: each time sample must be a whole number of bytes");
@@ -798,6 +1011,8 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
     // Derived geometry (lib/cpu/rfi_cpu.cpp:24-60).
     const double ws   = static_cast<double>(baseline) / layout.tsamp;
     window_len_       = std::max<uint64_t>(static_cast<uint64_t>(ws), kMinWindow);
+    require(layout.nchans * (layout.max_nsamps / window_len_ + 1) < (uint64_t{1} << 31),
+            "RFI: nchans * bandpass windows must be below 2^31");
     smooth_radius_    = std::max<uint64_t>(static_cast<uint64_t>(ws / 2), 1);
     time_fill_radius_ = std::max<uint64_t>(smooth_radius_ / 4, 1);
     chan_fill_radius_ = std::max<uint64_t>(nch_ / 60, 1);
@@ -848,6 +1063,7 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
     }
     kseg_.resize(4 * nch_);
     counters_.resize(8);
+    slow_tiles_.resize(2 * ((layout.max_nsamps + 63) / 64) * ((layout.nchans + 63) / 64));
     tail_loc_.resize(nch_);
     tail_scale_.resize(nch_);
     bad_index_.resize(1);
@@ -869,7 +1085,7 @@ uint64_t DeviceRfi::memory_bytes() const {
            clip_mean_.bytes() + clip_inv_.bytes() + mask_dev_.bytes() + flags_dev_.bytes() +
            kwhole_.bytes() + good_prefix_.bytes() + kspec_.bytes() + s_det_.bytes() +
            s_keep_.bytes() + row_std_.bytes() + cmin_.bytes() + cmax_.bytes() + kpos_.bytes() +
-           kneg_.bytes() + kseg_.bytes() + counters_.bytes() + tail_loc_.bytes() +
+           kneg_.bytes() + kseg_.bytes() + counters_.bytes() + slow_tiles_.bytes() + tail_loc_.bytes() +
            tail_scale_.bytes();
 }
 
@@ -1086,8 +1302,12 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
     const unsigned cblk = ceil_div(n, 64);
 
     FA_HIP(hipEventRecord(ev_[0], s));
-    bandpass_kernel<<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
-        g, loc_.get(), scale_.get(), inv_.get(), measured_.get());
+    if (layout_.dtype == FA_UINT8)
+        bandpass_kernel<FA_UINT8><<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
+            g, loc_.get(), scale_.get(), inv_.get(), measured_.get());
+    else
+        bandpass_kernel<-1><<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
+            g, loc_.get(), scale_.get(), inv_.get(), measured_.get());
     FA_HIP(hipGetLastError());
     scale_host_.resize(nwin_ * n);
     measured_host_.resize(nwin_ * n);
@@ -1096,8 +1316,12 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
     FA_HIP(hipMemcpyAsync(mask_dev_.get(), static_mask_.data(), n, hipMemcpyHostToDevice, s));
     FA_HIP(hipEventRecord(ev_[1], s));
     if (need_moments_) {
-        moments_kernel<<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
-            g, loc_.get(), inv_.get(), mask_dev_.get(), clip_, moments_.get());
+        if (layout_.dtype == FA_UINT8)
+            moments_kernel<FA_UINT8><<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
+                g, loc_.get(), inv_.get(), mask_dev_.get(), clip_, moments_.get());
+        else
+            moments_kernel<-1><<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
+                g, loc_.get(), inv_.get(), mask_dev_.get(), clip_, moments_.get());
         FA_HIP(hipGetLastError());
     }
     FA_HIP(hipStreamSynchronize(s));
@@ -1130,7 +1354,10 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
         a.kspec_span  = kspec ? kspec_.get() : nullptr;
         a.kwhole      = kspec ? kwhole_.get() : nullptr;
         const size_t lds = 64 * (kRowTileCh * bits_ / 32 + 1) * 4;
-        scan_rows_kernel<<<ceil_div(nsamps, 64), 64, lds, s>>>(a);
+        if (layout_.dtype == FA_UINT8)
+            scan_rows_kernel<FA_UINT8><<<ceil_div(nsamps, 64), 64, lds, s>>>(a);
+        else
+            scan_rows_kernel<-1><<<ceil_div(nsamps, 64), 64, lds, s>>>(a);
         FA_HIP(hipGetLastError());
         ColArgs b{};
         b.g         = g;
@@ -1147,7 +1374,10 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
         b.cmax      = cmax_.get();
         b.kpos      = kchan ? kpos_.get() : nullptr;
         b.kneg      = kchan ? kneg_.get() : nullptr;
-        scan_cols_kernel<<<dim3(static_cast<unsigned>(nchunks), cblk), 64, 0, s>>>(b);
+        if (layout_.dtype == FA_UINT8)
+            scan_cols_kernel<FA_UINT8><<<dim3(static_cast<unsigned>(nchunks), cblk), 64, 0, s>>>(b);
+        else
+            scan_cols_kernel<-1><<<dim3(static_cast<unsigned>(nchunks), cblk), 64, 0, s>>>(b);
         FA_HIP(hipGetLastError());
     }
     FA_HIP(hipEventRecord(ev_[3], s));
@@ -1178,7 +1408,15 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
     ap.stride      = stride;
     ap.offset      = offset;
     ap.counters    = counters_.get();
-    apply_kernel<<<dim3(ceil_div(capacity, 64), cblk), 256, 0, s>>>(ap);
+    // counters_[5] counts the slow tiles listed in slow_tiles_.
+    const dim3 tiles(ceil_div(capacity, 64), cblk);
+    auto* nslow = reinterpret_cast<unsigned int*>(counters_.get() + 5);
+    if (layout_.dtype == FA_UINT8)
+        apply_fast_kernel<FA_UINT8><<<tiles, 256, 0, s>>>(ap, slow_tiles_.get(), nslow);
+    else
+        apply_fast_kernel<-1><<<tiles, 256, 0, s>>>(ap, slow_tiles_.get(), nslow);
+    FA_HIP(hipGetLastError());
+    apply_slow_kernel<<<1024, 256, 0, s>>>(ap, slow_tiles_.get(), nslow);
     FA_HIP(hipGetLastError());
     // Last window's calibration for short tails.
     const uint64_t last = (nwin_ - 1) * n;

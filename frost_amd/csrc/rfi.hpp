@@ -107,6 +107,7 @@ private:
     DeviceBuffer<KadaneDev> kpos_, kneg_;
     DeviceBuffer<int64_t> kseg_; // [nch][4]: pos start/end, neg start/end
     DeviceBuffer<unsigned long long> counters_;
+    DeviceBuffer<uint32_t> slow_tiles_; // (time tile, channel tile) pairs needing fills
     DeviceBuffer<float> tail_loc_, tail_scale_;
     DeviceBuffer<unsigned long long> bad_index_;
 
