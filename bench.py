@@ -9,9 +9,10 @@ the host. The gulp is generated on the device before timing (synthetic,
 seeded) and stays resident in HBM.
 
 Prints one JSON line: DM-trial*samples/s, p50/p99 gulp latency as a fraction
-of the gulp's real-time duration, the FDMT merge kernel against the HBM
+of the gulp's real-time duration, the dominant FDMT gather kernel against the HBM
 roofline (live HIP-event timings), and the CPU oracle timed on a bounded
-sample on this host.
+sample on this host. --copy-probe runs only the 1 GiB device copy the HBM
+traffic counters are calibrated against.
 """
 from __future__ import annotations
 
@@ -35,17 +36,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def merge_traffic_from_profile():
-    """HBM bytes per merge launch measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE (committed summary), if present."""
-    p = ROOT / "profiles" / "fdmt_merge_4096ch_traffic.json"
-    if not p.exists():
+TRAFFIC_SUMMARY = ROOT / "profiles" / "fdmt_gather_r16_4096ch_traffic.json"
+
+
+def gather_traffic_from_profile():
+    """HBM bytes per radix-16 gather launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against --copy-probe), if
+    present."""
+    if not TRAFFIC_SUMMARY.exists():
         return None
     try:
-        d = json.loads(p.read_text())
+        d = json.loads(TRAFFIC_SUMMARY.read_text())
         return float(d["hbm_bytes_per_launch"])
     except Exception:
         return None
+
+
+def copy_probe() -> int:
+    """A plain 1 GiB device-to-device copy, three times (the reference the
+    FETCH_SIZE / WRITE_SIZE counters are scaled by)."""
+    from frost_amd import abi, search
+    n = 1 << 30
+    stream = search.Stream(0)
+    a, b = search.DeviceArray(n), search.DeviceArray(n)
+    for _ in range(3):
+        abi.check(abi.lib().fa_memcpy_d2d(b.ptr, a.ptr, n, stream.h))
+    print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "copies": 3}),
+          flush=True)
+    return 0
 
 
 def cpu_baseline(W, dev_gulp, stream, sample_rows):
@@ -88,7 +106,11 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=8192,
                     help="rows of the gulp timed on the CPU oracle (0 = skip)")
     ap.add_argument("--workload", default="4096ch_64us_gulp1M_dm2000_11planes")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the 1 GiB device copy used to calibrate HBM counters")
     a = ap.parse_args()
+    if a.copy_probe:
+        return copy_probe()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(beams shard across GPUs as independent processes, see DESIGN.md)",
@@ -107,6 +129,13 @@ def main() -> int:
     summ = plan.summary()
     rows = plan.level_rows().astype(np.float64)
     ndm = summ["ndm"]
+    # Dominant kernel: the FDMT gather launches that sum four tree levels
+    # (radix 16): levels src -> src + 4 in one pass.
+    src = beam.level_sources()
+    r16 = [l for l in range(len(src)) if src[l] >= 0 and l - src[l] == 4]
+    if not r16:  # other shapes: the deepest launches present
+        depth = max(l - src[l] for l in range(len(src)) if src[l] >= 0)
+        r16 = [l for l in range(len(src)) if src[l] >= 0 and l - src[l] == depth]
     for i in range(a.warmup):
         beam.execute_device(gulp, W.gulp, stream)
         log(f"warmup {i}: {beam.candidate_count()} candidates")
@@ -124,8 +153,8 @@ def main() -> int:
         nc = beam.candidate_count()
         ncand += nc
         sec, cnt, tile = beam.level_times()
-        merge_s += float(sec[1:].sum())
-        merge_launches += int(cnt[1:].sum())
+        merge_s += float(sec[r16].sum())
+        merge_launches += int(cnt[r16].sum())
         tm = beam.engine_timings()
         rep = beam.rfi_report()
         stage["rfi"] += rep.t_bandpass + rep.t_moments + rep.t_scan + rep.t_flags + rep.t_apply
@@ -135,12 +164,12 @@ def main() -> int:
             f"{rep.chans_masked} ch, flagged {rep.samps_flagged} samples")
     total = float(sum(lat))
     value = ndm * W.gulp * a.steps / total
-    # Dominant kernel: the FDMT merge. Algorithmic bytes of one full gulp pass:
-    # read the input rows and write the output rows of every level.
-    alg_bytes_gulp = 4.0 * W.gulp * float((rows[:-1] + rows[1:]).sum())
+    # Algorithmic bytes of the dominant launches over one gulp: each input
+    # row read once and each output row written once (float32, per sample).
+    alg_bytes_gulp = 4.0 * W.gulp * float(sum(rows[src[l]] + rows[l] for l in r16))
     achieved = alg_bytes_gulp * a.steps / merge_s / 1e9
     launches_per_gulp = merge_launches / a.steps
-    traffic = merge_traffic_from_profile()
+    traffic = gather_traffic_from_profile()
     out = {
         "metric": "DM-trial·samples/s at 4096 ch, 1 GPU; p99 gulp latency / real-time",
         "value": value,
@@ -171,7 +200,8 @@ def main() -> int:
         },
         "stage_seconds_per_gulp": {k: v / a.steps for k, v in stage.items()},
         "roofline": {
-            "kernel": "fdmt_merge_kernel",
+            "kernel": "fdmt_gather_kernel<16> (FDMT levels l-4 -> l in one pass)",
+            "levels": [[int(src[l]), int(l)] for l in r16],
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

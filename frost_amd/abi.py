@@ -293,6 +293,7 @@ EXPORTS = {
     "fa_device_free": (C.c_int, [P]),
     "fa_memcpy_h2d": (C.c_int, [P, P, U64, P]),
     "fa_memcpy_d2h": (C.c_int, [P, P, U64, P]),
+    "fa_memcpy_d2d": (C.c_int, [P, P, U64, P]),
     "fa_generate_dm_list": (C.c_int, [C.POINTER(DedispConfig), PF32, U64, PU64]),
     "fa_plan_create": (C.c_int, [C.POINTER(DedispConfig), C.POINTER(P)]),
     "fa_plan_destroy": (C.c_int, [P]),

@@ -144,6 +144,13 @@ int fa_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream) {
     });
 }
 
+int fa_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream) {
+    return guard([&] {
+        FA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, fa::as_stream(stream)));
+        FA_HIP(hipStreamSynchronize(fa::as_stream(stream)));
+    });
+}
+
 // ---- plan -------------------------------------------------------------------
 
 int fa_generate_dm_list(const fa_dedisp_config* cfg, float* out, uint64_t cap, uint64_t* n) {

@@ -363,6 +363,72 @@ __device__ KadaneDev kadane_combine(const KadaneDev& a, const KadaneDev& b) {
     return r;
 }
 
+// Register form of the append-only Kadane state used in the scans: the same
+// comparisons and FP64 additions as kadane_append, with 32-bit indices; the
+// prefix run always starts at `first`, and `last` is the last appended index
+// (set when the state is written out).
+struct KadaneReg {
+    double total, psum, ssum, bsum;
+    int32_t first, pend, sstart, send, bstart, bend;
+    bool empty;
+};
+
+__device__ __forceinline__ void kreg_init(KadaneReg& k) {
+    k.total = k.psum = k.ssum = k.bsum = 0.0;
+    k.first = 0;
+    k.pend = k.send = k.bend = -1;
+    k.sstart = k.bstart = 0;
+    k.empty  = true;
+}
+
+__device__ __forceinline__ void kreg_append(KadaneReg& k, int32_t i, double u) {
+    if (k.empty) {
+        k.empty  = false;
+        k.first  = i;
+        k.psum   = 0.0;
+        k.pend   = i - 1;
+        k.ssum   = 0.0;
+        k.sstart = i;
+        k.send   = i - 1;
+        k.bsum   = 0.0;
+        k.bstart = i;
+        k.bend   = i - 1;
+    }
+    k.total += u;
+    if (k.total > k.psum) {
+        k.psum = k.total;
+        k.pend = i;
+    }
+    if (k.ssum + u > 0.0) {
+        if (k.send - k.sstart + 1 <= 0) k.sstart = i;
+        k.ssum += u;
+        k.send = i;
+    } else {
+        k.ssum   = 0.0;
+        k.sstart = i + 1;
+        k.send   = i;
+    }
+    if (k.ssum > k.bsum) {
+        k.bsum   = k.ssum;
+        k.bstart = k.sstart;
+        k.bend   = k.send;
+    }
+}
+
+__device__ __forceinline__ KadaneDev kreg_state(const KadaneReg& k, int64_t last) {
+    KadaneDev d;
+    kadane_init(d);
+    if (k.empty) return d;
+    d.empty  = 0;
+    d.total  = k.total;
+    d.first  = k.first;
+    d.last   = last;
+    d.prefix = {k.psum, k.first, k.pend};
+    d.suffix = {k.ssum, k.sstart, k.send};
+    d.best   = {k.bsum, k.bstart, k.bend};
+    return d;
+}
+
 // ---- scan: rows ------------------------------------------------------------------
 
 struct ScanArgs {
@@ -384,10 +450,14 @@ struct ScanArgs {
 constexpr int kRowTileCh = 256;
 
 // 64 rows per block; channel tiles transposed through LDS so each lane walks
-// its own row in channel order (lib/cpu/rfi_cpu_measure.cpp:365-415).
+// its own row in channel order (lib/cpu/rfi_cpu_measure.cpp:365-415). The
+// tile's channel mask and calibration (for the one or two windows the 64 rows
+// span) are staged in LDS beside the samples.
 template <int DT>
 __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
     extern __shared__ uint32_t tile[]; // [64][words + 1]
+    __shared__ float s_lc[2][kRowTileCh], s_ic[2][kRowTileCh];
+    __shared__ uint8_t s_mask[kRowTileCh];
     const Geometry& g   = a.g;
     const uint64_t t0   = (uint64_t)blockIdx.x * 64;
     const int lane      = threadIdx.x;
@@ -397,70 +467,85 @@ __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
                          : g.dtype == FA_UINT8 ? 8 : g.dtype == FA_UINT16 ? 16 : 32;
     const int words     = kRowTileCh * bits / 32; // 32-bit words per row per tile
     const int pitch     = words + 1;
-    const uint64_t w    = active ? g.window_of(t) : 0;
-    const float* lc     = a.loc + w * g.nch;
-    const float* ic     = a.inv + w * g.nch;
-    const uint64_t t_last   = min(t0 + 63, g.nsamps - 1);
-    const uint64_t wu       = g.window_of(t0);
-    const bool w_uniform    = g.window_of(t_last) == wu;
+    const uint64_t t_last = min(t0 + 63, g.nsamps - 1);
+    const uint64_t wa     = g.window_of(t0);
+    const uint64_t wb     = g.window_of(t_last);
+    const bool staged     = wb - wa <= 1; // else (windows under 64 rows) read calibration directly
+    const uint64_t w      = active ? g.window_of(t) : wa;
+    const int set         = w == wa ? 0 : 1;
+    const bool aligned    = (g.row_bytes % 4) == 0 && (reinterpret_cast<uintptr_t>(g.block) & 3) == 0;
     double sum_det = 0, sum_keep = 0, n_keep = 0, sx = 0, sxx = 0, n_meas = 0;
-    KadaneDev sp, sn;
-    kadane_init(sp);
-    kadane_init(sn);
+    KadaneReg sp, sn;
+    kreg_init(sp);
+    kreg_init(sn);
     for (uint64_t c0 = 0; c0 < g.nch; c0 += kRowTileCh) {
         const uint64_t nc   = g.nch - c0 < kRowTileCh ? g.nch - c0 : kRowTileCh;
         const uint64_t nbyt = nc * bits / 8;
         __syncthreads();
-        for (int r = 0; r < 64; ++r) {
-            const uint64_t tr = t0 + r;
-            if (tr >= g.nsamps) break;
-            const uint8_t* rowp = g.block + tr * g.row_bytes + c0 * bits / 8;
-            for (int k = lane; k < words; k += 64) {
-                uint32_t v = 0;
-                const uint64_t byte = (uint64_t)k * 4;
-                if (byte + 4 <= nbyt) {
-                    if ((reinterpret_cast<uintptr_t>(rowp + byte) & 3) == 0)
-                        v = *reinterpret_cast<const uint32_t*>(rowp + byte);
-                    else
-                        for (int q = 0; q < 4; ++q) v |= (uint32_t)rowp[byte + q] << (8 * q);
-                } else {
+        if (aligned && nbyt % 4 == 0) {
+            // Whole words: 16 rows of loads in flight (rows past the block
+            // end read the last row and are never used).
+            for (int r0 = 0; r0 < 64; r0 += kRowBatch) {
+                for (int k = lane; k < words; k += 64) {
+                    uint32_t v[kRowBatch];
+#pragma unroll
+                    for (int q = 0; q < kRowBatch; ++q) {
+                        const uint64_t tr = min(t0 + r0 + q, g.nsamps - 1);
+                        const uint8_t* rp = g.block + tr * g.row_bytes + c0 * bits / 8;
+                        v[q] = (uint64_t)k * 4 < nbyt ? *reinterpret_cast<const uint32_t*>(rp + (uint64_t)k * 4) : 0U;
+                    }
+#pragma unroll
+                    for (int q = 0; q < kRowBatch; ++q) tile[(r0 + q) * pitch + k] = v[q];
+                }
+            }
+        } else {
+            for (int r = 0; r < 64; ++r) {
+                const uint64_t tr = t0 + r;
+                if (tr >= g.nsamps) break;
+                const uint8_t* rowp = g.block + tr * g.row_bytes + c0 * bits / 8;
+                for (int k = lane; k < words; k += 64) {
+                    uint32_t v = 0;
+                    const uint64_t byte = (uint64_t)k * 4;
                     for (uint64_t q = 0; byte + q < nbyt && q < 4; ++q)
                         v |= (uint32_t)rowp[byte + q] << (8 * q);
+                    tile[r * pitch + k] = v;
                 }
-                tile[r * pitch + k] = v;
+            }
+        }
+        for (uint64_t j = lane; j < nc; j += 64) {
+            s_mask[j] = a.mask[c0 + j];
+            if (staged) {
+                s_lc[0][j] = a.loc[wa * g.nch + c0 + j];
+                s_ic[0][j] = a.inv[wa * g.nch + c0 + j];
+                s_lc[1][j] = a.loc[wb * g.nch + c0 + j];
+                s_ic[1][j] = a.inv[wb * g.nch + c0 + j];
             }
         }
         __syncthreads();
         if (!active) continue;
         const uint8_t* myrow = reinterpret_cast<const uint8_t*>(tile + lane * pitch);
-        // Channel walk in order; lcp/icp are the window's calibration rows.
-        auto walk = [&](const float* lcp, const float* icp) {
-            for (uint64_t j = 0; j < nc; ++j) {
-                const uint64_t c = c0 + j;
-                if (a.mask[c] == 0) continue;
-                const float x = xat<DT>(myrow, j, g.dtype);
-                if (icp[c] == 0.0f) continue;
-                n_meas += 1.0;
-                const float zc = (x - lcp[c]) * icp[c];
-                const float cl = zc < -a.clip ? -a.clip : (a.clip < zc ? a.clip : zc);
-                sum_det += cl;
-                if (fabsf(zc) <= a.clip) {
-                    sum_keep += zc;
-                    n_keep += 1.0;
-                }
-                sx += x;
-                sxx += (double)x * x;
-                if (a.kspec) {
-                    kadane_append(sp, (int64_t)c, cl);
-                    kadane_append(sn, (int64_t)c, -cl);
-                }
+        const float* lcp     = staged ? s_lc[set] : a.loc + w * g.nch + c0;
+        const float* icp     = staged ? s_ic[set] : a.inv + w * g.nch + c0;
+        for (uint64_t j = 0; j < nc; ++j) {
+            if (s_mask[j] == 0) continue;
+            const float x = xat<DT>(myrow, j, g.dtype);
+            const float icv = icp[j];
+            if (icv == 0.0f) continue;
+            n_meas += 1.0;
+            const float zc = (x - lcp[j]) * icv;
+            const float cl = zc < -a.clip ? -a.clip : (a.clip < zc ? a.clip : zc);
+            sum_det += cl;
+            if (fabsf(zc) <= a.clip) {
+                sum_keep += zc;
+                n_keep += 1.0;
             }
-        };
-        if (w_uniform) {
-            // All 64 rows in one window: block-uniform calibration rows.
-            walk(a.loc + wu * g.nch, a.inv + wu * g.nch);
-        } else {
-            walk(lc, ic);
+            sx += x;
+            sxx += (double)x * x;
+            if (a.kspec) {
+                const int32_t c = (int32_t)(c0 + j);
+                kreg_append(sp, c, cl);
+                kreg_append(sn, c, -cl);
+            }
         }
     }
     if (!active) return;
@@ -473,7 +558,7 @@ __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
     if (a.kspec) {
         int32_t span[4] = {-1, -1, -1, -1};
         bool whole      = false;
-        const RunDev runs[2] = {sp.best, sn.best};
+        const RunDev runs[2] = {{sp.bsum, sp.bstart, sp.bend}, {sn.bsum, sn.bstart, sn.bend}};
         for (int k = 0; k < 2; ++k) {
             const RunDev& r = runs[k];
             if (r.end - r.start + 1 <= 0) continue;
@@ -521,9 +606,10 @@ __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
     const uint64_t o  = k * g.nch + c;
     float lo          = 3.402823466e+38f;
     float hi          = -3.402823466e+38f;
-    KadaneDev kp, kn;
-    kadane_init(kp);
-    kadane_init(kn);
+    KadaneReg kp, kn;
+    kreg_init(kp);
+    kreg_init(kn);
+    int64_t last = -1;
     if (a.mask[c] != 0) {
         const uint64_t t0 = k * a.chunk;
         const uint64_t t1 = t0 + a.chunk < g.nsamps ? t0 + a.chunk : g.nsamps;
@@ -570,17 +656,18 @@ __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
                 if (a.kchan) {
                     const bool outlier = fabsf(z) > a.clip;
                     const double u     = outlier ? 0.0 : (double)((z - cm) * ci);
-                    kadane_append(kp, (int64_t)t, u);
-                    kadane_append(kn, (int64_t)t, -u);
+                    kreg_append(kp, (int32_t)t, u);
+                    kreg_append(kn, (int32_t)t, -u);
                 }
             }
         }
+        last = (int64_t)t1 - 1;
     }
     a.cmin[o] = lo;
     a.cmax[o] = hi;
     if (a.kchan) {
-        a.kpos[o] = kp;
-        a.kneg[o] = kn;
+        a.kpos[o] = kreg_state(kp, last);
+        a.kneg[o] = kreg_state(kn, last);
     }
 }
 

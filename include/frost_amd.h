@@ -227,6 +227,9 @@ int fa_device_malloc(int device, uint64_t bytes, void** ptr);
 int fa_device_free(void* ptr);
 int fa_memcpy_h2d(void* dst, const void* src, uint64_t bytes, void* stream);
 int fa_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream);
+/* Device-to-device copy on `stream`, synchronised (also the plain copy the
+ * HBM traffic counters are calibrated against). */
+int fa_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream);
 
 /* ---- dedispersion plan -------------------------------------------------- */
 
