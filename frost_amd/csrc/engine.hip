@@ -163,9 +163,11 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wave < 4) {
         // Producer: two chunks of loads in flight (register sets A and B).
-        // Loads are unconditional at clamped in-range addresses (the value
-        // is replaced by 0 outside the input row's samples), so their count
-        // is static and waiting for set A leaves set B in flight.
+        // A window inside the input row's samples loads from a scalar base
+        // with lane offsets; edge windows load at clamped in-range addresses
+        // and zero the samples outside the row when staged. Both forms issue
+        // the same number of loads, so waiting for set A leaves set B in
+        // flight.
         float va[UMAX][M], vb[UMAX][M];
         auto load = [&](float (&v)[UMAX][M], int c) {
             const int tau0 = tau_first + c * kChunkG;
@@ -173,17 +175,22 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
             for (int u = 0; u < UMAX; ++u) {
                 const int sidx      = min(wave + 4 * u, B.nleaves - 1);
                 const GatherLeaf lf = leaves[B.leaf_first + sidx];
-                const float* row0   = in + lf.src + lf.shmax; // sample t0 of the input row
+                const int tp0       = tau0 - lf.shmax; // sample of window element 0
+                if (tp0 >= -lf.hc && tp0 + M * 64 <= g.tile_len) {
+                    const float* src = in + (lf.src + tau0);
 #pragma unroll
-                for (int m = 0; m < M; ++m) {
-                    const int tp  = tau0 + lane + 64 * m - lf.shmax;
-                    const int tpc = min(max(tp, -lf.hc), g.tile_len - 1);
-                    v[u][m]       = row0[tpc];
+                    for (int m = 0; m < M; ++m) v[u][m] = src[lane + 64 * m];
+                } else {
+                    const float* row0 = in + lf.src + lf.shmax; // sample t0 of the input row
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        const int tp  = tp0 + lane + 64 * m;
+                        const int tpc = min(max(tp, -lf.hc), g.tile_len - 1);
+                        v[u][m]       = row0[tpc];
+                    }
                 }
             }
         };
-        // Zero outside the input row's samples is applied here, where the
-        // loaded values are consumed, so the loads stay in flight until then.
         auto stage = [&](const float (&v)[UMAX][M], int c) {
             const int tau0 = tau_first + c * kChunkG;
             float* buf     = lds + (c & 1) * g.half;
@@ -192,12 +199,20 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                 const int sidx = wave + 4 * u;
                 if (sidx < B.nleaves) {
                     const GatherLeaf lf = leaves[B.leaf_first + sidx];
+                    const int tp0       = tau0 - lf.shmax;
+                    float* dst          = buf + lf.slot + lane;
+                    if (tp0 >= -lf.hc && tp0 + M * 64 <= g.tile_len) {
 #pragma unroll
-                    for (int m = 0; m < M; ++m) {
-                        const int j  = lane + 64 * m;
-                        const int tp = tau0 + j - lf.shmax;
-                        if (j < lf.len)
-                            buf[lf.slot + j] = (tp >= -lf.hc && tp < g.tile_len) ? v[u][m] : 0.0f;
+                        for (int m = 0; m < M; ++m)
+                            if (m < M - 1 || lane + 64 * m < lf.len) dst[64 * m] = v[u][m];
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < M; ++m) {
+                            const int j  = lane + 64 * m;
+                            const int tp = tp0 + j;
+                            if (j < lf.len)
+                                dst[64 * m] = (tp >= -lf.hc && tp < g.tile_len) ? v[u][m] : 0.0f;
+                        }
                     }
                 }
             }
@@ -215,51 +230,34 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
             }
         }
     } else {
-        // Consumer: wave 4-7 covers samples 64 * (wave - 4) .. + 64 of a chunk.
+        // Consumer: wave 4 + q sums rows q, q + 4, ... of the run; lane l
+        // covers samples l, l + 64, l + 128, l + 192 of the chunk, so one
+        // LDS address per (row, descendant) serves four reads.
         // Row record: [out lo, out hi, h, pad, delta_0 .. delta_{K-1}].
         constexpr int RS = 4 + K;
-        const int t      = threadIdx.x - 256;
+        const int q      = wave - 4;
         const int r_end  = B.row_first + B.nrows;
         for (int c = lo_c; c < hi_c; ++c) {
             lds_barrier();
-            const float* buf   = lds + (c & 1) * g.half;
-            const int tau      = tau_first + c * kChunkG + t;
-            const bool in_tile = tau < g.tile_len;
-            int r = B.row_first;
-            for (; r + 4 <= r_end; r += 4) {
-                int32_t rec[4][RS];
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-#pragma unroll
-                    for (int j = 0; j < RS; ++j) rec[q][j] = rows[(int64_t)(r + q) * RS + j];
-                // Every LDS offset stays inside the staged windows, so all four
-                // rows' sums are formed first and only the stores are guarded.
-                float sum[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    float w[K];
-#pragma unroll
-                    for (int j = 0; j < K; ++j) w[j] = buf[rec[q][4 + j] + t];
-                    sum[q] = tree_sum<K>(w);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (in_tile && tau >= -rec[q][2]) {
-                        const int64_t o = (int64_t)(uint32_t)rec[q][0] | ((int64_t)rec[q][1] << 32);
-                        out[o + tau]    = sum[q];
-                    }
-                }
-            }
-            for (; r < r_end; ++r) {
+            const float* buf = lds + (c & 1) * g.half + lane;
+            const int tau0   = tau_first + c * kChunkG + lane;
+            for (int r = B.row_first + q; r < r_end; r += 4) {
                 int32_t rec[RS];
 #pragma unroll
                 for (int j = 0; j < RS; ++j) rec[j] = rows[(int64_t)r * RS + j];
-                if (in_tile && tau >= -rec[2]) {
-                    float w[K];
+                float w[4][K];
 #pragma unroll
-                    for (int j = 0; j < K; ++j) w[j] = buf[rec[4 + j] + t];
-                    const int64_t o = (int64_t)(uint32_t)rec[0] | ((int64_t)rec[1] << 32);
-                    out[o + tau]    = tree_sum<K>(w);
+                for (int j = 0; j < K; ++j) {
+                    const float* p = buf + rec[4 + j];
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) w[s4][j] = p[64 * s4];
+                }
+                const int64_t o = (int64_t)(uint32_t)rec[0] | ((int64_t)rec[1] << 32);
+                float* dst      = out + o;
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4) {
+                    const int tau = tau0 + 64 * s4;
+                    if (tau < g.tile_len && tau >= -rec[2]) dst[tau] = tree_sum<K>(w[s4]);
                 }
             }
         }
@@ -281,8 +279,8 @@ void launch_gather_u(const MergeLaunch& ml, const float* in, float* out, const G
 int gather_chunks() {
     static const int n = [] {
         const char* v = std::getenv("FROST_AMD_GATHER_CHUNKS");
-        const int c   = v ? std::atoi(v) : 8;
-        return (c >= 1 && c <= 64) ? c : 8;
+        const int c   = v ? std::atoi(v) : 16;
+        return (c >= 1 && c <= 64) ? c : 16;
     }();
     return n;
 }
