@@ -586,10 +586,12 @@ __device__ __forceinline__ float back(const float (&cur)[4], const float (&prev)
     const int dl = (p >= 0) ? 0 : -((-p + 3) / 4); // floor(p / 4)
     const int jj = p - 4 * dl;                  // 0..3
     if (dl == 0) return cur[jj];
-    const int src  = lane + dl;
-    const float a  = __shfl(cur[jj], src & 63);
-    const float b  = __shfl(prev[jj], src & 63);
-    return src >= 0 ? a : b;
+    // Destination lane L reads source lane L - d (current chunk) or, for
+    // L < d, lane L - d + 64 of the previous chunk: choose at the source,
+    // then rotate once.
+    const int d     = -dl;
+    const float sel = lane < 64 - d ? cur[jj] : prev[jj];
+    return __shfl(sel, (lane - d) & 63);
 }
 
 template <int K, int KMAX, bool TRAP> struct LadderStep;
@@ -763,8 +765,7 @@ __global__ void __launch_bounds__(kBlock) search_reg_kernel(RegSearchArgs ra) {
             const int w = ra.widx[k];
             if (u < 0 || w < 0) continue;
             const uint32_t W = 1U << k;
-            float sv[4];
-            bool pred = false;
+            float sv[4], sums[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 float sum = D[k][j];
@@ -781,22 +782,33 @@ __global__ void __launch_bounds__(kBlock) search_reg_kernel(RegSearchArgs ra) {
                     }
                     sum = (sum - 0.5f * D[k - 2][j]) - 0.5f * e2;
                 }
-                float sn;
-                if (fast) {
-                    sn = sum * inv_c[k];
-                } else {
+                sums[j] = sum;
+            }
+            bool pred = false;
+            if (fast) {
+                // Multiplying by inv > 0 is monotone under rounding, so the
+                // largest of the four sums decides whether any S/N exceeds
+                // the threshold; the S/N values are formed only then.
+                const float m = fmaxf(fmaxf(sums[0], sums[1]), fmaxf(sums[2], sums[3]));
+                pred          = m * inv_c[k] > a.thr;
+                if (__ballot(pred) == 0 && !((open_mask >> w) & 1U)) continue;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sv[j] = sums[j] * inv_c[k];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
                     const int64_t t = tc0 + 4 * lane + j;
-                    sn              = -INFINITY;
+                    float sn        = -INFINITY;
                     if (base + j < tl) {
                         if (t < (int64_t)row.delay + (int64_t)W - 1) {
-                            if (!a.first) sn = sum * inv_o[k];
+                            if (!a.first) sn = sums[j] * inv_o[k];
                         } else {
-                            sn = sum * inv_c[k];
+                            sn = sums[j] * inv_c[k];
                         }
                     }
+                    sv[j] = sn;
+                    pred  = pred || sn > a.thr;
                 }
-                sv[j] = sn;
-                pred  = pred || sn > a.thr;
             }
             if (__ballot(pred) != 0 || ((open_mask >> w) & 1U)) {
 #pragma unroll
