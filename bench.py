@@ -166,7 +166,11 @@ def main() -> int:
     value = ndm * W.gulp * a.steps / total
     # Algorithmic bytes of the dominant launches over one gulp: each input
     # row read once and each output row written once (float32, per sample).
-    alg_bytes_gulp = 4.0 * W.gulp * float(sum(rows[src[l]] + rows[l] for l in r16))
+    # A launch reading level 0 when level 0 is not stored (src[0] == -2)
+    # forms it from the input waterfall: its input rows are the channels.
+    def in_rows(a):
+        return float(W.nchans) if (a == 0 and src[0] == -2) else float(rows[a])
+    alg_bytes_gulp = 4.0 * W.gulp * float(sum(in_rows(src[l]) + rows[l] for l in r16))
     achieved = alg_bytes_gulp * a.steps / merge_s / 1e9
     launches_per_gulp = merge_launches / a.steps
     traffic = gather_traffic_from_profile()

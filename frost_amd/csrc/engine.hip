@@ -3,6 +3,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -16,19 +17,25 @@ namespace fa {
 namespace {
 
 constexpr int kBlock       = 256;
-constexpr int kInitPer     = 4;    // outputs per thread per init block
+constexpr int kInitPer     = 8;    // outputs per thread per init block
 constexpr int kChunk       = 1024; // search samples per LDS pass
 constexpr uint32_t kMaxBox = 1024; // ladder fits LDS up to this width
 
 // ---- kernels ---------------------------------------------------------------
 
 // Level 0: per channel, the left-fold sums x(t) + x(t-1) + ... + x(t-D) for
-// each of its row delays D (ascending), over [t0 - hc, t0 + tile).
+// each of its row delays D (ascending), over [t0 - hc, t0 + tile). The
+// channel's row table is staged in LDS; each thread carries kInitPer running
+// sums through the rows in delay order.
+constexpr int kInitRows = 64; // level-0 rows per channel held in LDS
+
 __global__ void __launch_bounds__(kBlock)
 fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
                  const int32_t* __restrict__ chan, const int64_t* __restrict__ rows,
                  float* __restrict__ out, int tile_len, int nchans) {
     extern __shared__ float sx[];
+    __shared__ int64_t s_base[kInitRows];
+    __shared__ int32_t s_d[kInitRows], s_h[kInitRows];
     for (int c = blockIdx.y; c < nchans; c += gridDim.y) {
         const int rf = chan[4 * c], nr = chan[4 * c + 1], hc = chan[4 * c + 2],
                   dm = chan[4 * c + 3];
@@ -43,23 +50,36 @@ fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
             const int gi = i0 - dm + j;
             sx[j]        = gi < total ? xc[gi] : 0.0f;
         }
-        __syncthreads();
+        float acc[kInitPer];
+        int k = 0;
+        for (int rb = 0; rb < nr; rb += kInitRows) {
+            if (rb > 0) __syncthreads(); // previous batch of the row table consumed
+            const int nb = min(kInitRows, nr - rb);
+            for (int j = threadIdx.x; j < nb; j += kBlock) {
+                const int64_t* rw = rows + 3 * (int64_t)(rf + rb + j);
+                s_d[j]            = (int32_t)rw[0];
+                s_base[j]         = rw[1];
+                s_h[j]            = (int32_t)rw[2];
+            }
+            __syncthreads();
+            if (rb == 0) {
 #pragma unroll
-        for (int q = 0; q < kInitPer; ++q) {
-            const int i = i0 + threadIdx.x + q * kBlock;
-            if (i >= total) continue;
-            const int li = dm + (i - i0);
-            float acc    = sx[li];
-            int k        = 0;
-            for (int jr = 0; jr < nr; ++jr) {
-                const int64_t* rw = rows + 3 * (int64_t)(rf + jr);
-                const int D       = (int)rw[0];
-                while (k < D) {
-                    ++k;
-                    acc = acc + sx[li - k];
+                for (int q = 0; q < kInitPer; ++q) acc[q] = sx[dm + threadIdx.x + q * kBlock];
+            }
+            for (int jr = 0; jr < nb; ++jr) {
+                const int D = s_d[jr];
+                for (; k < D; ++k) {
+#pragma unroll
+                    for (int q = 0; q < kInitPer; ++q)
+                        acc[q] = acc[q] + sx[dm + threadIdx.x + q * kBlock - (k + 1)];
                 }
-                const int h = (int)rw[2];
-                if (i >= hc - h) out[rw[1] + (i - (hc - h))] = acc;
+                const int skip = hc - s_h[jr];
+                float* o       = out + s_base[jr] - skip;
+#pragma unroll
+                for (int q = 0; q < kInitPer; ++q) {
+                    const int i = i0 + threadIdx.x + q * kBlock;
+                    if (i < total && i >= skip) o[i] = acc[q];
+                }
             }
         }
     }
@@ -135,7 +155,21 @@ struct GatherDims {
     int tau_origin;
     int half;   // floats per LDS half
     int chunks; // consecutive chunks per block
+    // Level-0 runs (X0): x points at tile sample 0 of channel 0; input
+    // samples exist for tile samples [xlo, xhi].
+    const float* x;
+    int xlo, xhi;
 };
+
+constexpr int kX0Grid = 384; // grid elements per channel window (6 per lane)
+constexpr int kX0Win  = 448; // window samples (grid + delays, 7 per lane)
+constexpr int kX0Dmax = 16;  // largest level-0 delay formed from the input
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -143,12 +177,14 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int K, int UMAX>
+template <int K, int UMAX, bool X0>
 __global__ void __launch_bounds__(kGatherThreads)
 fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                    const GatherBlock* __restrict__ blocks, const GatherLeaf* __restrict__ leaves,
-                   const int32_t* __restrict__ rows, GatherDims g) {
+                   const int32_t* __restrict__ rows, const GatherChan* __restrict__ chans,
+                   GatherDims g) {
     extern __shared__ float lds[];
+    __shared__ float xwin[X0 ? 4 : 1][X0 ? kX0Win : 1]; // per producer wave
     constexpr int M     = (kChunkG + kSpreadG) / 64;
     const uint32_t id   = blockIdx.x;
     const uint32_t k    = id >> 3;
@@ -161,7 +197,129 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
     if (tau_first + kChunkG * g.chunks <= -B.hmax || lo_c >= hi_c) return; // uniform
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (wave < 4) {
+    if (X0 && wave < 4) {
+        // Producer of level-0 rows: per channel group, a window of input
+        // samples (two chunks of loads in flight, register sets A and B) is
+        // written to this wave's LDS window; the left-fold sums
+        // x(t) + x(t-1) + ... + x(t-D) are accumulated in that order on the
+        // channel's grid and each input row is emitted into its slot when its
+        // delay D is reached (rows sorted by delay).
+        constexpr int MX = kX0Win / 64, MG = kX0Grid / 64;
+        float va[UMAX][MX], vb[UMAX][MX];
+        auto load = [&](float (&v)[UMAX][MX], int c) {
+            const int tau0 = tau_first + c * kChunkG;
+#pragma unroll
+            for (int u = 0; u < UMAX; ++u) {
+                const int gi        = min(wave + 4 * u, B.nchan - 1);
+                const GatherChan ch = chans[B.chan_first + gi];
+                const int tw        = tau0 - ch.xs - ch.dmax; // tile sample of window element 0
+                const float* xr     = g.x + ch.xoff;
+                if (tw >= g.xlo && tw + kX0Win - 1 <= g.xhi) {
+                    const float* src = xr + tw;
+#pragma unroll
+                    for (int m = 0; m < MX; ++m) v[u][m] = src[lane + 64 * m];
+                } else {
+#pragma unroll
+                    for (int m = 0; m < MX; ++m)
+                        v[u][m] = xr[min(max(tw + lane + 64 * m, g.xlo), g.xhi)];
+                }
+            }
+        };
+        auto stage = [&](const float (&v)[UMAX][MX], int c) {
+            const int tau0 = tau_first + c * kChunkG;
+            float* buf     = lds + (c & 1) * g.half;
+            float* xw      = xwin[wave];
+#pragma unroll
+            for (int u = 0; u < UMAX; ++u) {
+                const int gi = wave + 4 * u;
+                if (gi < B.nchan) {
+                    const GatherChan ch = chans[B.chan_first + gi];
+                    wave_sync();
+#pragma unroll
+                    for (int m = 0; m < MX; ++m) xw[lane + 64 * m] = v[u][m];
+                    wave_sync();
+                    // Lane l holds grid elements 6l .. 6l + 5 and the kX0Dmax
+                    // samples before them: xl[q] = x(tile sample of grid 6l,
+                    // minus kX0Dmax, plus q), window element 6l + dmax - kX0Dmax + q.
+                    float xl[MG + kX0Dmax];
+                    const int e0 = MG * lane + ch.dmax - kX0Dmax;
+#pragma unroll
+                    for (int q = 0; q < MG + kX0Dmax; ++q) xl[q] = xw[max(e0 + q, 0)];
+                    float acc[MG];
+#pragma unroll
+                    for (int m = 0; m < MG; ++m) acc[m] = xl[m + kX0Dmax];
+                    // The group's rows in delay order; the first four are
+                    // held in registers (fetched before the sums), later
+                    // ones (rare) are fetched when reached.
+                    int fslot[4], flen[4], fsh[4], fhc[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const GatherLeaf& q = leaves[ch.leaf_first + min(k, ch.nleaf - 1)];
+                        fslot[k]            = q.slot;
+                        flen[k]             = q.len;
+                        fsh[k]              = q.shmax;
+                        fhc[k]              = q.hc;
+                    }
+                    int lk     = 0;
+                    auto emit  = [&]() {
+                        // lk is wave-uniform: select chains keep the fields in
+                        // registers; rows past the fourth are fetched (rare).
+                        auto pick = [&](const int (&f)[4]) {
+                            return lk == 0 ? f[0] : lk == 1 ? f[1] : lk == 2 ? f[2] : f[3];
+                        };
+                        GatherLeaf lf{};
+                        if (lk < 4) {
+                            lf.slot  = pick(fslot);
+                            lf.len   = pick(flen);
+                            lf.shmax = pick(fsh);
+                            lf.hc    = pick(fhc);
+                        } else {
+                            lf = leaves[ch.leaf_first + lk];
+                        }
+                        ++lk;
+                        const int j0  = MG * lane - (ch.xs - lf.shmax); // slot index of grid 6l
+                        const int tp0 = tau0 - ch.xs + MG * lane;
+                        const int ts  = tau0 - lf.shmax; // tile sample of slot element 0
+                        if (ts >= -lf.hc && ts + lf.len <= g.tile_len) {
+#pragma unroll
+                            for (int m = 0; m < MG; ++m) {
+                                const int j = j0 + m;
+                                if (j >= 0 && j < lf.len) buf[lf.slot + j] = acc[m];
+                            }
+                        } else {
+#pragma unroll
+                            for (int m = 0; m < MG; ++m) {
+                                const int j = j0 + m, tp = tp0 + m;
+                                if (j >= 0 && j < lf.len)
+                                    buf[lf.slot + j] =
+                                        (tp >= -lf.hc && tp < g.tile_len) ? acc[m] : 0.0f;
+                            }
+                        }
+                    };
+                    if (ch.dmask & 1U) emit();
+#pragma unroll
+                    for (int d = 1; d <= kX0Dmax; ++d) {
+                        if (d > ch.dmax) break;
+#pragma unroll
+                        for (int m = 0; m < MG; ++m) acc[m] = acc[m] + xl[m + kX0Dmax - d];
+                        if ((ch.dmask >> d) & 1U) emit();
+                    }
+                }
+            }
+        };
+        load(va, lo_c);
+        if (lo_c + 1 < hi_c) load(vb, lo_c + 1);
+        for (int c = lo_c; c < hi_c; c += 2) {
+            stage(va, c);
+            lds_barrier();
+            if (c + 2 < hi_c) load(va, c + 2);
+            if (c + 1 < hi_c) {
+                stage(vb, c + 1);
+                lds_barrier();
+                if (c + 3 < hi_c) load(vb, c + 3);
+            }
+        }
+    } else if (wave < 4) {
         // Producer: two chunks of loads in flight (register sets A and B).
         // A window inside the input row's samples loads from a scalar base
         // with lane offsets; edge windows load at clamped in-range addresses
@@ -264,16 +422,16 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
     }
 }
 
-template <int K, int UMAX>
+template <int K, int UMAX, bool X0>
 void launch_gather_u(const MergeLaunch& ml, const float* in, float* out, const GatherDims& g,
                      int tl, hipStream_t s) {
     const unsigned ngrp   = ceil_div(tl + ml.hmax, kChunkG * g.chunks);
     const uint64_t blocks = (uint64_t)((ngrp + 7) / 8) * 8 * ml.nblocks;
     if (blocks * kGatherThreads >= (uint64_t{1} << 32))
         throw std::invalid_argument("FDMT merge grid too large; reduce the time tile");
-    fdmt_gather_kernel<K, UMAX><<<static_cast<unsigned>(blocks), kGatherThreads,
-                                  2 * g.half * sizeof(float), s>>>(
-        in, out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), g);
+    fdmt_gather_kernel<K, UMAX, X0><<<static_cast<unsigned>(blocks), kGatherThreads,
+                                      2 * g.half * sizeof(float), s>>>(
+        in, out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), ml.chans.get(), g);
 }
 
 int gather_chunks() {
@@ -285,8 +443,11 @@ int gather_chunks() {
     return n;
 }
 
+// x0: tile sample 0 of channel 0 of the input; [xlo, xhi] the tile samples
+// held by the input buffer (used only by launches reading level 0 from x).
 template <int K>
-void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, hipStream_t s) {
+void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, const float* x0,
+                   int xlo, int xhi, hipStream_t s) {
     static_assert(kChunkG == 256, "four consumer waves cover one chunk");
     GatherDims g{};
     g.nblocks    = static_cast<int>(ml.nblocks);
@@ -294,11 +455,21 @@ void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, h
     g.tau_origin = -ml.hmax;
     g.half       = static_cast<int>(ml.lds_floats);
     g.chunks     = gather_chunks();
+    g.x          = x0;
+    g.xlo        = xlo;
+    g.xhi        = xhi;
+    if (ml.from_input) {
+        switch (ml.max_leaves) {
+        case 16: return launch_gather_u<K, 4, true>(ml, in, out, g, tl, s);
+        case 32: return launch_gather_u<K, 8, true>(ml, in, out, g, tl, s);
+        default: throw std::logic_error("staged FDMT gather: unsupported channels per block");
+        }
+    }
     switch (ml.max_leaves) {
-    case 16: return launch_gather_u<K, 4>(ml, in, out, g, tl, s);
-    case 32: return launch_gather_u<K, 8>(ml, in, out, g, tl, s);
-    case 48: return launch_gather_u<K, 12>(ml, in, out, g, tl, s);
-    case 64: return launch_gather_u<K, 16>(ml, in, out, g, tl, s);
+    case 16: return launch_gather_u<K, 4, false>(ml, in, out, g, tl, s);
+    case 32: return launch_gather_u<K, 8, false>(ml, in, out, g, tl, s);
+    case 48: return launch_gather_u<K, 12, false>(ml, in, out, g, tl, s);
+    case 64: return launch_gather_u<K, 16, false>(ml, in, out, g, tl, s);
     default: throw std::logic_error("staged FDMT gather: unsupported input rows per block");
     }
 }
@@ -633,11 +804,6 @@ __device__ __forceinline__ float trap_back(const float (&D)[9][4], const float (
     }
 }
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Serial run tracking of one width over one chunk (lane 0), the S/N values
 // staged in LDS. Returns the run's open flag afterwards.
@@ -948,12 +1114,15 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
 
     // Tuning: FROST_AMD_FUSE (levels per launch, 1-4), FROST_AMD_MERGE=direct
     // (gather from global memory instead of LDS staging), FROST_AMD_GATHER_LDS
-    // (LDS floats per staged block).
+    // (LDS floats per staged block), FROST_AMD_INIT=fused (form level 0 in
+    // the first gather launch instead of storing it with the init kernel).
     if (const char* fv = std::getenv("FROST_AMD_FUSE")) {
         const long f = std::atol(fv);
         if (f >= 1 && f <= 4) max_fuse_ = static_cast<uint32_t>(f);
     }
     if (const char* mv = std::getenv("FROST_AMD_MERGE")) staged_merge_ = std::string(mv) != "direct";
+    if (const char* iv = std::getenv("FROST_AMD_INIT")) x0_fused_ = std::string(iv) == "fused";
+    x0_fused_ = x0_fused_ && staged_merge_;
     if (!staged_merge_) max_fuse_ = std::min<uint32_t>(max_fuse_, 3);
     if (const char* gv = std::getenv("FROST_AMD_GATHER_LDS")) {
         const long f = std::atol(gv);
@@ -1064,6 +1233,7 @@ void Engine::build_programs() {
                                                     static_cast<uint64_t>(chan[4 * c + 3]));
         dmax0_   = std::max(dmax0_, chan[4 * c + 3]);
     }
+    xstride_ = round64(hist_in_ + cap_); // extended input row: H history + capacity
     init_chan_.resize(chan.size());
     FA_HIP(hipMemcpy(init_chan_.get(), chan.data(), chan.size() * 4, hipMemcpyHostToDevice));
     init_rows_.resize(rows.size());
@@ -1078,7 +1248,8 @@ void Engine::build_programs() {
     constexpr uint64_t kInf = ~uint64_t{0};
     std::vector<uint64_t> best(plan_.nlevels, kInf);
     std::vector<int32_t> prev(plan_.nlevels, -1);
-    best[0] = level_len_[0];
+    if (dmax0_ > kX0Dmax) x0_fused_ = false; // level-0 delays too long for the fused form
+    best[0] = x0_fused_ ? 0 : level_len_[0]; // fused: level 0 is formed from the input
     for (uint32_t l = 1; l <= L; ++l) {
         for (uint32_t j = l; j-- > 0 && l - j <= F;) {
             if (best[j] != kInf && best[j] + level_len_[l] < best[l]) {
@@ -1097,8 +1268,11 @@ void Engine::build_programs() {
     std::reverse(stored.begin(), stored.end());
     level_source_.assign(plan_.nlevels, -2);
     level_buf_.assign(plan_.nlevels, -1);
-    level_source_[0] = -1;
-    for (std::size_t i = 0; i < stored.size(); ++i) level_buf_[stored[i]] = static_cast<int>(i & 1);
+    level_source_[0] = x0_fused_ ? -2 : -1;
+    for (std::size_t i = 0, k = 0; i < stored.size(); ++i) {
+        if (stored[i] == 0 && x0_fused_) continue;
+        level_buf_[stored[i]] = static_cast<int>(k++ & 1);
+    }
     launches_.clear();
     for (std::size_t i = 1; i < stored.size(); ++i) {
         const uint32_t a = stored[i - 1], b = stored[i];
@@ -1107,8 +1281,9 @@ void Engine::build_programs() {
         ml.in_level  = a;
         ml.out_level = b;
         ml.radix     = 1 << (b - a);
-        ml.in_buf    = level_buf_[a];
-        ml.out_buf   = level_buf_[b];
+        ml.in_buf     = level_buf_[a];
+        ml.out_buf    = level_buf_[b];
+        ml.from_input = a == 0 && x0_fused_;
         const auto& lv = plan_.levels[b];
         ml.nrows       = static_cast<uint32_t>(lv.size());
         const int K    = ml.radix;
@@ -1209,6 +1384,33 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
     std::vector<std::pair<int32_t, int32_t>> rl(static_cast<std::size_t>(ml.nrows) * K);
     uint32_t first = 0, lds_max = 0;
     int32_t hmax = 0;
+    // Level-0 rows formed from the input: rows of one channel share a
+    // window, so a run is bounded by its channel count (16) and each
+    // channel's rows must fit one grid (shift range + chunk within kX0Grid,
+    // delays within the window).
+    const bool x0         = ml.from_input;
+    const auto& l0        = plan_.levels[0];
+    const std::size_t max_chans = 16; // four channel windows per producer wave
+    if (x0) ml.max_leaves = static_cast<uint32_t>(max_chans);
+    auto x0_ok = [&](const std::vector<Lf>& v) {
+        std::vector<std::array<int32_t, 4>> ch; // channel, min shmin, max shmax, max delay
+        for (const auto& x : v) {
+            const int32_t c = l0.chan_or_lo[x.c], d = l0.delay[x.c];
+            bool found      = false;
+            for (auto& e : ch)
+                if (e[0] == c) {
+                    e[1]  = std::min(e[1], x.shmin);
+                    e[2]  = std::max(e[2], x.shmax);
+                    e[3]  = std::max(e[3], d);
+                    found = true;
+                }
+            if (!found) ch.push_back({c, x.shmin, x.shmax, d});
+        }
+        if (ch.size() > max_chans) return false;
+        for (const auto& e : ch)
+            if (e[2] - e[1] + kChunkG > kX0Grid || e[3] > kX0Dmax) return false;
+        return true;
+    };
     auto need = [&](const std::vector<Lf>& v, bool& ok) {
         uint64_t n = 0;
         ok         = true;
@@ -1216,8 +1418,10 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
             n += kChunkG + (x.shmax - x.shmin);
             ok = ok && (x.shmax - x.shmin) <= kSpreadG;
         }
+        if (x0) ok = ok && x0_ok(v);
         return n;
     };
+    std::vector<GatherChan> gc;
     auto close = [&](uint32_t end) {
         GatherBlock b{};
         b.leaf_first = static_cast<int32_t>(gl.size());
@@ -1225,6 +1429,13 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
         b.row_first  = static_cast<int32_t>(first);
         b.nrows      = static_cast<int32_t>(end - first);
         b.hmax       = hmax;
+        b.chan_first = static_cast<int32_t>(gc.size());
+        b.nchan      = 0;
+        if (x0) // channel groups, delays ascending
+            std::sort(cur.begin(), cur.end(), [&](const Lf& p, const Lf& q) {
+                return std::make_pair(l0.chan_or_lo[p.c], l0.delay[p.c]) <
+                       std::make_pair(l0.chan_or_lo[q.c], l0.delay[q.c]);
+            });
         std::vector<int32_t> slot(cur.size());
         int32_t off = 0;
         for (std::size_t i = 0; i < cur.size(); ++i) {
@@ -1235,8 +1446,27 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
             f.len   = kChunkG + (x.shmax - x.shmin);
             f.shmax = x.shmax;
             f.hc    = hist_a[x.c];
+            f.delay = x0 ? l0.delay[x.c] : 0;
             slot[i] = off;
             off += f.len;
+            if (x0) {
+                const int32_t c = l0.chan_or_lo[x.c];
+                if (b.nchan == 0 || gc.back().xoff != static_cast<int64_t>(c) * xstride_) {
+                    GatherChan g{};
+                    g.xoff       = static_cast<int64_t>(c) * xstride_;
+                    g.xs         = x.shmax;
+                    g.dmax       = f.delay;
+                    g.leaf_first = static_cast<int32_t>(gl.size());
+                    g.nleaf      = 0;
+                    gc.push_back(g);
+                    ++b.nchan;
+                }
+                auto& g = gc.back();
+                g.xs    = std::max(g.xs, x.shmax);
+                g.dmax  = std::max(g.dmax, f.delay);
+                g.dmask |= 1U << f.delay;
+                ++g.nleaf;
+            }
             gl.push_back(f);
         }
         lds_max = std::max<uint32_t>(lds_max, static_cast<uint32_t>(off));
@@ -1274,7 +1504,7 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
         }
         bool ok          = true;
         const uint64_t n = need(tent, ok);
-        if (r > first && (!ok || n > gather_lds_ || tent.size() > max_leaves ||
+        if (r > first && (!ok || n > gather_lds_ || (!x0 && tent.size() > max_leaves) ||
                           r - first >= 128)) {
             close(r);
             first = r;
@@ -1302,6 +1532,11 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
     ml.blocks.resize(gb.size());
     ml.leaves.resize(gl.size());
     ml.rows.resize(rec.size());
+    if (x0) {
+        ml.chans.resize(gc.size());
+        FA_HIP(hipMemcpy(ml.chans.get(), gc.data(), gc.size() * sizeof(GatherChan),
+                         hipMemcpyHostToDevice));
+    }
     FA_HIP(hipMemcpy(ml.blocks.get(), gb.data(), gb.size() * sizeof(GatherBlock),
                      hipMemcpyHostToDevice));
     FA_HIP(hipMemcpy(ml.leaves.get(), gl.data(), gl.size() * sizeof(GatherLeaf),
@@ -1447,13 +1682,18 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                 for (const auto& m : launches_)
                     if (m.out_level == l) ml = &m;
                 if (!ml) throw std::logic_error("FDMT level without a launch");
-                const float* in = buf_[ml->in_buf].get();
+                const float* in = ml->from_input ? nullptr : buf_[ml->in_buf].get();
+                // Input samples of this tile: columns hist_in_ + t0 + tau,
+                // tau in [xlo, xhi] inside the row.
+                const float* x0 = x_.get() + hist_in_ + t0;
+                const int xlo   = -static_cast<int>(hist_in_ + t0);
+                const int xhi   = static_cast<int>(xstride_) - static_cast<int>(hist_in_ + t0) - 1;
                 if (staged_merge_) {
                     switch (ml->radix) {
-                    case 2: launch_gather<2>(*ml, in, out, tl, s); break;
-                    case 4: launch_gather<4>(*ml, in, out, tl, s); break;
-                    case 8: launch_gather<8>(*ml, in, out, tl, s); break;
-                    case 16: launch_gather<16>(*ml, in, out, tl, s); break;
+                    case 2: launch_gather<2>(*ml, in, out, tl, x0, xlo, xhi, s); break;
+                    case 4: launch_gather<4>(*ml, in, out, tl, x0, xlo, xhi, s); break;
+                    case 8: launch_gather<8>(*ml, in, out, tl, x0, xlo, xhi, s); break;
+                    case 16: launch_gather<16>(*ml, in, out, tl, x0, xlo, xhi, s); break;
                     default: throw std::logic_error("FDMT launch radix must be 2 to 16");
                     }
                 } else {

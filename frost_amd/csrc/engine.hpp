@@ -51,7 +51,9 @@ struct OpenRun {
 struct GatherBlock {
     int32_t leaf_first, nleaves;
     int32_t row_first, nrows;
-    int32_t hmax, pad;
+    int32_t hmax;
+    int32_t chan_first, nchan; // level-0 runs: channel groups of the input rows
+    int32_t pad;
 };
 
 // An input row staged for one row block: LDS [slot, slot + len) holds the
@@ -59,6 +61,21 @@ struct GatherBlock {
 struct GatherLeaf {
     int64_t src; // offset of sample t0 - shmax in the input level buffer
     int32_t slot, len, shmax, hc;
+    int32_t delay; // level-0 input rows computed from the input: the row's delay D
+    int32_t pad;
+};
+
+// Level-0 input rows formed from the input waterfall by the first launch:
+// one channel's rows in a run. Grid element i of the channel's window is
+// tile sample tau0 - xs + i; the window loads samples from tau0 - xs - dmax.
+struct GatherChan {
+    int64_t xoff;       // c * xstride: the channel's input row
+    int32_t xs;         // largest shmax of the channel's input rows in the run
+    int32_t dmax;       // largest delay among them
+    int32_t leaf_first; // the rows: leaves[leaf_first .. + nleaf), delay ascending
+    int32_t nleaf;
+    uint32_t dmask;     // bit D set: a row with delay D is among them
+    int32_t pad;
 };
 
 // One launch computes levels in_level+1 .. out_level (1 to 3 tree levels):
@@ -79,8 +96,10 @@ struct MergeLaunch {
     uint32_t nblocks    = 0;
     uint32_t lds_floats = 0;
     uint32_t max_leaves = 0; // input rows per block bound (16, 32, 48 or 64)
+    bool from_input     = false; // in_level 0 is not stored: producers form it from x
     DeviceBuffer<GatherBlock> blocks;
     DeviceBuffer<GatherLeaf> leaves;
+    DeviceBuffer<GatherChan> chans;
     // Per output row: [out offset of sample t0 (lo, hi), h, 0, K LDS offsets
     // of the descendants at chunk sample 0].
     DeviceBuffer<int32_t> rows;
@@ -158,6 +177,7 @@ private:
     std::vector<int> level_buf_;        // buffer holding a stored level
     uint32_t max_fuse_   = 4;
     bool staged_merge_   = true;
+    bool x0_fused_       = false; // level 0 formed inside the first gather launch
     uint32_t gather_lds_ = 8192; // LDS floats per staged row block (one half)
     void build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) const;
     // Level 0 program: per channel first row, row count, hc, Dmax; per row D,

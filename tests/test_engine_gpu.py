@@ -23,13 +23,14 @@ def stream():
 
 
 def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None,
-             search_path=None, fuse=None, merge=None):
+             search_path=None, fuse=None, merge=None, init=None):
     """Runs the GPU engine and the oracle on the same gulps; returns both
     result lists (per gulp + drain) and, if keep, the plane rows. `fuse` caps
     the FDMT levels summed per launch (default 4); merge="direct" gathers
-    from global memory instead of staging through LDS."""
+    from global memory instead of staging through LDS; init="fused" forms
+    level 0 from the input inside the first launch instead of storing it."""
     env = {"FROST_AMD_TILE": tile, "FROST_AMD_SEARCH": search_path, "FROST_AMD_FUSE": fuse,
-           "FROST_AMD_MERGE": merge}
+           "FROST_AMD_MERGE": merge, "FROST_AMD_INIT": init}
     for k, v in env.items():
         if v:
             os.environ[k] = str(v)
@@ -40,7 +41,7 @@ def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None
         for k in env:
             os.environ.pop(k, None)
     src = gpu.level_sources()
-    stored = [l for l in range(len(src)) if src[l] != -2]
+    stored = [0] + [l for l in range(1, len(src)) if src[l] >= 0]
     gaps = [b - a for a, b in zip(stored, stored[1:])]
     say(f"  stored levels {stored}, levels per launch {gaps}")
     assert max(gaps) <= int(fuse or 4)
@@ -109,12 +110,14 @@ def test_engine_single_gulp_small(oracle, stream):
     assert len(o[0][0]) > 0
 
 
-@pytest.mark.parametrize("depth,path,fuse,merge", [
-    (0, None, None, None), (1, None, None, None), (2, None, None, None), (2, "lds", None, None),
-    (2, None, 1, None), (2, None, 2, None), (2, None, 3, None), (2, None, 3, "direct"),
-    (1, None, 1, "direct")])
-def test_engine_gulps_and_planes(oracle, stream, depth, path, fuse, merge):
-    say(f"test_engine_gulps_and_planes depth={depth} search={path} fuse={fuse} merge={merge}")
+@pytest.mark.parametrize("depth,path,fuse,merge,init", [
+    (0, None, None, None, None), (1, None, None, None, None), (2, None, None, None, None),
+    (2, "lds", None, None, None), (2, None, 1, None, None), (2, None, 2, None, None),
+    (2, None, 3, None, None), (2, None, 3, "direct", None), (1, None, 1, "direct", None),
+    (2, None, None, None, "fused"), (2, None, 2, None, "fused"), (0, None, 1, None, "fused")])
+def test_engine_gulps_and_planes(oracle, stream, depth, path, fuse, merge, init):
+    say(f"test_engine_gulps_and_planes depth={depth} search={path} fuse={fuse} merge={merge} "
+        f"init={init}")
     dcfg = abi.dedisp_config(64, 256e-6, 1500.0, -300 / 64, 500.0, block_nsamps=4096,
                              subband_depth=depth)
     bcfg = abi.boxcar_config(max_boxcar=64, detect_thresh=5.0)
@@ -122,7 +125,8 @@ def test_engine_gulps_and_planes(oracle, stream, depth, path, fuse, merge):
     gulps = [pulse_waterfall(rng, 64, 4096, [(100.0 * (k + 1), 1000 + 500 * k, 8, 1.5),
                                              (320.0, 3900, 3, 3.0)], dcfg)
              for k in range(3)]
-    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, search_path=path, fuse=fuse, merge=merge)
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, search_path=path, fuse=fuse, merge=merge,
+                    init=init)
     assert_same(g, o)
 
 
@@ -186,6 +190,21 @@ def test_engine_tiles_and_short_gulp(oracle, stream):
              pulse_waterfall(rng, 64, 5000, [(600.0, 100, 2, 3.0)], dcfg),
              pulse_waterfall(rng, 64, 700, [], dcfg)]
     g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, tile=1111)
+    assert_same(g, o)
+
+
+@pytest.mark.parametrize("init", [None, "fused"])
+def test_engine_long_channel_delays(oracle, stream, init):
+    """Low band, wide channels: about 40 level-0 rows per channel (delays
+    above the fused form's bound of 16, so it falls back to the init
+    kernel)."""
+    say(f"test_engine_long_channel_delays init={init}")
+    dcfg = abi.dedisp_config(16, 2e-4, 400.0, -6.25, 60.0, block_nsamps=4000)
+    bcfg = abi.boxcar_config(max_boxcar=16, detect_thresh=5.0)
+    rng = np.random.default_rng(23)
+    gulps = [pulse_waterfall(rng, 16, 4000, [(40.0, 300 + 1600 * k, 3, 2.0)], dcfg)
+             for k in range(2)]
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, init=init)
     assert_same(g, o)
 
 
