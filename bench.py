@@ -164,15 +164,20 @@ def main() -> int:
             f"{rep.chans_masked} ch, flagged {rep.samps_flagged} samples")
     total = float(sum(lat))
     value = ndm * W.gulp * a.steps / total
-    # Algorithmic bytes of the dominant launches over one gulp: each input
-    # row read once and each output row written once (float32, per sample).
-    # A launch reading level 0 when level 0 is not stored (src[0] == -2)
-    # forms it from the input waterfall: its input rows are the channels.
-    def in_rows(a):
-        return float(W.nchans) if (a == 0 and src[0] == -2) else float(rows[a])
-    alg_bytes_gulp = 4.0 * W.gulp * float(sum(in_rows(src[l]) + rows[l] for l in r16))
+    # Algorithmic bytes (unfused FDMT): a launch covering levels a -> b does
+    # the work of b - a merge levels, each reading its input rows and writing
+    # its output rows once, float32 per sample: sum_{l=a}^{b-1} 4 N (R_l + R_{l+1}).
+    # A launch that also forms level 0 from the input adds the init pass,
+    # 4 N (N_ch + R_0).
+    def unfused_bytes(a, b):
+        n = sum(rows[l] + rows[l + 1] for l in range(a, b))
+        if a == 0 and src[0] == -2:
+            n += W.nchans + rows[0]
+        return 4.0 * W.gulp * float(n)
+    alg_bytes_gulp = sum(unfused_bytes(int(src[l]), l) for l in r16)
     achieved = alg_bytes_gulp * a.steps / merge_s / 1e9
     launches_per_gulp = merge_launches / a.steps
+    avg_launch_s = merge_s / merge_launches
     traffic = gather_traffic_from_profile()
     out = {
         "metric": "DM-trial·samples/s at 4096 ch, 1 GPU; p99 gulp latency / real-time",
@@ -213,7 +218,13 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "algorithmic_bytes_per_launch": alg_bytes_gulp / launches_per_gulp,
-            "avg_launch_ms": 1e3 * merge_s / merge_launches,
+            "avg_launch_ms": 1e3 * avg_launch_s,
+            # rocprof-measured HBM bytes per launch over the live launch time.
+            "measured_gbs": None if traffic is None else traffic / avg_launch_s / 1e9,
+            "measured_frac": None if traffic is None else traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS,
+            "note": "achieved counts the unfused per-level FDMT bytes of the levels each "
+                    "launch covers (fused launches move fewer, so frac may exceed 1); "
+                    "measured_gbs is the counted HBM traffic",
         },
     }
     if a.cpu_sample > 0:

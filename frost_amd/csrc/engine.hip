@@ -171,6 +171,33 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Raw buffer access (gfx9 resource word 3): byte offsets at or past
+// 4 * nfloats, negative ones included, read 0 and drop stores. The range
+// check applies to the per-lane (VGPR) offset only, not to an instruction
+// immediate, so every offset is passed whole in a VGPR (made opaque so the
+// compiler cannot split a constant part into the immediate).
+constexpr int kBufWord3 = 0x00020000;
+
+__device__ __forceinline__ int whole_offset(int off) {
+    asm volatile("" : "+v"(off));
+    return off;
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const float* base, int nfloats) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, nfloats * 4,
+                                             kBufWord3);
+}
+
+__device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t rs, int byte_off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, whole_offset(byte_off),
+                                                                          0, 0));
+}
+
+__device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t rs, int byte_off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), rs, whole_offset(byte_off), 0,
+                                          0);
+}
+
 __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
@@ -321,11 +348,10 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
         }
     } else if (wave < 4) {
         // Producer: two chunks of loads in flight (register sets A and B).
-        // A window inside the input row's samples loads from a scalar base
-        // with lane offsets; edge windows load at clamped in-range addresses
-        // and zero the samples outside the row when staged. Both forms issue
-        // the same number of loads, so waiting for set A leaves set B in
-        // flight.
+        // Each input row is read through a buffer resource bounded to its
+        // samples [t0 - hc, t0 + tile): loads outside it return 0, which is
+        // the value staged for samples outside the row. The load count is
+        // static, so waiting for set A leaves set B in flight.
         float va[UMAX][M], vb[UMAX][M];
         auto load = [&](float (&v)[UMAX][M], int c) {
             const int tau0 = tau_first + c * kChunkG;
@@ -333,45 +359,23 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
             for (int u = 0; u < UMAX; ++u) {
                 const int sidx      = min(wave + 4 * u, B.nleaves - 1);
                 const GatherLeaf lf = leaves[B.leaf_first + sidx];
-                const int tp0       = tau0 - lf.shmax; // sample of window element 0
-                if (tp0 >= -lf.hc && tp0 + M * 64 <= g.tile_len) {
-                    const float* src = in + (lf.src + tau0);
+                const auto rs       = row_rsrc(in + (lf.src + lf.shmax - lf.hc), lf.hc + g.tile_len);
+                const int off0      = (tau0 - lf.shmax + lf.hc + lane) * 4;
 #pragma unroll
-                    for (int m = 0; m < M; ++m) v[u][m] = src[lane + 64 * m];
-                } else {
-                    const float* row0 = in + lf.src + lf.shmax; // sample t0 of the input row
-#pragma unroll
-                    for (int m = 0; m < M; ++m) {
-                        const int tp  = tp0 + lane + 64 * m;
-                        const int tpc = min(max(tp, -lf.hc), g.tile_len - 1);
-                        v[u][m]       = row0[tpc];
-                    }
-                }
+                for (int m = 0; m < M; ++m) v[u][m] = buffer_load(rs, off0 + 256 * m);
             }
         };
         auto stage = [&](const float (&v)[UMAX][M], int c) {
-            const int tau0 = tau_first + c * kChunkG;
-            float* buf     = lds + (c & 1) * g.half;
+            float* buf = lds + (c & 1) * g.half;
 #pragma unroll
             for (int u = 0; u < UMAX; ++u) {
                 const int sidx = wave + 4 * u;
                 if (sidx < B.nleaves) {
                     const GatherLeaf lf = leaves[B.leaf_first + sidx];
-                    const int tp0       = tau0 - lf.shmax;
                     float* dst          = buf + lf.slot + lane;
-                    if (tp0 >= -lf.hc && tp0 + M * 64 <= g.tile_len) {
 #pragma unroll
-                        for (int m = 0; m < M; ++m)
-                            if (m < M - 1 || lane + 64 * m < lf.len) dst[64 * m] = v[u][m];
-                    } else {
-#pragma unroll
-                        for (int m = 0; m < M; ++m) {
-                            const int j  = lane + 64 * m;
-                            const int tp = tp0 + j;
-                            if (j < lf.len)
-                                dst[64 * m] = (tp >= -lf.hc && tp < g.tile_len) ? v[u][m] : 0.0f;
-                        }
-                    }
+                    for (int m = 0; m < M; ++m)
+                        if (m < M - 1 || lane + 64 * m < lf.len) dst[64 * m] = v[u][m];
                 }
             }
         };
@@ -390,7 +394,11 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
     } else {
         // Consumer: wave 4 + q sums rows q, q + 4, ... of the run; lane l
         // covers samples l, l + 64, l + 128, l + 192 of the chunk, so one
-        // LDS address per (row, descendant) serves four reads.
+        // LDS address per (row, descendant) serves four reads. Stores go
+        // through a buffer resource bounded to the row's samples
+        // [t0 - h, t0 + tile), so samples outside it are dropped without
+        // branches. The next row's record is requested before this row's
+        // LDS reads.
         // Row record: [out lo, out hi, h, pad, delta_0 .. delta_{K-1}].
         constexpr int RS = 4 + K;
         const int q      = wave - 4;
@@ -399,10 +407,17 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
             lds_barrier();
             const float* buf = lds + (c & 1) * g.half + lane;
             const int tau0   = tau_first + c * kChunkG + lane;
-            for (int r = B.row_first + q; r < r_end; r += 4) {
-                int32_t rec[RS];
+            int r            = B.row_first + q;
+            if (r >= r_end) continue;
+            int32_t rec[RS], nxt[RS];
 #pragma unroll
-                for (int j = 0; j < RS; ++j) rec[j] = rows[(int64_t)r * RS + j];
+            for (int j = 0; j < RS; ++j) rec[j] = rows[(int64_t)r * RS + j];
+            for (; r < r_end; r += 4) {
+                const bool more = r + 4 < r_end;
+                if (more) {
+#pragma unroll
+                    for (int j = 0; j < RS; ++j) nxt[j] = rows[(int64_t)(r + 4) * RS + j];
+                }
                 float w[4][K];
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
@@ -411,11 +426,14 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                     for (int s4 = 0; s4 < 4; ++s4) w[s4][j] = p[64 * s4];
                 }
                 const int64_t o = (int64_t)(uint32_t)rec[0] | ((int64_t)rec[1] << 32);
-                float* dst      = out + o;
+                const int h     = rec[2];
+                const auto rs   = row_rsrc(out + (o - h), h + g.tile_len);
+                const int off0  = (tau0 + h) * 4;
 #pragma unroll
-                for (int s4 = 0; s4 < 4; ++s4) {
-                    const int tau = tau0 + 64 * s4;
-                    if (tau < g.tile_len && tau >= -rec[2]) dst[tau] = tree_sum<K>(w[s4]);
+                for (int s4 = 0; s4 < 4; ++s4) buffer_store(rs, off0 + 256 * s4, tree_sum<K>(w[s4]));
+                if (more) {
+#pragma unroll
+                    for (int j = 0; j < RS; ++j) rec[j] = nxt[j];
                 }
             }
         }
