@@ -18,6 +18,8 @@ namespace {
 
 constexpr int kBlock       = 256;
 constexpr int kInitPer     = 8;    // outputs per thread per init block
+constexpr uint64_t kTileMax = uint64_t{1} << 20; // automatic time tile bounds
+constexpr uint64_t kTileMin = uint64_t{1} << 14;
 constexpr int kChunk       = 1024; // search samples per LDS pass
 constexpr uint32_t kMaxBox = 1024; // ladder fits LDS up to this width
 
@@ -1086,12 +1088,13 @@ double trapezoid_window_variance(int64_t a, int64_t W) {
     return total;
 }
 
+// FROST_AMD_TILE overrides the time tile; 0 means choose automatically.
 uint64_t env_tile() {
     if (const char* v = std::getenv("FROST_AMD_TILE")) {
         const long long t = std::atoll(v);
         if (t > 0) return static_cast<uint64_t>(t);
     }
-    return uint64_t{1} << 18;
+    return 0;
 }
 
 uint64_t round64(uint64_t n) { return (n + 63) & ~uint64_t{63}; }
@@ -1124,7 +1127,8 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     const char* sv = std::getenv("FROST_AMD_SEARCH");
     reg_search_ = pow2_widths_ && widths_.back() <= 256 && !(sv && std::string(sv) == "lds");
     cap_ = plan.cfg.block_nsamps;
-    tile_ = std::min<uint64_t>(cap_, tile ? tile : env_tile());
+    const uint64_t tile_req = tile ? tile : env_tile();
+    tile_ = std::min<uint64_t>(cap_, tile_req ? tile_req : kTileMax);
     const uint64_t nch = plan.cfg.nchans;
     ctor_mask_.assign(nch, 1);
     if (kill_mask) ctor_mask_.assign(kill_mask, kill_mask + nch);
@@ -1149,6 +1153,34 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
 
     DeviceGuard dg(device_);
     build_programs();
+    if (!tile_req) {
+        // Automatic tile: the largest power of two up to kTileMax (and the
+        // gulp capacity) whose input and two level buffers fit in 75 % of
+        // the device memory free now. Fewer, longer tiles recompute less
+        // look-back and launch fewer kernels per gulp.
+        size_t free_b = 0, total_b = 0;
+        FA_HIP(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t budget = free_b / 4 * 3;
+        const uint64_t xbytes = nch * round64(hist_in_ + cap_) * sizeof(float);
+        uint64_t pick         = std::min<uint64_t>(cap_, kTileMin);
+        for (uint64_t t = kTileMax; t >= kTileMin; t >>= 1) {
+            const uint64_t tt = std::min<uint64_t>(t, cap_);
+            uint64_t lmax     = 0;
+            for (uint32_t l = 0; l < plan_.nlevels; ++l) {
+                uint64_t len = 0;
+                for (const int32_t h : hist_[l]) len += round64(tt + static_cast<uint64_t>(h));
+                lmax = std::max(lmax, len);
+            }
+            if (xbytes + 2 * lmax * sizeof(float) <= budget) {
+                pick = tt;
+                break;
+            }
+        }
+        if (pick != tile_) {
+            tile_ = pick;
+            build_programs();
+        }
+    }
 
     // Extended input, zero history.
     xstride_ = round64(hist_in_ + cap_);
