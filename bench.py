@@ -141,6 +141,9 @@ def main() -> int:
         log(f"warmup {i}: {beam.candidate_count()} candidates")
     lat, merge_s, merge_launches, ncand = [], 0.0, 0, 0
     stage = {"rfi": 0.0, "init": 0.0, "merge": 0.0, "search": 0.0}
+    rfi_parts = {"bandpass": 0.0, "moments_and_mask": 0.0, "scans": 0.0, "flags": 0.0,
+                 "apply": 0.0}
+    collect_s, raw_cands = 0.0, 0
     stream.synchronize()
     for i in range(a.steps):
         # One step: RFI + FDMT + search + candidate rows on the host. The
@@ -156,8 +159,14 @@ def main() -> int:
         merge_s += float(sec[r16].sum())
         merge_launches += int(cnt[r16].sum())
         tm = beam.engine_timings()
+        cs, rc = beam.host_stats()
+        collect_s += cs
+        raw_cands += rc
         rep = beam.rfi_report()
         stage["rfi"] += rep.t_bandpass + rep.t_moments + rep.t_scan + rep.t_flags + rep.t_apply
+        for k, v in (("bandpass", rep.t_bandpass), ("moments_and_mask", rep.t_moments),
+                     ("scans", rep.t_scan), ("flags", rep.t_flags), ("apply", rep.t_apply)):
+            rfi_parts[k] += v
         for k in ("init", "merge", "search"):
             stage[k] += tm[k]
         log(f"step {i}: {lat[-1]:.3f} s, {nc} candidates, masked "
@@ -208,6 +217,9 @@ def main() -> int:
             "realtime_beams": W.gulp_seconds() / float(np.mean(lat)),
         },
         "stage_seconds_per_gulp": {k: v / a.steps for k, v in stage.items()},
+        "rfi_seconds_per_gulp": {k: v / a.steps for k, v in rfi_parts.items()},
+        "candidates": {"emitted_per_gulp": raw_cands / a.steps,
+                       "host_collect_seconds_per_gulp": collect_s / a.steps},
         "roofline": {
             "kernel": "fdmt_gather_kernel<16> (FDMT levels l-4 -> l in one pass)",
             "levels": [[int(src[l]), int(l)] for l in r16],

@@ -395,4 +395,13 @@ int fa_engine_level_sources(const fa_engine* engine, int32_t* source, uint64_t c
     });
 }
 
+int fa_engine_host_stats(const fa_engine* engine, double* collect_seconds,
+                         uint64_t* raw_candidates) {
+    return guard([&] {
+        require(engine != nullptr, "engine must not be null");
+        if (collect_seconds) *collect_seconds = engine->eng->collect_seconds();
+        if (raw_candidates) *raw_candidates = engine->eng->raw_candidates();
+    });
+}
+
 } // extern "C"

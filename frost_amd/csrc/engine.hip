@@ -3,6 +3,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <array>
 #include <cmath>
 #include <cstdlib>
@@ -1880,23 +1881,58 @@ void Engine::collect(hipStream_t s) {
     DeviceGuard dg(device_);
     FA_HIP(hipStreamSynchronize(s));
     if (!pending_collect_) return; // the last call's view stays valid
+    const auto h0    = std::chrono::steady_clock::now();
     pending_collect_ = false;
     result_.clear();
     dropped_global_ = dropped_per_dm_ = 0;
     uint64_t ctr[2];
     FA_HIP(hipMemcpy(ctr, counters_.get(), sizeof(ctr), hipMemcpyDeviceToHost));
     frontier_             = ctr[1];
+    raw_candidates_       = ctr[0];
     const uint64_t stored = std::min(ctr[0], cand_cap_);
     dropped_global_       = ctr[0] - stored;
-    std::vector<CandRecord> recs(stored);
+    recs_.resize(stored);
     if (stored)
-        FA_HIP(hipMemcpy(recs.data(), cands_.get(), stored * sizeof(CandRecord),
+        FA_HIP(hipMemcpy(recs_.data(), cands_.get(), stored * sizeof(CandRecord),
                          hipMemcpyDeviceToHost));
-    result_.reserve(stored);
-    for (const auto& r : recs) {
-        const auto it = std::upper_bound(plane_first_.begin(), plane_first_.end(), r.prow);
-        const auto p  = static_cast<std::size_t>(it - plane_first_.begin()) - 1;
-        const auto& pd = plan_.planes[p];
+    // Order of frost's caps: (subband level, subband idx, dm idx, filter idx,
+    // peak). Planes are numbered by (depth, idx) and a plane row is
+    // plane_first + dm idx, so (plane row, width, peak) is the same order;
+    // it is unique per call, so the order is deterministic. Sort the compact
+    // records, apply the caps, and convert only the rows kept.
+    // Counting pass on the plane row, then a small sort inside each row.
+    row_start_.assign(static_cast<std::size_t>(nprows_) + 1, 0);
+    for (const auto& r : recs_) ++row_start_[r.prow + 1];
+    for (std::size_t i = 1; i < row_start_.size(); ++i) row_start_[i] += row_start_[i - 1];
+    sorted_.resize(recs_.size());
+    {
+        std::vector<uint64_t> fill(row_start_.begin(), row_start_.end() - 1);
+        for (const auto& r : recs_) sorted_[fill[r.prow]++] = r;
+    }
+    for (std::size_t i = 0; i + 1 < row_start_.size(); ++i)
+        if (row_start_[i + 1] - row_start_[i] > 1)
+            std::sort(sorted_.begin() + static_cast<std::ptrdiff_t>(row_start_[i]),
+                      sorted_.begin() + static_cast<std::ptrdiff_t>(row_start_[i + 1]),
+                      [](const CandRecord& a, const CandRecord& b) {
+                          return a.w != b.w ? a.w < b.w : a.peak < b.peak;
+                      });
+    recs_.swap(sorted_);
+    result_.reserve(std::min<uint64_t>(stored, cfg_.max_candidates));
+    uint64_t run = 0;
+    for (std::size_t k = 0; k < recs_.size(); ++k) {
+        const CandRecord& r = recs_[k];
+        run                 = (k > 0 && r.prow == recs_[k - 1].prow) ? run + 1 : 0;
+        if (cfg_.max_cands_per_dm > 0 && run >= cfg_.max_cands_per_dm) {
+            ++dropped_per_dm_;
+            continue;
+        }
+        if (result_.size() >= cfg_.max_candidates) {
+            ++dropped_global_;
+            continue;
+        }
+        const auto it    = std::upper_bound(plane_first_.begin(), plane_first_.end(), r.prow);
+        const auto p     = static_cast<std::size_t>(it - plane_first_.begin()) - 1;
+        const auto& pd   = plan_.planes[p];
         const uint32_t i = r.prow - plane_first_[p];
         fa_raw_candidate c{};
         c.peak_sample   = r.peak;
@@ -1917,10 +1953,8 @@ void Engine::collect(hipStream_t s) {
         c.beam_idx      = 0;
         result_.push_back(c);
     }
-    uint64_t dg2 = 0, dd2 = 0;
-    finalize(result_, cfg_, dg2, dd2);
-    dropped_global_ += dg2;
-    dropped_per_dm_ = dd2;
+    collect_seconds_ =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
     timing_collect();
 }
 

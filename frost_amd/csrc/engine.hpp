@@ -151,6 +151,9 @@ public:
     std::vector<double> level_seconds, level_launches;
     double search_seconds = 0, total_seconds = 0;
     uint64_t tile() const { return tile_; }
+    // Host side of the last collect: seconds, and candidates the search emitted.
+    double collect_seconds() const { return collect_seconds_; }
+    uint64_t raw_candidates() const { return raw_candidates_; }
     const std::vector<int32_t>& level_source() const { return level_source_; }
     uint64_t hist_in() const { return hist_in_; }
 
@@ -210,6 +213,10 @@ private:
     bool pending_collect_ = false;
 
     std::vector<fa_raw_candidate> result_;
+    std::vector<CandRecord> recs_, sorted_;
+    std::vector<uint64_t> row_start_;
+    uint64_t raw_candidates_ = 0;
+    double collect_seconds_  = 0;
     uint64_t dropped_global_ = 0, dropped_per_dm_ = 0, frontier_ = 0;
 
     // Timing events: pairs per launch, tagged with a level (-1 = search).

@@ -12,6 +12,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 namespace fa {
 
@@ -1563,14 +1564,17 @@ void DeviceRfi::build_flags(hipStream_t s) {
     report_.chans_dead   = dead;
     report_.chans_masked = static_cast<uint64_t>(std::count(mask_.begin(), mask_.end(), 0));
     if (good_any) {
+        // The dropout median is independent of the broad-band test: computed
+        // on a second thread meanwhile; flags are applied in the CPU order.
+        float med_std = 0.0f;
+        std::thread med;
+        if (cfg_.dropout_enabled) med = std::thread([&] { med_std = kth_smallest(rstd, nsamps / 2); });
         if (cfg_.broad_enabled)
             report_.samps_broad = broadband_flags(sdet, smooth_radius_, cfg_.broad_threshold,
                                                   cfg_.broad_boxcar_max, time_flags_host_.data());
+        if (med.joinable()) med.join();
         if (cfg_.dropout_enabled) {
-            std::vector<float> spread = rstd;
-            std::nth_element(spread.begin(), spread.begin() + static_cast<std::ptrdiff_t>(nsamps / 2),
-                             spread.end());
-            const float cut = cfg_.dropout_min_std_fraction * spread[nsamps / 2];
+            const float cut = cfg_.dropout_min_std_fraction * med_std;
             std::vector<uint8_t> drop(nsamps, 0);
             for (uint64_t t = 0; t < nsamps; ++t)
                 if (rstd[t] < cut) {

@@ -122,6 +122,8 @@ private:
 // Host-side helpers shared with tests of the flag logic.
 std::vector<uint8_t> iqrm_flags(const std::vector<float>& x, uint64_t radius, float threshold,
                                 const std::vector<uint8_t>& ignore);
+// The value std::nth_element places at k (an exact order statistic).
+float kth_smallest(std::vector<float> v, std::size_t k);
 uint64_t broadband_flags(std::vector<float> series, uint64_t smooth_radius, float threshold,
                          uint64_t boxcar_max, uint8_t* flags);
 

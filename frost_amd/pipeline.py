@@ -79,6 +79,12 @@ class Beam:
     def level_sources(self):
         return search.engine_level_sources(self.engine_handle())
 
+    def host_stats(self):
+        """(collect seconds, candidates emitted before caps) of the last gulp."""
+        sec, n = C.c_double(), C.c_uint64()
+        check(lib().fa_engine_host_stats(self.engine_handle(), C.byref(sec), C.byref(n)))
+        return sec.value, n.value
+
     def level_times(self):
         e = self.engine_handle()
         nl, tile = C.c_uint64(), C.c_uint64()

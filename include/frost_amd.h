@@ -340,6 +340,12 @@ int fa_engine_level_times(const fa_engine* engine, double* seconds,
 int fa_engine_level_sources(const fa_engine* engine, int32_t* source, uint64_t cap,
                             uint64_t* nlevels);
 
+/* Host side of the last copy_result: wall seconds spent sorting, capping and
+ * converting candidates, and the candidates the search emitted before caps.
+ * No reference counterpart (diagnostics). */
+int fa_engine_host_stats(const fa_engine* engine, double* collect_seconds,
+                         uint64_t* raw_candidates);
+
 /* ---- device RFI (RFICleaner::execute_normalized / DeviceRfi) ------------- */
 
 /* RFICleaner(cfg, layout, baseline_length) (include/frost/prepare/rfi.hpp:260)
