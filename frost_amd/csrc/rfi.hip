@@ -1151,6 +1151,9 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
     }
     kseg_.resize(4 * nch_);
     counters_.resize(8);
+    moments_pin_.reserve(maxwin * nch_);
+    series_pin_.reserve(2 * layout.max_nsamps);
+    whole_pin_.reserve(layout.max_nsamps);
     slow_tiles_.resize(2 * ((layout.max_nsamps + 63) / 64) * ((layout.nchans + 63) / 64));
     tail_loc_.resize(nch_);
     tail_scale_.resize(nch_);
@@ -1252,13 +1255,13 @@ void DeviceRfi::build_channel_mask(hipStream_t s) {
     const uint64_t n = nch_, nw = nwin_;
     std::vector<uint8_t> dyn(n, 0);
     if (need_moments_) {
-        moments_host_.resize(nw * n);
-        FA_HIP(hipMemcpyAsync(moments_host_.data(), moments_.get(), nw * n * sizeof(MomentSumsDev),
+        FA_HIP(hipMemcpyAsync(moments_pin_.get(), moments_.get(), nw * n * sizeof(MomentSumsDev),
                               hipMemcpyDeviceToHost, s));
         FA_HIP(hipStreamSynchronize(s));
+        const MomentSumsDev* mh = moments_pin_.get(); // pinned staging
         std::vector<float> cm(nw * n, 0.0f), ci(nw * n, 0.0f);
         for (uint64_t i = 0; i < nw * n; ++i) {
-            const auto& m = moments_host_[i];
+            const auto& m = mh[i];
             if (m.nc > 1.0) {
                 const double mean = m.c1 / m.nc;
                 const double var  = (m.c2 / m.nc) - (mean * mean);
@@ -1275,7 +1278,7 @@ void DeviceRfi::build_channel_mask(hipStream_t s) {
                 if (mask_[c] == 0) continue;
                 double sn = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, sl = 0, pairs = 0;
                 for (uint64_t w = 0; w < nw; ++w) {
-                    const auto& m = moments_host_[w * n + c];
+                    const auto& m = mh[w * n + c];
                     sn += m.n;
                     s1 += m.s1;
                     s2 += m.s2;
@@ -1333,7 +1336,7 @@ void DeviceRfi::build_channel_mask(hipStream_t s) {
                 ignore[c]     = static_cast<uint8_t>(mask_[c] == 0 || unmeasured_[c] > 0);
                 double weight = 0.0, acc = 0.0;
                 for (uint64_t w = 0; w < nw; ++w) {
-                    const auto& m = moments_host_[w * n + c];
+                    const auto& m = mh[w * n + c];
                     if (m.n > 1.0) {
                         const double mean = m.s1 / m.n;
                         const double var  = (m.s2 / m.n) - (mean * mean);
@@ -1543,26 +1546,29 @@ void DeviceRfi::build_flags(hipStream_t s) {
         FA_HIP(hipGetLastError());
     }
     time_flags_host_.assign(nsamps, 0);
-    std::vector<float> sdet, rstd;
-    std::vector<uint8_t> whole;
-    if (good_any && cfg_.broad_enabled) {
-        sdet.resize(nsamps);
-        FA_HIP(hipMemcpyAsync(sdet.data(), s_det_.get(), nsamps * 4, hipMemcpyDeviceToHost, s));
-    }
-    if (good_any && cfg_.dropout_enabled) {
-        rstd.resize(nsamps);
-        FA_HIP(hipMemcpyAsync(rstd.data(), row_std_.get(), nsamps * 4, hipMemcpyDeviceToHost, s));
-    }
-    if (good_any && cfg_.kadane_spectrum_enabled) {
-        whole.resize(nsamps);
-        FA_HIP(hipMemcpyAsync(whole.data(), kwhole_.get(), nsamps, hipMemcpyDeviceToHost, s));
-    }
+    // Per-sample series through pinned staging (s_det, row_std, whole-band).
+    float* sdet_pin  = series_pin_.get();
+    float* rstd_pin  = series_pin_.get() + nsamps;
+    uint8_t* whole_p = whole_pin_.get();
+    if (good_any && cfg_.broad_enabled)
+        FA_HIP(hipMemcpyAsync(sdet_pin, s_det_.get(), nsamps * 4, hipMemcpyDeviceToHost, s));
+    if (good_any && cfg_.dropout_enabled)
+        FA_HIP(hipMemcpyAsync(rstd_pin, row_std_.get(), nsamps * 4, hipMemcpyDeviceToHost, s));
+    if (good_any && cfg_.kadane_spectrum_enabled)
+        FA_HIP(hipMemcpyAsync(whole_p, kwhole_.get(), nsamps, hipMemcpyDeviceToHost, s));
     unsigned long long dead = 0;
     FA_HIP(hipMemcpyAsync(&dead, counters_.get() + 4, 8, hipMemcpyDeviceToHost, s));
     FA_HIP(hipMemcpyAsync(mask_.data(), mask_dev_.get(), n, hipMemcpyDeviceToHost, s));
     FA_HIP(hipStreamSynchronize(s));
     report_.chans_dead   = dead;
     report_.chans_masked = static_cast<uint64_t>(std::count(mask_.begin(), mask_.end(), 0));
+    const std::vector<float> sdet =
+        good_any && cfg_.broad_enabled ? std::vector<float>(sdet_pin, sdet_pin + nsamps)
+                                       : std::vector<float>();
+    const std::vector<float> rstd =
+        good_any && cfg_.dropout_enabled ? std::vector<float>(rstd_pin, rstd_pin + nsamps)
+                                         : std::vector<float>();
+    const uint8_t* whole = whole_p;
     if (good_any) {
         // The dropout median is independent of the broad-band test: computed
         // on a second thread meanwhile; flags are applied in the CPU order.

@@ -115,7 +115,9 @@ private:
     std::vector<float> scale_host_;
     std::vector<uint8_t> measured_host_;
     std::vector<uint64_t> unmeasured_;
-    std::vector<MomentSumsDev> moments_host_;
+    PinnedBuffer<MomentSumsDev> moments_pin_; // staging of the per-window moment sums
+    PinnedBuffer<float> series_pin_;          // s_det, row_std
+    PinnedBuffer<uint8_t> whole_pin_;         // whole-band Kadane rows
     hipEvent_t ev_[6] = {};
 };
 

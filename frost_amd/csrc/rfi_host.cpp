@@ -195,34 +195,47 @@ std::vector<uint8_t> iqrm_flags(const std::vector<float>& x, uint64_t radius, fl
     if (n == 0) return out;
     auto skip = [&](std::size_t i) { return !ignore.empty() && ignore[i] != 0; };
     std::vector<std::vector<uint32_t>> casts(n), got(n);
-    std::vector<double> d(n), vals;
-    std::vector<uint8_t> ok(n);
     const auto last = static_cast<int64_t>(n) - 1;
-    uint64_t lag    = 1;
-    while (lag <= radius) {
-        for (const int64_t k : {static_cast<int64_t>(lag), -static_cast<int64_t>(lag)}) {
-            vals.clear();
-            for (int64_t i = 0; i <= last; ++i) {
-                const auto j = static_cast<std::size_t>(std::min(std::max<int64_t>(i - k, 0), last));
-                ok[i] = static_cast<uint8_t>(!skip(i) && !skip(j));
-                d[i]  = static_cast<double>(x[i]) - static_cast<double>(x[j]);
-                if (ok[i]) vals.push_back(d[i]);
-            }
-            if (vals.empty()) continue;
-            std::sort(vals.begin(), vals.end());
-            const double q1 = lin_quantile(vals, 0.25), q2 = lin_quantile(vals, 0.5),
-                         q3 = lin_quantile(vals, 0.75);
-            const double cut = static_cast<double>(threshold) * (q3 - q1) / 1.349;
-            for (int64_t i = 0; i <= last; ++i) {
-                if (!ok[i] || !(d[i] - q2 > cut)) continue;
-                const auto j = static_cast<std::size_t>(std::min(std::max<int64_t>(i - k, 0), last));
-                casts[j].push_back(static_cast<uint32_t>(i));
-                got[i].push_back(static_cast<uint32_t>(j));
-            }
-        }
+    // Signed lags 1, then max(lag + lag/2, lag + 1) up to the radius.
+    std::vector<int64_t> lags;
+    for (uint64_t lag = 1; lag <= radius;) {
+        lags.push_back(static_cast<int64_t>(lag));
+        lags.push_back(-static_cast<int64_t>(lag));
         if (lag >= n - 1) break;
         lag = std::max(lag + lag / 2, lag + 1);
     }
+    // Each signed lag casts its votes independently; the votes are merged
+    // into sets afterwards, so the split over threads does not matter.
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> votes(lags.size());
+    auto one_lag = [&](std::size_t li) {
+        const int64_t k = lags[li];
+        std::vector<double> d(n), vals;
+        std::vector<uint8_t> ok(n);
+        for (int64_t i = 0; i <= last; ++i) {
+            const auto j = static_cast<std::size_t>(std::min(std::max<int64_t>(i - k, 0), last));
+            ok[i] = static_cast<uint8_t>(!skip(i) && !skip(j));
+            d[i]  = static_cast<double>(x[i]) - static_cast<double>(x[j]);
+            if (ok[i]) vals.push_back(d[i]);
+        }
+        if (vals.empty()) return;
+        std::sort(vals.begin(), vals.end());
+        const double q1 = lin_quantile(vals, 0.25), q2 = lin_quantile(vals, 0.5),
+                     q3 = lin_quantile(vals, 0.75);
+        const double cut = static_cast<double>(threshold) * (q3 - q1) / 1.349;
+        for (int64_t i = 0; i <= last; ++i) {
+            if (!ok[i] || !(d[i] - q2 > cut)) continue;
+            const auto j = static_cast<std::size_t>(std::min(std::max<int64_t>(i - k, 0), last));
+            votes[li].emplace_back(static_cast<uint32_t>(i), static_cast<uint32_t>(j));
+        }
+    };
+    parallel_for(lags.size(), [&](std::size_t lo, std::size_t hi) {
+        for (std::size_t li = lo; li < hi; ++li) one_lag(li);
+    }, 1);
+    for (const auto& vl : votes)
+        for (const auto& [i, j] : vl) {
+            casts[j].push_back(i);
+            got[i].push_back(j);
+        }
     for (auto& v : casts) {
         std::sort(v.begin(), v.end());
         v.erase(std::unique(v.begin(), v.end()), v.end());
