@@ -43,15 +43,19 @@ fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
         const int rf = chan[4 * c], nr = chan[4 * c + 1], hc = chan[4 * c + 2],
                   dm = chan[4 * c + 3];
         const int total = tile_len + hc;
-        const int i0    = blockIdx.x * (kBlock * kInitPer);
+        // Blocks start at indices congruent to hc mod 64, so each wave's
+        // stores cover whole 256-byte lines of the rows (sample 0 of a row is
+        // 64-float aligned); block 0 starts up to 64 indices early.
+        const int i0    = blockIdx.x * (kBlock * kInitPer) + (hc & 63) - 64;
         if (i0 >= total) continue;
-        // xc[i] is the sample at t0 - hc + i.
+        // xc[i] is the sample at t0 - hc + i; indices from -dm are held.
         const float* xc = x + (uint64_t)c * xstride + (xcol0 - hc);
         const int span  = kBlock * kInitPer + dm;
         __syncthreads();
         for (int j = threadIdx.x; j < span; j += kBlock) {
             const int gi = i0 - dm + j;
-            sx[j]        = gi < total ? xc[gi] : 0.0f;
+            const float v = xc[min(max(gi, -dm), total - 1)];
+            sx[j]        = gi >= -dm && gi < total ? v : 0.0f;
         }
         float acc[kInitPer];
         int k = 0;
@@ -180,6 +184,7 @@ __device__ __forceinline__ void wave_sync() {
 // immediate, so every offset is passed whole in a VGPR (made opaque so the
 // compiler cannot split a constant part into the immediate).
 constexpr int kBufWord3 = 0x00020000;
+constexpr int kBufNone  = INT32_MIN; // an offset outside every buffer (reads 0, no fetch)
 
 __device__ __forceinline__ int whole_offset(int off) {
     asm volatile("" : "+v"(off));
@@ -365,7 +370,11 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                 const auto rs       = row_rsrc(in + (lf.src + lf.shmax - lf.hc), lf.hc + g.tile_len);
                 const int off0      = (tau0 - lf.shmax + lf.hc + lane) * 4;
 #pragma unroll
-                for (int m = 0; m < M; ++m) v[u][m] = buffer_load(rs, off0 + 256 * m);
+                for (int m = 0; m < M - 1; ++m) v[u][m] = buffer_load(rs, off0 + 256 * m);
+                // The last segment only holds the shift spread (len - 256
+                // samples): other lanes read out of range, which fetches nothing.
+                const bool tail    = lane + 64 * (M - 1) < lf.len;
+                v[u][M - 1]        = buffer_load(rs, tail ? off0 + 256 * (M - 1) : kBufNone);
             }
         };
         auto stage = [&](const float (&v)[UMAX][M], int c) {
@@ -446,7 +455,7 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
 template <int K, int UMAX, bool X0>
 void launch_gather_u(const MergeLaunch& ml, const float* in, float* out, const GatherDims& g,
                      int tl, hipStream_t s) {
-    const unsigned ngrp   = ceil_div(tl + ml.hmax, kChunkG * g.chunks);
+    const unsigned ngrp   = ceil_div(tl - g.tau_origin, kChunkG * g.chunks);
     const uint64_t blocks = (uint64_t)((ngrp + 7) / 8) * 8 * ml.nblocks;
     if (blocks * kGatherThreads >= (uint64_t{1} << 32))
         throw std::invalid_argument("FDMT merge grid too large; reduce the time tile");
@@ -473,7 +482,7 @@ void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, c
     GatherDims g{};
     g.nblocks    = static_cast<int>(ml.nblocks);
     g.tile_len   = tl;
-    g.tau_origin = -ml.hmax;
+    g.tau_origin = -((ml.hmax + 63) & ~63); // chunks start on 64-sample boundaries
     g.half       = static_cast<int>(ml.lds_floats);
     g.chunks     = gather_chunks();
     g.x          = x0;
@@ -1100,6 +1109,12 @@ uint64_t env_tile() {
 
 uint64_t round64(uint64_t n) { return (n + 63) & ~uint64_t{63}; }
 
+// Floats taken by a level row of look-back h over a tile: the row's sample 0
+// sits on a 64-float boundary, so the look-back and the tile are each rounded.
+uint64_t row_span(int32_t h, uint64_t tile) {
+    return round64(static_cast<uint64_t>(h)) + round64(tile);
+}
+
 } // namespace
 
 // ---- Engine ------------------------------------------------------------------
@@ -1169,7 +1184,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
             uint64_t lmax     = 0;
             for (uint32_t l = 0; l < plan_.nlevels; ++l) {
                 uint64_t len = 0;
-                for (const int32_t h : hist_[l]) len += round64(tt + static_cast<uint64_t>(h));
+                for (const int32_t h : hist_[l]) len += row_span(h, tt);
                 lmax = std::max(lmax, len);
             }
             if (xbytes + 2 * lmax * sizeof(float) <= budget) {
@@ -1256,9 +1271,11 @@ void Engine::build_programs() {
     level_len_.assign(plan_.nlevels, 0);
     for (uint32_t l = 0; l <= L; ++l) {
         uint64_t off = 0;
+        // Rows start h samples before their sample 0, which is placed on a
+        // 64-float boundary (whole 256-byte lines for every wave's stores).
         for (const int32_t h : hist_[l]) {
-            base_[l].push_back(static_cast<int64_t>(off));
-            off += round64(tile_ + static_cast<uint64_t>(h));
+            base_[l].push_back(static_cast<int64_t>(off + round64(static_cast<uint64_t>(h)) - h));
+            off += row_span(h, tile_);
         }
         level_len_[l] = off;
     }
@@ -1473,7 +1490,32 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
         return n;
     };
     std::vector<GatherChan> gc;
+    // Sub-band (node) of each input row, its largest delay, and a union-find
+    // over nodes that share a block (used for the dispatch order below).
+    const auto& la     = plan_.levels[a];
+    const int32_t nmax = la.node.empty() ? 0 : *std::max_element(la.node.begin(), la.node.end());
+    std::vector<int32_t> ndmax(static_cast<std::size_t>(nmax) + 1, 0), par(ndmax.size());
+    for (std::size_t c = 0; c < la.size(); ++c)
+        ndmax[la.node[c]] = std::max(ndmax[la.node[c]], la.delay[c]);
+    for (std::size_t i = 0; i < par.size(); ++i) par[i] = static_cast<int32_t>(i);
+    auto find = [&](int32_t x) {
+        while (par[x] != x) x = par[x] = par[par[x]];
+        return x;
+    };
+    std::vector<double> bpos; // mean DM position of a block's input rows
+    std::vector<int32_t> bnode;
     auto close = [&](uint32_t end) {
+        {
+            double pos = 0;
+            const int32_t r0 = find(la.node[cur[0].c]);
+            for (const auto& x : cur) {
+                const int32_t nd = la.node[x.c];
+                pos += static_cast<double>(la.delay[x.c]) / (ndmax[nd] + 1.0);
+                par[find(nd)] = r0;
+            }
+            bpos.push_back(pos / static_cast<double>(cur.size()));
+            bnode.push_back(la.node[cur[0].c]);
+        }
         GatherBlock b{};
         b.leaf_first = static_cast<int32_t>(gl.size());
         b.nleaves    = static_cast<int32_t>(cur.size());
@@ -1578,6 +1620,28 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
         hmax = std::max(hmax, h);
     }
     if (ml.nrows > first) close(ml.nrows);
+    {
+        // Dispatch order: each XCD sweeps the block table in order and its L2
+        // keeps the input rows of the blocks in flight. Blocks whose input
+        // sub-bands overlap (tree and staggered sub-band rows of one level)
+        // are interleaved by DM position, so that an input row both read is
+        // still cached at the second read; groups of blocks with disjoint
+        // inputs keep their order (stable sort).
+        std::vector<int32_t> gfirst(par.size(), std::numeric_limits<int32_t>::max());
+        for (std::size_t i = 0; i < gb.size(); ++i) {
+            auto& f = gfirst[find(bnode[i])];
+            f       = std::min(f, static_cast<int32_t>(i));
+        }
+        std::vector<uint32_t> order(gb.size());
+        for (std::size_t i = 0; i < order.size(); ++i) order[i] = static_cast<uint32_t>(i);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) {
+            const int32_t fp = gfirst[find(bnode[p])], fq = gfirst[find(bnode[q])];
+            return fp != fq ? fp < fq : bpos[p] < bpos[q];
+        });
+        std::vector<GatherBlock> sorted(gb.size());
+        for (std::size_t i = 0; i < order.size(); ++i) sorted[i] = gb[order[i]];
+        gb.swap(sorted);
+    }
     ml.nblocks    = static_cast<uint32_t>(gb.size());
     ml.lds_floats = lds_max;
     ml.blocks.resize(gb.size());
@@ -1722,7 +1786,7 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
             if (l == 0) {
                 int hcmax = 0;
                 for (const int32_t h : hist_[0]) hcmax = std::max(hcmax, h);
-                const unsigned gx = ceil_div(tl + hcmax, kBlock * kInitPer);
+                const unsigned gx = ceil_div(tl + hcmax + 64, kBlock * kInitPer);
                 const unsigned gy = static_cast<unsigned>(std::min<uint64_t>(nch, 65535));
                 const size_t lds  = (kBlock * kInitPer + dmax0_) * sizeof(float);
                 fdmt_init_kernel<<<dim3(gx, gy), kBlock, lds, s>>>(
