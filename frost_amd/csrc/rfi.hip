@@ -320,27 +320,6 @@ __device__ __forceinline__ void kadane_init(KadaneDev& k) {
     k.best   = {0.0, 0, -1};
 }
 
-__device__ __forceinline__ void kadane_append(KadaneDev& k, int64_t i, double u) {
-    if (k.empty) {
-        k.empty  = 0;
-        k.first  = i;
-        k.prefix = {0.0, i, i - 1};
-        k.suffix = {0.0, i, i - 1};
-        k.best   = k.suffix;
-    }
-    k.last = i;
-    k.total += u;
-    if (k.total > k.prefix.sum) k.prefix = {k.total, k.first, i};
-    if (k.suffix.sum + u > 0.0) {
-        if (k.suffix.end - k.suffix.start + 1 <= 0) k.suffix.start = i;
-        k.suffix.sum += u;
-        k.suffix.end = i;
-    } else {
-        k.suffix = {0.0, i + 1, i};
-    }
-    if (k.suffix.sum > k.best.sum) k.best = k.suffix;
-}
-
 __device__ KadaneDev kadane_combine(const KadaneDev& a, const KadaneDev& b) {
     if (a.empty) return b;
     if (b.empty) return a;
@@ -365,7 +344,8 @@ __device__ KadaneDev kadane_combine(const KadaneDev& a, const KadaneDev& b) {
 }
 
 // Register form of the append-only Kadane state used in the scans: the same
-// comparisons and FP64 additions as kadane_append, with 32-bit indices; the
+// comparisons and FP64 additions as the CPU cleaner's append
+// (lib/cpu/rfi_kernels.hpp:65-133), with 32-bit indices; the
 // prefix run always starts at `first`, and `last` is the last appended index
 // (set when the state is written out).
 struct KadaneReg {
@@ -382,24 +362,53 @@ __device__ __forceinline__ void kreg_init(KadaneReg& k) {
     k.empty  = true;
 }
 
-__device__ __forceinline__ void kreg_append(KadaneReg& k, int32_t i, double u) {
-    if (k.empty) {
-        k.empty  = false;
-        k.first  = i;
-        k.psum   = 0.0;
-        k.pend   = i - 1;
-        k.ssum   = 0.0;
-        k.sstart = i;
-        k.send   = i - 1;
-        k.bsum   = 0.0;
-        k.bstart = i;
-        k.bend   = i - 1;
-    }
+// The state before the first element, appended at index i.
+__device__ __forceinline__ void kreg_start(KadaneReg& k, int32_t i) {
+    k.empty  = false;
+    k.first  = i;
+    k.psum   = 0.0;
+    k.pend   = i - 1;
+    k.ssum   = 0.0;
+    k.sstart = i;
+    k.send   = i - 1;
+    k.bsum   = 0.0;
+    k.bstart = i;
+    k.bend   = i - 1;
+}
+
+// Appends element i of value u to a started state.
+__device__ __forceinline__ void kreg_extend(KadaneReg& k, int32_t i, double u) {
     k.total += u;
     if (k.total > k.psum) {
         k.psum = k.total;
         k.pend = i;
     }
+    if (k.ssum + u > 0.0) {
+        if (k.send - k.sstart + 1 <= 0) k.sstart = i;
+        k.ssum += u;
+        k.send = i;
+    } else {
+        k.ssum   = 0.0;
+        k.sstart = i + 1;
+        k.send   = i;
+    }
+    if (k.ssum > k.bsum) {
+        k.bsum   = k.ssum;
+        k.bstart = k.sstart;
+        k.bend   = k.send;
+    }
+}
+
+// Best run only (the row scans report no prefix or total). Starting from the
+// empty run at 0 gives the same best run as kreg_start at the first appended
+// index: that append sees an empty suffix either way, and a best run that is
+// never set stays empty, which the caller skips.
+struct KadaneBest {
+    double ssum = 0.0, bsum = 0.0;
+    int32_t sstart = 0, send = -1, bstart = 0, bend = -1;
+};
+
+__device__ __forceinline__ void kbest_append(KadaneBest& k, int32_t i, double u) {
     if (k.ssum + u > 0.0) {
         if (k.send - k.sstart + 1 <= 0) k.sstart = i;
         k.ssum += u;
@@ -448,7 +457,7 @@ struct ScanArgs {
     uint8_t* kwhole;
 };
 
-constexpr int kRowTileCh = 256;
+constexpr int kRowTileCh = 128; // channels per LDS tile (u8: 8.3 KB per wave)
 
 // 64 rows per block; channel tiles transposed through LDS so each lane walks
 // its own row in channel order (lib/cpu/rfi_cpu_measure.cpp:365-415). The
@@ -457,7 +466,8 @@ constexpr int kRowTileCh = 256;
 template <int DT>
 __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
     extern __shared__ uint32_t tile[]; // [64][words + 1]
-    __shared__ float s_lc[2][kRowTileCh], s_ic[2][kRowTileCh];
+    __shared__ __align__(16) float s_lc[2][kRowTileCh];
+    __shared__ __align__(16) float s_ic[2][kRowTileCh];
     __shared__ uint8_t s_mask[kRowTileCh];
     const Geometry& g   = a.g;
     const uint64_t t0   = (uint64_t)blockIdx.x * 64;
@@ -475,28 +485,38 @@ __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
     const uint64_t w      = active ? g.window_of(t) : wa;
     const int set         = w == wa ? 0 : 1;
     const bool aligned    = (g.row_bytes % 4) == 0 && (reinterpret_cast<uintptr_t>(g.block) & 3) == 0;
-    double sum_det = 0, sum_keep = 0, n_keep = 0, sx = 0, sxx = 0, n_meas = 0;
-    KadaneReg sp, sn;
-    kreg_init(sp);
-    kreg_init(sn);
+    // Counts, and for u8 samples the sums of x and x^2, are integers: exact
+    // in integer registers and equal to the FP64 running sums.
+    double sum_det = 0, sum_keep = 0, sx = 0, sxx = 0;
+    int32_t n_keep = 0, n_meas = 0;
+    uint64_t isx = 0, isxx = 0;
+    KadaneBest sp, sn;
     for (uint64_t c0 = 0; c0 < g.nch; c0 += kRowTileCh) {
         const uint64_t nc   = g.nch - c0 < kRowTileCh ? g.nch - c0 : kRowTileCh;
         const uint64_t nbyt = nc * bits / 8;
         __syncthreads();
         if (aligned && nbyt % 4 == 0) {
-            // Whole words: 16 rows of loads in flight (rows past the block
-            // end read the last row and are never used).
-            for (int r0 = 0; r0 < 64; r0 += kRowBatch) {
-                for (int k = lane; k < words; k += 64) {
-                    uint32_t v[kRowBatch];
+            // Whole words, lanes over (row, word) with kRowBatch loads in
+            // flight; rows past the block end read the last row, words past
+            // the tile read its last word (neither is used), zero is stored
+            // past the tile.
+            const int nw    = 64 * words; // words is a power of two
+            const int wlast = (int)(nbyt / 4) - 1;
+            for (int b0 = 0; b0 < nw; b0 += 64 * kRowBatch) {
+                uint32_t v[kRowBatch];
 #pragma unroll
-                    for (int q = 0; q < kRowBatch; ++q) {
-                        const uint64_t tr = min(t0 + r0 + q, g.nsamps - 1);
-                        const uint8_t* rp = g.block + tr * g.row_bytes + c0 * bits / 8;
-                        v[q] = (uint64_t)k * 4 < nbyt ? *reinterpret_cast<const uint32_t*>(rp + (uint64_t)k * 4) : 0U;
-                    }
+                for (int q = 0; q < kRowBatch; ++q) {
+                    const int idx     = min(b0 + lane + 64 * q, nw - 1);
+                    const int r       = idx / words, k = idx - r * words;
+                    const uint64_t tr = min(t0 + r, g.nsamps - 1);
+                    const uint8_t* rp = g.block + tr * g.row_bytes + c0 * bits / 8;
+                    v[q] = *reinterpret_cast<const uint32_t*>(rp + (uint64_t)min(k, wlast) * 4);
+                }
 #pragma unroll
-                    for (int q = 0; q < kRowBatch; ++q) tile[(r0 + q) * pitch + k] = v[q];
+                for (int q = 0; q < kRowBatch; ++q) {
+                    const int idx = b0 + lane + 64 * q;
+                    const int r   = idx / words, k = idx - r * words;
+                    if (idx < nw) tile[r * pitch + k] = k <= wlast ? v[q] : 0U;
                 }
             }
         } else {
@@ -513,46 +533,97 @@ __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
                 }
             }
         }
-        for (uint64_t j = lane; j < nc; j += 64) {
-            s_mask[j] = a.mask[c0 + j];
+        // Staged calibration: the inverse scale is 0 for masked channels, so
+        // one test skips both masked and unmeasured cells.
+        // Entries past nc (to a multiple of 4) are zero: skipped cells.
+        for (uint64_t j = lane; j < ((nc + 3) & ~uint64_t{3}); j += 64) {
+            const uint64_t jj = j < nc ? j : nc - 1;
+            const bool in     = j < nc;
+            s_mask[j]         = in ? a.mask[c0 + jj] : 0;
             if (staged) {
-                s_lc[0][j] = a.loc[wa * g.nch + c0 + j];
-                s_ic[0][j] = a.inv[wa * g.nch + c0 + j];
-                s_lc[1][j] = a.loc[wb * g.nch + c0 + j];
-                s_ic[1][j] = a.inv[wb * g.nch + c0 + j];
+                const bool on = in && a.mask[c0 + jj] != 0;
+                s_lc[0][j]    = a.loc[wa * g.nch + c0 + jj];
+                s_ic[0][j]    = on ? a.inv[wa * g.nch + c0 + jj] : 0.0f;
+                s_lc[1][j]    = a.loc[wb * g.nch + c0 + jj];
+                s_ic[1][j]    = on ? a.inv[wb * g.nch + c0 + jj] : 0.0f;
             }
         }
         __syncthreads();
         if (!active) continue;
         const uint8_t* myrow = reinterpret_cast<const uint8_t*>(tile + lane * pitch);
-        const float* lcp     = staged ? s_lc[set] : a.loc + w * g.nch + c0;
-        const float* icp     = staged ? s_ic[set] : a.inv + w * g.nch + c0;
-        for (uint64_t j = 0; j < nc; ++j) {
-            if (s_mask[j] == 0) continue;
-            const float x = xat<DT>(myrow, j, g.dtype);
-            const float icv = icp[j];
-            if (icv == 0.0f) continue;
-            n_meas += 1.0;
-            const float zc = (x - lcp[j]) * icv;
+        uint32_t tsx = 0, tsxx = 0; // this tile's integer sums (u8: < 2^32)
+        auto cell = [&](uint64_t j, float x, float lc, float icv) {
+            ++n_meas;
+            const float zc = (x - lc) * icv;
             const float cl = zc < -a.clip ? -a.clip : (a.clip < zc ? a.clip : zc);
             sum_det += cl;
             if (fabsf(zc) <= a.clip) {
                 sum_keep += zc;
-                n_keep += 1.0;
+                ++n_keep;
             }
-            sx += x;
-            sxx += (double)x * x;
+            if constexpr (DT == FA_UINT8) {
+                const uint32_t xi = (uint32_t)x;
+                tsx += xi;
+                tsxx += xi * xi;
+            } else {
+                sx += x;
+                sxx += (double)x * x;
+            }
             if (a.kspec) {
                 const int32_t c = (int32_t)(c0 + j);
-                kreg_append(sp, c, cl);
-                kreg_append(sn, c, -cl);
+                kbest_append(sp, c, cl);
+                kbest_append(sn, c, -cl);
+            }
+        };
+        if constexpr (DT == FA_UINT8) {
+            if (staged) {
+                // Four channels per step: one word of samples and four
+                // calibration pairs read together, cells taken in order.
+                const uint32_t* rw = tile + lane * pitch;
+                for (uint64_t j = 0; j < nc; j += 4) {
+                    const uint32_t xw = rw[j / 4];
+                    const float4 lc4  = *reinterpret_cast<const float4*>(&s_lc[set][j]);
+                    const float4 ic4  = *reinterpret_cast<const float4*>(&s_ic[set][j]);
+                    const float lcs[4] = {lc4.x, lc4.y, lc4.z, lc4.w};
+                    const float ics[4] = {ic4.x, ic4.y, ic4.z, ic4.w};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (ics[q] != 0.0f) cell(j + q, (float)((xw >> (8 * q)) & 0xffU), lcs[q], ics[q]);
+                }
+                isx += tsx;
+                isxx += tsxx;
+                continue;
             }
         }
+        if (staged) {
+            const float* lcp = s_lc[set];
+            const float* icp = s_ic[set];
+            for (uint64_t j = 0; j < nc; ++j) {
+                const float icv = icp[j];
+                if (icv == 0.0f) continue;
+                cell(j, xat<DT>(myrow, j, g.dtype), lcp[j], icv);
+            }
+        } else {
+            const float* lcp = a.loc + w * g.nch + c0;
+            const float* icp = a.inv + w * g.nch + c0;
+            for (uint64_t j = 0; j < nc; ++j) {
+                if (s_mask[j] == 0) continue;
+                const float icv = icp[j];
+                if (icv == 0.0f) continue;
+                cell(j, xat<DT>(myrow, j, g.dtype), lcp[j], icv);
+            }
+        }
+        isx += tsx;
+        isxx += tsxx;
     }
     if (!active) return;
-    const double dn = n_meas > 1.0 ? n_meas : 1.0;
+    if constexpr (DT == FA_UINT8) {
+        sx  = (double)isx;
+        sxx = (double)isxx;
+    }
+    const double dn = n_meas > 1 ? (double)n_meas : 1.0;
     a.s_det[t]      = (float)(sum_det / dn);
-    a.s_keep[t]     = (float)(n_keep > 0.0 ? sum_keep / n_keep : 0.0);
+    a.s_keep[t]     = (float)(n_keep > 0 ? sum_keep / (double)n_keep : 0.0);
     const double mx = sx / dn;
     const double vr = (sxx / dn) - (mx * mx);
     a.row_std[t]    = (float)sqrt(vr > 0.0 ? vr : 0.0);
@@ -629,6 +700,10 @@ __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
             }
         };
         load_w();
+        if (a.kchan && t0 < t1) { // every row of the chunk is appended, from t0
+            kreg_start(kp, (int32_t)t0);
+            kreg_start(kn, (int32_t)t0);
+        }
         const uint8_t* p = g.block + t0 * g.row_bytes;
         for (uint64_t tb = t0; tb < t1; tb += kRowBatch) {
             const int nq = (int)min((uint64_t)kRowBatch, t1 - tb);
@@ -657,8 +732,8 @@ __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
                 if (a.kchan) {
                     const bool outlier = fabsf(z) > a.clip;
                     const double u     = outlier ? 0.0 : (double)((z - cm) * ci);
-                    kreg_append(kp, (int32_t)t, u);
-                    kreg_append(kn, (int32_t)t, -u);
+                    kreg_extend(kp, (int32_t)t, u);
+                    kreg_extend(kn, (int32_t)t, -u);
                 }
             }
         }

@@ -84,6 +84,22 @@ def test_all_strategies_uint8(oracle, stream, seed):
             [(blk, n, n)])
 
 
+@pytest.mark.parametrize("nch,baseline", [(300, 0.3), (333, 0.3), (333, 0.003)])
+def test_ragged_channel_tiles(oracle, stream, nch, baseline):
+    """Channel counts that leave a partial last LDS tile of the row scan (300:
+    whole 4-byte words, 333: odd row bytes), and windows of under 64 rows
+    (0.003 s at 64 us), with every strategy on."""
+    say(f"test_ragged_channel_tiles nch={nch} baseline={baseline}")
+    n = 12000
+    blk = synth.filterbank(n, nch, seed=nch, rfi_chans=(5, 150, 151, nch - 2),
+                           impulses=(700, 5000), dead=(40,),
+                           bands=[(2000, 130, 140, 3.0), (2001, nch - 30, nch - 1, -3.0)])
+    cfg = abi.rfi_config(moments_enabled=1, iqrm_enabled=1, kadane_channel_enabled=1,
+                         kadane_spectrum_enabled=1, broad_boxcar_max=16, dropout_enabled=1)
+    compare(oracle, stream, cfg, layout(nch, n, tsamp=64e-6, df=-300 / nch), baseline,
+            [(blk, n, n)])
+
+
 def test_gulps_tails_and_padding(oracle, stream):
     say("test_gulps_tails_and_padding")
     nch, cap = 128, 12000
