@@ -211,6 +211,71 @@ class RawCandidate(C.Structure):
     ]
 
 
+class Giant(C.Structure):
+    """fa_giant (frost::search::Giant)."""
+    _fields_ = [
+        ("snr", C.c_float), ("dm", C.c_float), ("samp_idx", C.c_uint64),
+        ("filter_idx", C.c_uint64), ("filter_width", C.c_uint64), ("dm_idx", C.c_uint64),
+        ("begin", C.c_uint64), ("end", C.c_uint64), ("members", C.c_uint64),
+        ("f_low", C.c_float), ("f_high", C.c_float), ("chan_begin", C.c_uint64),
+        ("chan_end", C.c_uint64), ("plane", C.c_uint64), ("snr_best", C.c_float),
+        ("has_location", C.c_uint32), ("best_chan_begin", C.c_uint64),
+        ("best_chan_end", C.c_uint64), ("n_extents", C.c_uint64), ("loc_samp_idx", C.c_uint64),
+        ("loc_filter_width", C.c_uint64), ("loc_dm", C.c_float), ("loc_f_low", C.c_float),
+        ("loc_f_high", C.c_float), ("pad_", C.c_uint32),
+    ]
+
+
+GIANT_FIELDS = tuple(f for f, _ in Giant._fields_ if f != "pad_")
+
+
+def giant(**kw) -> Giant:
+    """A Giant with frost's member defaults (members = n_extents = 1)."""
+    g = Giant()
+    g.members = 1
+    g.n_extents = 1
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def giant_tuple(g: Giant) -> tuple:
+    return tuple(getattr(g, f) for f in GIANT_FIELDS)
+
+
+class ClusterTolerances(C.Structure):
+    _fields_ = [("time_tol", C.c_uint64), ("filter_tol", C.c_uint64), ("dm_tol", C.c_uint64),
+                ("dm_smear", C.c_double), ("tsamp", C.c_double), ("snr_floor", C.c_double)]
+
+
+def tolerances(time_tol, filter_tol, dm_tol, dm_smear=0.0, tsamp=0.0, snr_floor=0.0):
+    return ClusterTolerances(time_tol, filter_tol, dm_tol, dm_smear, tsamp, snr_floor)
+
+
+CLUSTER_CHAIN, CLUSTER_SEEDED = 0, 1
+DEDUP_TRANSITIVE, DEDUP_SEEDED = 0, 1
+
+
+class SearchConfig(C.Structure):
+    _fields_ = [
+        ("boxcar_max", C.c_uint64), ("max_candidates", C.c_uint64),
+        ("max_pending_candidates", C.c_uint64), ("cand_sep_time", C.c_uint64),
+        ("cand_sep_filter", C.c_uint64), ("cand_sep_dm", C.c_uint64),
+        ("detect_thresh", C.c_float), ("cand_sep_dm_smear", C.c_float),
+        ("max_giant_rate", C.c_float), ("cluster", C.c_uint32), ("subband_dedup", C.c_uint32),
+        ("pad_", C.c_uint32),
+    ]
+
+
+def search_config(**kw) -> SearchConfig:
+    """SearchConfig{} defaults, overridden by keyword."""
+    c = SearchConfig()
+    lib().fa_search_config_default(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
 CANDIDATE_FIELDS = ("peak_sample", "begin_sample", "end_sample", "snr", "dm", "dm_idx",
                     "filter_idx", "width", "chan_begin", "chan_end", "f_low_mhz",
                     "f_high_mhz", "subband_level", "subband_idx", "staggered",
@@ -337,6 +402,22 @@ EXPORTS = {
     "fa_beam_rfi_report": (C.c_int, [P, C.POINTER(RfiReport)]),
     "fa_beam_engine": (C.c_int, [P, C.POINTER(P)]),
     "fa_beam_rfi": (C.c_int, [P, C.POINTER(P)]),
+    "fa_search_config_default": (None, [C.POINTER(SearchConfig)]),
+    "fa_are_coincident": (C.c_int, [C.POINTER(Giant), C.POINTER(Giant),
+                                    C.POINTER(ClusterTolerances), C.POINTER(C.c_int)]),
+    "fa_cluster_labels": (C.c_int, [C.POINTER(Giant), U64, C.POINTER(ClusterTolerances),
+                                    C.c_int, C.c_int, PU64]),
+    "fa_merge_candidates": (C.c_int, [C.POINTER(Giant), PU64, U64, C.POINTER(Giant), PU64]),
+    "fa_subband_dedup": (C.c_int, [C.POINTER(Giant), U64, C.POINTER(ClusterTolerances), F32,
+                                   C.c_int, C.c_int, C.POINTER(Giant), PU64]),
+    "fa_subband_time_shift_samples": (C.c_double, [F32, F32, F32, F32]),
+    "fa_subband_dm_step": (C.c_double, [F32, F32, F32]),
+    "fa_postprocess_create": (C.c_int, [C.POINTER(SearchConfig), U64, C.c_double, C.c_double,
+                                        C.c_int, C.POINTER(P)]),
+    "fa_postprocess_destroy": (C.c_int, [P]),
+    "fa_postprocess_execute": (C.c_int, [P, C.POINTER(RawCandidate), U64, U64, U64, C.c_int, U64,
+                                         C.POINTER(C.POINTER(Giant)), PU64, PU64, PF64, PF64]),
+    "fa_cand_format": (C.c_int, [C.POINTER(Giant), U64, C.c_double, U64, C.c_char_p, U64, PU64]),
 }
 
 

@@ -463,6 +463,128 @@ int fa_beam_rfi_report(const fa_beam* beam, fa_rfi_report* out);
 int fa_beam_engine(fa_beam* beam, fa_engine** engine);
 int fa_beam_rfi(fa_beam* beam, fa_rfi** rfi);
 
+/* ---- candidate post-processing (frost::search) -------------------------- */
+
+/* Giant (include/frost/search/candidate.hpp:24-64). The loc_* fields hold
+ * Giant::location_best when has_location is 1 (a std::optional in frost). */
+typedef struct fa_giant {
+    float snr;
+    float dm;
+    uint64_t samp_idx;
+    uint64_t filter_idx;
+    uint64_t filter_width; /* 0: 1 << filter_idx (giant_boxcar_width) */
+    uint64_t dm_idx;
+    uint64_t begin;
+    uint64_t end;
+    uint64_t members;
+    float f_low;
+    float f_high;
+    uint64_t chan_begin; /* inclusive */
+    uint64_t chan_end;   /* inclusive */
+    uint64_t plane;
+    float snr_best;
+    uint32_t has_location;
+    uint64_t best_chan_begin;
+    uint64_t best_chan_end;
+    uint64_t n_extents;
+    uint64_t loc_samp_idx;
+    uint64_t loc_filter_width;
+    float loc_dm;
+    float loc_f_low;
+    float loc_f_high;
+    uint32_t pad_;
+} fa_giant;
+
+/* ClusterTolerances (include/frost/search/clustering.hpp:23-33). */
+typedef struct fa_cluster_tolerances {
+    uint64_t time_tol;
+    uint64_t filter_tol;
+    uint64_t dm_tol;
+    double dm_smear;
+    double tsamp;
+    double snr_floor;
+} fa_cluster_tolerances;
+
+enum fa_cluster_mode { FA_CLUSTER_CHAIN = 0, FA_CLUSTER_SEEDED = 1 };
+enum fa_dedup_mode { FA_DEDUP_TRANSITIVE = 0, FA_DEDUP_SEEDED = 1 };
+
+/* The post-processing fields of SearchConfig
+ * (include/frost/config/config.hpp:57-97). */
+typedef struct fa_search_config {
+    uint64_t boxcar_max;
+    uint64_t max_candidates;
+    uint64_t max_pending_candidates; /* 0: max_candidates */
+    uint64_t cand_sep_time;
+    uint64_t cand_sep_filter;
+    uint64_t cand_sep_dm;
+    float detect_thresh;
+    float cand_sep_dm_smear;
+    float max_giant_rate; /* <= 0 disables the rate limiter */
+    uint32_t cluster;
+    uint32_t subband_dedup; /* fa_dedup_mode */
+    uint32_t pad_;
+} fa_search_config;
+
+typedef struct fa_postprocess fa_postprocess;
+
+/* SearchConfig{} defaults (include/frost/config/config.hpp:57-97). */
+void fa_search_config_default(fa_search_config* cfg);
+
+/* are_coincident(a, b, ClusterTolerances) (lib/search/clustering.cpp:26-70):
+ * *out = 1 when a and b are one event. */
+int fa_are_coincident(const fa_giant* a, const fa_giant* b,
+                      const fa_cluster_tolerances* tol, int* out);
+
+/* ClusterLabelPlan::execute (lib/search/clustering.cpp:106-226): labels of
+ * n giants (host arrays). Chain labels (the smallest index of each
+ * friends-of-friends component) are computed on `device`. */
+int fa_cluster_labels(const fa_giant* giants, uint64_t n,
+                      const fa_cluster_tolerances* tol, int mode, int device,
+                      uint64_t* labels);
+
+/* MergeCandidatesPlan::execute (lib/search/merge.cpp:9-37): one row per
+ * label, in ascending label order. out holds n rows; *nout is set. */
+int fa_merge_candidates(const fa_giant* giants, const uint64_t* labels,
+                        uint64_t n, fa_giant* out, uint64_t* nout);
+
+/* SubbandDedupPlan::execute (lib/search/subband_dedup.cpp:194-315); the
+ * pair tests of the transitive mode run on `device`. out holds n rows. */
+int fa_subband_dedup(const fa_giant* giants, uint64_t n,
+                     const fa_cluster_tolerances* tol, float tsamp, int mode,
+                     int device, fa_giant* out, uint64_t* nout);
+
+/* subband_time_shift_samples / subband_dm_step
+ * (lib/search/subband_dedup.cpp:164-192). */
+double fa_subband_time_shift_samples(float dm, float f_wide, float f_narrow,
+                                     float tsamp);
+double fa_subband_dm_step(float f_low, float f_high, float tsamp);
+
+/* CandidatePostprocessPlan (lib/search/postprocess.cpp:45-276) with
+ * CandidatePostprocessLayout {nchans, tsamp, frequency_step}. Chain
+ * clustering and sub-band dedup pair tests run on `device`. */
+int fa_postprocess_create(const fa_search_config* cfg, uint64_t nchans,
+                          double tsamp, double frequency_step, int device,
+                          fa_postprocess** pp);
+int fa_postprocess_destroy(fa_postprocess* pp);
+
+/* CandidatePostprocessPlan::execute (lib/search/postprocess.cpp:62-170):
+ * the raw rows of one engine result (host memory). The released rows stay
+ * valid until the next call: *rows / *n; *pending is the retained member
+ * count; cluster / dedup seconds as frost reports them. */
+int fa_postprocess_execute(fa_postprocess* pp, const fa_raw_candidate* raw,
+                           uint64_t nraw, uint64_t samples_read,
+                           uint64_t duration_samples, int final,
+                           uint64_t candidate_frontier, const fa_giant** rows,
+                           uint64_t* n, uint64_t* pending, double* cluster_s,
+                           double* dedup_s);
+
+/* CandWriter::write (lib/output/cand_writer.cpp:39-75): the text of the
+ * rows' .cand lines (23 tab-separated columns) into buf. *len is the full
+ * length; nothing past cap is written (call with cap 0 to size). */
+int fa_cand_format(const fa_giant* rows, uint64_t n, double dt,
+                   uint64_t first_samp_idx, char* buf, uint64_t cap,
+                   uint64_t* len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -287,3 +287,128 @@ class Engine:
         out = np.zeros(info.ndm * nwidths, np.float32)
         check(lib().oracle_engine_inv_sigma(self.h, p, ptr(out), out.size))
         return out.reshape(info.ndm, nwidths)
+
+
+# ---- candidate post-processing (post_oracle.cpp) -----------------------------
+
+_post = None
+
+
+def post_lib() -> C.CDLL:
+    global _post
+    if _post is None:
+        L = lib()
+        G = C.POINTER(abi.Giant)
+        T = C.POINTER(abi.ClusterTolerances)
+        U = C.c_uint64
+        sig = {
+            "post_oracle_error": (C.c_char_p, []),
+            "post_oracle_sizeof": (C.c_uint64, [C.c_int]),
+            "post_oracle_coincident": (C.c_int, [G, G, T]),
+            "post_oracle_labels": (C.c_int, [G, U, T, C.c_int, C.POINTER(U)]),
+            "post_oracle_merge": (C.c_int, [G, C.POINTER(U), U, G, C.POINTER(U)]),
+            "post_oracle_dedup": (C.c_int, [G, U, T, C.c_float, C.c_int, G, C.POINTER(U)]),
+            "post_oracle_time_shift": (C.c_double, [C.c_float] * 4),
+            "post_oracle_dm_step": (C.c_double, [C.c_float] * 3),
+            "post_oracle_create": (P, [C.POINTER(abi.SearchConfig), U, C.c_double, C.c_double]),
+            "post_oracle_destroy": (None, [P]),
+            "post_oracle_execute": (C.c_int, [P, C.POINTER(abi.RawCandidate), U, U, U, C.c_int,
+                                              U, G, U, C.POINTER(U), C.POINTER(U)]),
+            "post_oracle_format": (U, [G, U, C.c_double, U, C.c_char_p, U]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _post = L
+    return _post
+
+
+def _post_check(rc: int) -> None:
+    if rc != 0:
+        msg = post_lib().post_oracle_error().decode()
+        if rc == abi.FA_INVALID_ARGUMENT:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+
+def _giants(gs):
+    arr = (abi.Giant * max(1, len(gs)))()
+    for i, g in enumerate(gs):
+        arr[i] = g
+    return arr
+
+
+def post_coincident(a, b, tol) -> bool:
+    return bool(post_lib().post_oracle_coincident(C.byref(a), C.byref(b), C.byref(tol)))
+
+
+def post_labels(gs, tol, mode=0) -> list:
+    n = len(gs)
+    out = (C.c_uint64 * max(1, n))()
+    _post_check(post_lib().post_oracle_labels(_giants(gs), n, C.byref(tol), mode, out))
+    return list(out[:n])
+
+
+def post_merge(gs, labels) -> list:
+    n = len(gs)
+    out = (abi.Giant * max(1, n))()
+    nout = C.c_uint64()
+    lab = (C.c_uint64 * max(1, len(labels)))(*labels)
+    if len(labels) != n:
+        raise ValueError("candidates/labels size mismatch")
+    _post_check(post_lib().post_oracle_merge(_giants(gs), lab, n, out, C.byref(nout)))
+    return [out[i] for i in range(nout.value)]
+
+
+def post_dedup(gs, tol, tsamp, mode=0) -> list:
+    n = len(gs)
+    out = (abi.Giant * max(1, n))()
+    nout = C.c_uint64()
+    _post_check(post_lib().post_oracle_dedup(_giants(gs), n, C.byref(tol), tsamp, mode, out,
+                                             C.byref(nout)))
+    return [out[i] for i in range(nout.value)]
+
+
+def post_time_shift(dm, fw, fn, tsamp) -> float:
+    return post_lib().post_oracle_time_shift(dm, fw, fn, tsamp)
+
+
+def post_dm_step(lo, hi, tsamp) -> float:
+    return post_lib().post_oracle_dm_step(lo, hi, tsamp)
+
+
+class PostPlan:
+    """CandidatePostprocessPlan restated on the CPU."""
+
+    def __init__(self, cfg, nchans, tsamp, df):
+        h = post_lib().post_oracle_create(C.byref(cfg), nchans, tsamp, df)
+        if not h:
+            msg = post_lib().post_oracle_error().decode()
+            raise ValueError(msg)
+        self.h = h
+        self.cap = max(1, cfg.max_candidates + (cfg.max_pending_candidates or cfg.max_candidates))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            post_lib().post_oracle_destroy(self.h)
+
+    def execute(self, raw, samples_read, duration, final, frontier):
+        n = len(raw)
+        arr = (abi.RawCandidate * max(1, n))()
+        for i, r in enumerate(raw):
+            arr[i] = r
+        out = (abi.Giant * self.cap)()
+        nout, pend = C.c_uint64(), C.c_uint64()
+        _post_check(post_lib().post_oracle_execute(self.h, arr, n, samples_read, duration,
+                                                   int(final), frontier, out, self.cap,
+                                                   C.byref(nout), C.byref(pend)))
+        return [abi.Giant.from_buffer_copy(out[i]) for i in range(nout.value)], pend.value
+
+
+def post_format(gs, dt, first=0) -> str:
+    arr = _giants(gs)
+    n = post_lib().post_oracle_format(arr, len(gs), dt, first, None, 0)
+    buf = C.create_string_buffer(n + 1)
+    post_lib().post_oracle_format(arr, len(gs), dt, first, buf, n)
+    return buf.raw[:n].decode()

@@ -14,7 +14,7 @@ HEADER = Path(__file__).resolve().parents[1] / "include" / "frost_amd.h"
 
 def declared_functions():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(fa_\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|const char\*)\s+(fa_\w+)\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -29,6 +29,34 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts():
     assert C.sizeof(abi.RawCandidate) == 72
     assert C.sizeof(abi.PlaneInfo) == 40
+    assert C.sizeof(abi.Giant) == 160
+    assert C.sizeof(abi.SearchConfig) == 72
+    assert C.sizeof(abi.ClusterTolerances) == 48
+
+
+def test_search_config_defaults():
+    """SearchConfig{} (include/frost/config/config.hpp:57-97)."""
+    c = abi.search_config()
+    assert (c.boxcar_max, c.max_candidates, c.max_pending_candidates) == (4096, 131072, 0)
+    assert (c.cand_sep_time, c.cand_sep_filter, c.cand_sep_dm) == (3, 3, 200)
+    assert (c.detect_thresh, c.cand_sep_dm_smear, c.max_giant_rate) == (6.0, 8.0, 0.0)
+    assert (c.cluster, c.subband_dedup) == (1, abi.DEDUP_TRANSITIVE)
+
+
+def test_cand_format_and_dedup_geometry_match_oracle(oracle):
+    """Host-only post-processing entry points (no GPU work): the .cand text
+    and the dispersion helpers equal the oracle's."""
+    from frost_amd import postprocess as pp
+    import post_cases as pc
+    g = pc.writer_giant()
+    assert pp.cand_format([g, g], 0.001, 7) == oracle.post_format([g, g], 0.001, 7)
+    for args in [(100.0, 1000.0, 1100.0, 0.001), (104.5, 704.0, 3776.0, 512e-6)]:
+        assert pp.subband_time_shift_samples(*args) == oracle.post_time_shift(*args)
+    for args in [(1100.0, 1164.0, 0.001), (3776.0, 4032.0, 512e-6), (5.0, 1.0, 1e-3)]:
+        assert pp.subband_dm_step(*args) == oracle.post_dm_step(*args)
+    a, b = pc.uwl_giant(10.0, 5000, 100), pc.uwl_giant(9.0, 5000, 106)
+    tol = abi.tolerances(3, 3, 200, 3.0, 512e-6)
+    assert pp.are_coincident(a, b, tol) == oracle.post_coincident(a, b, tol)
 
 
 CONFIGS = [
