@@ -150,13 +150,17 @@ __host__ __device__ inline bool same_curve(const fa_giant& a, const fa_giant& b,
 
 // ---- device union-find -----------------------------------------------------------
 
-__device__ inline uint32_t uf_root(const uint32_t* p, uint32_t x) {
-    uint32_t y = __atomic_load_n(&p[x], __ATOMIC_RELAXED);
-    while (y != x) {
-        x = y;
-        y = __atomic_load_n(&p[x], __ATOMIC_RELAXED);
+// Root of x with path halving: a non-root's parent is only ever replaced by
+// one of its ancestors, so concurrent halving and joining stay consistent.
+__device__ inline uint32_t uf_root(uint32_t* p, uint32_t x) {
+    for (;;) {
+        const uint32_t y = __atomic_load_n(&p[x], __ATOMIC_RELAXED);
+        if (y == x) return x;
+        const uint32_t z = __atomic_load_n(&p[y], __ATOMIC_RELAXED);
+        if (z == y) return y;
+        __atomic_store_n(&p[x], z, __ATOMIC_RELAXED);
+        x = z;
     }
-    return x;
 }
 
 // Hangs the larger root under the smaller; retried when another thread
@@ -183,37 +187,62 @@ __global__ void uf_init_kernel(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = i;
 }
 
-__global__ void uf_flatten_kernel(const uint32_t* p, uint32_t n, uint32_t* root) {
+__global__ void uf_flatten_kernel(uint32_t* p, uint32_t n, uint32_t* root) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) root[i] = uf_root(p, i);
 }
 
-// Fields the chain scan reads for every neighbour.
-struct ChainKey {
-    u64 samp, plane, dm_idx, filter_idx;
+// A row of the chain scan: the fields are_coincident reads, the boxcar
+// width, and subband_dm_step of the row's band (what the test computes when
+// the row is its first argument), in (plane, time) order.
+struct ChainRow {
+    u64 samp, plane, dm_idx, filter_idx, width, chan_begin, chan_end;
+    double step;
+    float dm, snr;
+    uint32_t wid, pad;
 };
 
-// Rows in (plane, time) order. Thread r scans the rows after it in its plane
-// while they lie within `reach` (the largest time reach of any pair) and
-// tests each pair with the wider giant first (ties: the earlier row), as
-// frost's sweep does; coincident pairs join the components of their work
-// indices `wid`.
-__global__ void chain_pairs_kernel(const ChainKey* __restrict__ key, const fa_giant* __restrict__ g,
-                                   const uint32_t* __restrict__ wid, uint32_t n, u64 reach,
+// are_coincident(a, b, tol) with a's DM step and both widths precomputed.
+__device__ inline bool coincident_rows(const ChainRow& a, const ChainRow& b,
+                                       const fa_cluster_tolerances& t) {
+    if (a.chan_begin != b.chan_begin || a.chan_end != b.chan_end) return false;
+    const u64 widest_w = umax(a.width, b.width);
+    if (absdiff(a.samp, b.samp) > scaled_reach(widest_w, t.time_tol)) return false;
+    if (!(t.dm_smear > 0.0) || !(t.tsamp > 0.0) || absdiff(a.dm_idx, b.dm_idx) <= 1) return true;
+    const double step = a.step;
+    if (!(step > 0.0)) return true;
+    const double widest = (double)widest_w;
+    const double ddm    = fabs((double)a.dm - (double)b.dm);
+    double growth       = 1.0;
+    if (t.snr_floor > 0.0) {
+        const double ratio = (double)(a.snr > b.snr ? a.snr : b.snr) / t.snr_floor;
+        const double g2    = (ratio * ratio) - 1.0;
+        const double root  = sqrt(g2 > 0.0 ? g2 : 0.0);
+        growth             = root > 1.0 ? root : 1.0;
+    }
+    return ddm <= t.dm_smear * widest * step * growth;
+}
+
+// Thread r scans the rows after it in its plane while they lie within
+// `reach` (the largest time reach of any pair). Each pair passing the trial
+// gates is skipped when both rows already share a root (as frost's sweep
+// does), else tested with the wider giant first (ties: the earlier row) and
+// joined when coincident.
+__global__ void chain_pairs_kernel(const ChainRow* __restrict__ rows, uint32_t n, u64 reach,
                                    fa_cluster_tolerances t, uint32_t* parent) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    const ChainKey kr = key[r];
+    const ChainRow a = rows[r];
     for (uint32_t q = r + 1; q < n; ++q) {
-        const ChainKey kq = key[q];
-        if (kq.plane != kr.plane || kq.samp - kr.samp > reach) break;
-        if (absdiff(kq.dm_idx, kr.dm_idx) > t.dm_tol ||
-            absdiff(kq.filter_idx, kr.filter_idx) > t.filter_tol)
+        const u64 plane = rows[q].plane, samp = rows[q].samp;
+        if (plane != a.plane || samp - a.samp > reach) break;
+        if (absdiff(rows[q].dm_idx, a.dm_idx) > t.dm_tol ||
+            absdiff(rows[q].filter_idx, a.filter_idx) > t.filter_tol)
             continue;
-        const fa_giant& a = g[r];
-        const fa_giant& b = g[q];
-        const bool hit = boxcar_width(a) >= boxcar_width(b) ? coincident(a, b, t) : coincident(b, a, t);
-        if (hit) uf_join(parent, wid[r], wid[q]);
+        if (uf_root(parent, a.wid) == uf_root(parent, rows[q].wid)) continue;
+        const ChainRow b = rows[q];
+        const bool hit = a.width >= b.width ? coincident_rows(a, b, t) : coincident_rows(b, a, t);
+        if (hit) uf_join(parent, a.wid, b.wid);
     }
 }
 
@@ -269,7 +298,7 @@ struct DevBuf {
 // per standalone call).
 struct PairGpu {
     int device = 0;
-    DevBuf rows, keys, wid, parent, roots;
+    DevBuf rows, parent, roots;
 
     // Chain labels of `work` (rows of one or more planes): each plane's rows
     // in time order (stable) are scanned; labels are the smallest work index
@@ -284,31 +313,32 @@ struct PairGpu {
         u64 widest = 1;
         for (const auto& g : work) widest = std::max(widest, boxcar_width(g));
         const u64 reach = scaled_reach(widest, t.time_tol);
-        std::vector<uint32_t> ord(n);
-        std::iota(ord.begin(), ord.end(), 0U);
-        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
-            return work[a].plane != work[b].plane ? work[a].plane < work[b].plane
-                                                  : work[a].samp_idx < work[b].samp_idx;
+        // (plane, time) order, ties by work index (a stable order).
+        struct Key {
+            u64 plane, samp;
+            uint32_t idx, pad;
+        };
+        std::vector<Key> order(n);
+        for (u64 i = 0; i < n; ++i)
+            order[i] = {work[i].plane, work[i].samp_idx, static_cast<uint32_t>(i), 0};
+        std::sort(order.begin(), order.end(), [](const Key& x, const Key& y) {
+            return std::tie(x.plane, x.samp, x.idx) < std::tie(y.plane, y.samp, y.idx);
         });
-        std::vector<fa_giant> rs(n);
-        std::vector<ChainKey> ks(n);
+        std::vector<ChainRow> rs(n);
+        const float ts = static_cast<float>(t.tsamp);
         for (u64 r = 0; r < n; ++r) {
-            const fa_giant& g = work[ord[r]];
-            rs[r]             = g;
-            ks[r]             = {g.samp_idx, g.plane, g.dm_idx, g.filter_idx};
+            const fa_giant& g = work[order[r].idx];
+            rs[r] = {g.samp_idx, g.plane, g.dm_idx, g.filter_idx, boxcar_width(g), g.chan_begin,
+                     g.chan_end, dm_step_of(g.f_low, g.f_high, ts), g.dm, g.snr, order[r].idx, 0};
         }
         DeviceGuard dg(device);
-        auto* drows = static_cast<fa_giant*>(rows.need(n * sizeof(fa_giant)));
-        auto* dkeys = static_cast<ChainKey*>(keys.need(n * sizeof(ChainKey)));
-        auto* dwid  = static_cast<uint32_t*>(wid.need(n * 4));
+        auto* drows = static_cast<ChainRow*>(rows.need(n * sizeof(ChainRow)));
         auto* dpar  = static_cast<uint32_t*>(parent.need(n * 4));
         auto* droot = static_cast<uint32_t*>(roots.need(n * 4));
-        FA_HIP(hipMemcpy(drows, rs.data(), n * sizeof(fa_giant), hipMemcpyHostToDevice));
-        FA_HIP(hipMemcpy(dkeys, ks.data(), n * sizeof(ChainKey), hipMemcpyHostToDevice));
-        FA_HIP(hipMemcpy(dwid, ord.data(), n * 4, hipMemcpyHostToDevice));
+        FA_HIP(hipMemcpy(drows, rs.data(), n * sizeof(ChainRow), hipMemcpyHostToDevice));
         const uint32_t n32 = static_cast<uint32_t>(n);
         uf_init_kernel<<<ceil_div(n, 256), 256>>>(dpar, n32);
-        chain_pairs_kernel<<<ceil_div(n, 256), 256>>>(dkeys, drows, dwid, n32, reach, t, dpar);
+        chain_pairs_kernel<<<ceil_div(n, 256), 256>>>(drows, n32, reach, t, dpar);
         uf_flatten_kernel<<<ceil_div(n, 256), 256>>>(dpar, n32, droot);
         FA_HIP(hipGetLastError());
         std::vector<uint32_t> root(n);
@@ -579,10 +609,27 @@ public:
             g.n_extents    = 1;
             incoming_.push_back(g);
         }
-        std::sort(incoming_.begin(), incoming_.end(), [](const fa_giant& a, const fa_giant& b) {
-            return std::tie(a.plane, a.dm_idx, a.filter_idx, a.samp_idx) <
-                   std::tie(b.plane, b.dm_idx, b.filter_idx, b.samp_idx);
-        });
+        // frost's std::sort by (plane, dm_idx, filter_idx, samp_idx), run on
+        // compact keys with the row index carried but not compared: the same
+        // comparisons give the same permutation as sorting the rows.
+        {
+            struct Key {
+                u64 plane, dm_idx, filter_idx, samp;
+                uint32_t idx, pad;
+            };
+            std::vector<Key> keys(incoming_.size());
+            for (u64 k = 0; k < incoming_.size(); ++k) {
+                const fa_giant& g = incoming_[k];
+                keys[k]           = {g.plane, g.dm_idx, g.filter_idx, g.samp_idx, static_cast<uint32_t>(k), 0};
+            }
+            std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
+                return std::tie(a.plane, a.dm_idx, a.filter_idx, a.samp) <
+                       std::tie(b.plane, b.dm_idx, b.filter_idx, b.samp);
+            });
+            sorted_.resize(incoming_.size());
+            for (u64 k = 0; k < keys.size(); ++k) sorted_[k] = incoming_[keys[k].idx];
+            incoming_.swap(sorted_);
+        }
         const u64 width     = cfg_.boxcar_max;
         const u64 reach     = cfg_.cand_sep_time >= (kU64Max / width) - 1
                                   ? kU64Max
@@ -690,7 +737,7 @@ private:
     u64 nchans_;
     double tsamp_, df_;
     PairGpu gpu_;
-    std::vector<fa_giant> incoming_, work_, ready_, pending_, next_;
+    std::vector<fa_giant> incoming_, sorted_, work_, ready_, pending_, next_;
 };
 
 namespace {

@@ -105,6 +105,12 @@ class CandidatePostprocessPlan:
         arr = (abi.RawCandidate * max(1, n))()
         for i, r in enumerate(raw):
             arr[i] = r
+        return self.execute_array(arr, n, samples_read, duration_samples, final,
+                                  candidate_frontier)
+
+    def execute_array(self, arr, n: int, samples_read: int, duration_samples: int, final: bool,
+                      candidate_frontier: int) -> PostprocessResult:
+        """execute() on the first n rows of a ctypes RawCandidate array."""
         rows = C.POINTER(abi.Giant)()
         nrows, pend = C.c_uint64(), C.c_uint64()
         cs, ds = C.c_double(), C.c_double()
