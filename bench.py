@@ -98,6 +98,54 @@ def cpu_baseline(W, dev_gulp, stream, sample_rows):
     }
 
 
+def beam_rows(beam):
+    """The last gulp's raw candidate rows as a ctypes array, their count and
+    the engine's candidate frontier."""
+    from frost_amd import abi
+    n, dg, dd, fr = (C.c_uint64() for _ in range(4))
+    abi.check(abi.lib().fa_beam_result(beam.h, None, 0, C.byref(n), C.byref(dg), C.byref(dd),
+                                       C.byref(fr)))
+    arr = (abi.RawCandidate * max(n.value, 1))()
+    abi.check(abi.lib().fa_beam_result(beam.h, arr, n.value, C.byref(n), C.byref(dg),
+                                       C.byref(dd), C.byref(fr)))
+    return arr, n.value, fr.value
+
+
+def postprocess_measure(W, arr, n, read, frontier, reps=3):
+    """frost's CandidatePostprocessPlan on one gulp's raw rows (then the final
+    flush), with SearchConfig defaults: wall time per gulp, median of reps,
+    each on a fresh plan."""
+    from frost_amd import abi, postprocess
+    cfg = abi.search_config(boxcar_max=W.boxcar_max)
+    times, out = [], None
+    for _ in range(reps):
+        plan = postprocess.CandidatePostprocessPlan(cfg, W.nchans, W.tsamp, W.df)
+        t0 = time.perf_counter()
+        r1 = plan.execute_array(arr, n, read, W.gulp, False, frontier)
+        r2 = plan.execute_array(arr, 0, read, 0, True, read)
+        times.append(time.perf_counter() - t0)
+        out = (r1, r2)
+    r1, r2 = out
+    return {"raw_rows_per_gulp": n, "rows_out": len(r1.candidates) + len(r2.candidates),
+            "seconds_per_gulp": float(np.median(times)),
+            "cluster_seconds": r1.cluster_s + r2.cluster_s,
+            "dedup_seconds": r1.dedup_s + r2.dedup_s,
+            "device_work": "chain-clustering pair tests and sub-band dedup pair tests"}
+
+
+def postprocess_cpu(W, arr, n, read, frontier):
+    """The oracle's CandidatePostprocessPlan (one thread) on the same rows."""
+    import oracle_ffi
+    from frost_amd import abi
+    cfg = abi.search_config(boxcar_max=W.boxcar_max)
+    plan = oracle_ffi.PostPlan(cfg, W.nchans, W.tsamp, W.df)
+    t0 = time.perf_counter()
+    plan.execute_array(arr, n, read, W.gulp, False, frontier)
+    plan.execute_array(arr, 0, read, 0, True, read)
+    return {"seconds_per_gulp": time.perf_counter() - t0, "cores": 1, "kind": "port",
+            "sample": f"the same {n} raw rows of one gulp"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +221,12 @@ def main() -> int:
             f"{rep.chans_masked} ch, flagged {rep.samps_flagged} samples")
     total = float(sum(lat))
     value = ndm * W.gulp * a.steps / total
+    # Candidate post-processing (outside the timed steps): the last gulp's rows.
+    rows_arr, rows_n, frontier = beam_rows(beam)
+    read = (a.warmup + a.steps) * W.gulp
+    post = postprocess_measure(W, rows_arr, rows_n, read, frontier)
+    log(f"postprocess: {rows_n} raw rows -> {post['rows_out']} in "
+        f"{post['seconds_per_gulp'] * 1e3:.1f} ms")
     # Algorithmic bytes (unfused FDMT): a launch covering levels a -> b does
     # the work of b - a merge levels, each reading its input rows and writing
     # its output rows once, float32 per sample: sum_{l=a}^{b-1} 4 N (R_l + R_{l+1}).
@@ -220,6 +274,7 @@ def main() -> int:
         "rfi_seconds_per_gulp": {k: v / a.steps for k, v in rfi_parts.items()},
         "candidates": {"emitted_per_gulp": raw_cands / a.steps,
                        "host_collect_seconds_per_gulp": collect_s / a.steps},
+        "postprocess": post,
         "roofline": {
             "kernel": "fdmt_gather_kernel<16> (FDMT levels l-4 -> l in one pass)",
             "levels": [[int(src[l]), int(l)] for l in r16],
@@ -242,6 +297,7 @@ def main() -> int:
     if a.cpu_sample > 0:
         log(f"CPU baseline on {a.cpu_sample} rows ...")
         out["cpu_baseline"] = cpu_baseline(W, gulp, stream, a.cpu_sample)
+        out["cpu_baseline"]["postprocess"] = postprocess_cpu(W, rows_arr, rows_n, read, frontier)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

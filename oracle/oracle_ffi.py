@@ -398,6 +398,9 @@ class PostPlan:
         arr = (abi.RawCandidate * max(1, n))()
         for i, r in enumerate(raw):
             arr[i] = r
+        return self.execute_array(arr, n, samples_read, duration, final, frontier)
+
+    def execute_array(self, arr, n, samples_read, duration, final, frontier):
         out = (abi.Giant * self.cap)()
         nout, pend = C.c_uint64(), C.c_uint64()
         _post_check(post_lib().post_oracle_execute(self.h, arr, n, samples_read, duration,
