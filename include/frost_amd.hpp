@@ -11,6 +11,11 @@
 //     (lib/pipeline/pipeline.cpp:346-367,603-608)
 //   Pipeline::Impl::execute_block (lib/pipeline/        frost_amd::Beam
 //     pipeline.cpp:529-619)
+//   search::CandidatePostprocessPlan (include/frost/    frost_amd::
+//     search/postprocess.hpp, lib/search/               CandidatePostprocessPlan
+//     postprocess.cpp:45-276)
+//   output::CandWriter::write (lib/output/             frost_amd::cand_rows
+//     cand_writer.cpp:39-75)
 //
 // Errors follow frost's conventions: std::invalid_argument for bad arguments
 // (thrown before any state changes), std::runtime_error otherwise, including
@@ -230,5 +235,55 @@ private:
     fa_beam* b_ = nullptr;
     std::vector<fa_raw_candidate> rows_;
 };
+
+/** CandidatePostprocessPlan::execute's result: rows borrowed until the next
+ * execute() (CandidatePostprocessResult, include/frost/search/
+ * postprocess.hpp:20-25). */
+struct CandidatePostprocessResult {
+    std::span<const fa_giant> candidates;
+    double cluster_s            = 0.0;
+    double dedup_s              = 0.0;
+    uint64_t pending_candidates = 0;
+};
+
+/** search::CandidatePostprocessPlan(cfg, {nchans, tsamp, frequency_step}):
+ * chain clustering and sub-band dedup pair tests on `device`. */
+class CandidatePostprocessPlan {
+public:
+    CandidatePostprocessPlan(const fa_search_config& cfg, uint64_t nchans, double tsamp,
+                             double frequency_step, int device = 0) {
+        check(fa_postprocess_create(&cfg, nchans, tsamp, frequency_step, device, &h_));
+    }
+    CandidatePostprocessPlan(const CandidatePostprocessPlan&)            = delete;
+    CandidatePostprocessPlan& operator=(const CandidatePostprocessPlan&) = delete;
+    ~CandidatePostprocessPlan() { fa_postprocess_destroy(h_); }
+
+    CandidatePostprocessResult execute(std::span<const fa_raw_candidate> raw,
+                                       uint64_t samples_read, uint64_t duration_samples,
+                                       bool final, uint64_t candidate_frontier) {
+        const fa_giant* rows = nullptr;
+        uint64_t n = 0;
+        CandidatePostprocessResult r;
+        check(fa_postprocess_execute(h_, raw.data(), raw.size(), samples_read, duration_samples,
+                                     final ? 1 : 0, candidate_frontier, &rows, &n,
+                                     &r.pending_candidates, &r.cluster_s, &r.dedup_s));
+        r.candidates = std::span<const fa_giant>(rows, n);
+        return r;
+    }
+
+private:
+    fa_postprocess* h_ = nullptr;
+};
+
+/** The .cand text CandWriter::write(rows, first_samp_idx) appends for `rows`
+ * (sample period dt). */
+inline std::string cand_rows(std::span<const fa_giant> rows, double dt,
+                             uint64_t first_samp_idx = 0) {
+    uint64_t len = 0;
+    check(fa_cand_format(rows.data(), rows.size(), dt, first_samp_idx, nullptr, 0, &len));
+    std::string s(len, '\0');
+    check(fa_cand_format(rows.data(), rows.size(), dt, first_samp_idx, s.data(), len, &len));
+    return s;
+}
 
 } // namespace frost_amd

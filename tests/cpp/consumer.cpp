@@ -55,6 +55,17 @@ int main(int argc, char** argv) {
         threw = true;
     }
     CHECK(threw);
+    {
+        // CandWriter's row (tests/cpp/api/cand_writer_test.cpp:21-98).
+        fa_giant g{};
+        g.snr = 12.5f, g.samp_idx = 100, g.filter_idx = 3, g.dm_idx = 7, g.dm = 42.5f;
+        g.members = 4, g.begin = 98, g.end = 102, g.f_low = 1200.5f, g.f_high = 1500.25f;
+        g.chan_begin = 4, g.chan_end = 19, g.n_extents = 1;
+        const std::vector<fa_giant> rows{g};
+        CHECK(frost_amd::cand_rows(rows, 0.001) ==
+              "12.5\t100\t0.1\t3\t7\t42.5\t4\t98\t102\t1200.5\t1500.25\t4\t19\t12.5\t4\t19\t1"
+              "\t100\t0.1\t42.5\t8\t1200.5\t1500.25\n");
+    }
     if (argc > 1 && std::strcmp(argv[1], "gpu") == 0) {
         // Undispersed width-4 pulse at 2000 in 32 channels: peak at 2003.
         fa_dedisp_config d = cfg;
@@ -90,6 +101,17 @@ int main(int argc, char** argv) {
         for (const auto& c : all)
             if (c.snr > best->snr) best = &c;
         CHECK(best->peak_sample == 2003 && best->width == 4);
+        // frost's post-processing of the same rows: one event, its winner the
+        // brightest excursion (channels ascending, df < 0 flips them).
+        fa_search_config sc{};
+        fa_search_config_default(&sc);
+        sc.boxcar_max = 16;
+        frost_amd::CandidatePostprocessPlan post(sc, 32, 1.0e-3, -1.0);
+        auto r = post.execute(all, 4096, 4096, true, 4096);
+        bool found = false;
+        for (const auto& c : r.candidates) found = found || (c.samp_idx == 2003 && c.members > 1);
+        CHECK(found);
+        CHECK(!frost_amd::cand_rows(r.candidates, 1.0e-3).empty());
     }
     std::printf("consumer ok\n");
     return 0;
