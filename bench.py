@@ -146,6 +146,41 @@ def postprocess_cpu(W, arr, n, read, frontier):
             "sample": f"the same {n} raw rows of one gulp"}
 
 
+def ingest_measure(W, beam, gulp, stream, ndm, steps=3):
+    """Host-resident gulps through the beam's pinned double-buffered ingest:
+    per-gulp wall time with the copy to the device before each gulp (serial)
+    and with gulp k + 1's copy submitted before gulp k runs (overlapped).
+    Both slots are filled once with the gulp's bytes (the read a file source
+    would do), outside the timed loops."""
+    host = np.zeros(W.gulp_bytes(), np.uint8)
+    gulp.to_host(host, stream)
+    stream.synchronize()
+    for _ in range(2):  # fill both slots
+        np.copyto(beam.ingest_slot(), host)
+        beam.ingest_submit(W.gulp)
+        beam.execute_next(stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        beam.ingest_slot()
+        beam.ingest_submit(W.gulp)
+        beam.execute_next(stream)
+    serial = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    beam.ingest_slot()
+    beam.ingest_submit(W.gulp)
+    for k in range(steps):
+        if k + 1 < steps:
+            beam.ingest_slot()
+            beam.ingest_submit(W.gulp)
+        beam.execute_next(stream)
+    overlapped = (time.perf_counter() - t0) / steps
+    return {"bytes_per_gulp": W.gulp_bytes(), "serial_ms_per_gulp": 1e3 * serial,
+            "overlapped_ms_per_gulp": 1e3 * overlapped,
+            "pcie_inclusive_value": ndm * W.gulp / overlapped,
+            "unit": "DM-trial*samples/s"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +262,9 @@ def main() -> int:
     post = postprocess_measure(W, rows_arr, rows_n, read, frontier)
     log(f"postprocess: {rows_n} raw rows -> {post['rows_out']} in "
         f"{post['seconds_per_gulp'] * 1e3:.1f} ms")
+    ingest = ingest_measure(W, beam, gulp, stream, ndm)
+    log(f"ingest from pinned host slots: serial {ingest['serial_ms_per_gulp']:.1f} ms, "
+        f"overlapped {ingest['overlapped_ms_per_gulp']:.1f} ms per gulp")
     # Algorithmic bytes (unfused FDMT): a launch covering levels a -> b does
     # the work of b - a merge levels, each reading its input rows and writing
     # its output rows once, float32 per sample: sum_{l=a}^{b-1} 4 N (R_l + R_{l+1}).
@@ -275,6 +313,7 @@ def main() -> int:
         "candidates": {"emitted_per_gulp": raw_cands / a.steps,
                        "host_collect_seconds_per_gulp": collect_s / a.steps},
         "postprocess": post,
+        "ingest_from_host": ingest,
         "roofline": {
             "kernel": "fdmt_gather_kernel<16> (FDMT levels l-4 -> l in one pass)",
             "levels": [[int(src[l]), int(l)] for l in r16],

@@ -16,6 +16,8 @@
 
 namespace fa {
 
+
+
 class Beam {
 public:
     Beam(const fa_rfi_config& rcfg, const fa_block_layout& layout, float baseline,
@@ -71,6 +73,58 @@ public:
         samples_read_ += nsamps;
     }
 
+    // Ingest with overlap (the counterpart of frost's GulpPrefetch,
+    // lib/pipeline/pipeline.cpp:109-262): two pinned host slots and two
+    // device slots. The caller fills the slot ingest_slot() returns and
+    // submits it; its copy to the device runs on the beam's copy stream
+    // while the previous gulp computes; execute_next() waits for the oldest
+    // submitted copy on the compute stream and runs that gulp.
+    uint8_t* ingest_slot(uint64_t* bytes) {
+        DeviceGuard dg(device_);
+        Slot& sl = slots_[fill_];
+        require(!sl.submitted, "both ingest slots hold submitted gulps; execute one first");
+        if (!copy_stream_) FA_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
+        if (!sl.copied) FA_HIP(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
+        sl.host.reserve(capacity_ * row_bytes_);
+        sl.dev.reserve(capacity_ * row_bytes_);
+        // The slot's previous copy finished before its gulp ran; wait anyway
+        // so a slot is never refilled under an outstanding copy.
+        FA_HIP(hipEventSynchronize(sl.copied));
+        if (bytes) *bytes = capacity_ * row_bytes_;
+        return sl.host.get();
+    }
+
+    void ingest_submit(uint64_t nsamps) {
+        require(nsamps >= 1 && nsamps <= capacity_, "nsamps must be in [1, capacity]");
+        Slot& sl = slots_[fill_];
+        require(sl.host.get() != nullptr && !sl.submitted,
+                "no ingest slot to submit (call fa_beam_ingest_slot first)");
+        DeviceGuard dg(device_);
+        FA_HIP(hipMemcpyAsync(sl.dev.get(), sl.host.get(), nsamps * row_bytes_,
+                              hipMemcpyHostToDevice, copy_stream_));
+        FA_HIP(hipEventRecord(sl.copied, copy_stream_));
+        sl.nsamps    = nsamps;
+        sl.submitted = true;
+        fill_ ^= 1;
+    }
+
+    void execute_next(hipStream_t s) {
+        require(s != nullptr, "an explicit non-null stream is required");
+        Slot& sl = slots_[next_];
+        require(sl.submitted, "no submitted gulp to execute");
+        DeviceGuard dg(device_);
+        FA_HIP(hipStreamWaitEvent(s, sl.copied, 0));
+        execute(sl.dev.get(), sl.nsamps, true, s);
+        sl.submitted = false;
+        next_ ^= 1;
+    }
+
+    ~Beam() {
+        for (auto& sl : slots_)
+            if (sl.copied) (void)hipEventDestroy(sl.copied);
+        if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
+    }
+
     void drain(hipStream_t s) { engine_->drain(s); }
 
     DeviceRfi& rfi() { return *rfi_; }
@@ -84,6 +138,16 @@ private:
     std::unique_ptr<Engine> engine_;
     std::vector<uint8_t> static_mask_, fdmt_mask_;
     DeviceBuffer<uint8_t> staging_;
+    struct Slot {
+        PinnedBuffer<uint8_t> host;
+        DeviceBuffer<uint8_t> dev;
+        hipEvent_t copied = nullptr;
+        uint64_t nsamps   = 0;
+        bool submitted    = false;
+    };
+    Slot slots_[2];
+    int fill_ = 0, next_ = 0;
+    hipStream_t copy_stream_ = nullptr;
 };
 
 } // namespace fa
@@ -123,6 +187,27 @@ int fa_beam_execute(fa_beam* beam, const void* gulp, uint64_t nsamps, int is_dev
     return guard([&] {
         require(beam != nullptr, "beam must not be null");
         beam->beam->execute(gulp, nsamps, is_device != 0, fa::as_stream(stream));
+    });
+}
+
+int fa_beam_ingest_slot(fa_beam* beam, void** host_slot, uint64_t* bytes) {
+    return guard([&] {
+        require(beam != nullptr && host_slot != nullptr, "arguments must not be null");
+        *host_slot = beam->beam->ingest_slot(bytes);
+    });
+}
+
+int fa_beam_ingest_submit(fa_beam* beam, uint64_t nsamps) {
+    return guard([&] {
+        require(beam != nullptr, "beam must not be null");
+        beam->beam->ingest_submit(nsamps);
+    });
+}
+
+int fa_beam_execute_next(fa_beam* beam, void* stream) {
+    return guard([&] {
+        require(beam != nullptr, "beam must not be null");
+        beam->beam->execute_next(fa::as_stream(stream));
     });
 }
 

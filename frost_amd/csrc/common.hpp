@@ -79,11 +79,11 @@ public:
     PinnedBuffer(const PinnedBuffer&)            = delete;
     PinnedBuffer& operator=(const PinnedBuffer&) = delete;
     ~PinnedBuffer() { reset(); }
-    void reserve(std::size_t n) {
+    void reserve(std::size_t n, unsigned flags = hipHostMallocDefault) {
         if (n <= n_) return;
         reset();
         void* p = nullptr;
-        FA_HIP(hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault));
+        FA_HIP(hipHostMalloc(&p, n * sizeof(T), flags));
         p_ = static_cast<T*>(p);
         n_ = n;
     }

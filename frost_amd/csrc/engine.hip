@@ -1949,16 +1949,21 @@ void Engine::collect(hipStream_t s) {
     pending_collect_ = false;
     result_.clear();
     dropped_global_ = dropped_per_dm_ = 0;
+    // Copies on the gulp's stream (a synchronous hipMemcpy would also wait
+    // for work on other streams, such as the next gulp's ingest copy).
     uint64_t ctr[2];
-    FA_HIP(hipMemcpy(ctr, counters_.get(), sizeof(ctr), hipMemcpyDeviceToHost));
+    FA_HIP(hipMemcpyAsync(ctr, counters_.get(), sizeof(ctr), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
     frontier_             = ctr[1];
     raw_candidates_       = ctr[0];
     const uint64_t stored = std::min(ctr[0], cand_cap_);
     dropped_global_       = ctr[0] - stored;
     recs_.resize(stored);
-    if (stored)
-        FA_HIP(hipMemcpy(recs_.data(), cands_.get(), stored * sizeof(CandRecord),
-                         hipMemcpyDeviceToHost));
+    if (stored) {
+        FA_HIP(hipMemcpyAsync(recs_.data(), cands_.get(), stored * sizeof(CandRecord),
+                              hipMemcpyDeviceToHost, s));
+        FA_HIP(hipStreamSynchronize(s));
+    }
     // Order of frost's caps: (subband level, subband idx, dm idx, filter idx,
     // peak). Planes are numbered by (depth, idx) and a plane row is
     // plane_first + dm idx, so (plane row, width, peak) is the same order;
@@ -2028,8 +2033,9 @@ void Engine::drain(hipStream_t s) {
     FA_HIP(hipStreamSynchronize(s));
     const std::size_t nw = widths_.size();
     std::vector<OpenRun> st(static_cast<std::size_t>(nprows_) * nw);
-    FA_HIP(hipMemcpy(st.data(), open_.get(), st.size() * sizeof(OpenRun),
-                     hipMemcpyDeviceToHost));
+    FA_HIP(hipMemcpyAsync(st.data(), open_.get(), st.size() * sizeof(OpenRun),
+                          hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
     result_.clear();
     for (std::size_t g = 0; g < st.size(); ++g) {
         if (!st[g].open) continue;

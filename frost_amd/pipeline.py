@@ -37,6 +37,23 @@ class Beam:
         check(lib().fa_beam_execute(self.h, g.ctypes.data_as(C.c_void_p), n, 0, stream.h))
         return self.result()
 
+    def ingest_slot(self) -> np.ndarray:
+        """The next pinned host slot (uint8, capacity x row bytes) to fill
+        with a gulp's rows before ingest_submit()."""
+        ptr, size = C.c_void_p(), C.c_uint64()
+        check(lib().fa_beam_ingest_slot(self.h, C.byref(ptr), C.byref(size)))
+        buf = (C.c_uint8 * size.value).from_address(ptr.value)
+        return np.frombuffer(buf, np.uint8)
+
+    def ingest_submit(self, nsamps: int):
+        """Starts the filled slot's copy to the device (beam copy stream)."""
+        check(lib().fa_beam_ingest_submit(self.h, nsamps))
+
+    def execute_next(self, stream: Stream):
+        """Runs the oldest submitted gulp once its copy is done."""
+        check(lib().fa_beam_execute_next(self.h, stream.h))
+        return self.result()
+
     def result(self):
         n, dg, dd, fr = (C.c_uint64() for _ in range(4))
         check(lib().fa_beam_result(self.h, None, 0, C.byref(n), C.byref(dg), C.byref(dd),

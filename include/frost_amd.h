@@ -451,6 +451,18 @@ int fa_beam_destroy(fa_beam* beam);
 int fa_beam_execute(fa_beam* beam, const void* gulp, uint64_t nsamps,
                     int is_device, void* stream);
 
+/* Ingest with overlap (replaces GulpPrefetch, lib/pipeline/pipeline.cpp:
+ * 109-262, and its pageable, non-overlapped copies): fa_beam_ingest_slot
+ * returns the next of two pinned host slots (*bytes = capacity x row bytes)
+ * for the caller to fill with a gulp's rows; fa_beam_ingest_submit starts
+ * its copy to the device on the beam's copy stream; fa_beam_execute_next runs
+ * the oldest submitted gulp (as fa_beam_execute with that slot) once its copy
+ * is done. Submitting gulp k + 1 before executing gulp k overlaps the copy
+ * with the compute. At most two gulps are submitted and not yet executed. */
+int fa_beam_ingest_slot(fa_beam* beam, void** host_slot, uint64_t* bytes);
+int fa_beam_ingest_submit(fa_beam* beam, uint64_t nsamps);
+int fa_beam_execute_next(fa_beam* beam, void* stream);
+
 /* Results of the last fa_beam_execute / fa_beam_drain. */
 int fa_beam_result(fa_beam* beam, fa_raw_candidate* out, uint64_t cap,
                    uint64_t* n, uint64_t* dropped_global,

@@ -76,3 +76,41 @@ def test_beam_rejects_short_gulp_capacity():
     dcfg = abi.dedisp_config(nch, 64e-6, 1500.0, -1.0, 1000.0, block_nsamps=1024)
     with pytest.raises(abi.InvalidArgument, match="max_delay"):
         pipeline.Beam(abi.rfi_config(), lay, 0.05, dcfg, abi.boxcar_config(max_boxcar=32))
+
+
+def test_ingest_overlap_matches_direct_execute():
+    """Gulps fed through the pinned double-buffered ingest (copy of gulp k+1
+    submitted before gulp k runs) give the same candidates, masks and
+    frontier as fa_beam_execute on the same host gulps."""
+    say("test_ingest_overlap_matches_direct_execute")
+    nch, cap = 64, 8192
+    tsamp, f0, df = 256e-6, 1500.0, -300.0 / 64
+    rcfg = abi.rfi_config(iqrm_enabled=1, kadane_channel_enabled=1)
+    lay = abi.BlockLayout(nch, cap, abi.UINT8, tsamp, f0, df)
+    dcfg = abi.dedisp_config(nch, tsamp, f0, df, 300.0, block_nsamps=cap, subband_depth=1)
+    bcfg = abi.boxcar_config(max_boxcar=32)
+    stream = search.Stream(0)
+    direct = pipeline.Beam(rcfg, lay, 0.25, dcfg, bcfg)
+    fed = pipeline.Beam(rcfg, lay, 0.25, dcfg, bcfg)
+    gulps = [synth.filterbank(cap, nch, seed=60 + k, tsamp=tsamp, f0=f0, df=df,
+                              pulses=[(100.0 + 50 * k, 3000, 4, 1.5)], rfi_chans=(5,),
+                              impulses=(1000,)) for k in range(4)]
+    sizes = [cap, cap, cap, 5000]
+
+    def fill(k):
+        slot = fed.ingest_slot()
+        n = sizes[k]
+        slot[:n * nch] = gulps[k][:n].reshape(-1)
+        fed.ingest_submit(n)
+
+    fill(0)
+    for k in range(4):
+        if k + 1 < 4:
+            fill(k + 1)  # gulp k + 1 copies while gulp k computes
+        got = fed.execute_next(stream)
+        want = direct.execute(gulps[k], stream, sizes[k])
+        assert got == want, k
+        assert (fed.channel_mask() == direct.channel_mask()).all(), k
+    with pytest.raises(abi.InvalidArgument, match="no submitted gulp"):
+        fed.execute_next(stream)
+    assert fed.drain(stream) == direct.drain(stream)
