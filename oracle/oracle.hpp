@@ -122,6 +122,16 @@ public:
     std::vector<Segment> kchan_pos, kchan_neg;
     std::vector<Kadane> kstate_pos, kstate_neg;
     std::vector<float> chunk_min, chunk_max;
+    // Zero-DM / equalisation (lib/cpu/rfi_cpu_detail.hpp:72-80,145-146,194).
+    struct ZdotSums {
+        double n = 0, sz = 0, ss = 0, sss = 0, szs = 0;
+    };
+    std::vector<ZdotSums> zdot_sums;
+    std::vector<float> zdot_a, zdot_b;
+    float eq_loc = 0.0F, eq_scale = 1.0F;
+    float zero_dm_term(SizeType c, SizeType t) const;
+    float level(SizeType w, SizeType c, float v) const;
+    float to_noise_units(SizeType w, SizeType c, float out) const;
 
 private:
     SizeType window_of(SizeType t) const {

@@ -1,7 +1,6 @@
 // CPU oracle, RFI part: TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 // A restatement of frost's CPU cleaner; each function cites the frost file
-// and lines it follows. Transforms (zero_dm, equalize) are not restated:
-// the cleaner rejects them.
+// and lines it follows.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -495,8 +494,10 @@ Cleaner::Cleaner(const fa_rfi_config& c, const fa_block_layout& l,
         throw std::invalid_argument("RFI: rows must be whole bytes");
     if (!(l.tsamp > 0) || !(baseline_length > 0))
         throw std::invalid_argument("RFI: tsamp and baseline_length must be > 0");
-    if (cfg.zero_dm != FA_ZERO_DM_OFF || cfg.equalize)
-        throw std::invalid_argument("oracle: zero_dm/equalize not restated");
+    // lib/cpu/rfi_cpu.cpp:317-325
+    if ((cfg.zero_dm != FA_ZERO_DM_OFF || cfg.equalize) && l.dtype != FA_FLOAT32 &&
+        bits_of(l.dtype) < 8)
+        throw std::invalid_argument("RFICleaner: zero_dm and equalize need >= 8-bit data");
     if (cfg.zap_chans)
         zap_chans.assign(cfg.zap_chans, cfg.zap_chans + 2 * cfg.n_zap_chans);
     if (cfg.zap_freqs)
@@ -510,9 +511,19 @@ Cleaner::Cleaner(const fa_rfi_config& c, const fa_block_layout& l,
     clip             = cfg.narrow_threshold;
     narrow_active    = cfg.narrow_enabled &&
                     (l.dtype == FA_FLOAT32 || bits_of(l.dtype) > 4);
-    transform    = false;
+    transform    = cfg.zero_dm != FA_ZERO_DM_OFF || cfg.equalize;
     need_moments = cfg.moments_enabled || cfg.iqrm_enabled ||
                    cfg.kadane_channel_enabled;
+    // lib/cpu/rfi_cpu.cpp:48-55: the output level and rms of equalisation
+    // come from the input's sample type (kept when normalising floats).
+    if (l.dtype == FA_FLOAT32) {
+        eq_loc   = 0.0F;
+        eq_scale = 1.0F;
+    } else {
+        const int nbits = bits_of(l.dtype);
+        eq_loc          = (float)(uint64_t{1} << (nbits - 1));
+        eq_scale        = (float)(uint64_t{1} << nbits) / 16.0F;
+    }
     // lib/cpu/rfi_cpu.cpp:334-345 + lib/cpu/rfi_masks_cpu.cpp:16-27
     static_mask.assign(nchans, 1);
     for (SizeType i = 0; i < cfg.n_zap_chans; ++i) {
@@ -735,7 +746,7 @@ void Cleaner::build_channel_mask() {
 }
 
 void Cleaner::scan_rows() {
-    // lib/cpu/rfi_cpu_measure.cpp:305-446 (zdot omitted: zero_dm rejected)
+    // lib/cpu/rfi_cpu_measure.cpp:305-446
     const SizeType n = nchans, nchunks = chunk_count();
     const SizeType chunk = (nsamps + nchunks - 1) / nchunks;
     const bool kchan = cfg.kadane_channel_enabled, kspec = cfg.kadane_spectrum_enabled;
@@ -750,6 +761,8 @@ void Cleaner::scan_rows() {
     kspec_pos.assign(kspec ? nsamps : 0, SpanC{});
     kspec_neg.assign(kspec ? nsamps : 0, SpanC{});
     kspec_whole.assign(nsamps, 0);
+    const bool zdot = cfg.zero_dm == FA_ZERO_DM_ZDOT;
+    zdot_sums.assign(zdot ? nchunks * n : 0, ZdotSums{});
     kstate_pos.assign(kchan ? nchunks * n : 0, Kadane{});
     kstate_neg.assign(kchan ? nchunks * n : 0, Kadane{});
     chunk_min.assign(nchunks * n, std::numeric_limits<float>::max());
@@ -802,8 +815,9 @@ void Cleaner::scan_rows() {
                 }
             }
             const double dn = std::max(n_meas, 1.0);
+            const double sk = n_keep > 0.0 ? sum_keep / n_keep : 0.0;
             s_det[t]        = (float)(sum_det / dn);
-            s_keep[t]       = (float)(n_keep > 0.0 ? sum_keep / n_keep : 0.0);
+            s_keep[t]       = (float)sk;
             const double mx = sx / dn;
             row_std[t] = (float)std::sqrt(std::max((sxx / dn) - (mx * mx), 0.0));
             if (kspec) {
@@ -811,22 +825,32 @@ void Cleaner::scan_rows() {
                 wh      = accept(sn.best, kspec_neg[t]) || wh;
                 kspec_whole[t] = (uint8_t)wh;
             }
-            if (!kchan) continue;
-            const float* cm = clip_mean.data() + w * n;
-            const float* ci = clip_inv_std.data() + w * n;
+            if (!zdot && !kchan) continue;
+            const float* cm = kchan ? clip_mean.data() + w * n : nullptr;
+            const float* ci = kchan ? clip_inv_std.data() + w * n : nullptr;
             for (SizeType c = 0; c < n; ++c) {
                 if (mask[c] == 0) continue;
                 const bool outlier = std::abs(z[c]) > clip;
-                const double u = outlier ? 0.0 : (double)((z[c] - cm[c]) * ci[c]);
-                kstate_pos[k * n + c].append((IndexType)t, u);
-                kstate_neg[k * n + c].append((IndexType)t, -u);
+                if (zdot && !outlier && ic[c] != 0.0F) {
+                    ZdotSums& a = zdot_sums[k * n + c];
+                    a.n += 1.0;
+                    a.sz += z[c];
+                    a.ss += sk;
+                    a.sss += sk * sk;
+                    a.szs += z[c] * sk;
+                }
+                if (kchan) {
+                    const double u = outlier ? 0.0 : (double)((z[c] - cm[c]) * ci[c]);
+                    kstate_pos[k * n + c].append((IndexType)t, u);
+                    kstate_neg[k * n + c].append((IndexType)t, -u);
+                }
             }
         }
     }
 }
 
 void Cleaner::build_flags() {
-    // lib/cpu/rfi_cpu.cpp:347-468 (zdot omitted)
+    // lib/cpu/rfi_cpu.cpp:347-468
     const SizeType n = nchans, nchunks = chunk_count();
     for (SizeType c = 0; c < n; ++c) {
         if (mask[c] == 0 || unmeasured[c] < nwin) continue;
@@ -874,6 +898,31 @@ void Cleaner::build_flags() {
             }
     report.samps_flagged =
         (uint64_t)std::count(time_flags.begin(), time_flags.end(), 1);
+    if (cfg.zero_dm == FA_ZERO_DM_ZDOT) {
+        // lib/cpu/rfi_cpu.cpp:413-437: per channel, the chunk sums in chunk
+        // order, then the regression of z on the kept band mean.
+        zdot_a.assign(n, 0.0F);
+        zdot_b.assign(n, 0.0F);
+        for (SizeType c = 0; c < n; ++c) {
+            ZdotSums s;
+            for (SizeType k = 0; k < nchunks; ++k) {
+                const ZdotSums& p = zdot_sums[k * n + c];
+                s.n += p.n;
+                s.sz += p.sz;
+                s.ss += p.ss;
+                s.sss += p.sss;
+                s.szs += p.szs;
+            }
+            if (s.n < 2.0) continue;
+            const double mz    = s.sz / s.n;
+            const double ms    = s.ss / s.n;
+            const double var_s = (s.sss / s.n) - (ms * ms);
+            const double cov   = (s.szs / s.n) - (mz * ms);
+            const double a     = var_s > 0.0 ? cov / var_s : 0.0;
+            zdot_a[c]          = (float)a;
+            zdot_b[c]          = (float)(mz - (a * ms));
+        }
+    }
     if (cfg.kadane_channel_enabled) {
         const double min_len =
             (double)cfg.kadane_channel_min_duration / layout.tsamp;
@@ -905,16 +954,39 @@ uint64_t Cleaner::draw(uint64_t stream, SizeType t, SizeType c,
                  (uint64_t)c, (uint64_t)attempt);
 }
 
+float Cleaner::zero_dm_term(SizeType c, SizeType t) const {
+    // lib/cpu/rfi_cpu.cpp:470-480
+    switch (cfg.zero_dm) {
+    case FA_ZERO_DM_SUBTRACT: return s_keep[t];
+    case FA_ZERO_DM_ZDOT: return (zdot_a[c] * s_keep[t]) + zdot_b[c];
+    default: return 0.0F;
+    }
+}
+
+float Cleaner::level(SizeType w, SizeType c, float v) const {
+    // lib/cpu/rfi_cpu.cpp:491-497
+    return cfg.equalize ? eq_loc + (eq_scale * v)
+                        : loc[w * nchans + c] + (scale[w * nchans + c] * v);
+}
+
+float Cleaner::to_noise_units(SizeType w, SizeType c, float out) const {
+    // lib/cpu/rfi_cpu.cpp:505-513
+    if (cfg.equalize) return (out - eq_loc) / eq_scale;
+    const SizeType i = w * nchans + c;
+    return (out - loc[i]) * inv_scale[i];
+}
+
 float Cleaner::fill_value(uint64_t stream, SizeType w, SizeType t,
                           SizeType c) const {
     // lib/cpu/rfi_cpu.cpp:493-524 (float codec: quantise is the identity)
     const float v = cfg.fill == FA_FILL_NOISE ? normal_draw(draw(stream, t, c, 0))
                                               : 0.0F;
-    return loc[w * nchans + c] + (scale[w * nchans + c] * v);
+    return level(w, c, v);
 }
 
 void Cleaner::apply() {
-    // lib/cpu/rfi_cpu.cpp:526-720 (float codec, no transform, no re-zero)
+    // lib/cpu/rfi_cpu.cpp:526-720 (float codec: quantise is the identity, so
+    // the re-zero carry is 0)
     const SizeType n = nchans;
     const bool kchan = cfg.kadane_channel_enabled, kspec = cfg.kadane_spectrum_enabled;
     const bool nbr   = cfg.fill == FA_FILL_NEIGHBOUR;
@@ -945,12 +1017,14 @@ void Cleaner::apply() {
                 state[c] = kKadaneChannel;
             else {
                 state[c] = kClean;
-                v[c]     = z[c] - 0.0F;
+                v[c]     = z[c] - zero_dm_term(c, t);
             }
         }
         for (SizeType c = 0; c < n; ++c) {
             switch (state[c]) {
             case kClean:
+                if (transform) out[c] = level(w, c, v[c]);
+                continue;
             case kUnmeasured: continue;
             case kMasked: out[c] = fill_value(kStreamMasked, w, t, c); continue;
             case kNarrow: ++cn; break;
@@ -974,41 +1048,58 @@ void Cleaner::apply() {
                 }
             }
             fb += found ? 0 : 1;
-            out[c] = lc[c] + (scale[w * n + c] * value);
+            out[c] = level(w, c, value);
         }
     }
     std::vector<SizeType> flagged;
     for (SizeType t = 0; t < nsamps; ++t)
         if (time_flags[t]) flagged.push_back(t);
+    // With zero-DM on, a refilled spectrum is rebuilt in noise units and its
+    // band mean removed (lib/cpu/rfi_cpu.cpp:631-712).
+    const bool rezero = cfg.zero_dm != FA_ZERO_DM_OFF;
     for (const SizeType t : flagged) {
         const SizeType w  = window_of(t);
         const SizeType r  = time_fill_radius;
         const SizeType lo = t > r ? t - r : 0;
         const SizeType hi = std::min(t + r, nsamps - 1);
         float* out        = row(t);
+        double sum = 0.0, count = 0.0;
         for (SizeType c = 0; c < n; ++c) {
             if (mask[c] == 0) {
                 out[c] = fill_value(kStreamMasked, w, t, c);
                 continue;
             }
-            float vc = 0.0F;
+            v[c] = 0.0F;
             if (!nbr) {
-                vc = cfg.fill == FA_FILL_NOISE
-                         ? normal_draw(draw(kStreamNoise, t, c, 0))
-                         : 0.0F;
+                v[c] = cfg.fill == FA_FILL_NOISE
+                           ? normal_draw(draw(kStreamNoise, t, c, 0))
+                           : 0.0F;
             } else {
                 bool found = false;
                 for (int a = 0; a < kMaxDraws && !found; ++a) {
                     const SizeType src =
                         uniform_index(draw(kStreamTime, t, c, a), lo, hi);
                     if (time_flags[src]) continue;
-                    found  = true;
-                    out[c] = row(src)[c];
+                    found = true;
+                    if (rezero)
+                        v[c] = to_noise_units(window_of(src), c, row(src)[c]);
+                    else
+                        out[c] = row(src)[c];
                 }
                 if (!found) ++fb;
-                if (found) continue;
+                if (found && !rezero) continue;
             }
-            out[c] = loc[w * n + c] + (scale[w * n + c] * vc);
+            if (rezero && inv_scale[w * n + c] != 0.0F) {
+                sum += v[c];
+                count += 1.0;
+            }
+            if (!rezero) out[c] = level(w, c, v[c]);
+        }
+        if (!rezero) continue;
+        const float mean = (float)(count > 0.0 ? sum / count : 0.0);
+        for (SizeType c = 0; c < n; ++c) {
+            if (mask[c] == 0) continue;
+            out[c] = level(w, c, v[c] - mean);
         }
     }
     report.cells_narrow          = cn;
@@ -1018,7 +1109,8 @@ void Cleaner::apply() {
 }
 
 void Cleaner::normalize_short(SizeType capacity, SizeType valid, float* out) {
-    // lib/cpu/rfi_cpu.cpp:262-315 (no transform: output = work value)
+    // lib/cpu/rfi_cpu.cpp:262-315: only the transforms run on a short tail,
+    // with the preceding block's calibration and zdot regression.
     report                    = fa_rfi_report{};
     report.nsamps             = valid;
     report.chans_zapped       = (uint64_t)std::count(static_mask.begin(),
@@ -1026,14 +1118,36 @@ void Cleaner::normalize_short(SizeType capacity, SizeType valid, float* out) {
     report.chans_masked       = (uint64_t)std::count(mask.begin(), mask.end(), 0);
     report.calibration_reused = 1;
     std::fill(out, out + capacity * nchans, 0.0F);
-    for (SizeType t = 0; t < valid; ++t)
+    std::vector<float> z(nchans);
+    for (SizeType t = 0; t < valid; ++t) {
+        double sum     = 0;
+        SizeType count = 0;
+        for (SizeType c = 0; c < nchans; ++c) {
+            z[c] = (work[t * nchans + c] - tail_loc[c]) *
+                   (tail_scale[c] > 0 ? 1.0F / tail_scale[c] : 1.0F);
+            if (mask[c] && tail_scale[c] > 0 && std::abs(z[c]) <= clip) {
+                sum += z[c];
+                ++count;
+            }
+        }
+        const float band_mean = count ? (float)(sum / (double)count) : 0;
         for (SizeType c = 0; c < nchans; ++c) {
             if (!mask[c]) continue;
-            const float transformed = work[t * nchans + c];
+            float term = 0;
+            if (cfg.zero_dm == FA_ZERO_DM_SUBTRACT)
+                term = band_mean;
+            else if (cfg.zero_dm == FA_ZERO_DM_ZDOT)
+                term = zdot_a[c] * band_mean + zdot_b[c];
+            const float transformed =
+                tail_scale[c] <= 0 ? work[t * nchans + c]
+                : cfg.equalize     ? eq_loc + eq_scale * (z[c] - term)
+                : transform        ? tail_loc[c] + tail_scale[c] * (z[c] - term)
+                                   : work[t * nchans + c];
             const float sc = tail_out_scale[c] > 0 ? 1.0F / tail_out_scale[c] : 1.0F;
             const SizeType dst = layout.df < 0 ? nchans - 1 - c : c;
             out[dst * capacity + t] = (transformed - tail_out_loc[c]) * sc;
         }
+    }
     ++block_index;
 }
 
@@ -1065,29 +1179,36 @@ void Cleaner::execute_normalized(const uint8_t* block, SizeType capacity,
     }
     nsamps = valid;
     execute_float();
+    const auto last = short_block ? 0 : (nwin - 1) * nchans;
     if (short_block) {
         nsamps = valid;
         nwin   = 1;
         loc.assign(nchans, 0.0F);
         scale.assign(nchans, 0.0F);
-        // Welford per channel (lib/cpu/rfi_cpu.cpp:222-239)
-        for (SizeType c = 0; c < nchans; ++c) {
-            double mean = 0.0, var = 0.0;
-            for (SizeType t = 0; t < nsamps; ++t) {
-                const double x     = work[t * nchans + c];
-                const double delta = x - mean;
-                mean += delta / (double)(t + 1);
-                var += delta * (x - mean);
-            }
-            loc[c]   = (float)mean;
-            scale[c] = nsamps > 0 ? (float)std::sqrt(var / (double)nsamps) : 0.0F;
-        }
     } else {
-        const auto last = (nwin - 1) * nchans;
         tail_loc.assign(loc.begin() + last, loc.end());
         tail_scale.assign(scale.begin() + last, scale.end());
-        tail_out_loc   = tail_loc;
-        tail_out_scale = tail_scale;
+    }
+    if (short_block || transform) {
+        // Welford per (window, channel) over the cleaned (transformed) rows
+        // (lib/cpu/rfi_cpu.cpp:217-239).
+        for (SizeType i = 0; i < nwin * nchans; ++i) {
+            const SizeType w = i / nchans, c = i % nchans;
+            const SizeType b = window_begin(w), e = window_end(w);
+            double mean = 0.0, var = 0.0;
+            for (SizeType t = b; t < e; ++t) {
+                const double x     = work[t * nchans + c];
+                const double delta = x - mean;
+                mean += delta / (double)(t - b + 1);
+                var += delta * (x - mean);
+            }
+            loc[i]   = (float)mean;
+            scale[i] = e > b ? (float)std::sqrt(var / (double)(e - b)) : 0.0F;
+        }
+    }
+    if (!short_block) {
+        tail_out_loc.assign(loc.begin() + last, loc.end());
+        tail_out_scale.assign(scale.begin() + last, scale.end());
     }
     std::fill(out, out + capacity * nchans, 0.0F);
     for (SizeType c = 0; c < nchans; ++c) {
