@@ -181,6 +181,28 @@ def ingest_measure(W, beam, gulp, stream, ndm, steps=3):
             "unit": "DM-trial*samples/s"}
 
 
+def rfi_transform_measure(W, gulp, stream, steps=2):
+    """The RFI stage alone on one gulp, with the workload's strategies, and
+    with zero-DM (zdot) and equalisation added: ms per gulp of each, into one
+    device output [nchans][gulp]."""
+    from frost_amd import abi, prepare, search
+    out = search.DeviceArray(W.nchans * W.gulp * 4)
+    res = {}
+    for name, extra in (("workload_rfi", {}),
+                        ("with_zdot_and_equalize", {"zero_dm": abi.ZERO_DM_ZDOT, "equalize": 1})):
+        cl = prepare.RFICleaner(abi.rfi_config(**{**W.rfi, **extra}), W.layout(), W.baseline)
+        cl.execute_normalized_device(gulp, W.gulp, W.gulp, out, W.gulp, 0, stream)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            cl.execute_normalized_device(gulp, W.gulp, W.gulp, out, W.gulp, 0, stream)
+        stream.synchronize()
+        res[name] = 1e3 * (time.perf_counter() - t0) / steps
+        del cl
+    del out
+    return {"ms_per_gulp": res, "steps": steps}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +287,8 @@ def main() -> int:
     ingest = ingest_measure(W, beam, gulp, stream, ndm)
     log(f"ingest from pinned host slots: serial {ingest['serial_ms_per_gulp']:.1f} ms, "
         f"overlapped {ingest['overlapped_ms_per_gulp']:.1f} ms per gulp")
+    rfi_tf = rfi_transform_measure(W, gulp, stream)
+    log(f"RFI stage alone: {rfi_tf['ms_per_gulp']}")
     # Algorithmic bytes (unfused FDMT): a launch covering levels a -> b does
     # the work of b - a merge levels, each reading its input rows and writing
     # its output rows once, float32 per sample: sum_{l=a}^{b-1} 4 N (R_l + R_{l+1}).
@@ -314,6 +338,7 @@ def main() -> int:
                        "host_collect_seconds_per_gulp": collect_s / a.steps},
         "postprocess": post,
         "ingest_from_host": ingest,
+        "rfi_transforms": rfi_tf,
         "roofline": {
             "kernel": "fdmt_gather_kernel<16> (FDMT levels l-4 -> l in one pass)",
             "levels": [[int(src[l]), int(l)] for l in r16],

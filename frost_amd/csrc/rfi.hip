@@ -21,10 +21,11 @@ namespace {
 constexpr uint64_t kMaxBandpassRows = 3125;
 constexpr uint64_t kMinWindow       = 5;
 constexpr uint64_t kMinBlockSamps   = 64;
+constexpr uint64_t kRowBlocks       = 512; // blocks of the per-row transform kernels
 constexpr int kMaxDraws             = 10;
 constexpr uint64_t kMaxChunks       = 64;
 constexpr uint64_t kMinChunkRows    = 1024;
-constexpr uint64_t kStreamCell = 1, kStreamTime = 2, kStreamNoise = 3;
+constexpr uint64_t kStreamCell = 1, kStreamTime = 2, kStreamNoise = 3, kStreamMasked = 4;
 
 int dtype_bits(int dtype) {
     switch (dtype) {
@@ -455,6 +456,7 @@ struct ScanArgs {
     float* row_std;
     int32_t* kspec_span; // [nsamps][4]: pos lo, hi, neg lo, hi
     uint8_t* kwhole;
+    double* s_keep64;    // the kept band mean in FP64 (zdot), or null
 };
 
 constexpr int kRowTileCh = 128; // channels per LDS tile (u8: 8.3 KB per wave)
@@ -623,7 +625,9 @@ __global__ void __launch_bounds__(64) scan_rows_kernel(ScanArgs a) {
     }
     const double dn = n_meas > 1 ? (double)n_meas : 1.0;
     a.s_det[t]      = (float)(sum_det / dn);
-    a.s_keep[t]     = (float)(n_keep > 0 ? sum_keep / (double)n_keep : 0.0);
+    const double sk = n_keep > 0 ? sum_keep / (double)n_keep : 0.0;
+    a.s_keep[t]     = (float)sk;
+    if (a.s_keep64) a.s_keep64[t] = sk;
     const double mx = sx / dn;
     const double vr = (sxx / dn) - (mx * mx);
     a.row_std[t]    = (float)sqrt(vr > 0.0 ? vr : 0.0);
@@ -665,6 +669,8 @@ struct ColArgs {
     float* cmax;
     KadaneDev* kpos;
     KadaneDev* kneg;
+    const double* s_keep64; // zdot: kept band mean of each row (FP64)
+    ZdotDev* zsums;         // zdot: [chunk][channel] sums, or null
 };
 
 // One lane per (chunk, channel), rows in order: min/max of unmeasured cells
@@ -682,6 +688,9 @@ __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
     kreg_init(kp);
     kreg_init(kn);
     int64_t last = -1;
+    // zdot regression sums (lib/cpu/rfi_cpu_measure.cpp:422-432): kept,
+    // measured cells in time order.
+    double zn = 0.0, zsz = 0.0, zss = 0.0, zsss = 0.0, zszs = 0.0;
     if (a.mask[c] != 0) {
         const uint64_t t0 = k * a.chunk;
         const uint64_t t1 = t0 + a.chunk < g.nsamps ? t0 + a.chunk : g.nsamps;
@@ -729,6 +738,14 @@ __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
                 } else {
                     z = (x - lc) * ic;
                 }
+                if (a.zsums && ic != 0.0f && !(fabsf(z) > a.clip)) {
+                    const double sk = a.s_keep64[t];
+                    zn += 1.0;
+                    zsz += z;
+                    zss += sk;
+                    zsss += sk * sk;
+                    zszs += z * sk;
+                }
                 if (a.kchan) {
                     const bool outlier = fabsf(z) > a.clip;
                     const double u     = outlier ? 0.0 : (double)((z - cm) * ci);
@@ -741,10 +758,42 @@ __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
     }
     a.cmin[o] = lo;
     a.cmax[o] = hi;
+    if (a.zsums) a.zsums[o] = ZdotDev{zn, zsz, zss, zsss, zszs};
     if (a.kchan) {
         a.kpos[o] = kreg_state(kp, last);
         a.kneg[o] = kreg_state(kn, last);
     }
+}
+
+// Zero-DM regression per channel (lib/cpu/rfi_cpu.cpp:413-437): chunk sums
+// in chunk order, then a = cov / var, b = mean(z) - a mean(s). No sums (no
+// good channel) leave a = b = 0.
+__global__ void zdot_fit_kernel(const ZdotDev* __restrict__ zs, uint64_t nchunks, uint64_t nch,
+                                int have_sums, float* __restrict__ za, float* __restrict__ zb) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    ZdotDev s{0.0, 0.0, 0.0, 0.0, 0.0};
+    if (have_sums)
+        for (uint64_t k = 0; k < nchunks; ++k) {
+            const ZdotDev p = zs[k * nch + c];
+            s.n += p.n;
+            s.sz += p.sz;
+            s.ss += p.ss;
+            s.sss += p.sss;
+            s.szs += p.szs;
+        }
+    float a = 0.0f, b = 0.0f;
+    if (s.n >= 2.0) {
+        const double mz    = s.sz / s.n;
+        const double ms    = s.ss / s.n;
+        const double var_s = (s.sss / s.n) - (ms * ms);
+        const double cov   = (s.szs / s.n) - (mz * ms);
+        const double ad    = var_s > 0.0 ? cov / var_s : 0.0;
+        a                  = (float)ad;
+        b                  = (float)(mz - (ad * ms));
+    }
+    za[c] = a;
+    zb[c] = b;
 }
 
 // Per channel: dead-channel test, then Kadane-along-time acceptance
@@ -815,7 +864,27 @@ struct ApplyArgs {
     float* out;
     uint64_t stride, offset;
     unsigned long long* counters; // narrow, kspec, kchan, fallbacks
+    // Transforms (lib/cpu/rfi_cpu.cpp:470-524): clean cells and fills are
+    // written as values (not normalised); flagged rows are rebuilt by
+    // rezero_rows_kernel when zero-DM is on.
+    int transform, zero_dm, equalize, rezero;
+    float eq_loc, eq_scale;
+    const float* s_keep; // kept band mean per row (float)
+    const float* zdot_a;
+    const float* zdot_b;
 };
+
+// zero_dm_term (lib/cpu/rfi_cpu.cpp:470-480).
+__device__ __forceinline__ float zterm(const ApplyArgs& a, uint64_t c, uint64_t t) {
+    if (a.zero_dm == FA_ZERO_DM_SUBTRACT) return a.s_keep[t];
+    if (a.zero_dm == FA_ZERO_DM_ZDOT) return (a.zdot_a[c] * a.s_keep[t]) + a.zdot_b[c];
+    return 0.0f;
+}
+
+// level (lib/cpu/rfi_cpu.cpp:491-497): noise units to output units.
+__device__ __forceinline__ float level(const ApplyArgs& a, uint64_t i, float v) {
+    return a.equalize ? a.eq_loc + (a.eq_scale * v) : a.loc[i] + (a.scale[i] * v);
+}
 
 enum : int { kClean = 0, kMasked = 1, kUnmeasured = 2, kNarrow = 3, kKSpec = 4, kKChan = 5 };
 
@@ -851,13 +920,14 @@ __device__ float cleaned_unflagged(const ApplyArgs& a, uint64_t t, uint64_t c, i
     if (state_out) *state_out = st;
     if (fallback) *fallback = false;
     const float x = a.g.x(t, c);
-    if (st == kClean || st == kUnmeasured || st == kMasked) return x;
     const uint64_t i = w * a.g.nch + c;
+    if (st == kClean) return a.transform ? level(a, i, z - zterm(a, c, t)) : x;
+    if (st == kUnmeasured || st == kMasked) return x;
     if (a.fill != FA_FILL_NEIGHBOUR) {
         const float v = a.fill == FA_FILL_NOISE
                             ? normal_draw(hash5(a.seed, a.block_index, (kStreamNoise << 56) ^ t, c, 0))
                             : 0.0f;
-        return a.loc[i] + (a.scale[i] * v);
+        return level(a, i, v);
     }
     const uint64_t lo = c > a.chan_r ? c - a.chan_r : 0;
     const uint64_t hi = c + a.chan_r < a.g.nch - 1 ? c + a.chan_r : a.g.nch - 1;
@@ -868,12 +938,12 @@ __device__ float cleaned_unflagged(const ApplyArgs& a, uint64_t t, uint64_t c, i
         const uint64_t src = lo + h % (hi - lo + 1);
         float zs           = 0.0f;
         if (cell_state(a, t, src, w, zs) == kClean) {
-            value = zs - 0.0f;
+            value = zs - zterm(a, src, t);
             found = true;
         }
     }
     if (fallback) *fallback = !found;
-    return a.loc[i] + (a.scale[i] * value);
+    return level(a, i, value);
 }
 
 // Cells that need a fill (flagged rows, narrow-band / Kadane cells): the
@@ -891,12 +961,14 @@ __device__ float apply_slow(const ApplyArgs& a, uint64_t t, uint64_t c,
         else if (st == kKSpec) ++local[1];
         else if (st == kKChan) ++local[2];
         if (fb) ++local[3];
+    } else if (a.rezero) {
+        return 0.0f; // rebuilt by rezero_rows_kernel
     } else if (a.fill != FA_FILL_NEIGHBOUR) {
         const float vv = a.fill == FA_FILL_NOISE
                              ? normal_draw(hash5(a.seed, a.block_index,
                                                  (kStreamNoise << 56) ^ t, c, 0))
                              : 0.0f;
-        cleaned = a.loc[i] + (a.scale[i] * vv);
+        cleaned = level(a, i, vv);
     } else {
         const uint64_t lo = t > a.time_r ? t - a.time_r : 0;
         const uint64_t hi = t + a.time_r < a.g.nsamps - 1 ? t + a.time_r : a.g.nsamps - 1;
@@ -912,9 +984,10 @@ __device__ float apply_slow(const ApplyArgs& a, uint64_t t, uint64_t c,
         }
         if (!found) {
             ++local[3];
-            cleaned = a.loc[i] + (a.scale[i] * 0.0f);
+            cleaned = level(a, i, 0.0f);
         }
     }
+    if (a.transform) return cleaned;
     const float sc = a.scale[i] > 0.0f ? 1.0f / a.scale[i] : 1.0f;
     return (cleaned - a.loc[i]) * sc;
 }
@@ -930,6 +1003,7 @@ __global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* 
                                                          unsigned int* __restrict__ nslow) {
     __shared__ float tile[64][65];
     __shared__ int32_t rowinfo[64][5]; // [row]: flagged, Kadane-spectrum spans lo0 hi0 lo1 hi1
+    __shared__ float rowkeep[64];      // kept band mean of each row (zero-DM)
     __shared__ int any_slow;
     const uint64_t t0  = (uint64_t)blockIdx.x * 64;
     const uint64_t c0  = (uint64_t)blockIdx.y * 64;
@@ -946,6 +1020,7 @@ __global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* 
                 for (int q = 0; q < 4; ++q) info[1 + q] = a.kspec[4 * t + q];
         }
         for (int q = 0; q < 5; ++q) rowinfo[threadIdx.x][q] = info[q];
+        rowkeep[threadIdx.x] = (a.zero_dm != FA_ZERO_DM_OFF && t < ns) ? a.s_keep[t] : 0.0f;
     }
     if (threadIdx.x == 0) any_slow = 0;
     // The tile's rows lie in window wa, or in wa then wb from row tb on.
@@ -956,6 +1031,7 @@ __global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* 
     const bool good    = c < a.g.nch && a.mask[c] != 0 && wb - wa <= 1;
     int64_t ks[4]      = {0, -1, 0, -1};
     float lca = 0.0f, ica = 0.0f, sca = 1.0f, lcb = 0.0f, icb = 0.0f, scb = 1.0f;
+    float rsa = 1.0f, rsb = 1.0f, za = 0.0f, zb = 0.0f; // transforms: scales, zdot fit
     if (good) {
         if (a.kseg)
             for (int q = 0; q < 4; ++q) ks[q] = a.kseg[4 * c + q];
@@ -966,6 +1042,12 @@ __global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* 
         lcb               = a.loc[ib];
         icb               = a.inv[ib];
         scb               = a.scale[ib] > 0.0f ? 1.0f / a.scale[ib] : 1.0f;
+        rsa               = a.scale[ia];
+        rsb               = a.scale[ib];
+        if (a.zero_dm == FA_ZERO_DM_ZDOT) {
+            za = a.zdot_a[c];
+            zb = a.zdot_b[c];
+        }
     }
     // Samples of this thread's rows, loaded up front at clamped in-range
     // addresses (out-of-range cells are discarded below).
@@ -1000,7 +1082,19 @@ __global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* 
                 if ((it >= ks[0] && it <= ks[1]) || (it >= ks[2] && it <= ks[3])) clean = false;
             }
             slow_cell = slow_cell || !clean;
-            v         = (x - lc) * sc;
+            if (!a.transform) {
+                v = (x - lc) * sc;
+            } else if (ic != 0.0f) {
+                // Clean cell as a value (a tile with any other cell is slow).
+                const float z    = (x - lc) * ic;
+                const float term = a.zero_dm == FA_ZERO_DM_SUBTRACT ? rowkeep[r]
+                                   : a.zero_dm == FA_ZERO_DM_ZDOT   ? (za * rowkeep[r]) + zb
+                                                                    : 0.0f;
+                const float vv   = z - term;
+                v = a.equalize ? a.eq_loc + (a.eq_scale * vv) : lc + ((inb ? rsb : rsa) * vv);
+            } else {
+                v = x;
+            }
         }
         tile[cl][r] = v;
     }
@@ -1058,6 +1152,196 @@ __global__ void __launch_bounds__(256) apply_slow_kernel(ApplyArgs a,
         }
         if (threadIdx.x < 4 && cnt[threadIdx.x])
             atomicAdd(&a.counters[threadIdx.x], cnt[threadIdx.x]);
+        __syncthreads();
+    }
+}
+
+// Flagged rows with zero-DM on (lib/cpu/rfi_cpu.cpp:631-712): each unmasked
+// cell's value in noise units (a time draw's source value, or the fill),
+// then the band mean over the cells measured in the row's window, summed in
+// channel order on one lane, and the row written as level(v - mean) (the
+// float output's rounding carry is 0). One block per row; the block's row
+// values go to rowbuf + blockIdx.x * nch (contiguous, so the one-lane sum
+// streams them).
+__global__ void __launch_bounds__(256)
+rezero_rows_kernel(ApplyArgs a, const uint32_t* __restrict__ rows, uint32_t nrows,
+                   float* __restrict__ rowbuf) {
+    float* vbuf = rowbuf + (uint64_t)blockIdx.x * a.g.nch;
+    __shared__ float s_mean;
+    __shared__ unsigned long long s_fb;
+    const uint64_t n = a.g.nch;
+    for (uint32_t k = blockIdx.x; k < nrows; k += gridDim.x) {
+        const uint64_t t  = rows[k];
+        const uint64_t w  = a.g.window_of(t);
+        const uint64_t lo = t > a.time_r ? t - a.time_r : 0;
+        const uint64_t hi = t + a.time_r < a.g.nsamps - 1 ? t + a.time_r : a.g.nsamps - 1;
+        if (threadIdx.x == 0) s_fb = 0;
+        __syncthreads();
+        unsigned long long fb = 0;
+        for (uint64_t c = threadIdx.x; c < n; c += blockDim.x) {
+            float v = 0.0f;
+            if (a.mask[c] != 0) {
+                if (a.fill != FA_FILL_NEIGHBOUR) {
+                    v = a.fill == FA_FILL_NOISE
+                            ? normal_draw(hash5(a.seed, a.block_index, (kStreamNoise << 56) ^ t, c, 0))
+                            : 0.0f;
+                } else {
+                    bool found = false;
+                    for (int d = 0; d < kMaxDraws && !found; ++d) {
+                        const uint64_t h =
+                            hash5(a.seed, a.block_index, (kStreamTime << 56) ^ t, c, (uint64_t)d);
+                        const uint64_t src = lo + h % (hi - lo + 1);
+                        if (a.flags[src] != 0) continue;
+                        found             = true;
+                        const float out   = cleaned_unflagged(a, src, c, nullptr, nullptr);
+                        const uint64_t is = a.g.window_of(src) * n + c;
+                        v = a.equalize ? (out - a.eq_loc) / a.eq_scale : (out - a.loc[is]) * a.inv[is];
+                    }
+                    if (!found) ++fb;
+                }
+            }
+            vbuf[c] = v;
+        }
+        if (fb) atomicAdd(&s_fb, fb);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sum = 0.0, count = 0.0;
+            for (uint64_t c = 0; c < n; ++c)
+                if (a.mask[c] != 0 && a.inv[w * n + c] != 0.0f) {
+                    sum += vbuf[c];
+                    count += 1.0;
+                }
+            s_mean = (float)(count > 0.0 ? sum / count : 0.0);
+            if (s_fb) atomicAdd(&a.counters[3], s_fb);
+        }
+        __syncthreads();
+        const float mean = s_mean;
+        for (uint64_t c = threadIdx.x; c < n; c += blockDim.x) {
+            if (a.mask[c] == 0) continue;
+            const uint64_t dst = a.flip ? n - 1 - c : c;
+            a.out[dst * a.stride + a.offset + t] = level(a, w * n + c, vbuf[c] - mean);
+        }
+        __syncthreads();
+    }
+}
+
+// Mean and population std of each (window, channel) of the cleaned values
+// (lib/cpu/rfi_cpu.cpp:217-239): Welford in time order, FP64, one lane per
+// channel walking its column of the output just written. A masked channel's
+// values are its fills, level(fill of the masked stream) in every row
+// (lib/cpu/rfi_cpu.cpp:493-524), computed here (its output column is 0).
+__global__ void welford_windows_kernel(ApplyArgs a, float* __restrict__ loc2,
+                                       float* __restrict__ scale2) {
+    const Geometry& g = a.g;
+    const uint64_t w  = blockIdx.x;
+    const uint64_t c  = (uint64_t)blockIdx.y * 64 + threadIdx.x;
+    if (c >= g.nch) return;
+    const uint64_t b   = g.window_begin(w), e = g.window_end(w);
+    const float* col   = a.out + (a.flip ? g.nch - 1 - c : c) * a.stride + a.offset;
+    const bool masked  = a.mask[c] == 0;
+    const bool noise   = a.fill == FA_FILL_NOISE;
+    double mean = 0.0, var = 0.0;
+    for (uint64_t t = b; t < e; ++t) {
+        float v = 0.0f;
+        if (masked)
+            v = level(a, w * g.nch + c,
+                      noise ? normal_draw(hash5(a.seed, a.block_index, (kStreamMasked << 56) ^ t, c, 0))
+                            : 0.0f);
+        const double x     = masked ? (double)v : (double)col[t];
+        const double delta = x - mean;
+        mean += delta / (double)(t - b + 1);
+        var += delta * (x - mean);
+    }
+    loc2[w * g.nch + c]   = (float)mean;
+    scale2[w * g.nch + c] = e > b ? (float)sqrt(var / (double)(e - b)) : 0.0f;
+}
+
+// Normalises the written values in place with the re-measured calibration
+// (lib/cpu/rfi_cpu.cpp:245-257); masked channels and padding rows are 0.
+__global__ void normalize_windows_kernel(float* __restrict__ out, uint64_t stride, uint64_t offset,
+                                         int flip, Geometry g, const float* __restrict__ loc2,
+                                         const float* __restrict__ scale2,
+                                         const uint8_t* __restrict__ mask, uint64_t capacity) {
+    const uint64_t c = blockIdx.y;
+    float* col       = out + (flip ? g.nch - 1 - c : c) * stride + offset;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < capacity;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        float v = 0.0f;
+        if (t < g.nsamps && mask[c] != 0) {
+            const uint64_t i = g.window_of(t) * g.nch + c;
+            const float sc   = scale2[i] > 0.0f ? 1.0f / scale2[i] : 1.0f;
+            v                = (col[t] - loc2[i]) * sc;
+        }
+        col[t] = v;
+    }
+}
+
+// Short tails with a transform (lib/cpu/rfi_cpu.cpp:262-315): z against the
+// last full block's calibration, the row's band mean over kept measured
+// cells (channel order, one lane), the transform, then normalisation with
+// the re-measured calibration of that block. One block per row t < nsamps
+// (padding rows are cleared by zero_rows_kernel); the row's z go to
+// rowbuf + blockIdx.x * nch.
+struct TailArgs {
+    Geometry g;
+    const float* loc;
+    const float* scale;
+    const float* out_loc;
+    const float* out_scale;
+    const uint8_t* mask;
+    const float* zdot_a;
+    const float* zdot_b;
+    float clip, eq_loc, eq_scale;
+    int zero_dm, equalize, flip;
+    uint64_t stride, offset;
+    float* out;
+};
+
+// Clears rows [from, to) of every channel's output.
+__global__ void zero_rows_kernel(float* __restrict__ out, uint64_t stride, uint64_t offset,
+                                 uint64_t from, uint64_t to) {
+    float* col = out + (uint64_t)blockIdx.y * stride + offset;
+    for (uint64_t t = from + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < to;
+         t += (uint64_t)gridDim.x * blockDim.x)
+        col[t] = 0.0f;
+}
+
+__global__ void __launch_bounds__(256) short_transform_kernel(TailArgs a, float* __restrict__ rowbuf) {
+    float* zbuf = rowbuf + (uint64_t)blockIdx.x * a.g.nch;
+    __shared__ float s_band;
+    const uint64_t n = a.g.nch;
+    for (uint64_t t = blockIdx.x; t < a.g.nsamps; t += gridDim.x) {
+        for (uint64_t c = threadIdx.x; c < n; c += blockDim.x)
+            zbuf[c] = (a.g.x(t, c) - a.loc[c]) * (a.scale[c] > 0 ? 1.0f / a.scale[c] : 1.0f);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sum     = 0;
+            uint64_t count = 0;
+            for (uint64_t c = 0; c < n; ++c)
+                if (a.mask[c] && a.scale[c] > 0 && fabsf(zbuf[c]) <= a.clip) {
+                    sum += zbuf[c];
+                    ++count;
+                }
+            s_band = count ? (float)(sum / (double)count) : 0.0f;
+        }
+        __syncthreads();
+        const float band = s_band;
+        for (uint64_t c = threadIdx.x; c < n; c += blockDim.x) {
+            float v = 0.0f;
+            if (a.mask[c]) {
+                float term = 0;
+                if (a.zero_dm == FA_ZERO_DM_SUBTRACT) term = band;
+                else if (a.zero_dm == FA_ZERO_DM_ZDOT) term = a.zdot_a[c] * band + a.zdot_b[c];
+                const float x = a.g.x(t, c);
+                const float transformed =
+                    a.scale[c] <= 0 ? x
+                    : a.equalize    ? a.eq_loc + a.eq_scale * (zbuf[c] - term)
+                                    : a.loc[c] + a.scale[c] * (zbuf[c] - term);
+                const float sc = a.out_scale[c] > 0 ? 1.0f / a.out_scale[c] : 1.0f;
+                v              = (transformed - a.out_loc[c]) * sc;
+            }
+            a.out[(a.flip ? n - 1 - c : c) * a.stride + a.offset + t] = v;
+        }
         __syncthreads();
     }
 }
@@ -1143,9 +1427,6 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
     require(cfg.broad_boxcar_max >= 1, "rfi.broad.boxcar_max must be >= 1");
     require(cfg.fill >= FA_FILL_NEIGHBOUR && cfg.fill <= FA_FILL_NOISE, "Invalid rfi.fill");
     require(cfg.zero_dm >= FA_ZERO_DM_OFF && cfg.zero_dm <= FA_ZERO_DM_ZDOT, "Invalid rfi.zero_dm");
-    if (cfg.zero_dm != FA_ZERO_DM_OFF || cfg.equalize)
-        throw std::invalid_argument(
-            "device RFI: zero_dm and equalize are not implemented on the device");
     if (cfg.zap_chans && cfg.n_zap_chans)
         zap_chans_.assign(cfg.zap_chans, cfg.zap_chans + 2 * cfg.n_zap_chans);
     if (cfg.zap_freqs && cfg.n_zap_freqs)
@@ -1163,6 +1444,19 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
     bits_ = dtype_bits(layout.dtype);
     require((layout.nchans * static_cast<uint64_t>(bits_)) % 8 == 0,
             "RFI: each time sample must be a whole number of bytes");
+    // lib/cpu/rfi_cpu.cpp:317-325
+    transform_ = cfg.zero_dm != FA_ZERO_DM_OFF || cfg.equalize;
+    if (transform_ && layout.dtype != FA_FLOAT32 && bits_ < 8)
+        throw std::invalid_argument("RFICleaner: zero_dm and equalize need >= 8-bit data");
+    // Output level and rms of equalisation from the sample type
+    // (lib/cpu/rfi_cpu.cpp:48-55).
+    if (layout.dtype == FA_FLOAT32) {
+        eq_loc_   = 0.0f;
+        eq_scale_ = 1.0f;
+    } else {
+        eq_loc_   = static_cast<float>(uint64_t{1} << (bits_ - 1));
+        eq_scale_ = static_cast<float>(uint64_t{1} << bits_) / 16.0f;
+    }
     require(std::isfinite(layout.tsamp) && layout.tsamp > 0 && std::isfinite(baseline) &&
                 baseline > 0,
             "RFI: tsamp and baseline_length must be finite and > 0");
@@ -1233,6 +1527,20 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
     tail_loc_.resize(nch_);
     tail_scale_.resize(nch_);
     bad_index_.resize(1);
+    if (cfg.zero_dm == FA_ZERO_DM_ZDOT) {
+        skeep64_.resize(layout.max_nsamps);
+        zsums_.resize(kMaxChunks * nch_);
+    }
+    zdot_a_.resize(nch_);
+    zdot_b_.resize(nch_);
+    if (transform_) {
+        loc2_.resize(maxwin * nch_);
+        scale2_.resize(maxwin * nch_);
+        tail_out_loc_.resize(nch_);
+        tail_out_scale_.resize(nch_);
+        rowbuf_.resize(kRowBlocks * nch_);
+        if (cfg.zero_dm != FA_ZERO_DM_OFF) flag_rows_.resize(layout.max_nsamps);
+    }
     for (auto& e : ev_) FA_HIP(hipEventCreate(&e));
 }
 
@@ -1252,14 +1560,20 @@ uint64_t DeviceRfi::memory_bytes() const {
            kwhole_.bytes() + good_prefix_.bytes() + kspec_.bytes() + s_det_.bytes() +
            s_keep_.bytes() + row_std_.bytes() + cmin_.bytes() + cmax_.bytes() + kpos_.bytes() +
            kneg_.bytes() + kseg_.bytes() + counters_.bytes() + slow_tiles_.bytes() + tail_loc_.bytes() +
-           tail_scale_.bytes();
+           tail_scale_.bytes() + skeep64_.bytes() + zsums_.bytes() + zdot_a_.bytes() +
+           zdot_b_.bytes() + loc2_.bytes() + scale2_.bytes() + tail_out_loc_.bytes() +
+           tail_out_scale_.bytes() + rowbuf_.bytes() + flag_rows_.bytes();
 }
 
 void DeviceRfi::calibration(float* loc, float* scale, float* inv) const {
     DeviceGuard dg(device_);
     const uint64_t n = nwin_ * nch_;
-    if (loc) FA_HIP(hipMemcpy(loc, loc_.get(), n * 4, hipMemcpyDeviceToHost));
-    if (scale) FA_HIP(hipMemcpy(scale, scale_.get(), n * 4, hipMemcpyDeviceToHost));
+    // With a transform, loc and scale are the re-measured ones after a full
+    // block (lib/cpu/rfi_cpu.cpp:217-239); inv_scale keeps the bandpass's.
+    const float* l = remeasured_ ? loc2_.get() : loc_.get();
+    const float* c = remeasured_ ? scale2_.get() : scale_.get();
+    if (loc) FA_HIP(hipMemcpy(loc, l, n * 4, hipMemcpyDeviceToHost));
+    if (scale) FA_HIP(hipMemcpy(scale, c, n * 4, hipMemcpyDeviceToHost));
     if (inv) FA_HIP(hipMemcpy(inv, inv_.get(), n * 4, hipMemcpyDeviceToHost));
 }
 
@@ -1293,7 +1607,37 @@ void DeviceRfi::execute_normalized(const uint8_t* block, uint64_t capacity, uint
         report_              = fa_rfi_report{};
         report_.nsamps       = nsamps;
         report_.chans_zapped = static_cast<uint64_t>(std::count(static_mask_.begin(), static_mask_.end(), 0));
-        if (tail_ready_) {
+        if (tail_ready_ && transform_) {
+            // The transforms with the last full block's calibration, zdot
+            // fit and re-measured calibration (lib/cpu/rfi_cpu.cpp:262-315).
+            report_.chans_masked       = static_cast<uint64_t>(std::count(mask_.begin(), mask_.end(), 0));
+            report_.calibration_reused = 1;
+            FA_HIP(hipMemcpyAsync(mask_dev_.get(), mask_.data(), nch_, hipMemcpyHostToDevice, s));
+            TailArgs ta{};
+            ta.g         = g;
+            ta.loc       = tail_loc_.get();
+            ta.scale     = tail_scale_.get();
+            ta.out_loc   = tail_out_loc_.get();
+            ta.out_scale = tail_out_scale_.get();
+            ta.mask      = mask_dev_.get();
+            ta.zdot_a    = zdot_a_.get();
+            ta.zdot_b    = zdot_b_.get();
+            ta.clip      = clip_;
+            ta.eq_loc    = eq_loc_;
+            ta.eq_scale  = eq_scale_;
+            ta.zero_dm   = cfg_.zero_dm;
+            ta.equalize  = cfg_.equalize ? 1 : 0;
+            ta.flip      = flip;
+            ta.stride    = stride;
+            ta.offset    = offset;
+            ta.out       = out;
+            const unsigned nb = static_cast<unsigned>(std::min<uint64_t>(nsamps, kRowBlocks));
+            short_transform_kernel<<<nb, 256, 0, s>>>(ta, rowbuf_.get());
+            if (capacity > nsamps)
+                zero_rows_kernel<<<dim3(ceil_div(capacity - nsamps, 256 * 8),
+                                        static_cast<unsigned>(nch_)),
+                                   256, 0, s>>>(out, stride, offset, nsamps, capacity);
+        } else if (tail_ready_) {
             // Reuse the last full block's final window and mask.
             report_.chans_masked       = static_cast<uint64_t>(std::count(mask_.begin(), mask_.end(), 0));
             report_.calibration_reused = 1;
@@ -1307,6 +1651,7 @@ void DeviceRfi::execute_normalized(const uint8_t* block, uint64_t capacity, uint
             report_.chans_masked = report_.chans_zapped;
             FA_HIP(hipMemcpyAsync(mask_dev_.get(), mask_.data(), nch_, hipMemcpyHostToDevice, s));
             welford_kernel<<<ceil_div(nch_, 64), 64, 0, s>>>(g, loc_.get(), scale_.get());
+            remeasured_ = false;
             dim3 grid(ceil_div(capacity, 256 * 8), static_cast<unsigned>(nch_));
             short_normalize_kernel<<<grid, 256, 0, s>>>(g, loc_.get(), scale_.get(),
                                                         mask_dev_.get(), capacity, flip, out,
@@ -1502,6 +1847,7 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
     const uint64_t chunk   = (nsamps + nchunks - 1) / nchunks;
     const bool good_any    = std::count(mask_.begin(), mask_.end(), 1) > 0;
     const bool kspec = cfg_.kadane_spectrum_enabled, kchan = cfg_.kadane_channel_enabled;
+    const bool zdot  = cfg_.zero_dm == FA_ZERO_DM_ZDOT;
     if (good_any) {
         ScanArgs a{};
         a.g           = g;
@@ -1519,6 +1865,7 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
         a.row_std     = row_std_.get();
         a.kspec_span  = kspec ? kspec_.get() : nullptr;
         a.kwhole      = kspec ? kwhole_.get() : nullptr;
+        a.s_keep64    = zdot ? skeep64_.get() : nullptr;
         const size_t lds = 64 * (kRowTileCh * bits_ / 32 + 1) * 4;
         if (layout_.dtype == FA_UINT8)
             scan_rows_kernel<FA_UINT8><<<ceil_div(nsamps, 64), 64, lds, s>>>(a);
@@ -1540,10 +1887,19 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
         b.cmax      = cmax_.get();
         b.kpos      = kchan ? kpos_.get() : nullptr;
         b.kneg      = kchan ? kneg_.get() : nullptr;
+        b.s_keep64  = zdot ? skeep64_.get() : nullptr;
+        b.zsums     = zdot ? zsums_.get() : nullptr;
         if (layout_.dtype == FA_UINT8)
             scan_cols_kernel<FA_UINT8><<<dim3(static_cast<unsigned>(nchunks), cblk), 64, 0, s>>>(b);
         else
             scan_cols_kernel<-1><<<dim3(static_cast<unsigned>(nchunks), cblk), 64, 0, s>>>(b);
+        FA_HIP(hipGetLastError());
+    }
+    if (zdot) {
+        // Per-channel regression of z on the kept band mean
+        // (lib/cpu/rfi_cpu.cpp:413-437).
+        zdot_fit_kernel<<<ceil_div(n, 256), 256, 0, s>>>(zsums_.get(), nchunks, n, good_any ? 1 : 0,
+                                                         zdot_a_.get(), zdot_b_.get());
         FA_HIP(hipGetLastError());
     }
     FA_HIP(hipEventRecord(ev_[3], s));
@@ -1574,6 +1930,15 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
     ap.stride      = stride;
     ap.offset      = offset;
     ap.counters    = counters_.get();
+    ap.transform   = transform_ ? 1 : 0;
+    ap.zero_dm     = cfg_.zero_dm;
+    ap.equalize    = cfg_.equalize ? 1 : 0;
+    ap.rezero      = cfg_.zero_dm != FA_ZERO_DM_OFF ? 1 : 0;
+    ap.eq_loc      = eq_loc_;
+    ap.eq_scale    = eq_scale_;
+    ap.s_keep      = s_keep_.get();
+    ap.zdot_a      = zdot_a_.get();
+    ap.zdot_b      = zdot_b_.get();
     // counters_[5] counts the slow tiles listed in slow_tiles_.
     const dim3 tiles(ceil_div(capacity, 64), cblk);
     auto* nslow = reinterpret_cast<unsigned int*>(counters_.get() + 5);
@@ -1588,6 +1953,38 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
     const uint64_t last = (nwin_ - 1) * n;
     FA_HIP(hipMemcpyAsync(tail_loc_.get(), loc_.get() + last, n * 4, hipMemcpyDeviceToDevice, s));
     FA_HIP(hipMemcpyAsync(tail_scale_.get(), scale_.get() + last, n * 4, hipMemcpyDeviceToDevice, s));
+    remeasured_ = transform_;
+    if (transform_) {
+        if (ap.rezero) {
+            // Flagged rows rebuilt with their band mean removed
+            // (lib/cpu/rfi_cpu.cpp:631-712).
+            flag_rows_host_.clear();
+            for (uint64_t t = 0; t < nsamps; ++t)
+                if (time_flags_host_[t]) flag_rows_host_.push_back(static_cast<uint32_t>(t));
+            const uint64_t nf = flag_rows_host_.size();
+            if (nf > 0) {
+                FA_HIP(hipMemcpyAsync(flag_rows_.get(), flag_rows_host_.data(), nf * 4,
+                                      hipMemcpyHostToDevice, s));
+                const unsigned nb = static_cast<unsigned>(std::min<uint64_t>(nf, kRowBlocks));
+                rezero_rows_kernel<<<nb, 256, 0, s>>>(ap, flag_rows_.get(),
+                                                      static_cast<uint32_t>(nf), rowbuf_.get());
+                FA_HIP(hipGetLastError());
+            }
+        }
+        // Re-measure each (window, channel) of the written values, then
+        // normalise in place with it (lib/cpu/rfi_cpu.cpp:217-257).
+        welford_windows_kernel<<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
+            ap, loc2_.get(), scale2_.get());
+        FA_HIP(hipGetLastError());
+        normalize_windows_kernel<<<dim3(ceil_div(capacity, 256 * 8), static_cast<unsigned>(n)), 256,
+                                   0, s>>>(out, stride, offset, ap.flip, g, loc2_.get(),
+                                           scale2_.get(), mask_dev_.get(), capacity);
+        FA_HIP(hipGetLastError());
+        FA_HIP(hipMemcpyAsync(tail_out_loc_.get(), loc2_.get() + last, n * 4,
+                              hipMemcpyDeviceToDevice, s));
+        FA_HIP(hipMemcpyAsync(tail_out_scale_.get(), scale2_.get() + last, n * 4,
+                              hipMemcpyDeviceToDevice, s));
+    }
     unsigned long long cnt[8];
     FA_HIP(hipMemcpyAsync(cnt, counters_.get(), sizeof(cnt), hipMemcpyDeviceToHost, s));
     FA_HIP(hipEventRecord(ev_[5], s));

@@ -17,7 +17,11 @@
 //  apply      device: cell states, neighbour fills with frost's counter
 //             hash, time fills, and the normalising transpose to
 //             channel-major float32 (ascending frequency) in one pass.
-// zero_dm and equalize are not implemented on the device and are rejected.
+//  transforms (zero_dm, equalize) device: zdot sums per (chunk, channel)
+//             in the column scan and a per-channel fit; the apply pass then
+//             writes transformed values (flagged rows re-zeroed per row with
+//             a channel-ordered band mean), a Welford pass re-measures each
+//             (window, channel), and a last pass normalises in place.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,6 +35,12 @@ namespace fa {
 
 struct MomentSumsDev {
     double n, s1, s2, s3, s4, lag, nc, c1, c2;
+};
+
+// Zero-DM regression sums of one (chunk, channel) (ZdotSums,
+// lib/cpu/rfi_cpu_detail.hpp:72-80).
+struct ZdotDev {
+    double n, sz, ss, sss, szs;
 };
 
 struct RunDev {
@@ -88,9 +98,12 @@ private:
              chan_fill_radius_ = 0;
     float clip_ = 0;
     bool narrow_active_ = false, need_moments_ = false;
+    bool transform_ = false; // zero_dm != off or equalize
+    float eq_loc_ = 0.0f, eq_scale_ = 1.0f;
     uint64_t block_index_ = 0;
     uint64_t nsamps_ = 0, nwin_ = 0;
     bool tail_ready_ = false;
+    bool remeasured_ = false; // loc2_/scale2_ hold the last block's calibration
 
     std::vector<uint8_t> static_mask_, mask_, time_flags_host_;
     fa_rfi_report report_{};
@@ -109,6 +122,14 @@ private:
     DeviceBuffer<unsigned long long> counters_;
     DeviceBuffer<uint32_t> slow_tiles_; // (time tile, channel tile) pairs needing fills
     DeviceBuffer<float> tail_loc_, tail_scale_;
+    // Transforms: row means in FP64, zdot sums and fit, re-measured
+    // calibration, its last window for short tails, flagged rows.
+    DeviceBuffer<double> skeep64_;
+    DeviceBuffer<ZdotDev> zsums_;
+    DeviceBuffer<float> zdot_a_, zdot_b_, loc2_, scale2_, tail_out_loc_, tail_out_scale_;
+    DeviceBuffer<uint32_t> flag_rows_;
+    DeviceBuffer<float> rowbuf_; // [kRowBlocks][nch] per-block row values
+    std::vector<uint32_t> flag_rows_host_;
     DeviceBuffer<unsigned long long> bad_index_;
 
     // Host mirrors.

@@ -145,9 +145,69 @@ def test_fill_modes(oracle, stream, fill):
             fill_tol=None if fill == abi.FILL_MEAN else 1e-3)
 
 
+TRANSFORMS = [(abi.ZERO_DM_SUBTRACT, 0), (abi.ZERO_DM_ZDOT, 0), (abi.ZERO_DM_OFF, 1),
+              (abi.ZERO_DM_SUBTRACT, 1), (abi.ZERO_DM_ZDOT, 1)]
+
+
+@pytest.mark.parametrize("zero_dm,equalize", TRANSFORMS)
+def test_transforms_with_flags_and_tails(oracle, stream, zero_dm, equalize):
+    """Zero-DM (subtract, zdot) and equalisation (lib/cpu/rfi_cpu.cpp:217-325,
+    413-437, 470-712) with every strategy on: broad-band and dropout rows
+    re-zeroed, narrow-band and Kadane cells refilled, the output re-measured
+    per window; then a second block, a 37-row warm tail, and a block shorter
+    than its capacity."""
+    say(f"test_transforms_with_flags_and_tails zero_dm={zero_dm} equalize={equalize}")
+    nch, cap = 256, 24000
+    cfg = abi.rfi_config(moments_enabled=1, iqrm_enabled=1, kadane_channel_enabled=1,
+                         kadane_spectrum_enabled=1, broad_boxcar_max=32, dropout_enabled=1,
+                         zap_chans=[(100, 103)], zero_dm=zero_dm, equalize=equalize)
+    lay = layout(nch, cap, tsamp=64e-6, df=-300 / nch)
+    b1 = synth.filterbank(cap, nch, seed=61, rfi_chans=(3, 17, 90, 200),
+                          impulses=(500, 7000, 15000), dead=(33,),
+                          gain=np.linspace(0.6, 1.8, nch), pulses=[(300.0, 4000, 8, 1.5)],
+                          steps=[(150, 2000, 9000, 0.4)], bands=[(3000, 20, 40, 3.0)])
+    b2 = synth.filterbank(cap, nch, seed=62, rfi_chans=(40,), impulses=(100, 101, 23000))
+    tail = np.zeros((cap, nch), np.uint8)
+    tail[:37] = synth.filterbank(37, nch, seed=63)
+    b4 = np.zeros((cap, nch), np.uint8)
+    b4[:17000] = synth.filterbank(17000, nch, seed=64, impulses=(16999,))
+    gpu, ora = compare(oracle, stream, cfg, lay, 0.3,
+                       [(b1, cap, cap), (b2, cap, cap), (tail, cap, 37), (b4, cap, 17000)])
+    assert gpu.report().samps_flagged > 0
+
+
+@pytest.mark.parametrize("dtype", [abi.UINT16, abi.FLOAT32])
+@pytest.mark.parametrize("zero_dm,equalize", [(abi.ZERO_DM_ZDOT, 1), (abi.ZERO_DM_SUBTRACT, 0)])
+def test_transforms_dtypes(oracle, stream, dtype, zero_dm, equalize):
+    """Equalisation levels of 16-bit and float samples (lib/cpu/rfi_cpu.cpp:48-55),
+    with a warm tail."""
+    say(f"test_transforms_dtypes dtype={dtype} zero_dm={zero_dm} equalize={equalize}")
+    nch, n = 64, 6000
+    kind = "u16" if dtype == abi.UINT16 else "f32"
+    blk = synth.filterbank(n, nch, seed=71, dtype=kind, rfi_chans=(9,), impulses=(100, 4000))
+    tail = synth.filterbank(n, nch, seed=72, dtype=kind)
+    cfg = abi.rfi_config(moments_enabled=1, kadane_spectrum_enabled=1, zero_dm=zero_dm,
+                         equalize=equalize)
+    compare(oracle, stream, cfg, layout(nch, n, dtype=dtype), 0.1, [(blk, n, n), (tail, n, 20)])
+
+
+@pytest.mark.parametrize("fill", [abi.FILL_MEAN, abi.FILL_NOISE])
+def test_transform_fill_modes(oracle, stream, fill):
+    """Re-zeroed rows with mean (zero) and noise fills."""
+    say(f"test_transform_fill_modes fill={fill}")
+    nch, n = 64, 5000
+    blk = synth.filterbank(n, nch, seed=81, rfi_chans=(4, 5), impulses=(800, 801, 2500))
+    cfg = abi.rfi_config(fill=fill, zero_dm=abi.ZERO_DM_ZDOT, equalize=1)
+    # Noise fills use libm log/cos on each side (1e-3 absolute, as above); the
+    # re-measured calibration spreads such differences over the window.
+    compare(oracle, stream, cfg, layout(nch, n), 0.1, [(blk, n, n)],
+            fill_tol=None if fill == abi.FILL_MEAN else 1e-3)
+
+
 def test_rejects_transforms_and_bad_input(stream):
-    with pytest.raises(abi.InvalidArgument):
-        prepare.RFICleaner(abi.rfi_config(zero_dm=abi.ZERO_DM_SUBTRACT), layout(16, 128), 0.01)
+    with pytest.raises(abi.InvalidArgument, match="8-bit"):
+        prepare.RFICleaner(abi.rfi_config(zero_dm=abi.ZERO_DM_SUBTRACT),
+                           layout(16, 128, abi.UINT4), 0.01)
     with pytest.raises(abi.InvalidArgument):
         prepare.RFICleaner(abi.rfi_config(narrow_threshold=0.0), layout(16, 128), 0.01)
     cl = prepare.RFICleaner(abi.rfi_config(), layout(16, 128, abi.FLOAT32), 0.01)
