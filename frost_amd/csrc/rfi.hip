@@ -1226,51 +1226,98 @@ rezero_rows_kernel(ApplyArgs a, const uint32_t* __restrict__ rows, uint32_t nrow
 }
 
 // Mean and population std of each (window, channel) of the cleaned values
-// (lib/cpu/rfi_cpu.cpp:217-239): Welford in time order, FP64, one lane per
-// channel walking its column of the output just written. A masked channel's
-// values are its fills, level(fill of the masked stream) in every row
-// (lib/cpu/rfi_cpu.cpp:493-524), computed here (its output column is 0).
-__global__ void welford_windows_kernel(ApplyArgs a, float* __restrict__ loc2,
-                                       float* __restrict__ scale2) {
-    const Geometry& g = a.g;
-    const uint64_t w  = blockIdx.x;
-    const uint64_t c  = (uint64_t)blockIdx.y * 64 + threadIdx.x;
-    if (c >= g.nch) return;
+// (lib/cpu/rfi_cpu.cpp:217-239): Welford in time order, FP64. One wave per
+// (window, 64 channels): tiles of 32 rows x 64 channels are read from the
+// output just written (a coalesced 128-byte row segment per channel, two
+// channels per load), the next tile's loads in flight while each lane runs
+// the recurrence of its channel over the current one from LDS. A masked
+// channel's values are its fills, level(fill of the masked stream) in every
+// row (lib/cpu/rfi_cpu.cpp:493-524), computed here (its output column is 0).
+// inv2 = 1/scale2 (1 where 0), the factor the normalisation applies.
+constexpr int kWelfordRows = 32;
+
+__global__ void __launch_bounds__(64) welford_windows_kernel(ApplyArgs a, float* __restrict__ loc2,
+                                                             float* __restrict__ scale2,
+                                                             float* __restrict__ inv2) {
+    __shared__ float tile[64][kWelfordRows + 1];
+    const Geometry& g  = a.g;
+    const uint64_t w   = blockIdx.x;
+    const uint64_t c0  = (uint64_t)blockIdx.y * 64;
+    const int l        = threadIdx.x;
+    const uint64_t c   = c0 + l;
+    const int64_t nc   = g.nch - c0 < 64 ? (int64_t)(g.nch - c0) : 64;
     const uint64_t b   = g.window_begin(w), e = g.window_end(w);
-    const float* col   = a.out + (a.flip ? g.nch - 1 - c : c) * a.stride + a.offset;
-    const bool masked  = a.mask[c] == 0;
+    const bool live    = c < g.nch;
+    const bool masked  = live && a.mask[c] == 0;
     const bool noise   = a.fill == FA_FILL_NOISE;
+    const uint64_t i   = w * g.nch + c;
+    const float mfill  = masked ? level(a, i, 0.0f) : 0.0f;
+    // Output column of channel c0 + j is col0 + j * dir.
+    const float* col0  = a.out + a.offset + (a.flip ? g.nch - 1 - c0 : c0) * a.stride;
+    const int64_t dir  = a.flip ? -(int64_t)a.stride : (int64_t)a.stride;
+    const int half     = l / kWelfordRows, r = l % kWelfordRows;
+    float nxt[32];
+    auto load = [&](uint64_t t0) {
+        const int64_t t = (int64_t)(t0 + r < e ? t0 + r : e - 1);
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const int64_t ch = 2 * j + half < nc ? 2 * j + half : nc - 1;
+            nxt[j]           = col0[ch * dir + t];
+        }
+    };
     double mean = 0.0, var = 0.0;
-    for (uint64_t t = b; t < e; ++t) {
-        float v = 0.0f;
-        if (masked)
-            v = level(a, w * g.nch + c,
-                      noise ? normal_draw(hash5(a.seed, a.block_index, (kStreamMasked << 56) ^ t, c, 0))
-                            : 0.0f);
-        const double x     = masked ? (double)v : (double)col[t];
-        const double delta = x - mean;
-        mean += delta / (double)(t - b + 1);
-        var += delta * (x - mean);
+    load(b);
+    for (uint64_t t0 = b; t0 < e; t0 += kWelfordRows) {
+        __syncthreads(); // the previous tile's reads are done
+#pragma unroll
+        for (int j = 0; j < 32; ++j) tile[2 * j + half][r] = nxt[j];
+        __syncthreads();
+        if (t0 + kWelfordRows < e) load(t0 + kWelfordRows);
+        const int kn = e - t0 < (uint64_t)kWelfordRows ? (int)(e - t0) : kWelfordRows;
+#pragma unroll 8
+        for (int k = 0; k < kn; ++k) {
+            const uint64_t t = t0 + k;
+            float v          = tile[l][k];
+            if (masked)
+                v = noise ? level(a, i, normal_draw(hash5(a.seed, a.block_index,
+                                                          (kStreamMasked << 56) ^ t, c, 0)))
+                          : mfill;
+            const double x     = v;
+            const double delta = x - mean;
+            mean += delta / (double)(t - b + 1);
+            var += delta * (x - mean);
+        }
     }
-    loc2[w * g.nch + c]   = (float)mean;
-    scale2[w * g.nch + c] = e > b ? (float)sqrt(var / (double)(e - b)) : 0.0f;
+    if (!live) return;
+    const float sc = e > b ? (float)sqrt(var / (double)(e - b)) : 0.0f;
+    loc2[i]        = (float)mean;
+    scale2[i]      = sc;
+    inv2[i]        = sc > 0.0f ? 1.0f / sc : 1.0f;
 }
 
 // Normalises the written values in place with the re-measured calibration
-// (lib/cpu/rfi_cpu.cpp:245-257); masked channels and padding rows are 0.
-__global__ void normalize_windows_kernel(float* __restrict__ out, uint64_t stride, uint64_t offset,
-                                         int flip, Geometry g, const float* __restrict__ loc2,
-                                         const float* __restrict__ scale2,
-                                         const uint8_t* __restrict__ mask, uint64_t capacity) {
-    const uint64_t c = blockIdx.y;
-    float* col       = out + (flip ? g.nch - 1 - c : c) * stride + offset;
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < capacity;
-         t += (uint64_t)gridDim.x * blockDim.x) {
+// (lib/cpu/rfi_cpu.cpp:245-257); masked channels and padding rows are 0. One
+// channel per grid row, 2048 consecutive samples per block.
+__global__ void __launch_bounds__(256)
+normalize_windows_kernel(float* __restrict__ out, uint64_t stride, uint64_t offset, int flip,
+                         Geometry g, const float* __restrict__ loc2,
+                         const float* __restrict__ inv2, const uint8_t* __restrict__ mask,
+                         uint64_t capacity) {
+    const uint64_t c   = blockIdx.y;
+    float* col         = out + (flip ? g.nch - 1 - c : c) * stride + offset;
+    const bool good    = mask[c] != 0;
+    const uint32_t wl  = (uint32_t)g.window_len;
+    const uint32_t wmx = (uint32_t)g.nwin - 1;
+    const uint64_t t0  = (uint64_t)blockIdx.x * 2048 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint64_t t = t0 + 256 * k;
+        if (t >= capacity) break;
         float v = 0.0f;
-        if (t < g.nsamps && mask[c] != 0) {
-            const uint64_t i = g.window_of(t) * g.nch + c;
-            const float sc   = scale2[i] > 0.0f ? 1.0f / scale2[i] : 1.0f;
-            v                = (col[t] - loc2[i]) * sc;
+        if (good && t < g.nsamps) {
+            const uint32_t w   = min((uint32_t)t / wl, wmx);
+            const uint64_t idx = (uint64_t)w * g.nch + c;
+            v                  = (col[t] - loc2[idx]) * inv2[idx];
         }
         col[t] = v;
     }
@@ -1536,6 +1583,7 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
     if (transform_) {
         loc2_.resize(maxwin * nch_);
         scale2_.resize(maxwin * nch_);
+        inv2_.resize(maxwin * nch_);
         tail_out_loc_.resize(nch_);
         tail_out_scale_.resize(nch_);
         rowbuf_.resize(kRowBlocks * nch_);
@@ -1562,7 +1610,7 @@ uint64_t DeviceRfi::memory_bytes() const {
            kneg_.bytes() + kseg_.bytes() + counters_.bytes() + slow_tiles_.bytes() + tail_loc_.bytes() +
            tail_scale_.bytes() + skeep64_.bytes() + zsums_.bytes() + zdot_a_.bytes() +
            zdot_b_.bytes() + loc2_.bytes() + scale2_.bytes() + tail_out_loc_.bytes() +
-           tail_out_scale_.bytes() + rowbuf_.bytes() + flag_rows_.bytes();
+           tail_out_scale_.bytes() + rowbuf_.bytes() + flag_rows_.bytes() + inv2_.bytes();
 }
 
 void DeviceRfi::calibration(float* loc, float* scale, float* inv) const {
@@ -1974,11 +2022,11 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
         // Re-measure each (window, channel) of the written values, then
         // normalise in place with it (lib/cpu/rfi_cpu.cpp:217-257).
         welford_windows_kernel<<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
-            ap, loc2_.get(), scale2_.get());
+            ap, loc2_.get(), scale2_.get(), inv2_.get());
         FA_HIP(hipGetLastError());
         normalize_windows_kernel<<<dim3(ceil_div(capacity, 256 * 8), static_cast<unsigned>(n)), 256,
                                    0, s>>>(out, stride, offset, ap.flip, g, loc2_.get(),
-                                           scale2_.get(), mask_dev_.get(), capacity);
+                                           inv2_.get(), mask_dev_.get(), capacity);
         FA_HIP(hipGetLastError());
         FA_HIP(hipMemcpyAsync(tail_out_loc_.get(), loc2_.get() + last, n * 4,
                               hipMemcpyDeviceToDevice, s));

@@ -126,7 +126,7 @@ private:
     // calibration, its last window for short tails, flagged rows.
     DeviceBuffer<double> skeep64_;
     DeviceBuffer<ZdotDev> zsums_;
-    DeviceBuffer<float> zdot_a_, zdot_b_, loc2_, scale2_, tail_out_loc_, tail_out_scale_;
+    DeviceBuffer<float> zdot_a_, zdot_b_, loc2_, scale2_, inv2_, tail_out_loc_, tail_out_scale_;
     DeviceBuffer<uint32_t> flag_rows_;
     DeviceBuffer<float> rowbuf_; // [kRowBlocks][nch] per-block row values
     std::vector<uint32_t> flag_rows_host_;
