@@ -256,6 +256,23 @@ CLUSTER_CHAIN, CLUSTER_SEEDED = 0, 1
 DEDUP_TRANSITIVE, DEDUP_SEEDED = 0, 1
 
 
+class MemoryEstimate(C.Structure):
+    """fa_memory_estimate: device bytes of one beam by owner
+    (MemoryEstimate, include/frost/pipeline/memory.hpp:17-26)."""
+    _fields_ = [("input_bytes", C.c_uint64), ("rfi_bytes", C.c_uint64),
+                ("fdmt_bytes", C.c_uint64), ("postprocess_bytes", C.c_uint64),
+                ("reserve_bytes", C.c_uint64), ("total_bytes", C.c_uint64),
+                ("tile", C.c_uint64)]
+
+
+class MemoryCheck(C.Structure):
+    """fa_memory_check (MemoryCheck, include/frost/pipeline/memory.hpp:54-62)."""
+    _fields_ = [("estimate", MemoryEstimate), ("requested_bytes", C.c_uint64),
+                ("available_bytes", C.c_uint64), ("gulp_requested", C.c_uint64),
+                ("gulp_used", C.c_uint64), ("lowered", C.c_int32),
+                ("device_memory", C.c_int32)]
+
+
 class SearchConfig(C.Structure):
     _fields_ = [
         ("boxcar_max", C.c_uint64), ("max_candidates", C.c_uint64),
@@ -405,6 +422,13 @@ EXPORTS = {
     "fa_beam_rfi_report": (C.c_int, [P, C.POINTER(RfiReport)]),
     "fa_beam_engine": (C.c_int, [P, C.POINTER(P)]),
     "fa_beam_rfi": (C.c_int, [P, C.POINTER(P)]),
+    "fa_beam_estimate_memory": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32,
+                                          C.POINTER(DedispConfig), C.POINTER(BoxcarConfig),
+                                          C.POINTER(MemoryEstimate)]),
+    "fa_beam_memory": (C.c_int, [P, C.POINTER(MemoryEstimate)]),
+    "fa_check_memory": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32,
+                                  C.POINTER(DedispConfig), C.POINTER(BoxcarConfig), C.c_int, U64,
+                                  U64, C.c_int, C.POINTER(MemoryCheck)]),
     "fa_search_config_default": (None, [C.POINTER(SearchConfig)]),
     "fa_are_coincident": (C.c_int, [C.POINTER(Giant), C.POINTER(Giant),
                                     C.POINTER(ClusterTolerances), C.POINTER(C.c_int)]),

@@ -27,6 +27,30 @@ inline void hip_check(hipError_t e, const char* what) {
 
 #define FA_HIP(expr) ::fa::hip_check((expr), #expr)
 
+// Sizing without a device: while a SizingScope is alive on this thread,
+// DeviceBuffer::resize and PinnedBuffer::reserve record their sizes without
+// allocating, upload() copies nothing and DeviceGuard leaves the current
+// device alone. Constructors run under it report what they would allocate
+// (the memory planner) and make no HIP call.
+inline thread_local bool t_sizing = false;
+inline bool sizing() { return t_sizing; }
+
+class SizingScope {
+public:
+    SizingScope() : prev_(t_sizing) { t_sizing = true; }
+    ~SizingScope() { t_sizing = prev_; }
+    SizingScope(const SizingScope&)            = delete;
+    SizingScope& operator=(const SizingScope&) = delete;
+
+private:
+    bool prev_;
+};
+
+// Host-to-device copy of a table (skipped while sizing).
+inline void upload(void* dst, const void* src, std::size_t bytes) {
+    if (!t_sizing && bytes) FA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+
 // Owning device allocation (hipMalloc / hipFree) on the current device.
 template <typename T> class DeviceBuffer {
 public:
@@ -45,6 +69,10 @@ public:
         if (n == n_) return;
         reset();
         if (n == 0) return;
+        if (t_sizing) {
+            n_ = n;
+            return;
+        }
         void* p = nullptr;
         FA_HIP(hipMalloc(&p, n * sizeof(T)));
         p_ = static_cast<T*>(p);
@@ -82,6 +110,10 @@ public:
     void reserve(std::size_t n, unsigned flags = hipHostMallocDefault) {
         if (n <= n_) return;
         reset();
+        if (t_sizing) {
+            n_ = n;
+            return;
+        }
         void* p = nullptr;
         FA_HIP(hipHostMalloc(&p, n * sizeof(T), flags));
         p_ = static_cast<T*>(p);
@@ -104,13 +136,18 @@ private:
 class DeviceGuard {
 public:
     explicit DeviceGuard(int device) {
+        if (t_sizing) return;
+        active_ = true;
         FA_HIP(hipGetDevice(&prev_));
         if (device != prev_) FA_HIP(hipSetDevice(device));
     }
-    ~DeviceGuard() { (void)hipSetDevice(prev_); }
+    ~DeviceGuard() {
+        if (active_) (void)hipSetDevice(prev_);
+    }
 
 private:
-    int prev_ = 0;
+    int prev_    = 0;
+    bool active_ = false;
 };
 
 void set_last_error(const std::string& msg);

@@ -19,8 +19,8 @@ namespace {
 
 constexpr int kBlock       = 256;
 constexpr int kInitPer     = 8;    // outputs per thread per init block
-constexpr uint64_t kTileMax = uint64_t{1} << 20; // automatic time tile bounds
-constexpr uint64_t kTileMin = uint64_t{1} << 14;
+constexpr uint64_t kTileMax = kAutoTileMax; // automatic time tile bounds
+constexpr uint64_t kTileMin = kAutoTileMin;
 constexpr int kChunk       = 1024; // search samples per LDS pass
 constexpr uint32_t kMaxBox = 1024; // ladder fits LDS up to this width
 
@@ -1169,7 +1169,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
 
     DeviceGuard dg(device_);
     build_programs();
-    if (!tile_req) {
+    if (!tile_req && !sizing()) {
         // Automatic tile: the largest power of two up to kTileMax (and the
         // gulp capacity) whose input and two level buffers fit in 75 % of
         // the device memory free now. Fewer, longer tiles recompute less
@@ -1201,7 +1201,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     // Extended input, zero history.
     xstride_ = round64(hist_in_ + cap_);
     x_.resize(nch * xstride_);
-    FA_HIP(hipMemset(x_.get(), 0, x_.bytes()));
+    if (!sizing()) FA_HIP(hipMemset(x_.get(), 0, x_.bytes()));
     uint64_t len0 = 0, len1 = 0;
     for (uint32_t l = 0; l < plan_.nlevels; ++l) {
         if (level_buf_[l] == 0) len0 = std::max(len0, level_len_[l]);
@@ -1213,15 +1213,16 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     inv_cur_.resize((uint64_t)nprows_ * nw);
     inv_or_.resize((uint64_t)nprows_ * nw);
     widths_dev_.resize(nw);
-    FA_HIP(hipMemcpy(widths_dev_.get(), widths_.data(), nw * 4, hipMemcpyHostToDevice));
+    upload(widths_dev_.get(), widths_.data(), nw * 4);
     open_.resize((uint64_t)nprows_ * nw);
-    FA_HIP(hipMemset(open_.get(), 0, open_.bytes()));
+    if (!sizing()) FA_HIP(hipMemset(open_.get(), 0, open_.bytes()));
     cand_cap_ = std::max<uint64_t>(4 * cfg.max_candidates, uint64_t{1} << 20);
     cands_.resize(cand_cap_);
     counters_.resize(2);
-    FA_HIP(hipMemset(counters_.get(), 0, counters_.bytes()));
+    if (!sizing()) FA_HIP(hipMemset(counters_.get(), 0, counters_.bytes()));
     level_seconds.assign(plan_.nlevels, 0.0);
     level_launches.assign(plan_.nlevels, 0.0);
+    if (sizing()) return;
     int lds_max = 0;
     FA_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
     const uint64_t lds_need = (uint64_t)(kmax_ + 1) * (kChunk + wm1_) * sizeof(float) +
@@ -1303,9 +1304,9 @@ void Engine::build_programs() {
     }
     xstride_ = round64(hist_in_ + cap_); // extended input row: H history + capacity
     init_chan_.resize(chan.size());
-    FA_HIP(hipMemcpy(init_chan_.get(), chan.data(), chan.size() * 4, hipMemcpyHostToDevice));
+    upload(init_chan_.get(), chan.data(), chan.size() * 4);
     init_rows_.resize(rows.size());
-    FA_HIP(hipMemcpy(init_rows_.get(), rows.data(), rows.size() * 8, hipMemcpyHostToDevice));
+    upload(init_rows_.get(), rows.data(), rows.size() * 8);
     // Stored levels: level 0, every level holding a plane, and enough others
     // that no launch spans more than max_fuse_ levels; the fewest stored
     // floats win (each stored level is written and read once per tile).
@@ -1383,8 +1384,7 @@ void Engine::build_programs() {
             row[K + 1] = h;
         }
         ml.prog.resize(prog.size());
-        FA_HIP(hipMemcpy(ml.prog.get(), prog.data(), prog.size() * sizeof(int64_t),
-                         hipMemcpyHostToDevice));
+        upload(ml.prog.get(), prog.data(), prog.size() * sizeof(int64_t));
         build_gather(ml, prog);
         launches_.push_back(std::move(ml));
     }
@@ -1409,8 +1409,7 @@ void Engine::build_programs() {
         search_count_[l] = static_cast<uint32_t>(sr.size());
         if (sr.empty()) continue;
         search_rows_[l].resize(sr.size());
-        FA_HIP(hipMemcpy(search_rows_[l].get(), sr.data(), sr.size() * sizeof(SearchRow),
-                         hipMemcpyHostToDevice));
+        upload(search_rows_[l].get(), sr.data(), sr.size() * sizeof(SearchRow));
     }
 }
 
@@ -1649,14 +1648,11 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
     ml.rows.resize(rec.size());
     if (x0) {
         ml.chans.resize(gc.size());
-        FA_HIP(hipMemcpy(ml.chans.get(), gc.data(), gc.size() * sizeof(GatherChan),
-                         hipMemcpyHostToDevice));
+        upload(ml.chans.get(), gc.data(), gc.size() * sizeof(GatherChan));
     }
-    FA_HIP(hipMemcpy(ml.blocks.get(), gb.data(), gb.size() * sizeof(GatherBlock),
-                     hipMemcpyHostToDevice));
-    FA_HIP(hipMemcpy(ml.leaves.get(), gl.data(), gl.size() * sizeof(GatherLeaf),
-                     hipMemcpyHostToDevice));
-    FA_HIP(hipMemcpy(ml.rows.get(), rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
+    upload(ml.blocks.get(), gb.data(), gb.size() * sizeof(GatherBlock));
+    upload(ml.leaves.get(), gl.data(), gl.size() * sizeof(GatherLeaf));
+    upload(ml.rows.get(), rec.data(), rec.size() * 4);
 }
 
 void Engine::compute_inv(const std::vector<uint8_t>& mask, std::vector<float>& out) const {
@@ -2072,9 +2068,15 @@ void Engine::drain(hipStream_t s) {
 }
 
 uint64_t Engine::memory_bytes() const {
+    uint64_t tables = 0;
+    for (const auto& ml : launches_)
+        tables += ml.prog.bytes() + ml.blocks.bytes() + ml.leaves.bytes() + ml.chans.bytes() +
+                  ml.rows.bytes();
+    for (const auto& sr : search_rows_) tables += sr.bytes();
     return x_.bytes() + buf_[0].bytes() + buf_[1].bytes() + inv_cur_.bytes() + inv_or_.bytes() +
            open_.bytes() + cands_.bytes() + keep_.bytes() + scratch_.bytes() +
-           init_chan_.bytes() + init_rows_.bytes();
+           init_chan_.bytes() + init_rows_.bytes() + widths_dev_.bytes() + counters_.bytes() +
+           tables;
 }
 
 void Engine::keep_planes(bool keep) { keep_planes_ = keep; }

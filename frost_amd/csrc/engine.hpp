@@ -34,6 +34,10 @@
 
 namespace fa {
 
+// Bounds of the automatically chosen FDMT time tile (samples per pass).
+constexpr uint64_t kAutoTileMin = uint64_t{1} << 14;
+constexpr uint64_t kAutoTileMax = uint64_t{1} << 20;
+
 struct CandRecord {
     uint64_t peak, begin, end;
     float snr;

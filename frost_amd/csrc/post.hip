@@ -555,6 +555,12 @@ double since(Clock::time_point t0) { return std::chrono::duration<double>(Clock:
 
 } // namespace
 
+// Device bytes of the pair searches over up to `rows` rows: the row copies
+// (chain rows or giants, whichever is larger), parents and roots.
+uint64_t postprocess_device_bytes(uint64_t rows) {
+    return rows * (std::max(sizeof(ChainRow), sizeof(fa_giant)) + 2 * sizeof(uint32_t));
+}
+
 // CandidatePostprocessPlan (lib/search/postprocess.cpp:45-276).
 class Postprocess {
 public:

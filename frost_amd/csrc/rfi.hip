@@ -1589,7 +1589,8 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
         rowbuf_.resize(kRowBlocks * nch_);
         if (cfg.zero_dm != FA_ZERO_DM_OFF) flag_rows_.resize(layout.max_nsamps);
     }
-    for (auto& e : ev_) FA_HIP(hipEventCreate(&e));
+    if (!sizing())
+        for (auto& e : ev_) FA_HIP(hipEventCreate(&e));
 }
 
 DeviceRfi::~DeviceRfi() {
@@ -1610,7 +1611,8 @@ uint64_t DeviceRfi::memory_bytes() const {
            kneg_.bytes() + kseg_.bytes() + counters_.bytes() + slow_tiles_.bytes() + tail_loc_.bytes() +
            tail_scale_.bytes() + skeep64_.bytes() + zsums_.bytes() + zdot_a_.bytes() +
            zdot_b_.bytes() + loc2_.bytes() + scale2_.bytes() + tail_out_loc_.bytes() +
-           tail_out_scale_.bytes() + rowbuf_.bytes() + flag_rows_.bytes() + inv2_.bytes();
+           tail_out_scale_.bytes() + rowbuf_.bytes() + flag_rows_.bytes() + inv2_.bytes() +
+           bad_index_.bytes();
 }
 
 void DeviceRfi::calibration(float* loc, float* scale, float* inv) const {

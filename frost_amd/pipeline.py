@@ -112,3 +112,33 @@ class Beam:
                                           cnt.ctypes.data_as(abi.PU32), nl.value, C.byref(tile),
                                           C.byref(nl)))
         return sec, cnt, tile.value
+
+    def memory(self) -> abi.MemoryEstimate:
+        """Device bytes this beam holds now, by owner."""
+        out = abi.MemoryEstimate()
+        check(lib().fa_beam_memory(self.h, C.byref(out)))
+        return out
+
+
+def estimate_memory(rfi_cfg: abi.RfiConfig, layout: abi.BlockLayout, baseline: float,
+                    dedisp: abi.DedispConfig, boxcar: abi.BoxcarConfig) -> abi.MemoryEstimate:
+    """Device bytes a beam of this configuration allocates (FDMT engine at its
+    smallest automatic time tile); needs no device."""
+    out = abi.MemoryEstimate()
+    check(lib().fa_beam_estimate_memory(C.byref(rfi_cfg), C.byref(layout), baseline,
+                                        C.byref(dedisp), C.byref(boxcar), C.byref(out)))
+    return out
+
+
+def check_memory(rfi_cfg: abi.RfiConfig, layout: abi.BlockLayout, baseline: float,
+                 dedisp: abi.DedispConfig, boxcar: abi.BoxcarConfig, gulp_is_default: bool,
+                 available_bytes: int = 0, memory_limit_mib: int = 0,
+                 device: int = 0) -> abi.MemoryCheck:
+    """check_memory (lib/pipeline/memory.cpp:294-508) on the device pool.
+    A lowered default gulp is written into `layout.max_nsamps` and
+    `dedisp.block_nsamps`; a gulp that cannot fit raises FrostAmdError."""
+    out = abi.MemoryCheck()
+    check(lib().fa_check_memory(C.byref(rfi_cfg), C.byref(layout), baseline, C.byref(dedisp),
+                                C.byref(boxcar), int(gulp_is_default), available_bytes,
+                                memory_limit_mib, device, C.byref(out)))
+    return out

@@ -475,6 +475,54 @@ int fa_beam_rfi_report(const fa_beam* beam, fa_rfi_report* out);
 int fa_beam_engine(fa_beam* beam, fa_engine** engine);
 int fa_beam_rfi(fa_beam* beam, fa_rfi** rfi);
 
+/* ---- device memory admission (lib/pipeline/memory.cpp) ------------------ */
+
+/* MemoryEstimate (include/frost/pipeline/memory.hpp:17-26) for this
+ * library's beam: device bytes it allocates, by owner. */
+typedef struct fa_memory_estimate {
+    uint64_t input_bytes;       /* two device ingest slots of the raw gulp */
+    uint64_t rfi_bytes;         /* device RFI cleaner */
+    uint64_t fdmt_bytes;        /* FDMT engine at `tile` */
+    uint64_t postprocess_bytes; /* device pair-search buffers for max_candidates rows */
+    uint64_t reserve_bytes;     /* 10 % of the above */
+    uint64_t total_bytes;
+    uint64_t tile;              /* FDMT time tile the figure is for */
+} fa_memory_estimate;
+
+/* MemoryCheck (include/frost/pipeline/memory.hpp:54-62). */
+typedef struct fa_memory_check {
+    fa_memory_estimate estimate;
+    uint64_t requested_bytes;
+    uint64_t available_bytes; /* 0: unknown, nothing was enforced */
+    uint64_t gulp_requested;
+    uint64_t gulp_used;
+    int32_t lowered;
+    int32_t device_memory;
+} fa_memory_check;
+
+/* Device bytes a beam of this configuration allocates with the FDMT engine
+ * at its smallest automatic time tile (the engine picks a longer tile when
+ * memory allows). No device is needed. Replaces the estimate lambda of
+ * check_memory (lib/pipeline/memory.cpp:374-460). */
+int fa_beam_estimate_memory(const fa_rfi_config* rfi_cfg, const fa_block_layout* layout,
+                            float baseline_length, const fa_dedisp_config* dedisp,
+                            const fa_boxcar_config* boxcar, fa_memory_estimate* out);
+/* Device bytes a live beam holds now (reserve and postprocess 0). */
+int fa_beam_memory(const fa_beam* beam, fa_memory_estimate* out);
+/* check_memory (lib/pipeline/memory.cpp:294-508): compares the estimate at
+ * layout->max_nsamps (== dedisp->block_nsamps, the gulp) with 90 % of
+ * `available_bytes` (0: the device's free memory), capped by
+ * memory_limit_mib (0: no cap). If it does not fit: with gulp_is_default
+ * the gulp is lowered in both configs to the largest multiple of 1024 that
+ * fits (out->lowered = 1); otherwise FA_RUNTIME_ERROR naming the estimate,
+ * the available memory and the largest gulp that fits. A lowered gulp never
+ * goes below max_delay + boxcar_max + 1 rounded up to 1024; when that does
+ * not fit either, FA_RUNTIME_ERROR naming max_delay. */
+int fa_check_memory(const fa_rfi_config* rfi_cfg, fa_block_layout* layout, float baseline_length,
+                    fa_dedisp_config* dedisp, const fa_boxcar_config* boxcar,
+                    int gulp_is_default, uint64_t available_bytes, uint64_t memory_limit_mib,
+                    int device, fa_memory_check* out);
+
 /* ---- candidate post-processing (frost::search) -------------------------- */
 
 /* Giant (include/frost/search/candidate.hpp:24-64). The loc_* fields hold

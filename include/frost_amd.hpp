@@ -16,6 +16,8 @@
 //     postprocess.cpp:45-276)
 //   output::CandWriter::write (lib/output/             frost_amd::cand_rows
 //     cand_writer.cpp:39-75)
+//   pipeline::check_memory (lib/pipeline/memory.cpp:    frost_amd::check_memory,
+//     294-508)                                         estimate_beam_memory
 //
 // Errors follow frost's conventions: std::invalid_argument for bad arguments
 // (thrown before any state changes), std::runtime_error otherwise, including
@@ -235,6 +237,31 @@ private:
     fa_beam* b_ = nullptr;
     std::vector<fa_raw_candidate> rows_;
 };
+
+/** Device bytes a beam of this configuration allocates (the FDMT engine at
+ * its smallest automatic tile); needs no device. */
+inline fa_memory_estimate estimate_beam_memory(const fa_rfi_config& rfi,
+                                               const fa_block_layout& layout, float baseline,
+                                               const fa_dedisp_config& dedisp,
+                                               const fa_boxcar_config& boxcar) {
+    fa_memory_estimate e{};
+    check(fa_beam_estimate_memory(&rfi, &layout, baseline, &dedisp, &boxcar, &e));
+    return e;
+}
+
+/** pipeline::check_memory (include/frost/pipeline/memory.hpp:77-89) on the
+ * device pool: a default gulp is lowered in `layout` and `dedisp`; a gulp
+ * that cannot fit throws std::runtime_error. */
+inline fa_memory_check check_memory(const fa_rfi_config& rfi, fa_block_layout& layout,
+                                    float baseline, fa_dedisp_config& dedisp,
+                                    const fa_boxcar_config& boxcar, bool gulp_is_default,
+                                    uint64_t available_bytes = 0, uint64_t memory_limit_mib = 0,
+                                    int device = 0) {
+    fa_memory_check c{};
+    check(fa_check_memory(&rfi, &layout, baseline, &dedisp, &boxcar, gulp_is_default ? 1 : 0,
+                          available_bytes, memory_limit_mib, device, &c));
+    return c;
+}
 
 /** CandidatePostprocessPlan::execute's result: rows borrowed until the next
  * execute() (CandidatePostprocessResult, include/frost/search/

@@ -66,6 +66,30 @@ int main(int argc, char** argv) {
               "12.5\t100\t0.1\t3\t7\t42.5\t4\t98\t102\t1200.5\t1500.25\t4\t19\t12.5\t4\t19\t1"
               "\t100\t0.1\t42.5\t8\t1200.5\t1500.25\n");
     }
+    {
+        // Device memory admission needs no device to plan
+        // (tests/cpp/api/memory_test.cpp:222-250).
+        fa_rfi_config rfi{};
+        fa_rfi_config_default(&rfi);
+        fa_boxcar_config box{};
+        fa_boxcar_config_default(&box);
+        box.max_boxcar       = 16;
+        fa_dedisp_config d   = cfg;
+        d.block_nsamps       = 4096;
+        fa_block_layout lay{16, 4096, FA_UINT8, 1.0e-3, 1400.0, -1.0};
+        const auto e = frost_amd::estimate_beam_memory(rfi, lay, 0.5f, d, box);
+        CHECK(e.total_bytes == e.input_bytes + e.rfi_bytes + e.fdmt_bytes + e.postprocess_bytes +
+                                   e.reserve_bytes);
+        const auto ok = frost_amd::check_memory(rfi, lay, 0.5f, d, box, false, uint64_t{4} << 30);
+        CHECK(!ok.lowered && ok.gulp_used == 4096 && ok.estimate.total_bytes == e.total_bytes);
+        bool refused = false;
+        try {
+            (void)frost_amd::check_memory(rfi, lay, 0.5f, d, box, false, uint64_t{4} << 30, 1);
+        } catch (const std::runtime_error&) {
+            refused = true;
+        }
+        CHECK(refused && lay.max_nsamps == 4096);
+    }
     if (argc > 1 && std::strcmp(argv[1], "gpu") == 0) {
         // Undispersed width-4 pulse at 2000 in 32 channels: peak at 2003.
         fa_dedisp_config d = cfg;

@@ -1,0 +1,101 @@
+"""Device memory admission (check_memory, lib/pipeline/memory.cpp:294-508)
+for this library's beam, on the host: the estimate is built by running the
+RFI cleaner's and the FDMT engine's constructors in sizing mode (no device),
+so these tests need no GPU. Cases follow frost's own memory tests
+(tests/cpp/api/memory_test.cpp:178-250)."""
+import pytest
+
+from frost_amd import abi, pipeline
+
+MiB, GiB = 1 << 20, 1 << 30
+
+
+def configs(nch=512, gulp=262144, dm_max=2000.0, boxcar=64, tsamp=1e-3, f0=1400.0, df=-1.0,
+            max_candidates=4096):
+    lay = abi.BlockLayout(nch, gulp, abi.UINT8, tsamp, f0, df)
+    dc = abi.dedisp_config(nch, tsamp, f0, df, dm_max, block_nsamps=gulp)
+    bc = abi.boxcar_config(max_boxcar=boxcar, max_candidates=max_candidates)
+    return abi.rfi_config(), lay, 0.5, dc, bc
+
+
+def estimate(gulp, **kw):
+    r, lay, base, dc, bc = configs(gulp=gulp, **kw)
+    return pipeline.estimate_memory(r, lay, base, dc, bc)
+
+
+def check(gulp, default, available, limit_mib=0, **kw):
+    r, lay, base, dc, bc = configs(gulp=gulp, **kw)
+    out = pipeline.check_memory(r, lay, base, dc, bc, default, available, limit_mib)
+    assert lay.max_nsamps == dc.block_nsamps == out.gulp_used
+    return out
+
+
+def test_components_sum_to_total():
+    """memory_test.cpp:253-263: the total is the sum of the components."""
+    e = estimate(65536)
+    comp = e.input_bytes + e.rfi_bytes + e.fdmt_bytes + e.postprocess_bytes
+    assert e.reserve_bytes == comp // 10
+    assert e.total_bytes == comp + e.reserve_bytes
+    assert e.input_bytes == 2 * 65536 * 512  # two u8 ingest slots
+    assert e.tile == 16384  # the smallest automatic tile
+    assert e.fdmt_bytes > 4 * 512 * 65536  # at least the engine's input columns
+
+
+def test_estimate_grows_with_the_gulp():
+    totals = [estimate(g).total_bytes for g in (8192, 16384, 40000, 65536, 131072, 262144)]
+    assert totals == sorted(totals) and len(set(totals)) == len(totals)
+
+
+def test_lowered_default_gulp_and_refusal():
+    """memory_test.cpp:178-218: a default gulp is lowered to a multiple of 1024
+    that fits and stays above max_delay + boxcar_max; budgets too small for
+    even that are refused naming max_delay."""
+    lowered = refused = False
+    for mib in range(256, 4097, 256):
+        try:
+            c = check(262144, True, mib * MiB)
+        except abi.FrostAmdError as e:
+            refused = True
+            assert "max_delay" in str(e)
+            continue
+        budget = mib * MiB - mib * MiB // 10
+        assert c.estimate.total_bytes <= budget
+        assert c.available_bytes == mib * MiB and c.device_memory == 1
+        if c.lowered:
+            lowered = True
+            assert c.gulp_used % 1024 == 0 and c.gulp_used < 262144
+            assert c.requested_bytes > budget
+            # The next multiple of 1024 would not fit.
+            assert estimate(c.gulp_used + 1024).total_bytes > budget
+        else:
+            assert c.gulp_used == 262144
+    assert lowered and refused
+
+
+def test_explicit_gulp_is_refused_with_the_largest_that_fits():
+    r = check(262144, True, 1 * GiB)
+    assert r.lowered
+    with pytest.raises(abi.FrostAmdError, match=f"largest nsamps_gulp that fits is {r.gulp_used} "):
+        check(262144, False, 1 * GiB)
+
+
+def test_memory_limit_caps_the_available_bytes():
+    """memory_test.cpp:222-250: the limit caps what is available."""
+    with pytest.raises(abi.FrostAmdError):
+        check(4096, False, 4 * GiB, limit_mib=1, nch=16, dm_max=40.0, boxcar=16)
+    c = check(4096, False, 4 * GiB, limit_mib=256, nch=16, dm_max=40.0, boxcar=16)
+    assert c.available_bytes == 256 * MiB
+    assert c.estimate.total_bytes < c.available_bytes and not c.lowered
+
+
+def test_fits_without_change():
+    c = check(65536, False, 64 * GiB)
+    assert not c.lowered and c.gulp_used == 65536
+    assert c.requested_bytes == c.estimate.total_bytes
+
+
+def test_rejects_mismatched_gulp():
+    r, lay, base, dc, bc = configs()
+    dc.block_nsamps = 1000
+    with pytest.raises(abi.InvalidArgument):
+        pipeline.estimate_memory(r, lay, base, dc, bc)
