@@ -217,8 +217,8 @@ def main() -> int:
     if a.copy_probe:
         return copy_probe()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(beams shard across GPUs as independent processes, see DESIGN.md)",
+        print(f"bench.py: measures one GPU; --gpus {a.gpus} is not supported (beams are "
+              "independent and shard across GPUs as one process per GPU)",
               file=sys.stderr)
         return 2
     from frost_amd import abi, pipeline, search, workloads
