@@ -1589,13 +1589,18 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
         rowbuf_.resize(kRowBlocks * nch_);
         if (cfg.zero_dm != FA_ZERO_DM_OFF) flag_rows_.resize(layout.max_nsamps);
     }
-    if (!sizing())
+    dead_pin_.reserve(1);
+    mask_pin_.reserve(nch_);
+    if (!sizing()) {
         for (auto& e : ev_) FA_HIP(hipEventCreate(&e));
+        FA_HIP(hipEventCreateWithFlags(&ev_series_, hipEventDisableTiming));
+    }
 }
 
 DeviceRfi::~DeviceRfi() {
     for (auto e : ev_)
         if (e) (void)hipEventDestroy(e);
+    if (ev_series_) (void)hipEventDestroy(ev_series_);
 }
 
 uint64_t DeviceRfi::chunk_count() const {
@@ -1922,6 +1927,9 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
         else
             scan_rows_kernel<-1><<<ceil_div(nsamps, 64), 64, lds, s>>>(a);
         FA_HIP(hipGetLastError());
+        // The per-row series go to the host now: the host's time flags
+        // (which need only them) are computed while the column scan runs.
+        copy_series(s);
         ColArgs b{};
         b.g         = g;
         b.loc       = loc_.get();
@@ -2068,22 +2076,17 @@ void DeviceRfi::build_flags(hipStream_t s) {
         FA_HIP(hipGetLastError());
     }
     time_flags_host_.assign(nsamps, 0);
-    // Per-sample series through pinned staging (s_det, row_std, whole-band).
+    // Per-sample series through pinned staging (s_det, row_std, whole-band),
+    // copied after the row scan (copy_series); the dead-channel count and
+    // mask come back when the stream reaches them, after the time flags are
+    // built on the host.
     float* sdet_pin  = series_pin_.get();
     float* rstd_pin  = series_pin_.get() + nsamps;
     uint8_t* whole_p = whole_pin_.get();
-    if (good_any && cfg_.broad_enabled)
-        FA_HIP(hipMemcpyAsync(sdet_pin, s_det_.get(), nsamps * 4, hipMemcpyDeviceToHost, s));
-    if (good_any && cfg_.dropout_enabled)
-        FA_HIP(hipMemcpyAsync(rstd_pin, row_std_.get(), nsamps * 4, hipMemcpyDeviceToHost, s));
-    if (good_any && cfg_.kadane_spectrum_enabled)
-        FA_HIP(hipMemcpyAsync(whole_p, kwhole_.get(), nsamps, hipMemcpyDeviceToHost, s));
-    unsigned long long dead = 0;
-    FA_HIP(hipMemcpyAsync(&dead, counters_.get() + 4, 8, hipMemcpyDeviceToHost, s));
-    FA_HIP(hipMemcpyAsync(mask_.data(), mask_dev_.get(), n, hipMemcpyDeviceToHost, s));
-    FA_HIP(hipStreamSynchronize(s));
-    report_.chans_dead   = dead;
-    report_.chans_masked = static_cast<uint64_t>(std::count(mask_.begin(), mask_.end(), 0));
+    unsigned long long* dead_pin = dead_pin_.get();
+    FA_HIP(hipMemcpyAsync(dead_pin, counters_.get() + 4, 8, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(mask_pin_.get(), mask_dev_.get(), n, hipMemcpyDeviceToHost, s));
+    if (good_any) FA_HIP(hipEventSynchronize(ev_series_));
     const std::vector<float> sdet =
         good_any && cfg_.broad_enabled ? std::vector<float>(sdet_pin, sdet_pin + nsamps)
                                        : std::vector<float>();
@@ -2125,8 +2128,24 @@ void DeviceRfi::build_flags(hipStream_t s) {
     }
     report_.samps_flagged =
         static_cast<uint64_t>(std::count(time_flags_host_.begin(), time_flags_host_.end(), 1));
+    FA_HIP(hipStreamSynchronize(s));
+    report_.chans_dead = *dead_pin;
+    std::copy(mask_pin_.get(), mask_pin_.get() + n, mask_.begin());
+    report_.chans_masked = static_cast<uint64_t>(std::count(mask_.begin(), mask_.end(), 0));
     FA_HIP(hipMemcpyAsync(flags_dev_.get(), time_flags_host_.data(), nsamps, hipMemcpyHostToDevice, s));
     FA_HIP(hipStreamSynchronize(s));
+}
+
+void DeviceRfi::copy_series(hipStream_t s) {
+    const uint64_t nsamps = nsamps_;
+    if (cfg_.broad_enabled)
+        FA_HIP(hipMemcpyAsync(series_pin_.get(), s_det_.get(), nsamps * 4, hipMemcpyDeviceToHost, s));
+    if (cfg_.dropout_enabled)
+        FA_HIP(hipMemcpyAsync(series_pin_.get() + nsamps, row_std_.get(), nsamps * 4,
+                              hipMemcpyDeviceToHost, s));
+    if (cfg_.kadane_spectrum_enabled)
+        FA_HIP(hipMemcpyAsync(whole_pin_.get(), kwhole_.get(), nsamps, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipEventRecord(ev_series_, s));
 }
 
 } // namespace fa

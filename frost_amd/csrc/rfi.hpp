@@ -81,6 +81,7 @@ private:
                     uint64_t stride, uint64_t offset, hipStream_t s);
     void build_channel_mask(hipStream_t s);
     void build_flags(hipStream_t s);
+    void copy_series(hipStream_t s);
     uint64_t window_of(uint64_t t) const {
         const uint64_t w = t / window_len_;
         return w < nwin_ - 1 ? w : nwin_ - 1;
@@ -139,7 +140,10 @@ private:
     PinnedBuffer<MomentSumsDev> moments_pin_; // staging of the per-window moment sums
     PinnedBuffer<float> series_pin_;          // s_det, row_std
     PinnedBuffer<uint8_t> whole_pin_;         // whole-band Kadane rows
+    PinnedBuffer<unsigned long long> dead_pin_; // dead-channel count
+    PinnedBuffer<uint8_t> mask_pin_;          // channel mask after dead channels
     hipEvent_t ev_[6] = {};
+    hipEvent_t ev_series_ = nullptr;          // the series copies are done
 };
 
 // Host-side helpers shared with tests of the flag logic.
