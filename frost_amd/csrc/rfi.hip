@@ -994,113 +994,119 @@ __device__ float apply_slow(const ApplyArgs& a, uint64_t t, uint64_t c,
 
 // 64 x 64 tile of (time, channel): each cell's cleaned value, written as
 // (cleaned - loc) / scale channel-major through LDS
-// (lib/cpu/rfi_cpu.cpp:245-257). The fast kernel handles tiles whose cells
-// are all clean cells of unflagged rows (or unmeasured), with the channel's
-// and window's values in registers; a tile with any cell that needs a fill
-// is appended to a list and written by apply_slow_kernel instead.
+// (lib/cpu/rfi_cpu.cpp:245-257). The fast kernel writes tiles whose cells are
+// all clean cells of unflagged rows (or unmeasured or masked); a tile that may
+// hold a cell needing a fill is appended to a list and written by
+// apply_slow_kernel, which follows the CPU cleaner cell by cell, so the split
+// only decides which kernel computes the same values. The tile-level tests
+// come first and are conservative (a flagged row; a Kadane-spectrum span of a
+// row reaching the tile's channels; a Kadane-channel segment of an unmasked
+// channel reaching the tile's rows; rows in more than two windows); the
+// narrow-band test is made per cell.
 template <int DT>
 __global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* __restrict__ slow,
                                                          unsigned int* __restrict__ nslow) {
     __shared__ float tile[64][65];
-    __shared__ int32_t rowinfo[64][5]; // [row]: flagged, Kadane-spectrum spans lo0 hi0 lo1 hi1
-    __shared__ float rowkeep[64];      // kept band mean of each row (zero-DM)
+    __shared__ float rowkeep[64]; // kept band mean of each row (zero-DM)
     __shared__ int any_slow;
-    const uint64_t t0  = (uint64_t)blockIdx.x * 64;
-    const uint64_t c0  = (uint64_t)blockIdx.y * 64;
-    const int wave     = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-    const int cl       = threadIdx.x % 64;
-    const uint64_t c   = c0 + cl;
-    const uint64_t ns  = a.g.nsamps;
-    if (threadIdx.x < 64) {
-        const uint64_t t = t0 + threadIdx.x;
-        int32_t info[5]  = {1, -1, -1, -1, -1};
-        if (t < ns) {
-            info[0] = a.flags[t] != 0;
-            if (a.kspec)
-                for (int q = 0; q < 4; ++q) info[1 + q] = a.kspec[4 * t + q];
-        }
-        for (int q = 0; q < 5; ++q) rowinfo[threadIdx.x][q] = info[q];
-        rowkeep[threadIdx.x] = (a.zero_dm != FA_ZERO_DM_OFF && t < ns) ? a.s_keep[t] : 0.0f;
-    }
-    if (threadIdx.x == 0) any_slow = 0;
+    const uint64_t t0 = (uint64_t)blockIdx.x * 64;
+    const uint64_t c0 = (uint64_t)blockIdx.y * 64;
+    const int wave    = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int cl      = threadIdx.x % 64;
+    const uint64_t c  = c0 + cl;
+    const uint64_t ns = a.g.nsamps;
+    const uint64_t tl = min(t0 + 63, ns - 1);
+    const uint64_t c1 = min(c0 + 63, a.g.nch - 1);
     // The tile's rows lie in window wa, or in wa then wb from row tb on.
-    const uint64_t tl  = min(t0 + 63, ns - 1);
-    const uint64_t wa  = a.g.window_of(t0);
-    const uint64_t wb  = a.g.window_of(tl);
-    const uint64_t tb  = a.g.window_begin(wb);
-    const bool good    = c < a.g.nch && a.mask[c] != 0 && wb - wa <= 1;
-    int64_t ks[4]      = {0, -1, 0, -1};
-    float lca = 0.0f, ica = 0.0f, sca = 1.0f, lcb = 0.0f, icb = 0.0f, scb = 1.0f;
-    float rsa = 1.0f, rsb = 1.0f, za = 0.0f, zb = 0.0f; // transforms: scales, zdot fit
-    if (good) {
-        if (a.kseg)
-            for (int q = 0; q < 4; ++q) ks[q] = a.kseg[4 * c + q];
-        const uint64_t ia = wa * a.g.nch + c, ib = wb * a.g.nch + c;
-        lca               = a.loc[ia];
-        ica               = a.inv[ia];
-        sca               = a.scale[ia] > 0.0f ? 1.0f / a.scale[ia] : 1.0f;
-        lcb               = a.loc[ib];
-        icb               = a.inv[ib];
-        scb               = a.scale[ib] > 0.0f ? 1.0f / a.scale[ib] : 1.0f;
-        rsa               = a.scale[ia];
-        rsb               = a.scale[ib];
-        if (a.zero_dm == FA_ZERO_DM_ZDOT) {
-            za = a.zdot_a[c];
-            zb = a.zdot_b[c];
+    const uint64_t wa = a.g.window_of(t0);
+    const uint64_t wb = a.g.window_of(tl);
+    const uint64_t tb = a.g.window_begin(wb);
+    if (threadIdx.x == 0) any_slow = wb - wa > 1;
+    __syncthreads();
+    if (wave == 0) {
+        // Row tests (lane = row of the tile).
+        const uint64_t t = t0 + cl;
+        bool s1          = false;
+        if (t < ns) {
+            s1 = a.flags[t] != 0;
+            if (a.kspec) {
+                const int32_t* sp = a.kspec + 4 * t;
+                s1 = s1 || (sp[0] >= 0 && (uint64_t)sp[0] <= c1 && (uint64_t)sp[1] >= c0) ||
+                     (sp[2] >= 0 && (uint64_t)sp[2] <= c1 && (uint64_t)sp[3] >= c0);
+            }
+            rowkeep[cl] = a.zero_dm != FA_ZERO_DM_OFF ? a.s_keep[t] : 0.0f;
         }
-    }
-    // Samples of this thread's rows, loaded up front at clamped in-range
-    // addresses (out-of-range cells are discarded below).
-    const uint64_t cc = c < a.g.nch ? c : a.g.nch - 1;
-    float xs[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const uint64_t t  = t0 + wave + 4 * k;
-        const uint64_t tc = t < ns ? t : ns - 1;
-        xs[k]             = xat<DT>(a.g.block + tc * a.g.row_bytes, cc, a.g.dtype);
+        if (s1) any_slow = 1;
+    } else if (wave == 1 && a.kseg && c < a.g.nch && a.mask[c] != 0) {
+        // Channel tests (lane = channel of the tile).
+        const int64_t* sg = a.kseg + 4 * c;
+        const int64_t lo = (int64_t)t0, hi = (int64_t)tl;
+        if ((sg[0] <= sg[1] && sg[0] <= hi && sg[1] >= lo) ||
+            (sg[2] <= sg[3] && sg[2] <= hi && sg[3] >= lo))
+            any_slow = 1;
     }
     __syncthreads();
-    bool slow_cell = c < a.g.nch && a.mask[c] != 0 && wb - wa > 1;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int r      = wave + 4 * k;
-        const uint64_t t = t0 + r;
-        float v          = 0.0f;
-        if (good && t < ns) {
-            const bool inb = t >= tb;
-            const float lc = inb ? lcb : lca, ic = inb ? icb : ica, sc = inb ? scb : sca;
-            const float x  = xs[k];
-            bool clean     = rowinfo[r][0] == 0;
-            if (clean && ic != 0.0f) {
-                const float z = (x - lc) * ic;
-                if (a.narrow && fabsf(z) > a.clip) clean = false;
-                const int32_t ci = (int32_t)c;
-                if ((rowinfo[r][1] >= 0 && ci >= rowinfo[r][1] && ci <= rowinfo[r][2]) ||
-                    (rowinfo[r][3] >= 0 && ci >= rowinfo[r][3] && ci <= rowinfo[r][4]))
-                    clean = false;
-                const int64_t it = (int64_t)t;
-                if ((it >= ks[0] && it <= ks[1]) || (it >= ks[2] && it <= ks[3])) clean = false;
-            }
-            slow_cell = slow_cell || !clean;
-            if (!a.transform) {
-                v = (x - lc) * sc;
-            } else if (ic != 0.0f) {
-                // Clean cell as a value (a tile with any other cell is slow).
-                const float z    = (x - lc) * ic;
-                const float term = a.zero_dm == FA_ZERO_DM_SUBTRACT ? rowkeep[r]
-                                   : a.zero_dm == FA_ZERO_DM_ZDOT   ? (za * rowkeep[r]) + zb
-                                                                    : 0.0f;
-                const float vv   = z - term;
-                v = a.equalize ? a.eq_loc + (a.eq_scale * vv) : lc + ((inb ? rsb : rsa) * vv);
-            } else {
-                v = x;
+    bool slow_tile = any_slow != 0;
+    if (!slow_tile) {
+        const bool good = c < a.g.nch && a.mask[c] != 0;
+        float lca = 0.0f, ica = 0.0f, sca = 1.0f, lcb = 0.0f, icb = 0.0f, scb = 1.0f;
+        float rsa = 1.0f, rsb = 1.0f, za = 0.0f, zb = 0.0f; // transforms: scales, zdot fit
+        if (good) {
+            const uint64_t ia = wa * a.g.nch + c, ib = wb * a.g.nch + c;
+            lca               = a.loc[ia];
+            ica               = a.inv[ia];
+            sca               = a.scale[ia] > 0.0f ? 1.0f / a.scale[ia] : 1.0f;
+            lcb               = a.loc[ib];
+            icb               = a.inv[ib];
+            scb               = a.scale[ib] > 0.0f ? 1.0f / a.scale[ib] : 1.0f;
+            rsa               = a.scale[ia];
+            rsb               = a.scale[ib];
+            if (a.zero_dm == FA_ZERO_DM_ZDOT) {
+                za = a.zdot_a[c];
+                zb = a.zdot_b[c];
             }
         }
-        tile[cl][r] = v;
+        // Samples of this thread's rows, loaded up front at clamped in-range
+        // addresses (out-of-range cells are discarded below).
+        const uint64_t cc = c < a.g.nch ? c : a.g.nch - 1;
+        float xs[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint64_t t  = t0 + wave + 4 * k;
+            const uint64_t tc = t < ns ? t : ns - 1;
+            xs[k]             = xat<DT>(a.g.block + tc * a.g.row_bytes, cc, a.g.dtype);
+        }
+        bool narrow_cell = false;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int r      = wave + 4 * k;
+            const uint64_t t = t0 + r;
+            float v          = 0.0f;
+            if (good && t < ns) {
+                const bool inb = t >= tb;
+                const float lc = inb ? lcb : lca, ic = inb ? icb : ica, sc = inb ? scb : sca;
+                const float x  = xs[k];
+                const float z  = (x - lc) * ic;
+                narrow_cell    = narrow_cell || (a.narrow && ic != 0.0f && fabsf(z) > a.clip);
+                if (!a.transform) {
+                    v = (x - lc) * sc;
+                } else if (ic != 0.0f) {
+                    const float term = a.zero_dm == FA_ZERO_DM_SUBTRACT ? rowkeep[r]
+                                       : a.zero_dm == FA_ZERO_DM_ZDOT   ? (za * rowkeep[r]) + zb
+                                                                        : 0.0f;
+                    const float vv   = z - term;
+                    v = a.equalize ? a.eq_loc + (a.eq_scale * vv) : lc + ((inb ? rsb : rsa) * vv);
+                } else {
+                    v = x;
+                }
+            }
+            tile[cl][r] = v;
+        }
+        if (narrow_cell) any_slow = 1;
+        __syncthreads();
+        slow_tile = any_slow != 0;
     }
-    if (slow_cell) any_slow = 1;
-    __syncthreads();
-    if (any_slow) {
+    if (slow_tile) {
         if (threadIdx.x == 0) {
             const unsigned int k = atomicAdd(nslow, 1U);
             slow[2 * k]          = blockIdx.x;
@@ -1108,13 +1114,19 @@ __global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* 
         }
         return;
     }
-    // Write: 64 channels x 64 samples, time contiguous per channel.
-    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-        const int cc2     = idx / 64, rr = idx % 64;
-        const uint64_t c2 = c0 + cc2, t2 = t0 + rr;
-        if (c2 >= a.g.nch || t2 >= a.capacity) continue;
-        const uint64_t dst = a.flip ? a.g.nch - 1 - c2 : c2;
-        a.out[dst * a.stride + a.offset + t2] = tile[cc2][rr];
+    // Write: thread (wave, lane) stores sample t0 + lane of channels
+    // c0 + wave + 4i, time contiguous per channel.
+    const uint64_t t2 = t0 + cl;
+    if (t2 >= a.capacity) return;
+    const int64_t step = a.flip ? -(int64_t)a.stride : (int64_t)a.stride;
+    const int64_t n    = (int64_t)a.g.nch;
+    int64_t c2         = (int64_t)c0 + wave;
+    int64_t at = (int64_t)(a.offset + t2) + (a.flip ? n - 1 - c2 : c2) * (int64_t)a.stride;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        if (c2 < n) a.out[at] = tile[wave + 4 * i][cl];
+        c2 += 4;
+        at += 4 * step;
     }
 }
 
