@@ -1009,9 +1009,11 @@ __global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* 
     __shared__ float tile[64][65];
     __shared__ float rowkeep[64]; // kept band mean of each row (zero-DM)
     __shared__ int any_slow;
-    const uint64_t t0 = (uint64_t)blockIdx.x * 64;
-    const uint64_t c0 = (uint64_t)blockIdx.y * 64;
-    const int wave    = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const uint32_t cblk = (uint32_t)((a.g.nch + 63) / 64);
+    const uint32_t tt   = blockIdx.x / cblk, ct = blockIdx.x % cblk;
+    const uint64_t t0   = (uint64_t)tt * 64;
+    const uint64_t c0   = (uint64_t)ct * 64;
+    const int wave      = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int cl      = threadIdx.x % 64;
     const uint64_t c  = c0 + cl;
     const uint64_t ns = a.g.nsamps;
@@ -1109,8 +1111,8 @@ __global__ void __launch_bounds__(256) apply_fast_kernel(ApplyArgs a, uint32_t* 
     if (slow_tile) {
         if (threadIdx.x == 0) {
             const unsigned int k = atomicAdd(nslow, 1U);
-            slow[2 * k]          = blockIdx.x;
-            slow[2 * k + 1]      = blockIdx.y;
+            slow[2 * k]          = tt;
+            slow[2 * k + 1]      = ct;
         }
         return;
     }
@@ -2010,7 +2012,12 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
     ap.zdot_a      = zdot_a_.get();
     ap.zdot_b      = zdot_b_.get();
     // counters_[5] counts the slow tiles listed in slow_tiles_.
-    const dim3 tiles(ceil_div(capacity, 64), cblk);
+    // One block per tile, channel tiles fastest: the two 64-byte halves of
+    // a 128-byte input line (channel tiles 2j, 2j + 1) are read by blocks
+    // launched next to each other.
+    const uint64_t ntiles = static_cast<uint64_t>(ceil_div(capacity, 64)) * cblk;
+    require(ntiles < (uint64_t{1} << 32), "RFI: too many 64 x 64 output tiles for one launch");
+    const unsigned tiles = static_cast<unsigned>(ntiles);
     auto* nslow = reinterpret_cast<unsigned int*>(counters_.get() + 5);
     if (layout_.dtype == FA_UINT8)
         apply_fast_kernel<FA_UINT8><<<tiles, 256, 0, s>>>(ap, slow_tiles_.get(), nslow);
