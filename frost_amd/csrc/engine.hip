@@ -873,7 +873,7 @@ __device__ uint32_t walk_chunk(const SearchArgs& a, float* snr_lds, OpenRun* st,
 }
 
 template <int KMAX, bool TRAP>
-__global__ void __launch_bounds__(kBlock) search_reg_kernel(RegSearchArgs ra) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRAP ? 1 : 6))) search_reg_kernel(RegSearchArgs ra) {
     const SearchArgs& a = ra.a;
     __shared__ float s_snr[kBlock / 64][kRegChunk];
     __shared__ OpenRun s_st[kBlock / 64][16];
@@ -939,27 +939,61 @@ __global__ void __launch_bounds__(kBlock) search_reg_kernel(RegSearchArgs ra) {
         }
         const int64_t tc0 = a.tile_t0 + u;
         const bool fast   = tc0 >= warm_max && u + kRegChunk <= tl;
+        // The ladder levels of this chunk.
 #pragma unroll
-        for (int k = 0; k <= KMAX; ++k) {
-            if (k > 0) {
+        for (int k = 1; k <= KMAX; ++k) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float bj;
+                switch (k) {
+                case 1: bj = ladder_back<1>(D, P, j, lane); break;
+                case 2: bj = ladder_back<2>(D, P, j, lane); break;
+                case 3: bj = ladder_back<3>(D, P, j, lane); break;
+                case 4: bj = ladder_back<4>(D, P, j, lane); break;
+                case 5: bj = ladder_back<5>(D, P, j, lane); break;
+                case 6: bj = ladder_back<6>(D, P, j, lane); break;
+                case 7: bj = ladder_back<7>(D, P, j, lane); break;
+                default: bj = ladder_back<8>(D, P, j, lane); break;
+                }
+                D[k][j] = D[k - 1][j] + bj;
+            }
+        }
+        // One test for the whole chunk first: with no run open and no S/N
+        // above the threshold at any searched width, nothing below can emit
+        // or change state (the per-width tests are the same comparisons).
+        bool quiet = u < 0; // the warm-up chunk before the tile is never searched
+        if (!quiet && fast && open_mask == 0) {
+            bool anyp = false;
+#pragma unroll
+            for (int k = 0; k <= KMAX; ++k) {
+                if (ra.widx[k] < 0) continue;
+                float m = -INFINITY;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    float bj;
-                    switch (k) {
-                    case 1: bj = ladder_back<1>(D, P, j, lane); break;
-                    case 2: bj = ladder_back<2>(D, P, j, lane); break;
-                    case 3: bj = ladder_back<3>(D, P, j, lane); break;
-                    case 4: bj = ladder_back<4>(D, P, j, lane); break;
-                    case 5: bj = ladder_back<5>(D, P, j, lane); break;
-                    case 6: bj = ladder_back<6>(D, P, j, lane); break;
-                    case 7: bj = ladder_back<7>(D, P, j, lane); break;
-                    default: bj = ladder_back<8>(D, P, j, lane); break;
+                    float sum = D[k][j];
+                    if (TRAP && k >= 2) {
+                        float e2;
+                        switch (k) {
+                        case 2: e2 = trap_back<2>(D, P, j, lane); break;
+                        case 3: e2 = trap_back<3>(D, P, j, lane); break;
+                        case 4: e2 = trap_back<4>(D, P, j, lane); break;
+                        case 5: e2 = trap_back<5>(D, P, j, lane); break;
+                        case 6: e2 = trap_back<6>(D, P, j, lane); break;
+                        case 7: e2 = trap_back<7>(D, P, j, lane); break;
+                        default: e2 = trap_back<8>(D, P, j, lane); break;
+                        }
+                        sum = (sum - 0.5f * D[k - 2][j]) - 0.5f * e2;
                     }
-                    D[k][j] = D[k - 1][j] + bj;
+                    m = j == 0 ? sum : fmaxf(m, sum);
                 }
+                anyp = anyp || m * inv_c[k] > a.thr;
             }
+            quiet = __ballot(anyp) == 0;
+        }
+#pragma unroll
+        for (int k = 0; k <= KMAX; ++k) {
             const int w = ra.widx[k];
-            if (u < 0 || w < 0) continue;
+            if (quiet || w < 0) continue;
             const uint32_t W = 1U << k;
             float sv[4], sums[4];
 #pragma unroll
