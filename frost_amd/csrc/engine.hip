@@ -1257,6 +1257,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     level_seconds.assign(plan_.nlevels, 0.0);
     level_launches.assign(plan_.nlevels, 0.0);
     if (sizing()) return;
+    FA_HIP(hipStreamCreateWithFlags(&search_stream_, hipStreamNonBlocking));
     int lds_max = 0;
     FA_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
     const uint64_t lds_need = (uint64_t)(kmax_ + 1) * (kChunk + wm1_) * sizeof(float) +
@@ -1268,6 +1269,11 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
 }
 
 Engine::~Engine() {
+    if (search_stream_) {
+        (void)hipStreamSynchronize(search_stream_);
+        (void)hipStreamDestroy(search_stream_);
+    }
+    for (auto e : sync_events_) (void)hipEventDestroy(e);
     for (auto e : events_) (void)hipEventDestroy(e);
     if (ev_begin_) (void)hipEventDestroy(ev_begin_);
     if (ev_end_) (void)hipEventDestroy(ev_end_);
@@ -1746,6 +1752,17 @@ std::vector<float> Engine::inv_sigma(uint32_t plane) const {
     return std::vector<float>(inv_cur_host_.begin() + b, inv_cur_host_.begin() + b + n);
 }
 
+hipEvent_t Engine::sync_event(hipStream_t s) {
+    if (sync_used_ == sync_events_.size()) {
+        hipEvent_t e;
+        FA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        sync_events_.push_back(e);
+    }
+    hipEvent_t e = sync_events_[sync_used_++];
+    FA_HIP(hipEventRecord(e, s));
+    return e;
+}
+
 void Engine::record_pair(int level, hipStream_t s, bool start) {
     if (start) {
         while (events_.size() < events_used_ + 2) {
@@ -1801,6 +1818,11 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
         keep_.reserve(static_cast<uint64_t>(nprows_) * cap_);
     }
     events_used_ = 0;
+    sync_used_   = 0;
+    // Searches still reading each level buffer (their last event), and the
+    // last search of the call.
+    hipEvent_t reading[2] = {nullptr, nullptr};
+    hipEvent_t last_search = nullptr;
     FA_HIP(hipEventRecord(ev_begin_, s));
     set_u64_kernel<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(counters_.get()), 0,
                                     processed_ + N);
@@ -1812,6 +1834,10 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
         for (uint32_t l = 0; l <= L; ++l) {
             if (level_source_[l] == -2) continue; // computed inside a fused launch
             float* out = buf_[level_buf_[l]].get();
+            if (hipEvent_t& r = reading[level_buf_[l]]) {
+                FA_HIP(hipStreamWaitEvent(s, r, 0)); // searches of the old contents done
+                r = nullptr;
+            }
             record_pair(static_cast<int>(l), s, true);
             if (l == 0) {
                 int hcmax = 0;
@@ -1875,7 +1901,12 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                 a.cap      = cand_cap_;
                 a.keep     = keep_planes_ ? keep_.get() : nullptr;
                 a.keep_stride = cap_;
-                record_pair(-1, s, true);
+                // The search of this level follows its launch on the search
+                // stream (in level and tile order there, which the runs'
+                // state across tiles needs).
+                FA_HIP(hipStreamWaitEvent(search_stream_, sync_event(s), 0));
+                hipStream_t ss = search_stream_;
+                record_pair(-1, ss, true);
                 if (reg_search_) {
                     RegSearchArgs ra{};
                     ra.a       = a;
@@ -1888,16 +1919,19 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                     }
                     const unsigned blocks = ceil_div(search_count_[l], kBlock / 64);
                     (cfg_.use_trapezoid ? reg_launcher<true>(static_cast<int>(kmax_))
-                                        : reg_launcher<false>(static_cast<int>(kmax_)))(ra, blocks, s);
+                                        : reg_launcher<false>(static_cast<int>(kmax_)))(ra, blocks, ss);
                 } else if (pow2_widths_) {
-                    search_kernel<true><<<search_count_[l], kBlock, lds_search, s>>>(a);
+                    search_kernel<true><<<search_count_[l], kBlock, lds_search, ss>>>(a);
                 } else {
-                    search_kernel<false><<<search_count_[l], kBlock, lds_search, s>>>(a);
+                    search_kernel<false><<<search_count_[l], kBlock, lds_search, ss>>>(a);
                 }
-                record_pair(-1, s, false);
+                record_pair(-1, ss, false);
+                last_search = sync_event(ss);
+                reading[level_buf_[l]] = last_search;
             }
         }
     }
+    if (last_search) FA_HIP(hipStreamWaitEvent(s, last_search, 0)); // every search done
     // Frontier: earliest start of an open run, else the next sample.
     const uint64_t nopen = static_cast<uint64_t>(nprows_) * nw;
     frontier_kernel<<<ceil_div(nopen, 256), 256, 0, s>>>(

@@ -229,6 +229,13 @@ private:
     size_t events_used_ = 0;
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
     void record_pair(int level, hipStream_t s, bool start);
+    // Plane searches run on their own stream, beside the next tree launches:
+    // a search waits for the launch that wrote its level; a launch that
+    // overwrites a buffer waits for the searches still reading it.
+    hipStream_t search_stream_ = nullptr;
+    std::vector<hipEvent_t> sync_events_;
+    size_t sync_used_ = 0;
+    hipEvent_t sync_event(hipStream_t s);
 };
 
 } // namespace fa
