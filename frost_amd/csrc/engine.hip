@@ -69,23 +69,39 @@ fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
                 s_h[j]            = (int32_t)rw[2];
             }
             __syncthreads();
+            // Thread tid holds samples i0 + 4 tid + 4 kBlock g + (0..3),
+            // g < kInitPer / 4: whole 16-byte groups, stored as one vector
+            // (o + i0 is 64-float aligned).
+            static_assert(kInitPer % 4 == 0, "init samples come in groups of four");
+            constexpr int kGroups = kInitPer / 4;
             if (rb == 0) {
 #pragma unroll
-                for (int q = 0; q < kInitPer; ++q) acc[q] = sx[dm + threadIdx.x + q * kBlock];
+                for (int q = 0; q < kInitPer; ++q)
+                    acc[q] = sx[dm + 4 * threadIdx.x + 4 * kBlock * (q / 4) + (q % 4)];
             }
             for (int jr = 0; jr < nb; ++jr) {
                 const int D = s_d[jr];
                 for (; k < D; ++k) {
 #pragma unroll
                     for (int q = 0; q < kInitPer; ++q)
-                        acc[q] = acc[q] + sx[dm + threadIdx.x + q * kBlock - (k + 1)];
+                        acc[q] = acc[q] +
+                                 sx[dm + 4 * threadIdx.x + 4 * kBlock * (q / 4) + (q % 4) - (k + 1)];
                 }
                 const int skip = hc - s_h[jr];
                 float* o       = out + s_base[jr] - skip;
 #pragma unroll
-                for (int q = 0; q < kInitPer; ++q) {
-                    const int i = i0 + threadIdx.x + q * kBlock;
-                    if (i < total && i >= skip) o[i] = acc[q];
+                for (int gq = 0; gq < kGroups; ++gq) {
+                    const int ib = i0 + 4 * threadIdx.x + 4 * kBlock * gq;
+                    if (ib >= skip && ib + 3 < total) {
+                        *reinterpret_cast<float4*>(o + ib) =
+                            make_float4(acc[4 * gq], acc[4 * gq + 1], acc[4 * gq + 2], acc[4 * gq + 3]);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int i = ib + j;
+                            if (i < total && i >= skip) o[i] = acc[4 * gq + j];
+                        }
+                    }
                 }
             }
         }
