@@ -237,3 +237,44 @@ def test_engine_rejects_bad_config():
     with pytest.raises(abi.InvalidArgument):
         search.FDMTBoxcar(plan, abi.boxcar_config(max_boxcar=16,
                                                   peak_mode=abi.PEAK_ACROSS_WIDTHS))
+
+
+def test_full_size_rows_exact_on_integer_input(oracle, stream, monkeypatch):
+    """The benchmark shape (4096 ch, gulp 2^20, DM 0-2000, 11 planes; 2^18
+    time tiles here): integer-valued input makes every FDMT sum exact, so
+    sampled rows of every plane, at times in the first tile and across a
+    tile boundary, equal brute-force sums along their delay paths."""
+    from frost_amd import workloads
+    say("test_full_size_rows_exact_on_integer_input")
+    W = workloads.PROD_4096
+    dcfg = W.dedisp_config()
+    bcfg = abi.boxcar_config(max_boxcar=W.boxcar_max, detect_thresh=1.0e6)
+    plan = search.make_fdmt_plan(dcfg)
+    monkeypatch.setenv("FROST_AMD_TILE", str(1 << 18))
+    eng = search.FDMTBoxcar(plan, bcfg)
+    monkeypatch.delenv("FROST_AMD_TILE")
+    eng.keep_planes(True)
+    nch, n = W.nchans, W.gulp
+    wf = np.random.default_rng(11).integers(-6, 7, (nch, n), dtype=np.int8).astype(np.float32)
+    dev = search.DeviceArray.from_host(wf, stream)
+    eng.execute_device(dev, n, n, stream)
+    eng.copy_result(stream)
+    del dev
+    hist = oracle.Engine(dcfg, bcfg).hist
+    o = oracle.Plan(dcfg)
+    span = 1536
+    starts = [hist + 4321, (1 << 18) - 700, 3 * (1 << 18) - 300]  # tiles 0, 0-1, 2-3
+    slices = {t0: np.ascontiguousarray(wf[:, t0 - hist:t0 + span]) for t0 in starts}
+    del wf
+    rng = np.random.default_rng(3)
+    for p in range(plan.summary()["nplanes"]):
+        rows = eng.plane_rows(p, n)
+        ndm = rows.shape[0]
+        picks = sorted({0, ndm - 1, ndm // 2, *rng.integers(0, ndm, 3).tolist()})
+        for r in picks:
+            for t0, xs in slices.items():
+                want = o.brute_force_row(p, r, xs, hist, span)
+                got = rows[r, t0:t0 + span]
+                assert (got == want).all(), (p, r, t0, np.argwhere(got != want)[:3])
+        del rows
+    say("  checked", plan.summary()["nplanes"], "planes")
