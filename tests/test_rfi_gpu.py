@@ -145,6 +145,25 @@ def test_fill_modes(oracle, stream, fill):
             fill_tol=None if fill == abi.FILL_MEAN else 1e-3)
 
 
+def test_benchmark_channel_count(oracle, stream):
+    """The benchmark's 4096 channels, 300 MHz band and strategies (moments,
+    IQRM, both Kadane searches) on a 16384-row block with RFI channels,
+    impulses and a dead channel: every LDS channel tile of the scans, apply
+    tiles over all 64 channel groups."""
+    say("test_benchmark_channel_count")
+    nch, n = 4096, 16384
+    rng = np.random.default_rng(44)
+    blk = synth.filterbank(n, nch, seed=44, df=-300.0 / nch,
+                           rfi_chans=tuple(int(c) for c in rng.choice(nch, 40, replace=False)),
+                           impulses=(1000, 9000, 9001), dead=(2047,),
+                           pulses=[(800.0, 6000, 16, 2.0)], bands=[(4000, 100, 900, 3.0)])
+    cfg = abi.rfi_config(moments_enabled=1, iqrm_enabled=1, kadane_channel_enabled=1,
+                         kadane_spectrum_enabled=1, dropout_enabled=1)
+    gpu, _ = compare(oracle, stream, cfg, layout(nch, n, tsamp=64e-6, df=-300.0 / nch), 0.25,
+                     [(blk, n, n)])
+    assert gpu.report().chans_masked > 0 and gpu.report().samps_flagged > 0
+
+
 TRANSFORMS = [(abi.ZERO_DM_SUBTRACT, 0), (abi.ZERO_DM_ZDOT, 0), (abi.ZERO_DM_OFF, 1),
               (abi.ZERO_DM_SUBTRACT, 1), (abi.ZERO_DM_ZDOT, 1)]
 
