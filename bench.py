@@ -87,10 +87,17 @@ def cpu_baseline(W, dev_gulp, stream, sample_rows):
     eng.execute(wf)
     el = time.perf_counter() - t0
     ndm = eng.plan.summary()["ndm"]
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": ndm * sample_rows / el,
         "unit": "DM-trial*samples/s",
         "cores": threads,
+        "cpu_model": cpu,
         "kind": "port",
         "sample": (f"first {sample_rows} rows of the same gulp (one cold gulp: RFI + FDMT over "
                    f"{sample_rows} samples plus the carried {eng.hist}-sample history + search "
@@ -330,6 +337,7 @@ def main() -> int:
             "gulp_seconds": W.gulp_seconds(),
             "p50_fraction_of_realtime": float(np.percentile(lat, 50)) / W.gulp_seconds(),
             "p99_fraction_of_realtime": float(np.percentile(lat, 99)) / W.gulp_seconds(),
+            "p999_fraction_of_realtime": float(np.percentile(lat, 99.9)) / W.gulp_seconds(),
             "realtime_beams": W.gulp_seconds() / float(np.mean(lat)),
         },
         "stage_seconds_per_gulp": {k: v / a.steps for k, v in stage.items()},
