@@ -1499,6 +1499,8 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
                     zap_freqs_[2 * i] <= zap_freqs_[2 * i + 1],
                 "rfi.zap_freqs ranges must be finite with lo <= hi");
     require(layout.nchans > 0 && layout.max_nsamps > 0, "RFI: nchans and max_nsamps must be > 0");
+    // Per-channel passes launch one grid row per channel (grid y <= 65535).
+    require(layout.nchans <= 65535, "RFI: nchans must be at most 65535");
     // Device index math: samples and (window, channel) cells fit 31 bits.
     require(layout.max_nsamps < (uint64_t{1} << 31), "RFI: max_nsamps must be below 2^31");
 
