@@ -165,13 +165,13 @@ def ingest_measure(W, beam, gulp, stream, ndm, steps=3):
     for _ in range(2):  # fill both slots
         np.copyto(beam.ingest_slot(), host)
         beam.ingest_submit(W.gulp)
-        beam.execute_next(stream)
+        beam.execute_next(stream, rows=False)
     stream.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         beam.ingest_slot()
         beam.ingest_submit(W.gulp)
-        beam.execute_next(stream)
+        beam.execute_next(stream, rows=False)
     serial = (time.perf_counter() - t0) / steps
     t0 = time.perf_counter()
     beam.ingest_slot()
@@ -180,7 +180,7 @@ def ingest_measure(W, beam, gulp, stream, ndm, steps=3):
         if k + 1 < steps:
             beam.ingest_slot()
             beam.ingest_submit(W.gulp)
-        beam.execute_next(stream)
+        beam.execute_next(stream, rows=False)
     overlapped = (time.perf_counter() - t0) / steps
     return {"bytes_per_gulp": W.gulp_bytes(), "serial_ms_per_gulp": 1e3 * serial,
             "overlapped_ms_per_gulp": 1e3 * overlapped,

@@ -49,10 +49,12 @@ class Beam:
         """Starts the filled slot's copy to the device (beam copy stream)."""
         check(lib().fa_beam_ingest_submit(self.h, nsamps))
 
-    def execute_next(self, stream: Stream):
-        """Runs the oldest submitted gulp once its copy is done."""
+    def execute_next(self, stream: Stream, rows: bool = True):
+        """Runs the oldest submitted gulp once its copy is done; returns its
+        candidate rows as Python tuples unless rows is False (the rows stay
+        in the beam, as after execute_device)."""
         check(lib().fa_beam_execute_next(self.h, stream.h))
-        return self.result()
+        return self.result() if rows else None
 
     def result(self):
         n, dg, dd, fr = (C.c_uint64() for _ in range(4))
