@@ -115,13 +115,14 @@ def test_gulps_tails_and_padding(oracle, stream):
                                             (b4, cap, 50)])
 
 
-@pytest.mark.parametrize("dtype,bits", [(abi.UINT2, 2), (abi.UINT4, 4), (abi.UINT16, 16),
-                                        (abi.FLOAT32, 32)])
+@pytest.mark.parametrize("dtype,bits", [(abi.UINT1, 1), (abi.UINT2, 2), (abi.UINT4, 4),
+                                        (abi.UINT16, 16), (abi.FLOAT32, 32)])
 def test_dtypes(oracle, stream, dtype, bits):
     say(f"test_dtypes bits={bits}")
     nch, n = 64, 6000
-    if dtype in (abi.UINT2, abi.UINT4):
-        raw = synth.filterbank(n, nch, seed=bits, mean=(1 << bits) / 2, sigma=(1 << bits) / 6,
+    if dtype in (abi.UINT1, abi.UINT2, abi.UINT4):
+        mean, sigma = ((0.5, 0.5) if bits == 1 else ((1 << bits) / 2, (1 << bits) / 6))
+        raw = synth.filterbank(n, nch, seed=bits, mean=mean, sigma=sigma,
                                rfi_chans=(9,), impulses=(100,))
         raw = np.clip(raw, 0, (1 << bits) - 1).astype(np.uint8)
         blk = synth.pack_bits(raw, bits)
