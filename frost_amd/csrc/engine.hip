@@ -93,8 +93,10 @@ fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
                 for (int gq = 0; gq < kGroups; ++gq) {
                     const int ib = i0 + 4 * threadIdx.x + 4 * kBlock * gq;
                     if (ib >= skip && ib + 3 < total) {
-                        *reinterpret_cast<float4*>(o + ib) =
-                            make_float4(acc[4 * gq], acc[4 * gq + 1], acc[4 * gq + 2], acc[4 * gq + 3]);
+                        // Streamed like the tree launches' stores.
+                        typedef float f4 __attribute__((ext_vector_type(4)));
+                        const f4 v4 = {acc[4 * gq], acc[4 * gq + 1], acc[4 * gq + 2], acc[4 * gq + 3]};
+                        __builtin_nontemporal_store(v4, reinterpret_cast<f4*>(o + ib));
                     } else {
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
@@ -217,9 +219,15 @@ __device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t rs, int byte
                                                                           0, 0));
 }
 
+// Stores of level rows are streamed (non-temporal cache policy, aux bit 1 on
+// gfx950): a row is read back only by a later launch, so its lines are not
+// kept in L2 at the expense of the input rows that neighbouring blocks
+// share.
+constexpr int kStoreStream = 2;
+
 __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t rs, int byte_off, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), rs, whole_offset(byte_off), 0,
-                                          0);
+                                          kStoreStream);
 }
 
 __device__ __forceinline__ void lds_barrier() {
