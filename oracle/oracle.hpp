@@ -232,6 +232,9 @@ public:
 
     std::vector<float> compute_inv(const std::vector<uint8_t>& m,
                                    uint32_t p) const;
+    void search_row(uint32_t p, uint32_t r, const std::vector<float>& row, SizeType N,
+                    bool first, std::vector<std::vector<float>>& ladder, Open* orow,
+                    std::vector<fa_raw_candidate>& out) const;
     void search_plane(uint32_t p,
                       const std::vector<std::vector<float>>& node_rows,
                       SizeType N, bool first);

@@ -289,6 +289,33 @@ int oracle_engine_plane_rows(void* h, uint32_t p, float* out, uint64_t cap) {
         std::memcpy(out, v.data(), v.size() * 4);
     });
 }
+// Row r of plane p searched alone, as on a first gulp (zero look-back,
+// samples before the row's warm-up unsearched), with the excursions still
+// open at the end emitted as drain() emits them. The engine must not have
+// executed a gulp; its own state is not changed.
+int oracle_engine_search_row(void* h, uint32_t p, uint32_t r, const float* row, uint64_t nsamps,
+                             fa_raw_candidate* out, uint64_t cap, uint64_t* n) {
+    return guard([&] {
+        auto* e = static_cast<Engine*>(h);
+        if (p >= e->plan.planes.size() || r >= e->plan.planes[p].delay.size())
+            throw std::invalid_argument("plane row out of range");
+        if (e->processed != 0) throw std::logic_error("search_row needs an engine before its first gulp");
+        if (!row || !n || nsamps == 0) throw std::invalid_argument("null argument or empty row");
+        const SizeType E = e->hist + nsamps;
+        std::vector<float> x(E, 0.0F);
+        std::memcpy(&x[e->hist], row, nsamps * sizeof(float));
+        int kmax = 0;
+        while ((1U << (kmax + 1)) <= e->widths.back()) ++kmax;
+        std::vector<std::vector<float>> ladder(kmax + 1, std::vector<float>(E, 0.0F));
+        std::vector<Engine::Open> opens(e->widths.size());
+        std::vector<fa_raw_candidate> res;
+        e->search_row(p, r, x, nsamps, true, ladder, opens.data(), res);
+        for (uint32_t w = 0; w < opens.size(); ++w)
+            if (opens[w].open) res.push_back(e->make_candidate(p, r, w, opens[w]));
+        *n = res.size();
+        if (out) std::memcpy(out, res.data(), std::min<uint64_t>(cap, res.size()) * sizeof(*out));
+    });
+}
 int oracle_engine_inv_sigma(void* h, uint32_t p, float* out, uint64_t cap) {
     return guard([&] {
         auto* e = static_cast<Engine*>(h);

@@ -78,6 +78,8 @@ def lib() -> C.CDLL:
             "oracle_engine_frontier": (C.c_uint64, [P]),
             "oracle_engine_plane_rows": (C.c_int, [P, C.c_uint32, P, C.c_uint64]),
             "oracle_engine_inv_sigma": (C.c_int, [P, C.c_uint32, P, C.c_uint64]),
+            "oracle_engine_search_row": (C.c_int, [P, C.c_uint32, C.c_uint32, P, C.c_uint64,
+                                                   P, C.c_uint64, P]),
             "oracle_engine_hist": (C.c_uint64, [P]),
             "oracle_sizeof_candidate": (C.c_uint64, []),
             "oracle_sizeof_rfi_config": (C.c_uint64, []),
@@ -281,6 +283,18 @@ class Engine:
         out = np.zeros(info.ndm * nsamps, np.float32)
         check(lib().oracle_engine_plane_rows(self.h, p, ptr(out), out.size))
         return out.reshape(info.ndm, nsamps)
+
+    def search_row(self, p, r, row: np.ndarray):
+        """Candidates of plane row r searched alone as on a first gulp, the
+        excursions open at its end included (search_oracle.cpp search_row)."""
+        row = np.ascontiguousarray(row, dtype=np.float32)
+        n = C.c_uint64()
+        check(lib().oracle_engine_search_row(self.h, p, r, ptr(row), row.size, None, 0,
+                                             C.byref(n)))
+        arr = (abi.RawCandidate * max(n.value, 1))()
+        check(lib().oracle_engine_search_row(self.h, p, r, ptr(row), row.size, C.cast(arr, P),
+                                             n.value, C.byref(n)))
+        return abi.candidate_tuples(arr, n.value)
 
     def inv_sigma(self, p, nwidths):
         info = self.plan.plane(p)

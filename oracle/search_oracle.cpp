@@ -450,6 +450,68 @@ static void finalize_result(std::vector<fa_raw_candidate>& res,
     res.swap(kept);
 }
 
+// One row of plane p: row holds the row's hist + N samples (look-back
+// first); orow its nw open excursions, updated in place; candidates closed
+// in this gulp are appended to out.
+void Engine::search_row(uint32_t p, uint32_t r, const std::vector<float>& row, SizeType N,
+                        bool first, std::vector<std::vector<float>>& ladder, Open* orow,
+                        std::vector<fa_raw_candidate>& out) const {
+    const PlanePlan& pp = plan.planes[p];
+    const auto nw       = widths.size();
+    const SizeType E    = hist + N;
+    const float thr     = bcfg.detect_thresh;
+    const int kmax      = static_cast<int>(ladder.size()) - 1;
+    ladder[0] = row;
+    for (int k = 1; k <= kmax; ++k) {
+        const SizeType s = SizeType{1} << (k - 1);
+        for (SizeType t = s; t < E; ++t)
+            ladder[k][t] = ladder[k - 1][t] + ladder[k - 1][t - s];
+    }
+    for (uint32_t w = 0; w < nw; ++w) {
+        const uint32_t W    = widths[w];
+        Open& o             = orow[w];
+        const uint64_t warm = (uint64_t)pp.delay[r] + W - 1;
+        for (SizeType t = 0; t < N; ++t) {
+            const uint64_t abs_t = processed + t;
+            const SizeType i     = hist + t;
+            float sum            = window_sum(ladder, i, W);
+            if (bcfg.use_trapezoid && W >= 3) {
+                const uint32_t e = (W + 1) / 4;
+                sum = (sum - 0.5F * window_sum(ladder, i, e)) -
+                      0.5F * window_sum(ladder, i - (W - e), e);
+            }
+            bool searched = true;
+            float inv     = inv_cur[p][r * nw + w];
+            if (t < warm) {
+                if (first)
+                    searched = false;
+                else
+                    inv = inv_or[p][r * nw + w];
+            }
+            const float snr  = sum * inv;
+            const bool above = searched && snr > thr;
+            if (bcfg.peak_mode == FA_PEAK_ALL) {
+                if (above) out.push_back(make_candidate(p, r, w, Open{1, abs_t, abs_t, abs_t, snr}));
+                continue;
+            }
+            if (above) {
+                if (!o.open) {
+                    o = Open{1, abs_t, abs_t, abs_t, snr};
+                } else {
+                    if (snr > o.snr) {
+                        o.snr  = snr;
+                        o.peak = abs_t;
+                    }
+                    o.end = abs_t;
+                }
+            } else if (o.open) {
+                out.push_back(make_candidate(p, r, w, o));
+                o = Open{};
+            }
+        }
+    }
+}
+
 void Engine::search_plane(uint32_t p,
                           const std::vector<std::vector<float>>& node_rows,
                           SizeType N, bool first) {
@@ -457,7 +519,6 @@ void Engine::search_plane(uint32_t p,
     const Node& nd      = plan.nodes[pp.node];
     const auto nw       = widths.size();
     const SizeType E    = hist + N;
-    const float thr     = bcfg.detect_thresh;
     int kmax            = 0;
     while ((1U << (kmax + 1)) <= widths.back()) ++kmax;
     if (keep_planes) plane_rows[p].assign(pp.delay.size() * N, 0.0F);
@@ -474,55 +535,7 @@ void Engine::search_plane(uint32_t p,
         const auto& row = node_rows[ir];
         if (keep_planes)
             std::memcpy(&plane_rows[p][r * N], &row[hist], N * sizeof(float));
-        ladder[0] = row;
-        for (int k = 1; k <= kmax; ++k) {
-            const SizeType s = SizeType{1} << (k - 1);
-            for (SizeType t = s; t < E; ++t)
-                ladder[k][t] = ladder[k - 1][t] + ladder[k - 1][t - s];
-        }
-        for (uint32_t w = 0; w < nw; ++w) {
-            const uint32_t W    = widths[w];
-            Open& o             = open[p][r * nw + w];
-            const uint64_t warm = (uint64_t)pp.delay[r] + W - 1;
-            for (SizeType t = 0; t < N; ++t) {
-                const uint64_t abs_t = processed + t;
-                const SizeType i     = hist + t;
-                float sum            = window_sum(ladder, i, W);
-                if (bcfg.use_trapezoid && W >= 3) {
-                    const uint32_t e = (W + 1) / 4;
-                    sum = (sum - 0.5F * window_sum(ladder, i, e)) -
-                          0.5F * window_sum(ladder, i - (W - e), e);
-                }
-                bool searched = true;
-                float inv     = inv_cur[p][r * nw + w];
-                if (t < warm) {
-                    if (first)
-                        searched = false;
-                    else
-                        inv = inv_or[p][r * nw + w];
-                }
-                const float snr  = sum * inv;
-                const bool above = searched && snr > thr;
-                if (bcfg.peak_mode == FA_PEAK_ALL) {
-                    if (above) mine.push_back(make_candidate(p, r, w, Open{1, abs_t, abs_t, abs_t, snr}));
-                    continue;
-                }
-                if (above) {
-                    if (!o.open) {
-                        o = Open{1, abs_t, abs_t, abs_t, snr};
-                    } else {
-                        if (snr > o.snr) {
-                            o.snr  = snr;
-                            o.peak = abs_t;
-                        }
-                        o.end = abs_t;
-                    }
-                } else if (o.open) {
-                    mine.push_back(make_candidate(p, r, w, o));
-                    o = Open{};
-                }
-            }
-        }
+        search_row(p, r, row, N, first, ladder, &open[p][r * nw], mine);
     }
 #pragma omp critical
     found.insert(found.end(), mine.begin(), mine.end());

@@ -243,12 +243,16 @@ def test_full_size_rows_exact_on_integer_input(oracle, stream, monkeypatch):
     """The benchmark shape (4096 ch, gulp 2^20, DM 0-2000, 11 planes; 2^18
     time tiles here): integer-valued input makes every FDMT sum exact, so
     sampled rows of every plane, at times in the first tile and across a
-    tile boundary, equal brute-force sums along their delay paths."""
+    tile boundary, equal brute-force sums along their delay paths. The
+    search ran on the same rows: for the rows of each plane with the most
+    candidates and two more, the candidates of this gulp and its drain
+    equal the oracle's search of that row alone. Uniform integers in
+    [-6, 6] have variance 14, so the threshold 22 is about 5.9 sigma."""
     from frost_amd import workloads
     say("test_full_size_rows_exact_on_integer_input")
     W = workloads.PROD_4096
     dcfg = W.dedisp_config()
-    bcfg = abi.boxcar_config(max_boxcar=W.boxcar_max, detect_thresh=1.0e6)
+    bcfg = abi.boxcar_config(max_boxcar=W.boxcar_max, detect_thresh=22.0)
     plan = search.make_fdmt_plan(dcfg)
     monkeypatch.setenv("FROST_AMD_TILE", str(1 << 18))
     eng = search.FDMTBoxcar(plan, bcfg)
@@ -258,9 +262,20 @@ def test_full_size_rows_exact_on_integer_input(oracle, stream, monkeypatch):
     wf = np.random.default_rng(11).integers(-6, 7, (nch, n), dtype=np.int8).astype(np.float32)
     dev = search.DeviceArray.from_host(wf, stream)
     eng.execute_device(dev, n, n, stream)
-    eng.copy_result(stream)
+    cands, dg, dd = eng.copy_result(stream)
+    tail, dg2, dd2 = eng.drain(stream)
     del dev
-    hist = oracle.Engine(dcfg, bcfg).hist
+    assert dg == dd == dg2 == dd2 == 0
+    cands = list(cands) + list(tail)
+    say("  candidates", len(cands))
+    assert 0 < len(cands) < bcfg.max_candidates
+    fi = abi.CANDIDATE_FIELDS.index
+    by_row = {}
+    for c in cands:
+        key = (c[fi("subband_level")], c[fi("subband_idx")], c[fi("staggered")], c[fi("dm_idx")])
+        by_row.setdefault(key, []).append(c)
+    ora = oracle.Engine(dcfg, bcfg)
+    hist = ora.hist
     o = oracle.Plan(dcfg)
     span = 1536
     starts = [hist + 4321, (1 << 18) - 700, 3 * (1 << 18) - 300]  # tiles 0, 0-1, 2-3
@@ -276,5 +291,14 @@ def test_full_size_rows_exact_on_integer_input(oracle, stream, monkeypatch):
                 want = o.brute_force_row(p, r, xs, hist, span)
                 got = rows[r, t0:t0 + span]
                 assert (got == want).all(), (p, r, t0, np.argwhere(got != want)[:3])
+        info = plan.plane(p)
+        ident = (info.subband_level, info.subband_idx, info.staggered)
+        counts = sorted(((len(v), k[3]) for k, v in by_row.items() if k[:3] == ident),
+                        reverse=True)
+        srch = {r for _, r in counts[:3]} | set(rng.integers(0, ndm, 2).tolist())
+        for r in sorted(srch):
+            want = sorted(ora.search_row(p, r, rows[r]))
+            got = sorted(by_row.get((*ident, r), []))
+            assert got == want, (p, r, len(got), len(want))
         del rows
     say("  checked", plan.summary()["nplanes"], "planes")

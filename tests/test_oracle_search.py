@@ -193,3 +193,40 @@ def test_caps_sorted_and_counted(oracle):
     assert len(cands) == 50 and dg > 0 and dd > 0
     keys = [(c[12], c[13], c[5], c[6], c[0]) for c in cands]
     assert keys == sorted(keys)
+
+
+@pytest.mark.parametrize("depth", [0, 2])
+def test_search_row_equals_engine_rows(oracle, depth):
+    """A plane row searched alone (the oracle entry point the full-size GPU
+    test compares against) gives the engine's own first-gulp candidates of
+    that row, drain included, for every row of every plane; raising one
+    sample of a row that has candidates changes them."""
+    cfg = small_cfg(nchans=32, depth=depth, dm_max=40.0, N=2048)
+    bc = abi.boxcar_config(max_boxcar=16, detect_thresh=3.0)
+    eng = oracle.Engine(cfg, bc, keep_planes=True)
+    fresh = oracle.Engine(cfg, bc)
+    rng = np.random.default_rng(40 + depth)
+    wf = rng.standard_normal((32, 2048)).astype(np.float32)
+    wf[:, 900:904] += 2.5
+    cands, dg, dd = eng.execute(wf)
+    rows_by_plane = [eng.plane_rows(p, 2048) for p in range(eng.plan.summary()["nplanes"])]
+    tail, _, _ = eng.drain()
+    assert dg == dd == 0
+    fi = abi.CANDIDATE_FIELDS.index
+    by_row = {}
+    for c in cands + tail:
+        key = (c[fi("subband_level")], c[fi("subband_idx")], c[fi("staggered")], c[fi("dm_idx")])
+        by_row.setdefault(key, []).append(c)
+    total, planted = 0, False
+    for p, rows in enumerate(rows_by_plane):
+        info = eng.plan.plane(p)
+        for r in range(info.ndm):
+            mine = sorted(by_row.get((info.subband_level, info.subband_idx, info.staggered, r), []))
+            assert sorted(fresh.search_row(p, r, rows[r])) == mine, (p, r)
+            total += len(mine)
+            if mine and not planted:
+                bad = rows[r].copy()
+                bad[mine[0][fi("peak_sample")]] += 10.0
+                assert sorted(fresh.search_row(p, r, bad)) != mine
+                planted = True
+    assert total > 0 and planted
