@@ -398,6 +398,7 @@ EXPORTS = {
     "fa_engine_timings": (C.c_int, [P, PF64, PF64, PF64, PF64]),
     "fa_engine_level_times": (C.c_int, [P, PF64, PU32, U64, PU64, PU64]),
     "fa_engine_level_sources": (C.c_int, [P, PI32, U64, PU64]),
+    "fa_engine_level_samples": (C.c_int, [P, PU64, U64, PU64]),
     "fa_engine_host_stats": (C.c_int, [P, PF64, PU64]),
     "fa_rfi_create": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32, C.c_int,
                                 C.POINTER(P)]),

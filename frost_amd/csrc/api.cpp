@@ -395,6 +395,17 @@ int fa_engine_level_sources(const fa_engine* engine, int32_t* source, uint64_t c
     });
 }
 
+int fa_engine_level_samples(const fa_engine* engine, uint64_t* samples, uint64_t cap,
+                            uint64_t* nlevels) {
+    return guard([&] {
+        require(engine != nullptr, "engine must not be null");
+        const auto v = engine->eng->level_samples();
+        if (nlevels) *nlevels = v.size();
+        for (std::size_t l = 0; l < std::min<uint64_t>(cap, v.size()); ++l)
+            if (samples) samples[l] = v[l];
+    });
+}
+
 int fa_engine_host_stats(const fa_engine* engine, double* collect_seconds,
                          uint64_t* raw_candidates) {
     return guard([&] {

@@ -21,8 +21,9 @@ constexpr int kBlock       = 256;
 constexpr int kInitPer     = 8;    // outputs per thread per init block
 constexpr uint64_t kTileMax = kAutoTileMax; // automatic time tile bounds
 constexpr uint64_t kTileMin = kAutoTileMin;
-constexpr int kChunk       = 1024; // search samples per LDS pass
-constexpr uint32_t kMaxBox = 1024; // ladder fits LDS up to this width
+constexpr uint32_t kMaxBox = 4096; // widest filter (frost's boxcar_max bound)
+constexpr uint32_t kMaxWidths = 32; // width trials per plane row
+constexpr uint64_t kCandCapMax = uint64_t{1} << 28; // candidate buffer growth bound
 
 // ---- kernels ---------------------------------------------------------------
 
@@ -566,29 +567,6 @@ void launch_merge(const float* in, float* out, const int64_t* prog, unsigned nro
     throw std::invalid_argument("unsupported FDMT merge shape (elements per thread, rows)");
 }
 
-__device__ __forceinline__ float ladder_sum(const float* L, int S, int j, uint32_t W) {
-    int k         = 31 - __clz(W);
-    float acc     = L[k * S + j];
-    int off       = 1 << k;
-    for (--k; k >= 0; --k) {
-        if ((W >> k) & 1U) {
-            acc = acc + L[k * S + j - off];
-            off += 1 << k;
-        }
-    }
-    return acc;
-}
-
-__device__ __forceinline__ float window_value(const float* L, int S, int j, uint32_t W,
-                                              int trap) {
-    float s = ladder_sum(L, S, j, W);
-    if (trap && W >= 3) {
-        const uint32_t e = (W + 1) / 4;
-        s = (s - 0.5f * ladder_sum(L, S, j, e)) - 0.5f * ladder_sum(L, S, j - (int)(W - e), e);
-    }
-    return s;
-}
-
 struct SearchArgs {
     const float* buf;
     const SearchRow* rows;
@@ -602,7 +580,6 @@ struct SearchArgs {
     int tile_len;
     int64_t tile_t0;   // gulp-local start of the tile
     uint64_t gulp_abs; // absolute sample of gulp-local 0
-    int first;
     float thr;
     int mode;
     int trap;
@@ -630,166 +607,221 @@ __device__ void emit(const SearchArgs& a, uint32_t prow, uint32_t w, uint64_t st
     }
 }
 
-// S/N of window ending at ladder index j for width slot w. POW2: every width
-// is a power of two (window = one ladder level, trapezoid edges another).
-template <bool POW2>
-__device__ __forceinline__ float width_sum(const float* L, int S, int j, uint32_t W, int kw,
-                                           int trap) {
-    if (POW2) {
-        float s = L[kw * S + j];
+// Run tracking of one sample (per_width and across_widths peaks): an
+// above-threshold sample opens or extends the run (the first maximum is
+// its peak, w the width index reported with it); a sample below closes it.
+__device__ __forceinline__ void track_run(const SearchArgs& a, OpenRun& o, uint32_t prow,
+                                          uint32_t w, uint64_t abs_t, bool above, float snr) {
+    if (above) {
+        if (!o.open) {
+            o.open  = 1;
+            o.w     = static_cast<uint16_t>(w);
+            o.start = o.peak = o.end = abs_t;
+            o.snr   = snr;
+        } else {
+            if (snr > o.snr) {
+                o.snr  = snr;
+                o.peak = abs_t;
+                o.w    = static_cast<uint16_t>(w);
+            }
+            o.end = abs_t;
+        }
+    } else if (o.open) {
+        emit(a, prow, o.w, o.start, o.peak, o.end, o.snr);
+        o.open = 0;
+    }
+}
+
+// ---- ring-ladder search (any widths up to 4096) ------------------------------
+//
+// One block of 256 threads per plane row, one sample per thread per chunk of
+// 256. Ladder level b (D_0 = row, D_b(t) = D_{b-1}(t) + D_{b-1}(t - 2^{b-1}))
+// lives in an LDS ring of R_b = 256 + lookback_b floats, where lookback_b is
+// the furthest level-b value any window or the next level reads behind the
+// sample (the host computes it from the widths). Sample t of the tile sits at
+// ring position (t - u0) mod R_b; a chunk overwrites only values older than
+// every read. Warm-up chunks from u0 <= -(W_max - 1) fill the rings with the
+// row's look-back; values covering samples before -(W_max - 1) are never read
+// by a searched window.
+constexpr int kRingChunk  = 256;
+constexpr int kRingLevels = 13; // widths up to 4096
+
+struct RingSearchArgs {
+    SearchArgs a;
+    int off[kRingLevels];  // LDS float offset of level b's ring
+    int size[kRingLevels]; // its length R_b
+    int u0;                // first chunk start (tile-relative, multiple of 256)
+    int lds_floats;        // sum of R_b
+};
+
+__device__ __forceinline__ int ring_pos(int rel, int R) {
+    // rel in (-R, 2R)
+    rel = rel >= R ? rel - R : rel;
+    return rel < 0 ? rel + R : rel;
+}
+
+struct Ring {
+    const float* lds;
+    const int* off;
+    const int* size;
+    const int* base;
+    // D_b at (sample i of the chunk) - d
+    __device__ __forceinline__ float at(int b, int i, int d) const {
+        return lds[off[b] + ring_pos(base[b] + i - d, size[b])];
+    }
+    // Window sum of width W ending d samples before sample i: its set bits,
+    // highest first.
+    __device__ __forceinline__ float wsum(uint32_t W, int i, int d) const {
+        int k     = 31 - __clz(W);
+        float acc = at(k, i, d);
+        int off   = d + (1 << k);
+        for (--k; k >= 0; --k) {
+            if ((W >> k) & 1U) {
+                acc = acc + at(k, i, off);
+                off += 1 << k;
+            }
+        }
+        return acc;
+    }
+    __device__ __forceinline__ float value(uint32_t W, int i, int trap) const {
+        float s = wsum(W, i, 0);
         if (trap && W >= 3) {
-            // e = (W + 1) / 4 = W / 4 = 2^(kw - 2)
-            const float* Le = L + (kw - 2) * S;
-            s = (s - 0.5f * Le[j]) - 0.5f * Le[j - (int)(W - (W >> 2))];
+            const uint32_t e = (W + 1) / 4;
+            s = (s - 0.5f * wsum(e, i, 0)) - 0.5f * wsum(e, i, (int)(W - e));
         }
         return s;
     }
-    return window_value(L, S, j, W, trap);
-}
+};
 
-// One block per plane row: ladder in LDS per chunk of kChunk samples, S/N per
-// width, per-width run tracking (thread w walks width w when needed).
-template <bool POW2>
-__global__ void __launch_bounds__(kBlock) search_kernel(SearchArgs a) {
+__global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs ra) {
     extern __shared__ float lds[];
+    const SearchArgs& a = ra.a;
     __shared__ OpenRun st[32];
     __shared__ float s_inv[32], s_invor[32];
     __shared__ uint32_t s_w[32];
-    __shared__ int s_kw[32];
     __shared__ int64_t s_warm[32];
+    __shared__ float s_best[kRingChunk];
+    __shared__ uint8_t s_arg[kRingChunk];
+    __shared__ int s_off[kRingLevels], s_size[kRingLevels], s_base[kRingLevels];
     __shared__ uint32_t flags;
-    const int S = kChunk + a.wm1;
-    float* L    = lds;
+    const int i         = threadIdx.x;
     const SearchRow row = a.rows[blockIdx.x];
     const uint32_t prow = row.prow;
     const int nw        = a.nw;
-    if (threadIdx.x < nw) {
-        const int w  = threadIdx.x;
-        st[w]        = a.open[(uint64_t)prow * nw + w];
-        s_inv[w]     = a.inv_cur[(uint64_t)prow * nw + w];
-        s_invor[w]   = a.inv_or[(uint64_t)prow * nw + w];
-        s_w[w]       = a.widths[w];
-        s_kw[w]      = 31 - __clz(a.widths[w]);
-        s_warm[w]    = (int64_t)row.delay + a.widths[w] - 1;
+    const bool across   = a.mode == FA_PEAK_ACROSS_WIDTHS;
+    if (i < nw) {
+        st[i]     = a.open[(uint64_t)prow * nw + i];
+        s_inv[i]  = a.inv_cur[(uint64_t)prow * nw + i];
+        s_invor[i] = a.inv_or[(uint64_t)prow * nw + i];
+        s_w[i]    = a.widths[i];
+        s_warm[i] = (int64_t)row.delay + a.widths[i] - 1;
     }
-    if (threadIdx.x == 0) flags = 0;
+    if (i < kRingLevels) {
+        s_off[i]  = ra.off[i];
+        s_size[i] = ra.size[i];
+        s_base[i] = 0;
+    }
+    if (i == 0) flags = 0;
+    for (int j = i; j < ra.lds_floats; j += kRingChunk) lds[j] = 0.0f;
     __syncthreads();
-    const int64_t warm_max = s_warm[nw - 1] > s_warm[0] ? s_warm[nw - 1] : s_warm[0];
-    const float* src = a.buf + row.in;
-    for (int u = 0; u < a.tile_len; u += kChunk) {
-        const int clen = min(kChunk, a.tile_len - u);
-        const int span = clen + a.wm1;
-        __syncthreads();
-        for (int j = threadIdx.x; j < span; j += kBlock) L[j] = src[u + j];
-        for (int k = 1; k <= a.kmax; ++k) {
+    const Ring R{lds, s_off, s_size, s_base};
+    const float* src = a.buf + row.in + a.wm1; // tile sample 0
+    const int tl     = a.tile_len;
+    for (int u = ra.u0; u < tl; u += kRingChunk) {
+        const int t   = u + i;
+        const float v = (t >= -a.wm1 && t < tl) ? src[t] : 0.0f;
+        lds[s_off[0] + ring_pos(s_base[0] + i, s_size[0])] = v;
+        if (a.keep && t >= 0 && t < tl)
+            a.keep[(uint64_t)prow * a.keep_stride + a.tile_t0 + t] = v;
+        for (int b = 1; b <= a.kmax; ++b) {
             __syncthreads();
-            const int s = 1 << (k - 1);
-            float* Lk       = L + k * S;
-            const float* Lp = L + (k - 1) * S;
-            for (int j = threadIdx.x + s; j < span; j += kBlock) Lk[j] = Lp[j] + Lp[j - s];
+            const float x = R.at(b - 1, i, 0) + R.at(b - 1, i, 1 << (b - 1));
+            lds[s_off[b] + ring_pos(s_base[b] + i, s_size[b])] = x;
         }
         __syncthreads();
-        if (a.keep) {
-            for (int i = threadIdx.x; i < clen; i += kBlock)
-                a.keep[(uint64_t)prow * a.keep_stride + a.tile_t0 + u + i] = L[a.wm1 + i];
-        }
-        const int64_t tc0 = a.tile_t0 + u;
-        uint32_t bits     = 0;
-        if (tc0 >= warm_max) {
-            // Whole chunk past the cold / OR-mask window of every width.
-            for (int w = 0; w < nw; ++w) {
+        if (u >= 0) {
+            const int clen  = min(kRingChunk, tl - u);
+            const int64_t tg = a.tile_t0 + t; // gulp-local sample
+            const int64_t ab = (int64_t)a.gulp_abs + tg;
+            // S/N of width w at this thread's sample; false when the
+            // window is cold (before delay + W - 1 in absolute samples);
+            // windows reaching into the previous gulp use the OR-mask noise.
+            auto snr_of = [&](int w, float& sn) {
+                if (i >= clen || ab < s_warm[w]) return false;
+                const float inv = tg < s_warm[w] ? s_invor[w] : s_inv[w];
+                sn              = R.value(s_w[w], i, a.trap) * inv;
+                return true;
+            };
+            if (across) {
+                bool any   = false;
+                float best = 0.0f;
+                int bw     = 0;
+                for (int w = 0; w < nw; ++w) {
+                    float sn;
+                    if (!snr_of(w, sn)) continue;
+                    if (!any || sn > best) {
+                        best = sn;
+                        bw   = w;
+                    }
+                    any = true;
+                }
+                s_best[i] = any ? best : -INFINITY;
+                s_arg[i]  = static_cast<uint8_t>(bw);
+                if (any && best > a.thr) atomicOr(&flags, 1U);
+            } else {
+                uint32_t bits = 0;
+                for (int w = 0; w < nw; ++w) {
+                    float sn;
+                    if (snr_of(w, sn) && sn > a.thr) bits |= 1U << w;
+                }
+                if (bits) atomicOr(&flags, bits);
+            }
+            __syncthreads();
+            if (across) {
+                if (i == 0 && (flags || st[0].open)) {
+                    OpenRun o = st[0];
+                    for (int q = 0; q < clen; ++q) {
+                        const uint64_t abs_t = a.gulp_abs + (uint64_t)(a.tile_t0 + u + q);
+                        const float sn       = s_best[q];
+                        track_run(a, o, prow, s_arg[q], abs_t, sn > a.thr, sn);
+                    }
+                    st[0] = o;
+                }
+            } else if (i < nw && (((flags >> i) & 1U) || st[i].open)) {
+                // Thread w walks width w over the chunk in time order.
+                const int w      = i;
                 const uint32_t W = s_w[w];
-                const int kw     = s_kw[w];
-                const float inv  = s_inv[w];
-                bool any         = false;
-#pragma unroll
-                for (int q = 0; q < kChunk / kBlock; ++q) {
-                    const int i = threadIdx.x + q * kBlock;
-                    if (i < clen) {
-                        const float snr = width_sum<POW2>(L, S, a.wm1 + i, W, kw, a.trap) * inv;
-                        any |= snr > a.thr;
+                OpenRun o        = st[w];
+                for (int q = 0; q < clen; ++q) {
+                    const int64_t tq  = a.tile_t0 + u + q;
+                    const int64_t abq = (int64_t)a.gulp_abs + tq;
+                    const uint64_t abs_t = (uint64_t)abq;
+                    bool above       = false;
+                    float sn         = 0.0f;
+                    if (abq >= s_warm[w]) {
+                        const float inv = tq < s_warm[w] ? s_invor[w] : s_inv[w];
+                        sn              = R.value(W, q, a.trap) * inv;
+                        above           = sn > a.thr;
                     }
-                }
-                if (any) bits |= 1U << w;
-            }
-        } else {
-            for (int w = 0; w < nw; ++w) {
-                const uint32_t W    = s_w[w];
-                const int kw        = s_kw[w];
-                const int64_t warm  = s_warm[w];
-                bool any            = false;
-#pragma unroll
-                for (int q = 0; q < kChunk / kBlock; ++q) {
-                    const int i = threadIdx.x + q * kBlock;
-                    if (i < clen) {
-                        const int64_t t = tc0 + i;
-                        float inv       = s_inv[w];
-                        bool searched   = true;
-                        if (t < warm) {
-                            if (a.first) searched = false;
-                            else inv = s_invor[w];
-                        }
-                        if (searched) {
-                            const float snr =
-                                width_sum<POW2>(L, S, a.wm1 + i, W, kw, a.trap) * inv;
-                            any |= snr > a.thr;
-                        }
+                    if (a.mode == FA_PEAK_ALL) {
+                        if (above) emit(a, prow, w, abs_t, abs_t, abs_t, sn);
+                        continue;
                     }
+                    track_run(a, o, prow, w, abs_t, above, sn);
                 }
-                if (any) bits |= 1U << w;
+                st[w] = o;
             }
+            __syncthreads();
+            if (i == 0) flags = 0;
         }
-        if (bits) atomicOr(&flags, bits);
-        __syncthreads();
-        const int w = threadIdx.x;
-        if (w < nw && (((flags >> w) & 1U) || st[w].open)) {
-            const uint32_t W   = s_w[w];
-            const int kw       = s_kw[w];
-            const int64_t warm = s_warm[w];
-            OpenRun o          = st[w];
-            for (int i = 0; i < clen; ++i) {
-                const int64_t t      = tc0 + i;
-                const uint64_t abs_t = a.gulp_abs + (uint64_t)t;
-                bool searched        = true;
-                float inv            = s_inv[w];
-                if (t < warm) {
-                    if (a.first) searched = false;
-                    else inv = s_invor[w];
-                }
-                float snr  = 0.0f;
-                bool above = false;
-                if (searched) {
-                    snr   = width_sum<POW2>(L, S, a.wm1 + i, W, kw, a.trap) * inv;
-                    above = snr > a.thr;
-                }
-                if (a.mode == FA_PEAK_ALL) {
-                    if (above) emit(a, prow, w, abs_t, abs_t, abs_t, snr);
-                    continue;
-                }
-                if (above) {
-                    if (!o.open) {
-                        o.open  = 1;
-                        o.start = o.peak = o.end = abs_t;
-                        o.snr   = snr;
-                    } else {
-                        if (snr > o.snr) {
-                            o.snr  = snr;
-                            o.peak = abs_t;
-                        }
-                        o.end = abs_t;
-                    }
-                } else if (o.open) {
-                    emit(a, prow, w, o.start, o.peak, o.end, o.snr);
-                    o.open = 0;
-                }
-            }
-            st[w] = o;
+        // Next chunk: every ring advances by one chunk.
+        if (i < kRingLevels) {
+            const int nb = s_base[i] + kRingChunk;
+            s_base[i]    = nb >= s_size[i] ? nb - s_size[i] : nb;
         }
         __syncthreads();
-        if (threadIdx.x == 0) flags = 0;
     }
-    __syncthreads();
-    if (threadIdx.x < nw) a.open[(uint64_t)prow * nw + threadIdx.x] = st[threadIdx.x];
+    if (i < nw) a.open[(uint64_t)prow * nw + i] = st[i];
 }
 
 // ---- register-ladder search (power-of-two widths up to 256) -----------------
@@ -861,8 +893,10 @@ __device__ __forceinline__ float trap_back(const float (&D)[9][4], const float (
 
 // Serial run tracking of one width over one chunk (lane 0), the S/N values
 // staged in LDS. Returns the run's open flag afterwards.
-__device__ uint32_t walk_chunk(const SearchArgs& a, float* snr_lds, OpenRun* st, uint32_t prow,
-                               int w, int64_t tc0, int clen, int lane) {
+__device__ uint32_t walk_chunk(const SearchArgs& a, const float* snr_lds, const uint8_t* arg_lds,
+                               OpenRun* st, uint32_t prow, int w, int64_t tc0, int clen,
+                               int lane) {
+    // arg_lds: the width index per sample (across_widths), else null.
     if (lane == 0) {
         OpenRun o = *st;
         for (int i = 0; i < clen; ++i) {
@@ -873,22 +907,7 @@ __device__ uint32_t walk_chunk(const SearchArgs& a, float* snr_lds, OpenRun* st,
                 if (above) emit(a, prow, w, abs_t, abs_t, abs_t, sn);
                 continue;
             }
-            if (above) {
-                if (!o.open) {
-                    o.open  = 1;
-                    o.start = o.peak = o.end = abs_t;
-                    o.snr   = sn;
-                } else {
-                    if (sn > o.snr) {
-                        o.snr  = sn;
-                        o.peak = abs_t;
-                    }
-                    o.end = abs_t;
-                }
-            } else if (o.open) {
-                emit(a, prow, w, o.start, o.peak, o.end, o.snr);
-                o.open = 0;
-            }
+            track_run(a, o, prow, arg_lds ? arg_lds[i] : w, abs_t, above, sn);
         }
         *st = o;
     }
@@ -900,6 +919,7 @@ template <int KMAX, bool TRAP>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRAP ? 1 : 6))) search_reg_kernel(RegSearchArgs ra) {
     const SearchArgs& a = ra.a;
     __shared__ float s_snr[kBlock / 64][kRegChunk];
+    __shared__ uint8_t s_arg[kBlock / 64][kRegChunk];
     __shared__ OpenRun s_st[kBlock / 64][16];
     const int lane = threadIdx.x % 64;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
@@ -1014,10 +1034,71 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
             }
             quiet = __ballot(anyp) == 0;
         }
+        if (a.mode == FA_PEAK_ACROSS_WIDTHS) {
+            // One run per row: per sample the largest S/N over the widths
+            // searched at it (the lowest width index on ties).
+            if (u >= 0 && (!quiet || open_mask != 0)) {
+                float best[4];
+                int arg[4];
+                bool any[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    best[j] = 0.0f;
+                    arg[j]  = 0;
+                    any[j]  = false;
+                }
+#pragma unroll
+                for (int k = 0; k <= KMAX; ++k) {
+                    const int w = ra.widx[k];
+                    if (w < 0) continue;
+                    const int64_t warm = (int64_t)row.delay + (int64_t)(1 << k) - 1;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        float sum = D[k][j];
+                        if (TRAP && k >= 2) {
+                            float e2;
+                            switch (k) {
+                            case 2: e2 = trap_back<2>(D, P, j, lane); break;
+                            case 3: e2 = trap_back<3>(D, P, j, lane); break;
+                            case 4: e2 = trap_back<4>(D, P, j, lane); break;
+                            case 5: e2 = trap_back<5>(D, P, j, lane); break;
+                            case 6: e2 = trap_back<6>(D, P, j, lane); break;
+                            case 7: e2 = trap_back<7>(D, P, j, lane); break;
+                            default: e2 = trap_back<8>(D, P, j, lane); break;
+                            }
+                            sum = (sum - 0.5f * D[k - 2][j]) - 0.5f * e2;
+                        }
+                        const int64_t t = tc0 + 4 * lane + j;
+                        if (base + j >= tl || (int64_t)a.gulp_abs + t < warm) continue;
+                        const float sn = sum * (t < warm ? inv_o[k] : inv_c[k]);
+                        if (!any[j] || sn > best[j]) {
+                            best[j] = sn;
+                            arg[j]  = w;
+                        }
+                        any[j] = true;
+                    }
+                }
+                bool pred = false;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float sn             = any[j] ? best[j] : -INFINITY;
+                    pred                       = pred || sn > a.thr;
+                    s_snr[wave][4 * lane + j] = sn;
+                    s_arg[wave][4 * lane + j] = static_cast<uint8_t>(arg[j]);
+                }
+                if (__ballot(pred) != 0 || open_mask != 0) {
+                    wave_sync();
+                    const int clen = min(kRegChunk, tl - u);
+                    const uint32_t op =
+                        walk_chunk(a, s_snr[wave], s_arg[wave], &s_st[wave][0], prow, 0, tc0, clen, lane);
+                    open_mask = op ? 1U : 0U;
+                }
+            }
+        }
 #pragma unroll
         for (int k = 0; k <= KMAX; ++k) {
             const int w = ra.widx[k];
-            if (quiet || w < 0) continue;
+            if (a.mode == FA_PEAK_ACROSS_WIDTHS || quiet || w < 0) continue;
             const uint32_t W = 1U << k;
             float sv[4], sums[4];
 #pragma unroll
@@ -1055,7 +1136,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
                     float sn        = -INFINITY;
                     if (base + j < tl) {
                         if (t < (int64_t)row.delay + (int64_t)W - 1) {
-                            if (!a.first) sn = sums[j] * inv_o[k];
+                            // Cold until the absolute sample passes the warm-up.
+                            if ((int64_t)a.gulp_abs + t >= (int64_t)row.delay + (int64_t)W - 1)
+                                sn = sums[j] * inv_o[k];
                         } else {
                             sn = sums[j] * inv_c[k];
                         }
@@ -1070,7 +1153,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
                 wave_sync();
                 const int clen = min(kRegChunk, tl - u);
                 const uint32_t op =
-                    walk_chunk(a, s_snr[wave], &s_st[wave][w], prow, w, tc0, clen, lane);
+                    walk_chunk(a, s_snr[wave], nullptr, &s_st[wave][w], prow, w, tc0, clen, lane);
                 open_mask = op ? (open_mask | (1U << w)) : (open_mask & ~(1U << w));
             }
         }
@@ -1119,7 +1202,7 @@ __global__ void frontier_kernel(const OpenRun* open, uint64_t n, unsigned long l
 
 __global__ void zero_open_kernel(OpenRun* open, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) open[i] = OpenRun{0, 0, 0, 0.0f, 0};
+    if (i < n) open[i] = OpenRun{0, 0, 0, 0.0f, 0, 0};
 }
 
 __global__ void copy_cols_kernel(const float* __restrict__ src, uint64_t sstride,
@@ -1181,25 +1264,58 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
                int device, uint64_t tile)
     : plan_(plan), cfg_(cfg), device_(device) {
     require(cfg.max_boxcar >= 1, "max_boxcar must be >= 1");
-    require(cfg.max_boxcar <= kMaxBox,
-            "max_boxcar above 1024 is not supported by the device search");
+    require(cfg.max_boxcar <= kMaxBox, "max_boxcar must be in [1, 4096]");
     require(std::isfinite(cfg.detect_thresh) && cfg.detect_thresh > 0.0f,
             "detect_thresh must be finite and > 0");
-    require(cfg.peak_mode == FA_PEAK_PER_WIDTH || cfg.peak_mode == FA_PEAK_ALL,
-            "peak_mode across_widths is not supported");
+    require(cfg.peak_mode == FA_PEAK_PER_WIDTH || cfg.peak_mode == FA_PEAK_ACROSS_WIDTHS ||
+                cfg.peak_mode == FA_PEAK_ALL,
+            "unknown peak_mode");
     require(cfg.max_candidates >= 1, "max_candidates must be >= 1");
     require(plan.cfg.block_nsamps >= 1, "block_nsamps (gulp capacity) must be >= 1");
     widths_ = box_width_trials(cfg.max_boxcar, cfg.wtsp);
-    require(widths_.size() <= 32, "too many width trials (at most 32)");
+    require(widths_.size() <= kMaxWidths,
+            "too many width trials (at most 32): raise wtsp or lower max_boxcar");
     kmax_ = 0;
     while ((1U << (kmax_ + 1)) <= widths_.back()) ++kmax_;
     wm1_ = widths_.back() - 1;
     pow2_widths_ = true;
     for (const uint32_t w : widths_) pow2_widths_ = pow2_widths_ && (w & (w - 1)) == 0;
     // Register ladder: power-of-two widths up to 256 (shifts within one
-    // 256-sample chunk). FROST_AMD_SEARCH=lds forces the LDS ladder.
+    // 256-sample chunk); other widths use the LDS ring ladder.
+    // FROST_AMD_SEARCH=ring forces the ring ladder.
     const char* sv = std::getenv("FROST_AMD_SEARCH");
-    reg_search_ = pow2_widths_ && widths_.back() <= 256 && !(sv && std::string(sv) == "lds");
+    reg_search_ = pow2_widths_ && widths_.back() <= 256 &&
+                  !(sv && (std::string(sv) == "ring" || std::string(sv) == "lds"));
+    {
+        // Ring lengths: level b holds one chunk plus the furthest look-back
+        // read from it (2^b by level b + 1; each set bit's offset inside a
+        // window sum; the trapezoid's shifted edge window).
+        std::vector<int32_t> look(kmax_ + 1, 0);
+        for (uint32_t b = 0; b < kmax_; ++b) look[b] = std::max<int32_t>(look[b], 1 << b);
+        auto add = [&](uint32_t W, int32_t d0) {
+            for (uint32_t b = 0; b <= kmax_; ++b)
+                if ((W >> b) & 1U) {
+                    const int32_t off = static_cast<int32_t>(W - (W % (2U << b)));
+                    look[b]           = std::max(look[b], d0 + off);
+                }
+        };
+        for (const uint32_t W : widths_) {
+            add(W, 0);
+            if (cfg.use_trapezoid && W >= 3) {
+                const uint32_t e = (W + 1) / 4;
+                add(e, 0);
+                add(e, static_cast<int32_t>(W - e));
+            }
+        }
+        ring_off_.assign(kmax_ + 1, 0);
+        ring_size_.assign(kmax_ + 1, 0);
+        ring_floats_ = 0;
+        for (uint32_t b = 0; b <= kmax_; ++b) {
+            ring_off_[b]  = ring_floats_;
+            ring_size_[b] = 256 + look[b];
+            ring_floats_ += ring_size_[b];
+        }
+    }
     cap_ = plan.cfg.block_nsamps;
     const uint64_t tile_req = tile ? tile : env_tile();
     tile_ = std::min<uint64_t>(cap_, tile_req ? tile_req : kTileMax);
@@ -1273,6 +1389,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     widths_dev_.resize(nw);
     upload(widths_dev_.get(), widths_.data(), nw * 4);
     open_.resize((uint64_t)nprows_ * nw);
+    open_saved_.resize((uint64_t)nprows_ * nw);
     if (!sizing()) FA_HIP(hipMemset(open_.get(), 0, open_.bytes()));
     cand_cap_ = std::max<uint64_t>(4 * cfg.max_candidates, uint64_t{1} << 20);
     cands_.resize(cand_cap_);
@@ -1284,10 +1401,11 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     FA_HIP(hipStreamCreateWithFlags(&search_stream_, hipStreamNonBlocking));
     int lds_max = 0;
     FA_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
-    const uint64_t lds_need = (uint64_t)(kmax_ + 1) * (kChunk + wm1_) * sizeof(float) +
-                              32 * sizeof(OpenRun) + 64;
-    if (lds_need > (uint64_t)lds_max)
-        throw std::invalid_argument("max_boxcar too large for the search kernel's LDS");
+    const uint64_t lds_need = static_cast<uint64_t>(ring_floats_) * sizeof(float) + 4096;
+    if (!reg_search_ && lds_need > (uint64_t)lds_max)
+        throw std::invalid_argument(
+            "width trials need more LDS than the device offers for the ring-ladder search; "
+            "raise wtsp or lower max_boxcar");
     FA_HIP(hipEventCreate(&ev_begin_));
     FA_HIP(hipEventCreate(&ev_end_));
 }
@@ -1808,6 +1926,8 @@ void Engine::execute(const float* wf, uint64_t stride, uint64_t offset, uint64_t
     require(wf != nullptr, "waterfall must not be null");
     require(nsamps >= 1 && nsamps <= cap_, "nsamps must be in [1, block_nsamps]");
     require(stride >= offset + nsamps, "waterfall stride too small");
+    require(!pending_collect_,
+            "the previous gulp's candidates were not collected (call copy_result first)");
     DeviceGuard dg(device_);
     FA_HIP(hipMemcpy2DAsync(input_columns(), xstride_ * 4, wf + offset, stride * 4, nsamps * 4,
                             plan_.cfg.nchans, hipMemcpyDeviceToDevice, s));
@@ -1817,6 +1937,8 @@ void Engine::execute(const float* wf, uint64_t stride, uint64_t offset, uint64_t
 void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t s) {
     require(N >= 1 && N <= cap_, "nsamps must be in [1, block_nsamps]");
     require(s != nullptr, "an explicit non-null stream is required");
+    require(!pending_collect_,
+            "the previous gulp's candidates were not collected (call copy_result first)");
     DeviceGuard dg(device_);
     const uint64_t nch = plan_.cfg.nchans;
     const std::vector<uint8_t> m =
@@ -1841,6 +1963,10 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
     if (keep_planes_) {
         keep_.reserve(static_cast<uint64_t>(nprows_) * cap_);
     }
+    // Open runs before this gulp: a re-run after a candidate overflow starts
+    // from them.
+    FA_HIP(hipMemcpyAsync(open_saved_.get(), open_.get(), open_.bytes(), hipMemcpyDeviceToDevice,
+                          s));
     events_used_ = 0;
     sync_used_   = 0;
     // Searches still reading each level buffer (their last event), and the
@@ -1850,8 +1976,6 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
     FA_HIP(hipEventRecord(ev_begin_, s));
     set_u64_kernel<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(counters_.get()), 0,
                                     processed_ + N);
-    const int lds_search =
-        static_cast<int>((kmax_ + 1) * (kChunk + wm1_) * sizeof(float));
     const uint32_t L = plan_.nlevels - 1;
     for (uint64_t t0 = 0; t0 < N; t0 += tile_) {
         const int tl = static_cast<int>(std::min<uint64_t>(tile_, N - t0));
@@ -1874,8 +1998,8 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                     init_rows_.get(), out, tl, static_cast<int>(nch));
             } else {
                 const MergeLaunch* ml = nullptr;
-                for (const auto& m : launches_)
-                    if (m.out_level == l) ml = &m;
+                for (const auto& mlc : launches_)
+                    if (mlc.out_level == l) ml = &mlc;
                 if (!ml) throw std::logic_error("FDMT level without a launch");
                 const float* in = ml->from_input ? nullptr : buf_[ml->in_buf].get();
                 // Input samples of this tile: columns hist_in_ + t0 + tau,
@@ -1902,55 +2026,14 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
             }
             record_pair(static_cast<int>(l), s, false);
             if (search_count_[l]) {
-                SearchArgs a{};
-                a.buf      = out;
-                a.rows     = search_rows_[l].get();
-                a.nrows    = static_cast<int>(search_count_[l]);
-                a.inv_cur  = inv_cur_.get();
-                a.inv_or   = use_or ? inv_or_.get() : inv_cur_.get();
-                a.widths   = widths_dev_.get();
-                a.nw       = static_cast<int>(nw);
-                a.kmax     = static_cast<int>(kmax_);
-                a.wm1      = static_cast<int>(wm1_);
-                a.tile_len = tl;
-                a.tile_t0  = static_cast<int64_t>(t0);
-                a.gulp_abs = processed_;
-                a.first    = first ? 1 : 0;
-                a.thr      = cfg_.detect_thresh;
-                a.mode     = cfg_.peak_mode;
-                a.trap     = cfg_.use_trapezoid ? 1 : 0;
-                a.open     = open_.get();
-                a.cands    = cands_.get();
-                a.counter  = reinterpret_cast<unsigned long long*>(counters_.get());
-                a.cap      = cand_cap_;
-                a.keep     = keep_planes_ ? keep_.get() : nullptr;
-                a.keep_stride = cap_;
                 // The search of this level follows its launch on the search
                 // stream (in level and tile order there, which the runs'
                 // state across tiles needs).
                 FA_HIP(hipStreamWaitEvent(search_stream_, sync_event(s), 0));
-                hipStream_t ss = search_stream_;
-                record_pair(-1, ss, true);
-                if (reg_search_) {
-                    RegSearchArgs ra{};
-                    ra.a       = a;
-                    ra.present = 0;
-                    for (int k = 0; k < 16; ++k) ra.widx[k] = -1;
-                    for (std::size_t w = 0; w < nw; ++w) {
-                        const int k = 31 - __builtin_clz(widths_[w]);
-                        ra.present |= 1U << k;
-                        ra.widx[k] = static_cast<int>(w);
-                    }
-                    const unsigned blocks = ceil_div(search_count_[l], kBlock / 64);
-                    (cfg_.use_trapezoid ? reg_launcher<true>(static_cast<int>(kmax_))
-                                        : reg_launcher<false>(static_cast<int>(kmax_)))(ra, blocks, ss);
-                } else if (pow2_widths_) {
-                    search_kernel<true><<<search_count_[l], kBlock, lds_search, ss>>>(a);
-                } else {
-                    search_kernel<false><<<search_count_[l], kBlock, lds_search, ss>>>(a);
-                }
-                record_pair(-1, ss, false);
-                last_search = sync_event(ss);
+                record_pair(-1, search_stream_, true);
+                launch_search(l, out, tl, t0, use_or, search_stream_);
+                record_pair(-1, search_stream_, false);
+                last_search = sync_event(search_stream_);
                 reading[level_buf_[l]] = last_search;
             }
         }
@@ -1960,7 +2043,72 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
     const uint64_t nopen = static_cast<uint64_t>(nprows_) * nw;
     frontier_kernel<<<ceil_div(nopen, 256), 256, 0, s>>>(
         open_.get(), nopen, reinterpret_cast<unsigned long long*>(counters_.get()) + 1);
-    // Carry the last H input samples of each channel to the front.
+    FA_HIP(hipGetLastError());
+    FA_HIP(hipEventRecord(ev_end_, s));
+    before_mask_ = prev_mask_;
+    last_mask_   = m;
+    prev_mask_   = m;
+    processed_ += N;
+    last_n_          = N;
+    pending_collect_ = true;
+}
+
+void Engine::launch_search(uint32_t l, const float* buf, int tl, uint64_t t0, bool use_or,
+                           hipStream_t ss) {
+    const std::size_t nw = widths_.size();
+    SearchArgs a{};
+    a.buf      = buf;
+    a.rows     = search_rows_[l].get();
+    a.nrows    = static_cast<int>(search_count_[l]);
+    a.inv_cur  = inv_cur_.get();
+    a.inv_or   = use_or ? inv_or_.get() : inv_cur_.get();
+    a.widths   = widths_dev_.get();
+    a.nw       = static_cast<int>(nw);
+    a.kmax     = static_cast<int>(kmax_);
+    a.wm1      = static_cast<int>(wm1_);
+    a.tile_len = tl;
+    a.tile_t0  = static_cast<int64_t>(t0);
+    a.gulp_abs = processed_;
+    a.thr      = cfg_.detect_thresh;
+    a.mode     = cfg_.peak_mode;
+    a.trap     = cfg_.use_trapezoid ? 1 : 0;
+    a.open     = open_.get();
+    a.cands    = cands_.get();
+    a.counter  = reinterpret_cast<unsigned long long*>(counters_.get());
+    a.cap      = cand_cap_;
+    a.keep     = keep_planes_ ? keep_.get() : nullptr;
+    a.keep_stride = cap_;
+    if (reg_search_) {
+        RegSearchArgs ra{};
+        ra.a       = a;
+        ra.present = 0;
+        for (int k = 0; k < 16; ++k) ra.widx[k] = -1;
+        for (std::size_t w = 0; w < nw; ++w) {
+            const int k = 31 - __builtin_clz(widths_[w]);
+            ra.present |= 1U << k;
+            ra.widx[k] = static_cast<int>(w);
+        }
+        const unsigned blocks = ceil_div(search_count_[l], kBlock / 64);
+        (cfg_.use_trapezoid ? reg_launcher<true>(static_cast<int>(kmax_))
+                            : reg_launcher<false>(static_cast<int>(kmax_)))(ra, blocks, ss);
+        return;
+    }
+    RingSearchArgs rs{};
+    rs.a = a;
+    for (int b = 0; b < kRingLevels; ++b) {
+        rs.off[b]  = b <= static_cast<int>(kmax_) ? ring_off_[b] : 0;
+        rs.size[b] = b <= static_cast<int>(kmax_) ? ring_size_[b] : 1;
+    }
+    rs.u0         = -static_cast<int>((wm1_ + kRingChunk - 1) / kRingChunk * kRingChunk);
+    rs.lds_floats = ring_floats_;
+    search_ring_kernel<<<search_count_[l], kRingChunk, ring_floats_ * sizeof(float), ss>>>(rs);
+}
+
+// Carries the last H input samples of each channel to the front of the
+// extended input (the next gulp's look-back).
+void Engine::carry_history(hipStream_t s) {
+    const uint64_t nch = plan_.cfg.nchans;
+    const uint64_t N   = last_n_;
     if (N >= hist_in_) {
         if (hist_in_ > 0)
             FA_HIP(hipMemcpy2DAsync(x_.get(), xstride_ * 4, x_.get() + N, xstride_ * 4,
@@ -1974,11 +2122,6 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
             scratch_.get(), hist_in_, x_.get(), xstride_, hist_in_, nch);
     }
     FA_HIP(hipGetLastError());
-    FA_HIP(hipEventRecord(ev_end_, s));
-    prev_mask_ = m;
-    processed_ += N;
-    last_n_          = N;
-    pending_collect_ = true;
 }
 
 void Engine::timing_collect() {
@@ -2034,7 +2177,6 @@ void Engine::collect(hipStream_t s) {
     FA_HIP(hipStreamSynchronize(s));
     if (!pending_collect_) return; // the last call's view stays valid
     const auto h0    = std::chrono::steady_clock::now();
-    pending_collect_ = false;
     result_.clear();
     dropped_global_ = dropped_per_dm_ = 0;
     // Copies on the gulp's stream (a synchronous hipMemcpy would also wait
@@ -2042,10 +2184,34 @@ void Engine::collect(hipStream_t s) {
     uint64_t ctr[2];
     FA_HIP(hipMemcpyAsync(ctr, counters_.get(), sizeof(ctr), hipMemcpyDeviceToHost, s));
     FA_HIP(hipStreamSynchronize(s));
+    if (ctr[0] > cand_cap_) {
+        // More records than the buffer holds: which ones were stored depends
+        // on the order of the atomic appends. Grow the buffer and search the
+        // gulp again from the runs open before it (its input is still in
+        // place: the history carry waits for this collect).
+        if (ctr[0] > kCandCapMax)
+            throw std::runtime_error(
+                "the search emitted " + std::to_string(ctr[0]) +
+                " raw candidates in one gulp, more than the " + std::to_string(kCandCapMax) +
+                " the device buffer may hold; raise detect_thresh");
+        cand_cap_ = ctr[0];
+        cands_.resize(cand_cap_);
+        FA_HIP(hipMemcpyAsync(open_.get(), open_saved_.get(), open_.bytes(),
+                              hipMemcpyDeviceToDevice, s));
+        processed_ -= last_n_;
+        prev_mask_       = before_mask_;
+        pending_collect_ = false;
+        const std::vector<uint8_t> m = last_mask_;
+        ++reruns_;
+        execute_prepared(last_n_, m.data(), s);
+        collect(s);
+        return;
+    }
+    pending_collect_ = false;
+    carry_history(s);
     frontier_             = ctr[1];
     raw_candidates_       = ctr[0];
-    const uint64_t stored = std::min(ctr[0], cand_cap_);
-    dropped_global_       = ctr[0] - stored;
+    const uint64_t stored = ctr[0];
     recs_.resize(stored);
     if (stored) {
         FA_HIP(hipMemcpyAsync(recs_.data(), cands_.get(), stored * sizeof(CandRecord),
@@ -2116,8 +2282,9 @@ void Engine::collect(hipStream_t s) {
 }
 
 void Engine::drain(hipStream_t s) {
+    require(!pending_collect_,
+            "the last gulp's candidates were not collected (call copy_result before drain)");
     DeviceGuard dg(device_);
-    if (pending_collect_) collect(s);
     FA_HIP(hipStreamSynchronize(s));
     const std::size_t nw = widths_.size();
     std::vector<OpenRun> st(static_cast<std::size_t>(nprows_) * nw);
@@ -2128,7 +2295,9 @@ void Engine::drain(hipStream_t s) {
     for (std::size_t g = 0; g < st.size(); ++g) {
         if (!st[g].open) continue;
         const uint32_t prow = static_cast<uint32_t>(g / nw);
-        const uint32_t w    = static_cast<uint32_t>(g % nw);
+        const uint32_t w    = cfg_.peak_mode == FA_PEAK_ACROSS_WIDTHS
+                                  ? st[g].w
+                                  : static_cast<uint32_t>(g % nw);
         const auto it = std::upper_bound(plane_first_.begin(), plane_first_.end(), prow);
         const auto p  = static_cast<std::size_t>(it - plane_first_.begin()) - 1;
         const auto& pd = plan_.planes[p];
@@ -2166,9 +2335,16 @@ uint64_t Engine::memory_bytes() const {
                   ml.rows.bytes();
     for (const auto& sr : search_rows_) tables += sr.bytes();
     return x_.bytes() + buf_[0].bytes() + buf_[1].bytes() + inv_cur_.bytes() + inv_or_.bytes() +
-           open_.bytes() + cands_.bytes() + keep_.bytes() + scratch_.bytes() +
+           open_.bytes() + open_saved_.bytes() + cands_.bytes() + keep_.bytes() + scratch_.bytes() +
            init_chan_.bytes() + init_rows_.bytes() + widths_dev_.bytes() + counters_.bytes() +
            tables;
+}
+
+std::vector<uint64_t> Engine::level_samples() const {
+    std::vector<uint64_t> out(plan_.nlevels, 0);
+    for (uint32_t l = 0; l < plan_.nlevels; ++l)
+        for (const int32_t h : hist_[l]) out[l] += tile_ + static_cast<uint64_t>(h);
+    return out;
 }
 
 void Engine::keep_planes(bool keep) { keep_planes_ = keep; }

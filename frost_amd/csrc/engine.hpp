@@ -11,10 +11,13 @@
 //    T = (B_W(t) - 0.5*B_e(t)) - 0.5*B_e(t-W+e). S/N = sum * inv_sigma.
 //  * inv_sigma = float(1/sqrt(var)), var = sum over active channels of the
 //    squared weights of (init window) * (filter), exact in double.
-//  * Windows ending at gulp-local t < delay + W - 1 are not searched in the
-//    first gulp and use the OR-mask noise in later gulps.
+//  * A window is searched once its absolute end sample reaches
+//    delay + W - 1; windows ending at gulp-local t < delay + W - 1 after
+//    that reach into the previous gulp and use the OR-mask noise.
 //  * per_width peaks: maximal above-threshold runs per (plane, row, width),
-//    first maximum, continued across tiles and gulps.
+//    first maximum, continued across tiles and gulps. across_widths: one run
+//    per (plane, row) over the per-sample maximum of the searched widths
+//    (lowest width index on ties). all: every sample above threshold.
 //
 // Device layout: the input is an extended channel-major buffer
 // x[nchans][H + capacity] whose first H columns carry the previous gulp's
@@ -46,10 +49,13 @@ struct CandRecord {
     uint32_t pad;
 };
 
+// An open above-threshold run of one (plane row, width), or of one plane
+// row in across_widths mode (slot 0; w is the width index of its peak).
 struct OpenRun {
     uint64_t start, peak, end;
     float snr;
-    uint32_t open;
+    uint16_t open;
+    uint16_t w;
 };
 
 struct GatherBlock {
@@ -133,9 +139,15 @@ public:
     // Copies a channel-major waterfall in first.
     void execute(const float* wf, uint64_t stride, uint64_t offset, uint64_t nsamps,
                  const uint8_t* kill_mask, hipStream_t s);
-    // Synchronises `s`, then sorts and caps the last call's candidates.
+    // Synchronises `s`, then sorts and caps the last call's candidates and
+    // carries the input history forward. When the search emitted more
+    // records than the candidate buffer holds, the buffer grows and the gulp
+    // is searched again from its saved open runs, so the kept rows and drop
+    // counts are those of a full sort.
     void collect(hipStream_t s);
+    // Closes every open run; the last gulp must have been collected.
     void drain(hipStream_t s);
+    bool pending() const { return pending_collect_; }
 
     const std::vector<fa_raw_candidate>& result() const { return result_; }
     uint64_t dropped_global() const { return dropped_global_; }
@@ -158,13 +170,21 @@ public:
     // Host side of the last collect: seconds, and candidates the search emitted.
     double collect_seconds() const { return collect_seconds_; }
     uint64_t raw_candidates() const { return raw_candidates_; }
+    // Gulps searched a second time because the candidate buffer overflowed.
+    uint64_t reruns() const { return reruns_; }
     const std::vector<int32_t>& level_source() const { return level_source_; }
+    // Per level: samples one full time tile holds over the level's rows,
+    // sum over rows of (tile + look-back).
+    std::vector<uint64_t> level_samples() const;
     uint64_t hist_in() const { return hist_in_; }
 
 private:
     void compute_inv(const std::vector<uint8_t>& mask, std::vector<float>& out) const;
     void build_programs();
     void timing_collect();
+    void launch_search(uint32_t level, const float* buf, int tl, uint64_t t0, bool use_or,
+                       hipStream_t ss);
+    void carry_history(hipStream_t s);
 
     FdmtPlan plan_;
     fa_boxcar_config cfg_{};
@@ -202,7 +222,7 @@ private:
     DeviceBuffer<float> buf_[2];
     DeviceBuffer<float> inv_cur_, inv_or_;
     DeviceBuffer<uint32_t> widths_dev_;
-    DeviceBuffer<OpenRun> open_;
+    DeviceBuffer<OpenRun> open_, open_saved_; // runs now / before the last gulp
     DeviceBuffer<CandRecord> cands_;
     DeviceBuffer<uint64_t> counters_; // [0] candidate count, [1] frontier
     DeviceBuffer<float> keep_;
@@ -215,6 +235,12 @@ private:
     uint64_t processed_ = 0;
     uint64_t last_n_    = 0;
     bool pending_collect_ = false;
+    // The last gulp's mask and the mask before it (a re-run restores them).
+    std::vector<uint8_t> last_mask_, before_mask_;
+    // Ring-ladder search: LDS ring offset and length per ladder level.
+    std::vector<int32_t> ring_off_, ring_size_;
+    int32_t ring_floats_ = 0;
+    uint64_t reruns_     = 0;
 
     std::vector<fa_raw_candidate> result_;
     std::vector<CandRecord> recs_, sorted_;

@@ -98,6 +98,9 @@ class Beam:
     def level_sources(self):
         return search.engine_level_sources(self.engine_handle())
 
+    def level_samples(self):
+        return search.engine_level_samples(self.engine_handle())
+
     def host_stats(self):
         """(collect seconds, candidates emitted before caps) of the last gulp."""
         sec, n = C.c_double(), C.c_uint64()

@@ -146,6 +146,16 @@ def engine_level_sources(handle) -> np.ndarray:
     return src
 
 
+def engine_level_samples(handle) -> np.ndarray:
+    """Per FDMT level: samples one full time tile holds over its rows."""
+    nl = C.c_uint64()
+    check(lib().fa_engine_level_samples(handle, None, 0, C.byref(nl)))
+    out = np.zeros(nl.value, np.uint64)
+    check(lib().fa_engine_level_samples(handle, out.ctypes.data_as(abi.PU64), nl.value,
+                                        C.byref(nl)))
+    return out
+
+
 class FDMTBoxcar:
     def __init__(self, plan: FDMTPlan, cfg: abi.BoxcarConfig, kill_mask=None,
                  device: int = 0):
@@ -244,7 +254,16 @@ class FDMTBoxcar:
     def level_sources(self) -> np.ndarray:
         return engine_level_sources(self.h)
 
+    def level_samples(self) -> np.ndarray:
+        return engine_level_samples(self.h)
+
     def memory_bytes(self) -> int:
         b = C.c_uint64()
         check(lib().fa_engine_memory_bytes(self.h, C.byref(b)))
         return b.value
+
+    def host_stats(self):
+        """(collect seconds, raw candidates the last gulp's search emitted)."""
+        sec, raw = C.c_double(), C.c_uint64()
+        check(lib().fa_engine_host_stats(self.h, C.byref(sec), C.byref(raw)))
+        return sec.value, raw.value

@@ -151,7 +151,9 @@ typedef struct fa_dedisp_config {
     uint64_t subband_depth; /* 0, 1 or 2 */
 } fa_dedisp_config;
 
-/* dmt::BoxcarConfig as frost fills it (lib/pipeline/pipeline.cpp:316-330). */
+/* dmt::BoxcarConfig as frost fills it (lib/pipeline/pipeline.cpp:316-330).
+ * max_boxcar in [1, 4096] (frost's kMaxBoxcar, lib/config/config.cpp:652);
+ * at most 32 width trials. */
 typedef struct fa_boxcar_config {
     uint64_t max_boxcar;
     float wtsp;
@@ -279,7 +281,9 @@ int fa_engine_destroy(fa_engine* engine);
  * (lib/pipeline/pipeline.cpp:603). waterfall: device, [nchans][stride]
  * channel-major float32, ascending frequency, nsamps valid columns starting
  * at column `offset`. kill_mask: host or NULL (= constructor mask). The
- * call is asynchronous on `stream`; fa_engine_copy_result synchronises. */
+ * call is asynchronous on `stream`; fa_engine_copy_result synchronises and
+ * must be called before the next execute (the input history is carried
+ * forward there; an execute with an uncollected gulp is refused). */
 int fa_engine_execute(fa_engine* engine, const float* waterfall,
                       uint64_t stride, uint64_t offset, uint64_t nsamps,
                       const uint8_t* kill_mask, void* stream);
@@ -287,13 +291,16 @@ int fa_engine_execute(fa_engine* engine, const float* waterfall,
 /* FDMTBoxcar::copy_result(Stream) -> CandidateResultView
  * (lib/pipeline/pipeline.cpp:605). Candidates are sorted by (plane, dm_idx,
  * filter_idx, peak_sample). *n is the number available; at most cap are
- * written. */
+ * written. Every raw candidate of the gulp takes part in the caps: when the
+ * search emits more than its device buffer holds, the buffer grows and the
+ * gulp is searched again (FA_RUNTIME_ERROR above 2^28 raw candidates). */
 int fa_engine_copy_result(fa_engine* engine, void* stream,
                           fa_raw_candidate* out, uint64_t cap, uint64_t* n,
                           uint64_t* dropped_global, uint64_t* dropped_per_dm);
 
 /* FDMTBoxcar::drain() (lib/pipeline/pipeline.cpp:517): closes every open
- * excursion; results as fa_engine_copy_result. */
+ * excursion; results as fa_engine_copy_result. The last gulp must have been
+ * collected with fa_engine_copy_result (refused otherwise). */
 int fa_engine_drain(fa_engine* engine, void* stream, fa_raw_candidate* out,
                     uint64_t cap, uint64_t* n, uint64_t* dropped_global,
                     uint64_t* dropped_per_dm);
@@ -338,6 +345,13 @@ int fa_engine_level_times(const fa_engine* engine, double* seconds,
  * stored. A launch reading level a and writing level b sums 2^(b-a) rows of
  * level a per output row. No reference counterpart (diagnostics). */
 int fa_engine_level_sources(const fa_engine* engine, int32_t* source, uint64_t cap,
+                            uint64_t* nlevels);
+
+/* Per FDMT level: samples one full time tile holds over the level's rows,
+ * the sum over rows of (tile + look-back). A launch reading stored level a
+ * and writing level b reads at least 4 x samples[a] bytes and writes
+ * 4 x samples[b] per tile. No reference counterpart (diagnostics). */
+int fa_engine_level_samples(const fa_engine* engine, uint64_t* samples, uint64_t cap,
                             uint64_t* nlevels);
 
 /* Host side of the last copy_result: wall seconds spent sorting, capping and

@@ -213,19 +213,24 @@ public:
     fa_boxcar_config bcfg;
     std::vector<uint32_t> widths;
     std::vector<uint8_t> mask, prev_mask;
+    std::vector<uint8_t> inv_mask; // the mask inv_cur was computed for
     std::vector<std::vector<float>> inv_cur, inv_or; // [plane][row*nw+w]
     std::vector<fa_raw_candidate> result;
     uint64_t dropped_global = 0, dropped_per_dm = 0;
     uint64_t processed      = 0; // absolute samples consumed
+    double t_fdmt = 0, t_search = 0; // wall seconds of the last execute by stage
     SizeType hist           = 0; // carried input samples per channel
     std::vector<float> carry;    // [nchans][hist]
     bool keep_planes = false;
     std::vector<std::vector<float>> plane_rows; // [plane][row][nsamps]
     // open excursions: [plane][row][width]
+    // One open run; in across_widths mode one per row (slot 0) and w is
+    // the width index of its peak.
     struct Open {
         int32_t open = 0;
         uint64_t start = 0, peak = 0, end = 0;
         float snr = 0;
+        uint32_t w = 0;
     };
     std::vector<std::vector<Open>> open;
     std::vector<std::vector<uint32_t>> per_dm_count; // [plane][row]
@@ -233,11 +238,11 @@ public:
     std::vector<float> compute_inv(const std::vector<uint8_t>& m,
                                    uint32_t p) const;
     void search_row(uint32_t p, uint32_t r, const std::vector<float>& row, SizeType N,
-                    bool first, std::vector<std::vector<float>>& ladder, Open* orow,
+                    std::vector<std::vector<float>>& ladder, Open* orow,
                     std::vector<fa_raw_candidate>& out) const;
     void search_plane(uint32_t p,
                       const std::vector<std::vector<float>>& node_rows,
-                      SizeType N, bool first);
+                      SizeType N);
     double cached_variance(int32_t len, uint32_t w) const;
     mutable std::vector<double> var_table; // [len][width], -1 = not yet computed
     void emit(uint32_t p, uint32_t row, uint32_t w, const Open& o);

@@ -309,9 +309,9 @@ int oracle_engine_search_row(void* h, uint32_t p, uint32_t r, const float* row, 
         std::vector<std::vector<float>> ladder(kmax + 1, std::vector<float>(E, 0.0F));
         std::vector<Engine::Open> opens(e->widths.size());
         std::vector<fa_raw_candidate> res;
-        e->search_row(p, r, x, nsamps, true, ladder, opens.data(), res);
+        e->search_row(p, r, x, nsamps, ladder, opens.data(), res);
         for (uint32_t w = 0; w < opens.size(); ++w)
-            if (opens[w].open) res.push_back(e->make_candidate(p, r, w, opens[w]));
+            if (opens[w].open) res.push_back(e->make_candidate(p, r, opens[w].w, opens[w]));
         *n = res.size();
         if (out) std::memcpy(out, res.data(), std::min<uint64_t>(cap, res.size()) * sizeof(*out));
     });
@@ -325,6 +325,10 @@ int oracle_engine_inv_sigma(void* h, uint32_t p, float* out, uint64_t cap) {
     });
 }
 uint64_t oracle_engine_hist(void* h) { return static_cast<Engine*>(h)->hist; }
+void oracle_engine_stage_seconds(void* h, double* fdmt, double* search) {
+    *fdmt   = static_cast<Engine*>(h)->t_fdmt;
+    *search = static_cast<Engine*>(h)->t_search;
+}
 uint64_t oracle_sizeof_candidate(void) { return sizeof(fa_raw_candidate); }
 uint64_t oracle_sizeof_rfi_config(void) { return sizeof(fa_rfi_config); }
 

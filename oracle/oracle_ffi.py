@@ -81,6 +81,7 @@ def lib() -> C.CDLL:
             "oracle_engine_search_row": (C.c_int, [P, C.c_uint32, C.c_uint32, P, C.c_uint64,
                                                    P, C.c_uint64, P]),
             "oracle_engine_hist": (C.c_uint64, [P]),
+            "oracle_engine_stage_seconds": (None, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "oracle_sizeof_candidate": (C.c_uint64, []),
             "oracle_sizeof_rfi_config": (C.c_uint64, []),
         }
@@ -277,6 +278,12 @@ class Engine:
 
     def frontier(self) -> int:
         return int(lib().oracle_engine_frontier(self.h))
+
+    def stage_seconds(self):
+        """(FDMT, search) wall seconds of the last execute."""
+        a, b = C.c_double(), C.c_double()
+        lib().oracle_engine_stage_seconds(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def plane_rows(self, p, nsamps):
         info = self.plan.plane(p)

@@ -34,15 +34,21 @@
 //    squared weights of (init window) * (filter), exact in double.
 //    S/N = window sum * float(1/sqrt(variance)).
 //  * A window ending at absolute sample t is searched once t >= delay + W - 1
-//    (cold prefix); in later gulps windows reaching into the previous gulp
-//    (gulp-local t < delay + W - 1) use the variance of the OR of the two
-//    gulps' good-channel masks.
+//    (cold prefix); windows reaching into the previous gulp (gulp-local
+//    t < delay + W - 1, absolute t past the cold prefix) use the variance of
+//    the OR of the two gulps' good-channel masks.
 //  * per_width peaks: maximal runs of S/N > threshold per (plane, row,
 //    width); the peak is the first maximum; runs continue across gulps.
+//  * across_widths peaks: per (plane, row), the sample's S/N is the largest
+//    over the widths searched at it (the lowest width index on ties); a
+//    run is a maximal stretch of samples above threshold; its peak is the
+//    first maximum, reported with the width that gave it.
+//  * all: every (plane, row, width, sample) above threshold.
 //  * Caps: candidates of one call are sorted by (plane, dm_idx, filter_idx,
 //    peak_sample); the first max_cands_per_dm of each (plane, dm_idx) and the
 //    first max_candidates overall are kept, the rest counted as dropped.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -318,8 +324,9 @@ void brute_force_row(const Plan& plan, uint32_t plane, uint32_t row,
 Engine::Engine(const fa_dedisp_config& dcfg, const fa_boxcar_config& b,
                const uint8_t* kill_mask)
     : plan(dcfg), bcfg(b) {
-    if (b.peak_mode == FA_PEAK_ACROSS_WIDTHS)
-        throw std::invalid_argument("peak_mode across_widths is not supported");
+    if (b.peak_mode != FA_PEAK_PER_WIDTH && b.peak_mode != FA_PEAK_ACROSS_WIDTHS &&
+        b.peak_mode != FA_PEAK_ALL)
+        throw std::invalid_argument("unknown peak_mode");
     if (!(b.detect_thresh > 0.0F))
         throw std::invalid_argument("detect_thresh must be > 0");
     widths = box_width_trials(b.max_boxcar, b.wtsp);
@@ -328,7 +335,8 @@ Engine::Engine(const fa_dedisp_config& dcfg, const fa_boxcar_config& b,
     prev_mask = mask;
     inv_cur.resize(plan.planes.size());
     for (uint32_t p = 0; p < plan.planes.size(); ++p) inv_cur[p] = compute_inv(mask, p);
-    inv_or = inv_cur;
+    inv_or   = inv_cur;
+    inv_mask = mask;
     hist   = (SizeType)plan.dt_max + widths.back() - 1;
     carry.assign(dcfg.nchans * hist, 0.0F);
     open.resize(plan.planes.size());
@@ -451,10 +459,11 @@ static void finalize_result(std::vector<fa_raw_candidate>& res,
 }
 
 // One row of plane p: row holds the row's hist + N samples (look-back
-// first); orow its nw open excursions, updated in place; candidates closed
-// in this gulp are appended to out.
+// first); orow its open excursions (nw of them; slot 0 only in across_widths
+// mode), updated in place; candidates closed in this gulp are appended to
+// out. `processed` is the absolute sample of the row's sample 0.
 void Engine::search_row(uint32_t p, uint32_t r, const std::vector<float>& row, SizeType N,
-                        bool first, std::vector<std::vector<float>>& ladder, Open* orow,
+                        std::vector<std::vector<float>>& ladder, Open* orow,
                         std::vector<fa_raw_candidate>& out) const {
     const PlanePlan& pp = plan.planes[p];
     const auto nw       = widths.size();
@@ -467,54 +476,82 @@ void Engine::search_row(uint32_t p, uint32_t r, const std::vector<float>& row, S
         for (SizeType t = s; t < E; ++t)
             ladder[k][t] = ladder[k - 1][t] + ladder[k - 1][t - s];
     }
-    for (uint32_t w = 0; w < nw; ++w) {
+    // S/N of width w at gulp-local t, or nothing when the window is cold:
+    // searched once the absolute end sample reaches delay + W - 1; windows
+    // reaching into the previous gulp (local t below that) use the OR-mask
+    // noise.
+    auto snr_at = [&](uint32_t w, SizeType t, float& snr) {
         const uint32_t W    = widths[w];
-        Open& o             = orow[w];
         const uint64_t warm = (uint64_t)pp.delay[r] + W - 1;
+        const uint64_t abs_t = processed + t;
+        if (abs_t < warm) return false;
+        const SizeType i = hist + t;
+        float sum        = window_sum(ladder, i, W);
+        if (bcfg.use_trapezoid && W >= 3) {
+            const uint32_t e = (W + 1) / 4;
+            sum = (sum - 0.5F * window_sum(ladder, i, e)) -
+                  0.5F * window_sum(ladder, i - (W - e), e);
+        }
+        const float inv = t < warm ? inv_or[p][r * nw + w] : inv_cur[p][r * nw + w];
+        snr             = sum * inv;
+        return true;
+    };
+    auto track = [&](Open& o, uint32_t w, uint64_t abs_t, bool above, float snr) {
+        if (above) {
+            if (!o.open) {
+                o = Open{1, abs_t, abs_t, abs_t, snr, w};
+            } else {
+                if (snr > o.snr) {
+                    o.snr  = snr;
+                    o.peak = abs_t;
+                    o.w    = w;
+                }
+                o.end = abs_t;
+            }
+        } else if (o.open) {
+            out.push_back(make_candidate(p, r, o.w, o));
+            o = Open{};
+        }
+    };
+    if (bcfg.peak_mode == FA_PEAK_ACROSS_WIDTHS) {
+        // One run per row over all widths: at each sample the largest S/N
+        // of the searched widths (the first width on ties) decides.
+        Open& o = orow[0];
+        for (SizeType t = 0; t < N; ++t) {
+            bool any   = false;
+            float best = 0.0F;
+            uint32_t bw = 0;
+            for (uint32_t w = 0; w < nw; ++w) {
+                float snr = 0.0F;
+                if (!snr_at(w, t, snr)) continue;
+                if (!any || snr > best) {
+                    best = snr;
+                    bw   = w;
+                }
+                any = true;
+            }
+            track(o, bw, processed + t, any && best > thr, best);
+        }
+        return;
+    }
+    for (uint32_t w = 0; w < nw; ++w) {
+        Open& o = orow[w];
         for (SizeType t = 0; t < N; ++t) {
             const uint64_t abs_t = processed + t;
-            const SizeType i     = hist + t;
-            float sum            = window_sum(ladder, i, W);
-            if (bcfg.use_trapezoid && W >= 3) {
-                const uint32_t e = (W + 1) / 4;
-                sum = (sum - 0.5F * window_sum(ladder, i, e)) -
-                      0.5F * window_sum(ladder, i - (W - e), e);
-            }
-            bool searched = true;
-            float inv     = inv_cur[p][r * nw + w];
-            if (t < warm) {
-                if (first)
-                    searched = false;
-                else
-                    inv = inv_or[p][r * nw + w];
-            }
-            const float snr  = sum * inv;
-            const bool above = searched && snr > thr;
+            float snr            = 0.0F;
+            const bool above     = snr_at(w, t, snr) && snr > thr;
             if (bcfg.peak_mode == FA_PEAK_ALL) {
-                if (above) out.push_back(make_candidate(p, r, w, Open{1, abs_t, abs_t, abs_t, snr}));
+                if (above) out.push_back(make_candidate(p, r, w, Open{1, abs_t, abs_t, abs_t, snr, w}));
                 continue;
             }
-            if (above) {
-                if (!o.open) {
-                    o = Open{1, abs_t, abs_t, abs_t, snr};
-                } else {
-                    if (snr > o.snr) {
-                        o.snr  = snr;
-                        o.peak = abs_t;
-                    }
-                    o.end = abs_t;
-                }
-            } else if (o.open) {
-                out.push_back(make_candidate(p, r, w, o));
-                o = Open{};
-            }
+            track(o, w, abs_t, above, snr);
         }
     }
 }
 
 void Engine::search_plane(uint32_t p,
                           const std::vector<std::vector<float>>& node_rows,
-                          SizeType N, bool first) {
+                          SizeType N) {
     const PlanePlan& pp = plan.planes[p];
     const Node& nd      = plan.nodes[pp.node];
     const auto nw       = widths.size();
@@ -535,7 +572,7 @@ void Engine::search_plane(uint32_t p,
         const auto& row = node_rows[ir];
         if (keep_planes)
             std::memcpy(&plane_rows[p][r * N], &row[hist], N * sizeof(float));
-        search_row(p, r, row, N, first, ladder, &open[p][r * nw], mine);
+        search_row(p, r, row, N, ladder, &open[p][r * nw], mine);
     }
 #pragma omp critical
     found.insert(found.end(), mine.begin(), mine.end());
@@ -548,14 +585,14 @@ void Engine::execute(const float* wf, SizeType N, const uint8_t* kill_mask) {
     // Constructor mask unless this gulp passes its own.
     const std::vector<uint8_t> m =
         kill_mask ? std::vector<uint8_t>(kill_mask, kill_mask + nch) : mask;
-    const bool first   = processed == 0;
-    const bool changed = m != prev_mask;
+    const bool changed = processed > 0 && m != prev_mask;
     std::vector<uint8_t> mor(nch);
     for (SizeType c = 0; c < nch; ++c) mor[c] = (uint8_t)(m[c] | prev_mask[c]);
     for (uint32_t p = 0; p < plan.planes.size(); ++p) {
-        inv_cur[p] = compute_inv(m, p);
-        inv_or[p]  = changed ? compute_inv(mor, p) : inv_cur[p];
+        if (m != inv_mask) inv_cur[p] = compute_inv(m, p); // tables of this mask
+        inv_or[p] = changed ? compute_inv(mor, p) : inv_cur[p];
     }
+    inv_mask = m;
     // Extended input [nch][hist + N]: carried history, then this gulp.
     const SizeType E = hist + N;
     std::vector<float> X(nch * E);
@@ -566,10 +603,13 @@ void Engine::execute(const float* wf, SizeType N, const uint8_t* kill_mask) {
     // FDMT tree level by level over the extended range (zero before -hist);
     // two levels resident; planes are searched as their level completes.
     result.clear();
+    t_fdmt = t_search = 0;
+    using clock = std::chrono::steady_clock;
     if (keep_planes) plane_rows.assign(plan.planes.size(), {});
     std::vector<std::vector<std::vector<float>>> prev_rows(plan.nodes.size()),
         cur_rows(plan.nodes.size());
     for (uint32_t l = 0; l < plan.nlevels; ++l) {
+        const auto tl0 = clock::now();
         std::vector<std::pair<std::size_t, std::size_t>> work; // (node, row)
         for (std::size_t id = 0; id < plan.nodes.size(); ++id) {
             const Node& nd = plan.nodes[id];
@@ -607,9 +647,12 @@ void Engine::execute(const float* wf, SizeType N, const uint8_t* kill_mask) {
             for (SizeType t = 0; t < E; ++t)
                 out[t] = a[t] + ((int64_t)t - dl >= 0 ? b[t - dl] : 0.0F);
         }
+        const auto tl1 = clock::now();
         for (uint32_t p = 0; p < plan.planes.size(); ++p)
             if (plan.planes[p].level == l)
-                search_plane(p, cur_rows[plan.planes[p].node], N, first);
+                search_plane(p, cur_rows[plan.planes[p].node], N);
+        t_fdmt += std::chrono::duration<double>(tl1 - tl0).count();
+        t_search += std::chrono::duration<double>(clock::now() - tl1).count();
         prev_rows.swap(cur_rows);
         for (auto& v : cur_rows) v.clear();
     }
@@ -628,7 +671,7 @@ std::vector<fa_raw_candidate> Engine::drain() {
         for (uint32_t r = 0; r < plan.planes[p].delay.size(); ++r)
             for (uint32_t w = 0; w < nw; ++w) {
                 Open& o = open[p][r * nw + w];
-                if (o.open) emit(p, r, w, o);
+                if (o.open) emit(p, r, o.w, o);
                 o = Open{};
             }
     finalize_result(result, bcfg, dropped_global, dropped_per_dm, 0);

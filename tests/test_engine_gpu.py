@@ -112,7 +112,7 @@ def test_engine_single_gulp_small(oracle, stream):
 
 @pytest.mark.parametrize("depth,path,fuse,merge,init", [
     (0, None, None, None, None), (1, None, None, None, None), (2, None, None, None, None),
-    (2, "lds", None, None, None), (2, None, 1, None, None), (2, None, 2, None, None),
+    (2, "ring", None, None, None), (2, None, 1, None, None), (2, None, 2, None, None),
     (2, None, 3, None, None), (2, None, 3, "direct", None), (1, None, 1, "direct", None),
     (2, None, None, None, "fused"), (2, None, 2, None, "fused"), (0, None, 1, None, "fused")])
 def test_engine_gulps_and_planes(oracle, stream, depth, path, fuse, merge, init):
@@ -234,9 +234,126 @@ def test_engine_rejects_bad_config():
     plan = search.make_fdmt_plan(dcfg)
     with pytest.raises(abi.InvalidArgument):
         search.FDMTBoxcar(plan, abi.boxcar_config(max_boxcar=16, detect_thresh=-1.0))
+    with pytest.raises(abi.InvalidArgument):  # frost's bound: boxcar_max <= 4096
+        search.FDMTBoxcar(plan, abi.boxcar_config(max_boxcar=4097))
+    with pytest.raises(abi.InvalidArgument):  # 64 width trials
+        search.FDMTBoxcar(plan, abi.boxcar_config(max_boxcar=64, wtsp=1.0))
+
+
+def test_engine_requires_collect_between_gulps(stream):
+    """A gulp's rows must be collected before the next execute or a drain
+    (the history carry waits for the collect); both refusals leave the
+    engine usable."""
+    dcfg = abi.dedisp_config(16, 1e-3, 1400.0, -1.0, 60.0, block_nsamps=1024)
+    gpu = search.FDMTBoxcar(search.make_fdmt_plan(dcfg), abi.boxcar_config(max_boxcar=16))
+    wf = np.zeros((16, 1024), np.float32)
+    d = search.DeviceArray.from_host(wf, stream)
+    gpu.execute_device(d, 1024, 1024, stream)
     with pytest.raises(abi.InvalidArgument):
-        search.FDMTBoxcar(plan, abi.boxcar_config(max_boxcar=16,
-                                                  peak_mode=abi.PEAK_ACROSS_WIDTHS))
+        gpu.execute_device(d, 1024, 1024, stream)
+    with pytest.raises(abi.InvalidArgument):
+        gpu.drain(stream)
+    assert gpu.copy_result(stream)[0] == []
+    gpu.execute_device(d, 1024, 1024, stream)
+    gpu.copy_result(stream)
+    assert gpu.drain(stream)[0] == []
+
+
+@pytest.mark.parametrize("wtsp,trap,path", [(2.0, False, None), (2.0, True, None),
+                                            (1.5, False, None), (2.0, False, "ring")])
+def test_engine_across_widths(oracle, stream, wtsp, trap, path):
+    """across_widths peaks (one run per plane row over the per-sample maximum
+    of the widths), register and ring searches, a mask change between gulps
+    (OR-mask noise near the boundary) and runs continued across gulps."""
+    say(f"test_engine_across_widths wtsp={wtsp} trap={trap} path={path}")
+    dcfg = abi.dedisp_config(32, 512e-6, 1400.0, -2.0, 300.0, block_nsamps=2048,
+                             subband_depth=1)
+    bcfg = abi.boxcar_config(max_boxcar=64, wtsp=wtsp, use_trapezoid=trap, detect_thresh=4.0,
+                             peak_mode=abi.PEAK_ACROSS_WIDTHS)
+    rng = np.random.default_rng(71)
+    m0 = np.ones(32, np.uint8)
+    m1 = m0.copy()
+    m1[5:11] = 0
+    gulps = [pulse_waterfall(rng, 32, 2048, [(120.0, 300 + 700 * k, 3 + 9 * k, 2.0),
+                                             (200.0, 2030, 30, 1.2)], dcfg) for k in range(3)]
+    masks = [m0, m1, m1]
+    for wf, m in zip(gulps, masks):
+        wf[m == 0] = 0.0
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, masks=masks, search_path=path, tile=900)
+    assert_same(g, o)
+    assert sum(len(x[0]) for x in o) > 10
+    # Across widths: at most one run per plane row ends at any sample.
+    rows = [(c[12], c[13], c[5], c[2]) for x in o for c in x[0]]
+    assert len(rows) == len(set(rows))
+
+
+@pytest.mark.parametrize("wtsp,trap,mode", [(2.0, False, abi.PEAK_PER_WIDTH),
+                                            (2.0, True, abi.PEAK_PER_WIDTH),
+                                            (1.5, False, abi.PEAK_PER_WIDTH),
+                                            (1.5, True, abi.PEAK_ACROSS_WIDTHS),
+                                            (2.0, True, abi.PEAK_ALL)])
+def test_engine_widths_to_4096(oracle, stream, wtsp, trap, mode):
+    """frost's default and largest boxcar_max, 4096 (include/frost/config/
+    config.hpp:58): the ring-ladder search, two tiles per gulp, three gulps,
+    wide and narrow pulses."""
+    say(f"test_engine_widths_to_4096 wtsp={wtsp} trap={trap} mode={mode}")
+    dcfg = abi.dedisp_config(32, 256e-6, 1500.0, -4.0, 200.0, block_nsamps=9000)
+    bcfg = abi.boxcar_config(max_boxcar=4096, wtsp=wtsp, use_trapezoid=trap,
+                             detect_thresh=4.5 if mode == abi.PEAK_ALL else 4.0, peak_mode=mode,
+                             max_candidates=20000 if mode == abi.PEAK_ALL else 500000,
+                             max_cands_per_dm=300 if mode == abi.PEAK_ALL else 100000)
+    rng = np.random.default_rng(41)
+    gulps = [pulse_waterfall(rng, 32, 9000, [(150.0, 500 + 2500 * k, 1500 + 900 * k, 0.12),
+                                             (60.0, 8000, 5, 2.0)], dcfg) for k in range(3)]
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, tile=4600)
+    assert_same(g, o)
+    if mode != abi.PEAK_ALL:  # in `all` mode the caps fill with narrow widths first
+        wide = [c for x in o for c in x[0] if c[7] >= 1024]
+        assert wide, "no candidate at widths >= 1024"
+
+
+def test_engine_short_first_gulp(oracle, stream):
+    """A first gulp shorter than the cold prefix: windows still reaching
+    before the stream's first sample stay unsearched in the second gulp
+    (absolute-sample rule), the rest use the OR-mask noise."""
+    say("test_engine_short_first_gulp")
+    dcfg = abi.dedisp_config(32, 512e-6, 1400.0, -2.0, 300.0, block_nsamps=2048)
+    bcfg = abi.boxcar_config(max_boxcar=32, detect_thresh=3.5)
+    rng = np.random.default_rng(13)
+    m0 = np.ones(32, np.uint8)
+    m1 = m0.copy()
+    m1[2:6] = 0
+    gulps = [pulse_waterfall(rng, 32, 40, [], dcfg),
+             pulse_waterfall(rng, 32, 2048, [(100.0, 20, 4, 2.5)], dcfg),
+             pulse_waterfall(rng, 32, 2048, [(250.0, 500, 8, 2.0)], dcfg)]
+    masks = [m0, m1, m1]
+    for wf, m in zip(gulps, masks):
+        wf[m == 0] = 0.0
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, masks=masks)
+    assert_same(g, o)
+
+
+def test_engine_candidate_overflow_deterministic(oracle, stream):
+    """More raw candidates than the device buffer's 2^20 records: the engine
+    grows the buffer and searches the gulp again, so the kept rows and both
+    drop counters equal the oracle's sort-then-cap."""
+    say("test_engine_candidate_overflow_deterministic")
+    dcfg = abi.dedisp_config(64, 256e-6, 1500.0, -300 / 64, 400.0, block_nsamps=8192)
+    bcfg = abi.boxcar_config(max_boxcar=64, detect_thresh=0.5, peak_mode=abi.PEAK_ALL,
+                             max_candidates=3000, max_cands_per_dm=40)
+    plan = search.make_fdmt_plan(dcfg)
+    gpu = search.FDMTBoxcar(plan, bcfg)
+    ora = oracle.Engine(dcfg, bcfg)
+    rng = np.random.default_rng(99)
+    for k in range(2):
+        wf = rng.standard_normal((64, 8192)).astype(np.float32)
+        a = gpu.execute(wf, stream)
+        raw = gpu.host_stats()[1]
+        b = ora.execute(wf)
+        say(f"  gulp {k}: raw {raw}, kept {len(a[0])}, dropped {a[1]}/{a[2]}")
+        assert raw > (1 << 20), raw
+        assert a == b
+    assert gpu.drain(stream) == ora.drain()
 
 
 def test_full_size_rows_exact_on_integer_input(oracle, stream, monkeypatch):
