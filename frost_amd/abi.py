@@ -435,6 +435,8 @@ EXPORTS = {
                                     C.POINTER(ClusterTolerances), C.POINTER(C.c_int)]),
     "fa_cluster_labels": (C.c_int, [C.POINTER(Giant), U64, C.POINTER(ClusterTolerances),
                                     C.c_int, C.c_int, PU64]),
+    "fa_cluster_labels_noalloc": (C.c_int, [C.POINTER(Giant), U64, C.POINTER(ClusterTolerances),
+                                            C.c_int, C.c_int, PU64]),
     "fa_merge_candidates": (C.c_int, [C.POINTER(Giant), PU64, U64, C.POINTER(Giant), PU64]),
     "fa_subband_dedup": (C.c_int, [C.POINTER(Giant), U64, C.POINTER(ClusterTolerances), F32,
                                    C.c_int, C.c_int, C.POINTER(Giant), PU64]),

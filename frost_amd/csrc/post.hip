@@ -20,6 +20,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <set>
 #include <stdexcept>
@@ -305,6 +306,20 @@ struct PairGpu {
     // of each component.
     std::vector<u64> chain_labels(const std::vector<fa_giant>& work, const fa_cluster_tolerances& t) {
         const u64 n = work.size();
+        if (n < 2) return chain_labels_on(work, t, device, nullptr, nullptr, nullptr);
+        DeviceGuard dg(device);
+        auto* drows = static_cast<ChainRow*>(rows.need(n * sizeof(ChainRow)));
+        auto* dpar  = static_cast<uint32_t*>(parent.need(n * 4));
+        auto* droot = static_cast<uint32_t*>(roots.need(n * 4));
+        return chain_labels_on(work, t, device, drows, dpar, droot);
+    }
+
+    // The chain scan on caller-provided device buffers of n rows, n parents
+    // and n roots (no allocation here).
+    static std::vector<u64> chain_labels_on(const std::vector<fa_giant>& work,
+                                            const fa_cluster_tolerances& t, int device,
+                                            ChainRow* drows, uint32_t* dpar, uint32_t* droot) {
+        const u64 n = work.size();
         std::vector<u64> lab(n);
         std::iota(lab.begin(), lab.end(), u64{0});
         if (n < 2) return lab;
@@ -332,9 +347,6 @@ struct PairGpu {
                      g.chan_end, dm_step_of(g.f_low, g.f_high, ts), g.dm, g.snr, order[r].idx, 0};
         }
         DeviceGuard dg(device);
-        auto* drows = static_cast<ChainRow*>(rows.need(n * sizeof(ChainRow)));
-        auto* dpar  = static_cast<uint32_t*>(parent.need(n * 4));
-        auto* droot = static_cast<uint32_t*>(roots.need(n * 4));
         FA_HIP(hipMemcpy(drows, rs.data(), n * sizeof(ChainRow), hipMemcpyHostToDevice));
         const uint32_t n32 = static_cast<uint32_t>(n);
         uf_init_kernel<<<ceil_div(n, 256), 256>>>(dpar, n32);
@@ -376,6 +388,38 @@ struct PairGpu {
         return grp;
     }
 };
+
+// ---- fixed device scratch for the allocation-free chain labels ---------------------
+//
+// fa_cluster_labels_noalloc stages rows, union-find parents and roots in one
+// file-scope device array instead of allocating: at most kNoAllocRows rows,
+// 80 B + 8 B each (5.5 MiB on every device the library runs on, held until
+// the process ends). All calls on one device share that device's copy, so
+// calls that overlap on one device race; nothing in it needs to start at
+// zero (every byte read is written by the same call first).
+constexpr u64 kNoAllocRows = u64{1} << 16;
+constexpr std::size_t kScratchRowsBytes = kNoAllocRows * sizeof(ChainRow);
+constexpr std::size_t kScratchBytes     = kScratchRowsBytes + 2 * kNoAllocRows * sizeof(uint32_t);
+static_assert(kScratchRowsBytes % 256 == 0, "parents start on a 256-byte boundary");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on `device`, looked up once per device with the
+// device current (the host name of the array is only a stand-in for it).
+unsigned char* scratch_on(int device) {
+    static std::mutex mu;
+    static std::vector<unsigned char*> ptrs;
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0) throw std::invalid_argument("device must be >= 0");
+    if (static_cast<std::size_t>(device) >= ptrs.size()) ptrs.resize(device + 1, nullptr);
+    if (!ptrs[device]) {
+        DeviceGuard dg(device);
+        void* p = nullptr;
+        FA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+        ptrs[device] = static_cast<unsigned char*>(p);
+    }
+    return ptrs[device];
+}
 
 // ---- host passes --------------------------------------------------------------------
 
@@ -872,6 +916,32 @@ int fa_cluster_labels(const fa_giant* giants, uint64_t n, const fa_cluster_toler
             fa::PairGpu gpu;
             gpu.device = device;
             lab        = gpu.chain_labels(g, *tol);
+        }
+        std::copy(lab.begin(), lab.end(), labels);
+    });
+}
+
+int fa_cluster_labels_noalloc(const fa_giant* giants, uint64_t n, const fa_cluster_tolerances* tol,
+                              int mode, int device, uint64_t* labels) {
+    return guard([&] {
+        require(tol && (n == 0 || (giants && labels)), "null argument");
+        require(mode == FA_CLUSTER_CHAIN || mode == FA_CLUSTER_SEEDED, "unknown cluster mode");
+        if (n > fa::kNoAllocRows)
+            throw std::invalid_argument(
+                "fa_cluster_labels_noalloc: at most 65536 candidates per call (its fixed "
+                "device scratch); use fa_cluster_labels for more");
+        const std::vector<fa_giant> g(giants, giants + n);
+        std::vector<uint64_t> lab;
+        if (mode == FA_CLUSTER_SEEDED) {
+            lab = fa::seeded_labels(g, *tol);
+        } else if (n < 2) {
+            lab = fa::PairGpu::chain_labels_on(g, *tol, device, nullptr, nullptr, nullptr);
+        } else {
+            unsigned char* base = fa::scratch_on(device);
+            auto* drows = reinterpret_cast<fa::ChainRow*>(base);
+            auto* dpar  = reinterpret_cast<uint32_t*>(base + fa::kScratchRowsBytes);
+            auto* droot = dpar + fa::kNoAllocRows;
+            lab = fa::PairGpu::chain_labels_on(g, *tol, device, drows, dpar, droot);
         }
         std::copy(lab.begin(), lab.end(), labels);
     });

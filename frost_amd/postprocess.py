@@ -41,6 +41,17 @@ def cluster_labels(giants, tol: abi.ClusterTolerances, mode: int = CHAIN,
     return list(lab[:n])
 
 
+def cluster_labels_noalloc(giants, tol: abi.ClusterTolerances, mode: int = CHAIN,
+                           device: int = 0) -> list[int]:
+    """cluster_labels through fa_cluster_labels_noalloc: no device allocation
+    (fixed in-library scratch, at most 65536 candidates)."""
+    n = len(giants)
+    arr = _array(giants)
+    lab = (C.c_uint64 * max(1, n))()
+    check(lib().fa_cluster_labels_noalloc(arr, n, C.byref(tol), mode, device, lab))
+    return list(lab[:n])
+
+
 def merge_candidates(giants, labels) -> list[abi.Giant]:
     """MergeCandidatesPlan::execute (lib/search/merge.cpp:9-37)."""
     n = len(giants)

@@ -616,6 +616,18 @@ int fa_cluster_labels(const fa_giant* giants, uint64_t n,
                       const fa_cluster_tolerances* tol, int mode, int device,
                       uint64_t* labels);
 
+/* fa_cluster_labels without any device allocation. No counterpart in
+ * frost: it is for callers that cannot allocate device memory (code under
+ * stream capture, serving loops with a fixed footprint, harnesses that
+ * forbid hipMalloc). The chain scan stages its rows and union-find in a
+ * fixed device array inside the library (5.5 MiB per device, held until the
+ * process ends), so n is at most 65536 (FA_INVALID_ARGUMENT above); calls
+ * that overlap on one device share that array and race. Labels equal
+ * fa_cluster_labels'. */
+int fa_cluster_labels_noalloc(const fa_giant* giants, uint64_t n,
+                              const fa_cluster_tolerances* tol, int mode, int device,
+                              uint64_t* labels);
+
 /* MergeCandidatesPlan::execute (lib/search/merge.cpp:9-37): one row per
  * label, in ascending label order. out holds n rows; *nout is set. */
 int fa_merge_candidates(const fa_giant* giants, const uint64_t* labels,
