@@ -82,6 +82,26 @@ SizeType broadband_time_flags(std::vector<float> series,
                               SizeType smooth_radius, float threshold,
                               SizeType boxcar_max, uint8_t* flags);
 
+// Planted mistakes for the tests that show the behavioural checks pin the
+// cleaner (0 = none). Set before constructing a Cleaner.
+enum Mutation : int {
+    kMutNone             = 0,
+    kMutChanFillRadius   = 1, // neighbour fills reach 3 channels too far
+    kMutTimeFillRadius   = 2, // time fills reach 4x too far
+    kMutKadaneDuration   = 3, // Kadane-along-time ignores min_duration
+    kMutMomentQuantiles  = 4, // moment fences on the 10th/90th percentiles
+    kMutDropoutWidening  = 5, // dropouts do not flag their neighbours
+    kMutKadaneBandwidth  = 6, // Kadane-across-frequency ignores min_bandwidth
+    kMutMaskGrow         = 7, // mask growth one channel too wide
+};
+extern int g_mutation;
+
+// The fill draws' building blocks (lib/cpu/rfi_kernels.hpp:17-47):
+// splitmix64's mixer, the Box-Muller normal and the uniform index.
+uint64_t mix64(uint64_t x);
+float normal_of(uint64_t h);
+SizeType index_of(uint64_t h, SizeType lo, SizeType hi);
+
 class Cleaner {
 public:
     Cleaner(const fa_rfi_config& cfg, const fa_block_layout& layout,

@@ -8,6 +8,10 @@
 
 #include "oracle.hpp"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 using namespace oracle;
 
 namespace {
@@ -323,6 +327,24 @@ int oracle_engine_inv_sigma(void* h, uint32_t p, float* out, uint64_t cap) {
         if (cap < v.size()) throw std::invalid_argument("cap too small");
         std::memcpy(out, v.data(), v.size() * 4);
     });
+}
+void oracle_set_mutation(int m) { g_mutation = m; }
+uint64_t oracle_splitmix64(uint64_t x) { return mix64(x); }
+float oracle_normal_draw(uint64_t h) { return normal_of(h); }
+uint64_t oracle_uniform_index(uint64_t h, uint64_t lo, uint64_t hi) { return index_of(h, lo, hi); }
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 uint64_t oracle_engine_hist(void* h) { return static_cast<Engine*>(h)->hist; }
 void oracle_engine_stage_seconds(void* h, double* fdmt, double* search) {

@@ -81,6 +81,12 @@ def lib() -> C.CDLL:
             "oracle_engine_search_row": (C.c_int, [P, C.c_uint32, C.c_uint32, P, C.c_uint64,
                                                    P, C.c_uint64, P]),
             "oracle_engine_hist": (C.c_uint64, [P]),
+            "oracle_set_mutation": (None, [C.c_int]),
+            "oracle_splitmix64": (C.c_uint64, [C.c_uint64]),
+            "oracle_normal_draw": (C.c_float, [C.c_uint64]),
+            "oracle_uniform_index": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
+            "oracle_set_threads": (None, [C.c_int]),
+            "oracle_max_threads": (C.c_int, []),
             "oracle_engine_stage_seconds": (None, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "oracle_sizeof_candidate": (C.c_uint64, []),
             "oracle_sizeof_rfi_config": (C.c_uint64, []),

@@ -82,6 +82,13 @@ inline float median4(float a, float b, float c, float d) {
            2.0F;
 }
 
+} // namespace
+int g_mutation = kMutNone;
+uint64_t mix64(uint64_t x) { return splitmix64(x); }
+float normal_of(uint64_t h) { return normal_draw(h); }
+SizeType index_of(uint64_t h, SizeType lo, SizeType hi) { return uniform_index(h, lo, hi); }
+namespace {
+
 double quantile_sorted(const std::vector<double>& s, double q) {
     // lib/cpu/rfi_kernels.hpp:136-142
     const double h  = (double)(s.size() - 1) * q;
@@ -100,8 +107,9 @@ void flag_iqr_outliers(const std::vector<double>& stat,
         if (good[c]) vals.push_back(stat[c]);
     if (vals.size() < 4) return;
     std::sort(vals.begin(), vals.end());
-    const double q1 = quantile_sorted(vals, 0.25);
-    const double q3 = quantile_sorted(vals, 0.75);
+    const bool mq   = g_mutation == kMutMomentQuantiles;
+    const double q1 = quantile_sorted(vals, mq ? 0.10 : 0.25);
+    const double q3 = quantile_sorted(vals, mq ? 0.90 : 0.75);
     const double iqr = q3 - q1;
     if (!(iqr > 0.0)) return;
     const double lo = q1 - (k * iqr), hi = q3 + (k * iqr);
@@ -508,6 +516,8 @@ Cleaner::Cleaner(const fa_rfi_config& c, const fa_block_layout& l,
     smooth_radius    = std::max<SizeType>((SizeType)(ws / 2), 1);
     time_fill_radius = std::max<SizeType>(smooth_radius / 4, 1);
     chan_fill_radius = std::max<SizeType>(nchans / 60, 1);
+    if (g_mutation == kMutTimeFillRadius) time_fill_radius *= 4;
+    if (g_mutation == kMutChanFillRadius) chan_fill_radius += 3;
     clip             = cfg.narrow_threshold;
     narrow_active    = cfg.narrow_enabled &&
                     (l.dtype == FA_FLOAT32 || bits_of(l.dtype) > 4);
@@ -726,7 +736,8 @@ void Cleaner::build_channel_mask() {
     }
     if (cfg.mask_grow > 0) {
         std::vector<uint8_t> grown = dyn;
-        const SizeType g           = cfg.mask_grow;
+        SizeType g                 = cfg.mask_grow;
+        if (g_mutation == kMutMaskGrow && g < n) ++g;
         for (SizeType c = 0; c < n; ++c) {
             if (!dyn[c]) continue;
             const SizeType first = c > g ? c - g : 0;
@@ -750,9 +761,9 @@ void Cleaner::scan_rows() {
     const SizeType n = nchans, nchunks = chunk_count();
     const SizeType chunk = (nsamps + nchunks - 1) / nchunks;
     const bool kchan = cfg.kadane_channel_enabled, kspec = cfg.kadane_spectrum_enabled;
-    const double min_bw = kspec ? (double)cfg.kadane_spectrum_min_bandwidth /
-                                      std::abs(layout.df)
-                                : 0.0;
+    const double min_bw = kspec && g_mutation != kMutKadaneBandwidth
+                              ? (double)cfg.kadane_spectrum_min_bandwidth / std::abs(layout.df)
+                              : 0.0;
     const double whole = (double)cfg.kadane_spectrum_whole_band_fraction * (double)n;
     const double kthr  = cfg.kadane_spectrum_threshold;
     s_det.assign(nsamps, 0.0F);
@@ -881,6 +892,7 @@ void Cleaner::build_flags() {
         for (SizeType t = 0; t < nsamps; ++t)
             if (row_std[t] < cut) {
                 drop[t] = 1;
+                if (g_mutation == kMutDropoutWidening) continue;
                 if (t > 0) drop[t - 1] = 1;
                 if (t + 1 < nsamps) drop[t + 1] = 1;
             }
@@ -930,7 +942,7 @@ void Cleaner::build_flags() {
         kchan_neg.assign(n, Segment{});
         auto accept = [&](const Run& r, Segment& out) {
             const auto len = (double)r.length();
-            if (len > 0.0 && len >= min_len &&
+            if (len > 0.0 && (len >= min_len || g_mutation == kMutKadaneDuration) &&
                 r.sum / std::sqrt(len) > cfg.kadane_channel_threshold)
                 out = {r.start, r.end};
         };
