@@ -84,7 +84,7 @@ def cpu_model() -> str:
         return ""
 
 
-def cpu_baseline(W, dev_gulp, stream, sample_rows, gulps=4):
+def cpu_baseline(W, host, sample_rows, gulps=4):
     """The CPU oracle (RFI restatement + FDMT/search rules) streaming
     consecutive `sample_rows`-row gulps of the same data on this host's
     cores: the first gulp (cold: its windows reaching before the stream are
@@ -96,8 +96,6 @@ def cpu_baseline(W, dev_gulp, stream, sample_rows, gulps=4):
     from frost_amd import abi
     if gulps * sample_rows > W.gulp:
         raise ValueError("cpu sample larger than the device gulp")
-    host = np.zeros(W.gulp_bytes(), np.uint8)
-    dev_gulp.to_host(host, stream)
     rows = host.reshape(W.gulp, W.nchans)
     lay = abi.BlockLayout(W.nchans, sample_rows, W.dtype, W.tsamp, W.f0, W.df)
     dcfg = abi.dedisp_config(W.nchans, W.tsamp, W.f0, W.df, W.dm_max, block_nsamps=sample_rows,
@@ -254,6 +252,148 @@ def rfi_transform_measure(W, gulp, stream, steps=2):
     return {"ms_per_gulp": res, "steps": steps}
 
 
+def serving_beam(nch, gulp=1 << 17, dm_max=2000.0):
+    """One beam of the serving workload: 1200-1500 MHz, 64 us, DM 0-2000, 11
+    planes, widths to 256, frost's default RFI."""
+    from frost_amd import abi
+    f0, df = 1500.0 - 150.0 / nch, -300.0 / nch
+    lay = abi.BlockLayout(nch, gulp, abi.UINT8, 64e-6, f0, df)
+    d = abi.dedisp_config(nch, 64e-6, f0, df, dm_max, block_nsamps=gulp, subband_depth=2)
+    return abi.rfi_config(), lay, 2.0, d, abi.boxcar_config(max_boxcar=256)
+
+
+def serving_measure(stream, seconds, nbeams=32, lanes=2, budget=200e9, seed=11):
+    """Live beams on one GPU: nbeams beams of 1024/2048/4096 channels, gulps
+    of 2^17 samples (8.39 s) arriving as independent Poisson streams at the
+    real-time rate times U(0.8, 1.2) per beam, run on `lanes` shared lanes.
+    After a third of the run four beams leave; at half four new ones join.
+    Each gulp's latency (submission to candidate rows on the host) is
+    reported as a fraction of its real-time duration."""
+    import threading
+    from frost_amd import abi, search, serving, workloads
+    rng = np.random.default_rng(seed)
+    shapes = [1024, 2048, 4096]
+    req = {n: serving.beam_requirements(*serving_beam(n)) for n in shapes}
+    arena = max(r[1] for r in req.values())
+    staging = max(r[2] for r in req.values())
+    srv = serving.Server(lanes=lanes, arena_bytes=arena, staging_bytes=staging,
+                         hbm_budget_bytes=int(budget))
+    gulp_s = (1 << 17) * 64e-6
+    # One device-resident gulp per shape (synthetic, seeded), reused by every
+    # beam of that shape: inputs stay in HBM, as in the main measurement.
+    gulps = {}
+    for n in shapes:
+        W = workloads.Workload(f"serve_{n}", n, 64e-6, 1500.0 - 150.0 / n, 300.0, 1 << 17, 2000.0,
+                               2, 256, npulses=2, seed=100 + n)
+        d = search.DeviceArray(W.gulp_bytes())
+        sc = W.synth_config(0)
+        abi.check(abi.lib().fa_synth_filterbank(C.byref(sc), d.ptr, W.gulp, stream.h))
+        gulps[n] = d
+    stream.synchronize()
+    ndm = {}
+    for n in shapes:
+        ndm[n] = int(search.make_fdmt_plan(serving_beam(n)[3]).summary()["ndm"])
+    t_setup = time.perf_counter()
+    beams = {}  # id -> (nch, rate)
+
+    def join(k):
+        n = shapes[k % 3]
+        b = srv.add_beam(*serving_beam(n))
+        beams[b] = (n, rng.uniform(0.8, 1.2) / gulp_s)
+        return b
+
+    ids = [join(k) for k in range(nbeams)]
+    setup_s = time.perf_counter() - t_setup
+    # Arrival schedule on the wall clock from t = 0.
+    events = []  # (time, kind, beam)
+    for b in ids:
+        t = rng.exponential(1.0 / beams[b][1])
+        while t < seconds:
+            events.append((t, "gulp", b))
+            t += rng.exponential(1.0 / beams[b][1])
+    leaving = ids[:4]
+    events = [e for e in events if not (e[2] in leaving and e[0] >= seconds / 3)]
+    events += [(seconds / 3, "leave", b) for b in leaving]
+    events += [(seconds / 2, "join", k) for k in range(nbeams, nbeams + 4)]
+    events.sort(key=lambda e: e[0])
+    done, lock = [], threading.Lock()
+    expected = [0]
+    finished = threading.Event()
+
+    def collector():
+        while True:
+            c = srv.wait(1000)
+            if c is None:
+                if finished.is_set():
+                    with lock:
+                        if len(done) >= expected[0]:
+                            return
+                continue
+            srv.release(c.ticket)
+            with lock:
+                done.append((c.beam, c.status, c.latency_s, c.queue_s, c.run_s, c.lane,
+                             c.ncandidates))
+
+    th = threading.Thread(target=collector, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    joined = []
+    for when, kind, b in events:
+        now = time.perf_counter() - t0
+        if when > now:
+            time.sleep(when - now)
+        if kind == "gulp":
+            n = beams[b][0]
+            srv.submit_device(b, gulps[n].ptr, 1 << 17)
+            expected[0] += 1
+        elif kind == "leave":
+            srv.remove_beam(b)
+        else:
+            nb = join(b)
+            joined.append(nb)
+            t = when + rng.exponential(1.0 / beams[nb][1])
+            extra = []
+            while t < seconds:
+                extra.append((t, "gulp", nb))
+                t += rng.exponential(1.0 / beams[nb][1])
+            events.extend(extra)
+            events.sort(key=lambda e: e[0])
+    finished.set()
+    th.join(timeout=600)
+    wall = time.perf_counter() - t0
+    ok = [d for d in done if d[1] == abi.FA_OK]
+    lat = np.array([d[2] for d in ok]) / gulp_s
+    run = np.array([d[4] for d in ok])
+    trials = sum(ndm[beams[d[0]][0]] * (1 << 17) for d in ok)
+    busy = float(run.sum()) / (wall * lanes)
+    st = srv.stats()
+    srv.close()
+    mean_run = {n: float(np.mean([d[4] for d in ok if beams[d[0]][0] == n] or [np.nan]))
+                for n in shapes}
+    log(f"serving: {len(ok)} gulps of {len(ids) + len(joined)} beams in {wall:.1f} s, "
+        f"p50 {np.percentile(lat, 50):.4f}, p99 {np.percentile(lat, 99):.4f} of real time, "
+        f"lane occupancy {busy:.2f}")
+    return {
+        "workload": f"{nbeams} live beams, 1024/2048/4096 ch, 64 us, gulps of 2^17 (8.39 s), "
+                    "DM 0-2000, 11 planes, widths to 256, default RFI; Poisson arrivals at the "
+                    "real-time rate x U(0.8, 1.2) per beam; 4 beams leave at 1/3 of the run, "
+                    "4 join at 1/2",
+        "lanes": lanes, "hbm_budget_bytes": budget, "arena_bytes_per_lane": arena,
+        "staging_bytes_per_lane": staging, "resident_bytes_end": st.resident_bytes,
+        "beam_setup_seconds": setup_s, "run_seconds": wall,
+        "gulps_completed": len(ok), "gulps_failed": len(done) - len(ok),
+        "latency_fraction_of_realtime": {
+            "p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+            "p999": float(np.percentile(lat, 99.9)), "max": float(lat.max())},
+        "mean_device_seconds_per_gulp": mean_run,
+        "throughput_dm_trial_samples_per_s": trials / wall,
+        "lane_occupancy": busy,
+        "realtime_beams_at_full_occupancy": float(
+            lanes * gulp_s / np.mean(run)) if len(run) else None,
+        "data": "one synthetic device-resident gulp per shape, reused (inputs in HBM)",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +402,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=131072,
                     help="rows per CPU-oracle gulp; 4 such gulps are streamed (0 = skip)")
     ap.add_argument("--workload", default="4096ch_64us_gulp1M_dm2000_11planes")
+    ap.add_argument("--serve-seconds", type=float, default=30.0,
+                    help="wall seconds of the 32-beam serving measurement (0 = skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate HBM counters")
     a = ap.parse_args()
@@ -443,9 +585,17 @@ def main() -> int:
             },
         },
     }
+    host_gulp = None
+    if a.cpu_sample > 0:
+        host_gulp = np.zeros(W.gulp_bytes(), np.uint8)
+        gulp.to_host(host_gulp, stream)
+        stream.synchronize()
+    if a.serve_seconds > 0:
+        del beam, gulp  # free the single beam's buffers before the served beams
+        out["serving"] = serving_measure(stream, a.serve_seconds)
     if a.cpu_sample > 0:
         log(f"CPU baseline on {a.cpu_sample} rows ...")
-        out["cpu_baseline"] = cpu_baseline(W, gulp, stream, a.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(W, host_gulp, a.cpu_sample)
         out["cpu_baseline"]["postprocess"] = postprocess_cpu(W, rows_arr, rows_n, read, frontier)
     else:
         out["cpu_baseline"] = None

@@ -273,6 +273,29 @@ class MemoryCheck(C.Structure):
                 ("device_memory", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    """fa_server_config."""
+    _fields_ = [("device", C.c_int32), ("lanes", C.c_uint32), ("arena_bytes", C.c_uint64),
+                ("staging_bytes", C.c_uint64), ("hbm_budget_bytes", C.c_uint64)]
+
+
+class ServerCompletion(C.Structure):
+    """fa_server_completion."""
+    _fields_ = [("ticket", C.c_uint64), ("beam", C.c_uint32), ("status", C.c_int32),
+                ("gulp_index", C.c_uint64), ("nsamps", C.c_uint64),
+                ("ncandidates", C.c_uint64), ("dropped_global", C.c_uint64),
+                ("dropped_per_dm", C.c_uint64), ("frontier", C.c_uint64),
+                ("latency_s", C.c_double), ("queue_s", C.c_double), ("run_s", C.c_double),
+                ("lane", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    """fa_server_stats."""
+    _fields_ = [("beams", C.c_uint64), ("queued", C.c_uint64), ("completed", C.c_uint64),
+                ("budget_bytes", C.c_uint64), ("lane_bytes", C.c_uint64),
+                ("resident_bytes", C.c_uint64), ("lanes", C.c_uint32), ("pad_", C.c_uint32)]
+
+
 class SearchConfig(C.Structure):
     _fields_ = [
         ("boxcar_max", C.c_uint64), ("max_candidates", C.c_uint64),
@@ -365,6 +388,21 @@ PU8 = C.POINTER(C.c_uint8)
 
 EXPORTS = {
     "fa_last_error": (C.c_char_p, []),
+    "fa_server_beam_requirements": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32,
+                                              C.POINTER(DedispConfig), C.POINTER(BoxcarConfig),
+                                              U64, U64, PU64, PU64, PU64, PU64]),
+    "fa_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(P)]),
+    "fa_server_destroy": (C.c_int, [P]),
+    "fa_server_add_beam": (C.c_int, [P, C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32,
+                                     C.POINTER(DedispConfig), C.POINTER(BoxcarConfig), PU32]),
+    "fa_server_remove_beam": (C.c_int, [P, U32, C.POINTER(RawCandidate), U64, PU64]),
+    "fa_server_submit": (C.c_int, [P, U32, P, U64, C.c_int, PU64]),
+    "fa_server_wait": (C.c_int, [P, C.c_int64, C.POINTER(ServerCompletion),
+                                 C.POINTER(C.c_int)]),
+    "fa_server_rows": (C.c_int, [P, U64, C.POINTER(RawCandidate), U64, PU64]),
+    "fa_server_message": (C.c_int, [P, U64, C.c_char_p, U64]),
+    "fa_server_release": (C.c_int, [P, U64]),
+    "fa_server_get_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
     "fa_rfi_config_default": (None, [C.POINTER(RfiConfig)]),
     "fa_boxcar_config_default": (None, [C.POINTER(BoxcarConfig)]),
     "fa_device_count": (C.c_int, [C.POINTER(C.c_int)]),

@@ -1261,7 +1261,7 @@ uint64_t row_span(int32_t h, uint64_t tile) {
 // ---- Engine ------------------------------------------------------------------
 
 Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t* kill_mask,
-               int device, uint64_t tile)
+               int device, uint64_t tile, uint64_t arena_bytes)
     : plan_(plan), cfg_(cfg), device_(device) {
     require(cfg.max_boxcar >= 1, "max_boxcar must be >= 1");
     require(cfg.max_boxcar <= kMaxBox, "max_boxcar must be in [1, 4096]");
@@ -1343,7 +1343,29 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
 
     DeviceGuard dg(device_);
     build_programs();
-    if (!tile_req && !sizing()) {
+    shared_ = arena_bytes > 0;
+    if (shared_) {
+        // The largest power-of-two tile (up to kTileMax and the capacity)
+        // whose input and level buffers, laid out as in the arena, fit it.
+        // The stored levels are re-planned for the tile picked, so its need
+        // is checked again after the programs are rebuilt.
+        uint64_t pick = 0;
+        for (uint64_t t = kTileMax; t >= kTileMin && pick == 0; t >>= 1) {
+            const uint64_t tt = std::min<uint64_t>(t, cap_);
+            if (tile_req && tt != tile_) continue;
+            if (layout_need(tt) > arena_bytes) continue;
+            if (tt != tile_) {
+                tile_ = tt;
+                build_programs();
+            }
+            if (layout_need(tile_) <= arena_bytes) pick = tile_;
+        }
+        if (pick == 0)
+            throw OutOfMemory("the FDMT input and level buffers need " +
+                              std::to_string(layout_need(std::min<uint64_t>(cap_, kTileMin))) +
+                              " bytes at the smallest time tile, more than the " +
+                              std::to_string(arena_bytes) + "-byte shared arena");
+    } else if (!tile_req && !sizing()) {
         // Automatic tile: the largest power of two up to kTileMax (and the
         // gulp capacity) whose input and two level buffers fit in 75 % of
         // the device memory free now. Fewer, longer tiles recompute less
@@ -1351,17 +1373,10 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
         size_t free_b = 0, total_b = 0;
         FA_HIP(hipMemGetInfo(&free_b, &total_b));
         const uint64_t budget = free_b / 4 * 3;
-        const uint64_t xbytes = nch * round64(hist_in_ + cap_) * sizeof(float);
         uint64_t pick         = std::min<uint64_t>(cap_, kTileMin);
         for (uint64_t t = kTileMax; t >= kTileMin; t >>= 1) {
             const uint64_t tt = std::min<uint64_t>(t, cap_);
-            uint64_t lmax     = 0;
-            for (uint32_t l = 0; l < plan_.nlevels; ++l) {
-                uint64_t len = 0;
-                for (const int32_t h : hist_[l]) len += row_span(h, tt);
-                lmax = std::max(lmax, len);
-            }
-            if (xbytes + 2 * lmax * sizeof(float) <= budget) {
+            if (buffers_bound(tt) <= budget) {
                 pick = tt;
                 break;
             }
@@ -1372,17 +1387,32 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
         }
     }
 
-    // Extended input, zero history.
+    // Extended input (zero history) and level buffers: owned, or laid out
+    // in the shared arena as [x | level buffer 0 | level buffer 1].
     xstride_ = round64(hist_in_ + cap_);
-    x_.resize(nch * xstride_);
-    if (!sizing()) FA_HIP(hipMemset(x_.get(), 0, x_.bytes()));
     uint64_t len0 = 0, len1 = 0;
     for (uint32_t l = 0; l < plan_.nlevels; ++l) {
         if (level_buf_[l] == 0) len0 = std::max(len0, level_len_[l]);
         if (level_buf_[l] == 1) len1 = std::max(len1, level_len_[l]);
     }
-    buf_[0].resize(len0);
-    buf_[1].resize(std::max<uint64_t>(len1, 1));
+    len1 = std::max<uint64_t>(len1, 1);
+    if (shared_) {
+        auto up256 = [](uint64_t b) { return (b + 255) & ~uint64_t{255}; };
+        buf_off_[0] = up256(nch * xstride_ * sizeof(float));
+        buf_off_[1] = up256(buf_off_[0] + len0 * sizeof(float));
+        arena_need_ = buf_off_[1] + len1 * sizeof(float);
+        hist_store_.resize(std::max<uint64_t>(nch * hist_in_, 1));
+        if (!sizing()) FA_HIP(hipMemset(hist_store_.get(), 0, hist_store_.bytes()));
+    } else {
+        x_.resize(nch * xstride_);
+        if (!sizing()) FA_HIP(hipMemset(x_.get(), 0, x_.bytes()));
+        buf_[0].resize(len0);
+        buf_[1].resize(len1);
+        x_ptr_      = x_.get();
+        buf_ptr_[0] = buf_[0].get();
+        buf_ptr_[1] = buf_[1].get();
+        arena_need_ = x_.bytes() + buf_[0].bytes() + buf_[1].bytes();
+    }
     const uint64_t nw = widths_.size();
     inv_cur_.resize((uint64_t)nprows_ * nw);
     inv_or_.resize((uint64_t)nprows_ * nw);
@@ -1408,6 +1438,46 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
             "raise wtsp or lower max_boxcar");
     FA_HIP(hipEventCreate(&ev_begin_));
     FA_HIP(hipEventCreate(&ev_end_));
+}
+
+uint64_t Engine::buffers_bound(uint64_t tile) const {
+    // x plus two level buffers each as long as the longest level, with the
+    // arena's alignment slack.
+    const uint64_t nch    = plan_.cfg.nchans;
+    const uint64_t xbytes = nch * round64(hist_in_ + cap_) * sizeof(float);
+    uint64_t lmax         = 0;
+    for (uint32_t l = 0; l < plan_.nlevels; ++l) {
+        uint64_t len = 0;
+        for (const int32_t h : hist_[l]) len += row_span(h, tile);
+        lmax = std::max(lmax, len);
+    }
+    return xbytes + 2 * lmax * sizeof(float) + 512;
+}
+
+uint64_t Engine::layout_need(uint64_t tile) const {
+    // [x | level buffer 0 | level buffer 1], each buffer on a 256-byte
+    // boundary, with the levels stored in each buffer as planned now.
+    auto up256        = [](uint64_t b) { return (b + 255) & ~uint64_t{255}; };
+    const uint64_t nch = plan_.cfg.nchans;
+    uint64_t len[2]    = {0, 1};
+    for (uint32_t l = 0; l < plan_.nlevels; ++l) {
+        if (level_buf_[l] < 0) continue;
+        uint64_t n = 0;
+        for (const int32_t h : hist_[l]) n += row_span(h, tile);
+        len[level_buf_[l]] = std::max(len[level_buf_[l]], n);
+    }
+    return up256(up256(nch * round64(hist_in_ + cap_) * sizeof(float)) + len[0] * sizeof(float)) +
+           len[1] * sizeof(float);
+}
+
+void Engine::bind_arena(void* arena) {
+    require(shared_, "bind_arena: the engine owns its buffers");
+    require(arena != nullptr, "bind_arena: null arena");
+    require(!pending_collect_, "bind_arena: the last gulp has not been collected");
+    auto* base  = static_cast<unsigned char*>(arena);
+    x_ptr_      = reinterpret_cast<float*>(base);
+    buf_ptr_[0] = reinterpret_cast<float*>(base + buf_off_[0]);
+    buf_ptr_[1] = reinterpret_cast<float*>(base + buf_off_[1]);
 }
 
 Engine::~Engine() {
@@ -1928,6 +1998,7 @@ void Engine::execute(const float* wf, uint64_t stride, uint64_t offset, uint64_t
     require(stride >= offset + nsamps, "waterfall stride too small");
     require(!pending_collect_,
             "the previous gulp's candidates were not collected (call copy_result first)");
+    require(x_ptr_ != nullptr, "no shared arena is bound to the engine");
     DeviceGuard dg(device_);
     FA_HIP(hipMemcpy2DAsync(input_columns(), xstride_ * 4, wf + offset, stride * 4, nsamps * 4,
                             plan_.cfg.nchans, hipMemcpyDeviceToDevice, s));
@@ -1963,6 +2034,12 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
     if (keep_planes_) {
         keep_.reserve(static_cast<uint64_t>(nprows_) * cap_);
     }
+    if (shared_) {
+        require(x_ptr_ != nullptr, "no shared arena is bound to the engine");
+        if (hist_in_ > 0)
+            FA_HIP(hipMemcpy2DAsync(x_ptr_, xstride_ * 4, hist_store_.get(), hist_in_ * 4,
+                                    hist_in_ * 4, nch, hipMemcpyDeviceToDevice, s));
+    }
     // Open runs before this gulp: a re-run after a candidate overflow starts
     // from them.
     FA_HIP(hipMemcpyAsync(open_saved_.get(), open_.get(), open_.bytes(), hipMemcpyDeviceToDevice,
@@ -1981,7 +2058,7 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
         const int tl = static_cast<int>(std::min<uint64_t>(tile_, N - t0));
         for (uint32_t l = 0; l <= L; ++l) {
             if (level_source_[l] == -2) continue; // computed inside a fused launch
-            float* out = buf_[level_buf_[l]].get();
+            float* out = buf_ptr_[level_buf_[l]];
             if (hipEvent_t& r = reading[level_buf_[l]]) {
                 FA_HIP(hipStreamWaitEvent(s, r, 0)); // searches of the old contents done
                 r = nullptr;
@@ -1994,17 +2071,17 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                 const unsigned gy = static_cast<unsigned>(std::min<uint64_t>(nch, 65535));
                 const size_t lds  = (kBlock * kInitPer + dmax0_) * sizeof(float);
                 fdmt_init_kernel<<<dim3(gx, gy), kBlock, lds, s>>>(
-                    x_.get(), xstride_, static_cast<int64_t>(hist_in_ + t0), init_chan_.get(),
+                    x_ptr_, xstride_, static_cast<int64_t>(hist_in_ + t0), init_chan_.get(),
                     init_rows_.get(), out, tl, static_cast<int>(nch));
             } else {
                 const MergeLaunch* ml = nullptr;
                 for (const auto& mlc : launches_)
                     if (mlc.out_level == l) ml = &mlc;
                 if (!ml) throw std::logic_error("FDMT level without a launch");
-                const float* in = ml->from_input ? nullptr : buf_[ml->in_buf].get();
+                const float* in = ml->from_input ? nullptr : buf_ptr_[ml->in_buf];
                 // Input samples of this tile: columns hist_in_ + t0 + tau,
                 // tau in [xlo, xhi] inside the row.
-                const float* x0 = x_.get() + hist_in_ + t0;
+                const float* x0 = x_ptr_ + hist_in_ + t0;
                 const int xlo   = -static_cast<int>(hist_in_ + t0);
                 const int xhi   = static_cast<int>(xstride_) - static_cast<int>(hist_in_ + t0) - 1;
                 if (staged_merge_) {
@@ -2109,17 +2186,24 @@ void Engine::launch_search(uint32_t l, const float* buf, int tl, uint64_t t0, bo
 void Engine::carry_history(hipStream_t s) {
     const uint64_t nch = plan_.cfg.nchans;
     const uint64_t N   = last_n_;
+    if (hist_in_ == 0) return;
+    if (shared_) {
+        // Columns [N, N + H) of the extended input are the next history
+        // (the old history's tail first when N < H).
+        FA_HIP(hipMemcpy2DAsync(hist_store_.get(), hist_in_ * 4, x_ptr_ + N, xstride_ * 4,
+                                hist_in_ * 4, nch, hipMemcpyDeviceToDevice, s));
+        return;
+    }
     if (N >= hist_in_) {
-        if (hist_in_ > 0)
-            FA_HIP(hipMemcpy2DAsync(x_.get(), xstride_ * 4, x_.get() + N, xstride_ * 4,
-                                    hist_in_ * 4, nch, hipMemcpyDeviceToDevice, s));
+        FA_HIP(hipMemcpy2DAsync(x_ptr_, xstride_ * 4, x_ptr_ + N, xstride_ * 4, hist_in_ * 4, nch,
+                                hipMemcpyDeviceToDevice, s));
     } else {
         scratch_.reserve(nch * hist_in_);
         const unsigned gy = static_cast<unsigned>(std::min<uint64_t>(nch, 65535));
         copy_cols_kernel<<<dim3(ceil_div(hist_in_, 256), gy), 256, 0, s>>>(
-            x_.get() + N, xstride_, scratch_.get(), hist_in_, hist_in_, nch);
+            x_ptr_ + N, xstride_, scratch_.get(), hist_in_, hist_in_, nch);
         copy_cols_kernel<<<dim3(ceil_div(hist_in_, 256), gy), 256, 0, s>>>(
-            scratch_.get(), hist_in_, x_.get(), xstride_, hist_in_, nch);
+            scratch_.get(), hist_in_, x_ptr_, xstride_, hist_in_, nch);
     }
     FA_HIP(hipGetLastError());
 }
@@ -2334,8 +2418,8 @@ uint64_t Engine::memory_bytes() const {
         tables += ml.prog.bytes() + ml.blocks.bytes() + ml.leaves.bytes() + ml.chans.bytes() +
                   ml.rows.bytes();
     for (const auto& sr : search_rows_) tables += sr.bytes();
-    return x_.bytes() + buf_[0].bytes() + buf_[1].bytes() + inv_cur_.bytes() + inv_or_.bytes() +
-           open_.bytes() + open_saved_.bytes() + cands_.bytes() + keep_.bytes() + scratch_.bytes() +
+    return x_.bytes() + buf_[0].bytes() + buf_[1].bytes() + hist_store_.bytes() +
+           inv_cur_.bytes() + inv_or_.bytes() + open_.bytes() + open_saved_.bytes() + cands_.bytes() + keep_.bytes() + scratch_.bytes() +
            init_chan_.bytes() + init_rows_.bytes() + widths_dev_.bytes() + counters_.bytes() +
            tables;
 }

@@ -123,14 +123,25 @@ struct SearchRow {
 
 class Engine {
 public:
+    // arena_bytes > 0: the extended input and the level buffers live in a
+    // device arena of that size shared with other engines (bind_arena before
+    // each gulp); the engine keeps only its carried input history, open runs,
+    // noise tables and candidate buffer, and picks the largest time tile
+    // whose buffers fit the arena.
     Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t* kill_mask,
-           int device, uint64_t tile = 0);
+           int device, uint64_t tile = 0, uint64_t arena_bytes = 0);
     ~Engine();
     Engine(const Engine&)            = delete;
     Engine& operator=(const Engine&) = delete;
 
     // Device columns where a gulp's samples go: x + H, row stride xstride().
-    float* input_columns() const { return x_.get() + hist_in_; }
+    float* input_columns() const { return x_ptr_ + hist_in_; }
+    // Shared-arena engines: the arena the next gulp runs in (the same
+    // arena must stay bound from the input write to the collect).
+    void bind_arena(void* arena);
+    bool shared() const { return shared_; }
+    // Bytes of the extended input and level buffers at the chosen tile.
+    uint64_t arena_need() const { return arena_need_; }
     uint64_t xstride() const { return xstride_; }
     uint64_t capacity() const { return cap_; }
 
@@ -220,6 +231,14 @@ private:
 
     DeviceBuffer<float> x_;
     DeviceBuffer<float> buf_[2];
+    // The buffers in use: owned above, or parts of a bound arena.
+    float* x_ptr_      = nullptr;
+    float* buf_ptr_[2] = {nullptr, nullptr};
+    bool shared_       = false;
+    uint64_t arena_need_ = 0, buf_off_[2] = {0, 0};
+    DeviceBuffer<float> hist_store_; // shared arena: the carried history [nch][H]
+    uint64_t buffers_bound(uint64_t tile) const; // x + two level buffers, upper bound
+    uint64_t layout_need(uint64_t tile) const;   // arena bytes at a tile, as planned now
     DeviceBuffer<float> inv_cur_, inv_or_;
     DeviceBuffer<uint32_t> widths_dev_;
     DeviceBuffer<OpenRun> open_, open_saved_; // runs now / before the last gulp

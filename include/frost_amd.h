@@ -537,6 +537,93 @@ int fa_check_memory(const fa_rfi_config* rfi_cfg, fa_block_layout* layout, float
                     int gulp_is_default, uint64_t available_bytes, uint64_t memory_limit_mib,
                     int device, fa_memory_check* out);
 
+/* ---- many beams on one GPU: the serving loop ------------------------------
+ *
+ * No frost counterpart as a whole: frost runs one beam per process
+ * (docs/roadmap.md:178-180) with an I/O prefetch thread (GulpPrefetch,
+ * lib/pipeline/pipeline.cpp:109-262) and admits it with check_memory
+ * (lib/pipeline/memory.cpp:294-508). A server keeps many beams resident on
+ * one device: each beam holds only its FDMT input history, open runs, noise
+ * tables, RFI calibration and candidate buffer; a gulp's extended input and
+ * FDMT level buffers live in a lane's arena. Each lane (a worker thread with
+ * its own stream, arena and staging buffer) takes queued gulps in arrival
+ * order, one gulp per beam at a time and in order, so lanes overlap gulps of
+ * different beams. A beam is admitted when all lanes plus every admitted
+ * beam's resident bytes, its own included, fit the HBM budget. */
+
+typedef struct fa_server_config {
+    int32_t device;
+    uint32_t lanes;            /* 1..16 concurrent gulps */
+    uint64_t arena_bytes;      /* FDMT input + level buffers of one gulp, per lane */
+    uint64_t staging_bytes;    /* raw gulp bytes of one gulp, per lane */
+    uint64_t hbm_budget_bytes; /* 0: 90 % of the device's free memory at creation */
+} fa_server_config;
+
+typedef struct fa_server_completion {
+    uint64_t ticket;
+    uint32_t beam;
+    int32_t status;      /* FA_OK or the error code of the gulp (fa_server_message) */
+    uint64_t gulp_index; /* 0-based count of the beam's submitted gulps */
+    uint64_t nsamps;
+    uint64_t ncandidates;
+    uint64_t dropped_global;
+    uint64_t dropped_per_dm;
+    uint64_t frontier;
+    double latency_s; /* submission to candidate rows on the host */
+    double queue_s;   /* submission to the start on a lane */
+    double run_s;     /* start on a lane to rows on the host */
+    uint32_t lane;
+    uint32_t pad_;
+} fa_server_completion;
+
+typedef struct fa_server_stats {
+    uint64_t beams;
+    uint64_t queued;
+    uint64_t completed;
+    uint64_t budget_bytes;
+    uint64_t lane_bytes;     /* lanes x (arena + staging) */
+    uint64_t resident_bytes; /* admitted beams' resident state */
+    uint32_t lanes;
+    uint32_t pad_;
+} fa_server_stats;
+
+typedef struct fa_server fa_server;
+
+/* Device bytes a beam keeps resident on a server whose lanes have
+ * arena_bytes of arena (0: unlimited), the arena bytes its gulps use and
+ * the time tile they run at (tile 0: the largest that fits), and the
+ * staging bytes of one raw gulp. No device is needed. */
+int fa_server_beam_requirements(const fa_rfi_config* rfi_cfg, const fa_block_layout* layout,
+                                float baseline_length, const fa_dedisp_config* dedisp,
+                                const fa_boxcar_config* boxcar, uint64_t arena_bytes,
+                                uint64_t tile, uint64_t* resident_bytes, uint64_t* arena_need,
+                                uint64_t* staging_bytes, uint64_t* tile_used);
+int fa_server_create(const fa_server_config* cfg, fa_server** server);
+/* Stops the lanes after their current gulps; queued gulps are dropped. */
+int fa_server_destroy(fa_server* server);
+/* Pipeline::Impl construction for one beam, admitted against the budget
+ * (FA_OUT_OF_MEMORY when it does not fit). */
+int fa_server_add_beam(fa_server* server, const fa_rfi_config* rfi_cfg,
+                       const fa_block_layout* layout, float baseline_length,
+                       const fa_dedisp_config* dedisp, const fa_boxcar_config* boxcar,
+                       uint32_t* beam);
+/* Waits for the beam's queued gulps, drains its open runs (FDMTBoxcar::drain,
+ * rows into out, *n of them) and releases its memory. */
+int fa_server_remove_beam(fa_server* server, uint32_t beam, fa_raw_candidate* out, uint64_t cap,
+                          uint64_t* n);
+/* Queues one gulp of nsamps rows (execute_block). The memory at gulp must
+ * stay valid and unchanged until the gulp's completion is returned. */
+int fa_server_submit(fa_server* server, uint32_t beam, const void* gulp, uint64_t nsamps,
+                     int is_device, uint64_t* ticket);
+/* The oldest completion not returned yet: *got = 1, or 0 after timeout_ms
+ * (negative: no limit). Its rows stay available until fa_server_release. */
+int fa_server_wait(fa_server* server, int64_t timeout_ms, fa_server_completion* out, int* got);
+int fa_server_rows(fa_server* server, uint64_t ticket, fa_raw_candidate* out, uint64_t cap,
+                   uint64_t* n);
+int fa_server_message(fa_server* server, uint64_t ticket, char* out, uint64_t cap);
+int fa_server_release(fa_server* server, uint64_t ticket);
+int fa_server_get_stats(fa_server* server, fa_server_stats* out);
+
 /* ---- candidate post-processing (frost::search) -------------------------- */
 
 /* Giant (include/frost/search/candidate.hpp:24-64). The loc_* fields hold

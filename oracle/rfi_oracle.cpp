@@ -287,6 +287,7 @@ void measure_bandpass_rows(const float* rows, SizeType m, SizeType n,
         while (count > 1) {
             const SizeType nc = count / 5;
             next.assign(nc * n, 0.0F);
+#pragma omp parallel for schedule(static)
             for (SizeType o = 0; o < nc; ++o)
                 for (SizeType c = 0; c < n; ++c) {
                     const float* r = cur.data() + 5 * o * n + c;
@@ -300,6 +301,7 @@ void measure_bandpass_rows(const float* rows, SizeType m, SizeType n,
     };
     const auto bp = remedian(rows, m);
     std::vector<float> ad(m * n);
+#pragma omp parallel for schedule(static)
     for (SizeType i = 0; i < m; ++i)
         for (SizeType c = 0; c < n; ++c)
             ad[i * n + c] = std::abs(rows[i * n + c] - bp[c]);
@@ -598,6 +600,7 @@ void Cleaner::measure_bandpass() {
                         n * sizeof(float));
         measure_bandpass_rows(rows.data(), m, n, loc.data() + w * n,
                               scale.data() + w * n);
+#pragma omp parallel for schedule(static)
         for (SizeType c = 0; c < n; ++c) {
             if (refine_bandpass(rows.data() + c, m, n, loc[w * n + c],
                                 scale[w * n + c]))
@@ -614,6 +617,7 @@ void Cleaner::measure_moments() {
     moments.assign(nwin * n, MomentSums{});
     for (SizeType w = 0; w < nwin; ++w) {
         const SizeType b = window_begin(w), e = window_end(w);
+#pragma omp parallel for schedule(dynamic, 16)
         for (SizeType c = 0; c < n; ++c) {
             const float lc = loc[w * n + c], ic = inv_scale[w * n + c];
             if (static_mask[c] == 0 || ic == 0.0F) continue;
@@ -791,7 +795,10 @@ void Cleaner::scan_rows() {
         }
         return false;
     };
+#pragma omp parallel
+    {
     std::vector<float> z(n);
+#pragma omp for schedule(dynamic)
     for (SizeType k = 0; k < nchunks; ++k) {
         const SizeType t0 = k * chunk, t1 = std::min(t0 + chunk, nsamps);
         for (SizeType t = t0; t < t1; ++t) {
@@ -858,6 +865,7 @@ void Cleaner::scan_rows() {
             }
         }
     }
+    } // omp parallel
 }
 
 void Cleaner::build_flags() {
@@ -1003,8 +1011,11 @@ void Cleaner::apply() {
     const bool kchan = cfg.kadane_channel_enabled, kspec = cfg.kadane_spectrum_enabled;
     const bool nbr   = cfg.fill == FA_FILL_NEIGHBOUR;
     uint64_t cn = 0, cks = 0, ckc = 0, fb = 0;
+#pragma omp parallel reduction(+ : cn, cks, ckc, fb)
+    {
     std::vector<float> z(n), v(n);
     std::vector<uint8_t> state(n);
+#pragma omp for schedule(static)
     for (SizeType t = 0; t < nsamps; ++t) {
         if (time_flags[t]) continue;
         const SizeType w = window_of(t);
@@ -1063,13 +1074,19 @@ void Cleaner::apply() {
             out[c] = level(w, c, value);
         }
     }
+    } // omp parallel
     std::vector<SizeType> flagged;
     for (SizeType t = 0; t < nsamps; ++t)
         if (time_flags[t]) flagged.push_back(t);
     // With zero-DM on, a refilled spectrum is rebuilt in noise units and its
     // band mean removed (lib/cpu/rfi_cpu.cpp:631-712).
     const bool rezero = cfg.zero_dm != FA_ZERO_DM_OFF;
-    for (const SizeType t : flagged) {
+#pragma omp parallel reduction(+ : fb)
+    {
+    std::vector<float> v(n);
+#pragma omp for schedule(static)
+    for (std::size_t ft = 0; ft < flagged.size(); ++ft) {
+        const SizeType t  = flagged[ft];
         const SizeType w  = window_of(t);
         const SizeType r  = time_fill_radius;
         const SizeType lo = t > r ? t - r : 0;
@@ -1114,6 +1131,7 @@ void Cleaner::apply() {
             out[c] = level(w, c, v[c] - mean);
         }
     }
+    } // omp parallel
     report.cells_narrow          = cn;
     report.cells_kadane_spectrum = cks;
     report.cells_kadane_channel  = ckc;
@@ -1181,6 +1199,7 @@ void Cleaner::execute_normalized(const uint8_t* block, SizeType capacity,
         }
     }
     work.resize(valid * nchans);
+#pragma omp parallel for schedule(static)
     for (SizeType t = 0; t < valid; ++t)
         for (SizeType c = 0; c < nchans; ++c)
             work[t * nchans + c] = unpack_sample(block + t * row_bytes, c, layout.dtype);
@@ -1204,6 +1223,7 @@ void Cleaner::execute_normalized(const uint8_t* block, SizeType capacity,
     if (short_block || transform) {
         // Welford per (window, channel) over the cleaned (transformed) rows
         // (lib/cpu/rfi_cpu.cpp:217-239).
+#pragma omp parallel for schedule(static)
         for (SizeType i = 0; i < nwin * nchans; ++i) {
             const SizeType w = i / nchans, c = i % nchans;
             const SizeType b = window_begin(w), e = window_end(w);
@@ -1223,6 +1243,7 @@ void Cleaner::execute_normalized(const uint8_t* block, SizeType capacity,
         tail_out_scale.assign(scale.begin() + last, scale.end());
     }
     std::fill(out, out + capacity * nchans, 0.0F);
+#pragma omp parallel for schedule(static)
     for (SizeType c = 0; c < nchans; ++c) {
         if (mask[c] == 0) continue;
         const SizeType dst = layout.df < 0.0 ? nchans - 1 - c : c;
