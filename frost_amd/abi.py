@@ -296,6 +296,15 @@ class ServerStats(C.Structure):
                 ("resident_bytes", C.c_uint64), ("lanes", C.c_uint32), ("pad_", C.c_uint32)]
 
 
+class EngineTuning(C.Structure):
+    """fa_engine_tuning."""
+    _fields_ = [("tile", C.c_uint64), ("gather_chunks", C.c_uint32), ("search_kernel", C.c_uint32)]
+
+    def as_dict(self):
+        return {"tile": self.tile, "gather_chunks": self.gather_chunks,
+                "search_kernel": "register" if self.search_kernel in (0, 1) else "ring"}
+
+
 class SearchConfig(C.Structure):
     _fields_ = [
         ("boxcar_max", C.c_uint64), ("max_candidates", C.c_uint64),
@@ -388,6 +397,18 @@ PU8 = C.POINTER(C.c_uint8)
 
 EXPORTS = {
     "fa_last_error": (C.c_char_p, []),
+    "fa_autotune_engine": (C.c_int, [C.POINTER(DedispConfig), C.POINTER(BoxcarConfig), U64,
+                                     C.c_int, U32, C.POINTER(EngineTuning), PF64,
+                                     C.POINTER(EngineTuning), PF64, PU32]),
+    "fa_batch_create": (C.c_int, [C.c_int, U32, U64, C.c_int, C.POINTER(P)]),
+    "fa_batch_destroy": (C.c_int, [P]),
+    "fa_batch_add_beam": (C.c_int, [P, C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32,
+                                    C.POINTER(DedispConfig), C.POINTER(BoxcarConfig),
+                                    C.POINTER(EngineTuning), PU32]),
+    "fa_batch_execute": (C.c_int, [P, C.POINTER(P), PU64]),
+    "fa_batch_result": (C.c_int, [P, U32, C.POINTER(RawCandidate), U64, PU64, PU64, PU64, PU64]),
+    "fa_batch_drain": (C.c_int, [P, U32]),
+    "fa_batch_beam_tuning": (C.c_int, [P, U32, C.POINTER(EngineTuning)]),
     "fa_server_beam_requirements": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32,
                                               C.POINTER(DedispConfig), C.POINTER(BoxcarConfig),
                                               U64, U64, PU64, PU64, PU64, PU64]),

@@ -28,7 +28,10 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 double event_seconds(hipEvent_t a, hipEvent_t b) {
     if (!a || !b) return 0.0;
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError(); // an event never recorded: no time, and no stale error
+        return 0.0;
+    }
     return static_cast<double>(ms) * 1e-3;
 }
 

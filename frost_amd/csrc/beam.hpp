@@ -29,7 +29,7 @@ public:
     // device arena of that size, bound with bind_arena before each gulp.
     Beam(const fa_rfi_config& rcfg, const fa_block_layout& layout, float baseline,
          const fa_dedisp_config& dcfg, const fa_boxcar_config& bcfg, int device,
-         uint64_t arena_bytes = 0)
+         uint64_t arena_bytes = 0, uint64_t tile = 0)
         : device_(device) {
         require(dcfg.nchans == layout.nchans, "dedisp nchans must match the block layout");
         require(dcfg.block_nsamps == layout.max_nsamps,
@@ -46,7 +46,7 @@ public:
         // (lib/pipeline/pipeline.cpp:352-357).
         require(dcfg.block_nsamps > static_cast<uint64_t>(plan.dt_max) + bcfg.max_boxcar,
                 "nsamps_gulp must exceed the dedispersion max_delay plus boxcar_max");
-        engine_ = std::make_unique<Engine>(plan, bcfg, fdmt.data(), device, 0, arena_bytes);
+        engine_ = std::make_unique<Engine>(plan, bcfg, fdmt.data(), device, tile, arena_bytes);
         DeviceGuard dg(device_);
         const uint64_t bits = layout.dtype == FA_UINT1 ? 1 : layout.dtype == FA_UINT2 ? 2
                               : layout.dtype == FA_UINT4 ? 4 : layout.dtype == FA_UINT8 ? 8
@@ -66,19 +66,23 @@ public:
             FA_HIP(hipMemcpyAsync(staging_.get(), gulp, nsamps * row_bytes_, hipMemcpyHostToDevice, s));
             dev = staging_.get();
         }
-        // Clean + calibrate into the engine's input columns; padding is zero.
-        rfi_->execute_normalized(dev, capacity_, nsamps, engine_->input_columns(),
-                                 engine_->xstride(), 0, s);
-        const auto& mask = rfi_->channel_mask();
-        const uint8_t* km = nullptr;
-        if (mask != static_mask_) {
-            fdmt_mask_ = mask;
-            if (descending_) std::reverse(fdmt_mask_.begin(), fdmt_mask_.end());
-            km = fdmt_mask_.data();
-        }
+        const uint8_t* km = clean(dev, nsamps, s);
         engine_->execute_prepared(capacity_, km, s);
         engine_->collect(s);
         samples_read_ += nsamps;
+    }
+
+    // Clean + calibrate a device gulp into the engine's input columns
+    // (padding zero); returns the gulp's FDMT kill mask, or null when it is
+    // the static one.
+    const uint8_t* clean(const uint8_t* dev, uint64_t nsamps, hipStream_t s) {
+        rfi_->execute_normalized(dev, capacity_, nsamps, engine_->input_columns(),
+                                 engine_->xstride(), 0, s);
+        const auto& mask = rfi_->channel_mask();
+        if (mask == static_mask_) return nullptr;
+        fdmt_mask_ = mask;
+        if (descending_) std::reverse(fdmt_mask_.begin(), fdmt_mask_.end());
+        return fdmt_mask_.data();
     }
 
     // Ingest with overlap (the counterpart of frost's GulpPrefetch,

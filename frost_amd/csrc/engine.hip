@@ -489,7 +489,7 @@ void launch_gather_u(const MergeLaunch& ml, const float* in, float* out, const G
         in, out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), ml.chans.get(), g);
 }
 
-int gather_chunks() {
+int default_gather_chunks() {
     static const int n = [] {
         const char* v = std::getenv("FROST_AMD_GATHER_CHUNKS");
         const int c   = v ? std::atoi(v) : 16;
@@ -502,14 +502,14 @@ int gather_chunks() {
 // held by the input buffer (used only by launches reading level 0 from x).
 template <int K>
 void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, const float* x0,
-                   int xlo, int xhi, hipStream_t s) {
+                   int xlo, int xhi, int chunks, hipStream_t s) {
     static_assert(kChunkG == 256, "four consumer waves cover one chunk");
     GatherDims g{};
     g.nblocks    = static_cast<int>(ml.nblocks);
     g.tile_len   = tl;
     g.tau_origin = -((ml.hmax + 63) & ~63); // chunks start on 64-sample boundaries
     g.half       = static_cast<int>(ml.lds_floats);
-    g.chunks     = gather_chunks();
+    g.chunks     = chunks;
     g.x          = x0;
     g.xlo        = xlo;
     g.xhi        = xhi;
@@ -579,7 +579,7 @@ struct SearchArgs {
     int wm1;
     int tile_len;
     int64_t tile_t0;   // gulp-local start of the tile
-    uint64_t gulp_abs; // absolute sample of gulp-local 0
+    const uint64_t* gulp_abs_p; // device: absolute sample of gulp-local 0
     float thr;
     int mode;
     int trap;
@@ -709,6 +709,7 @@ __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs 
     const uint32_t prow = row.prow;
     const int nw        = a.nw;
     const bool across   = a.mode == FA_PEAK_ACROSS_WIDTHS;
+    const uint64_t gabs = *a.gulp_abs_p;
     if (i < nw) {
         st[i]     = a.open[(uint64_t)prow * nw + i];
         s_inv[i]  = a.inv_cur[(uint64_t)prow * nw + i];
@@ -742,7 +743,7 @@ __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs 
         if (u >= 0) {
             const int clen  = min(kRingChunk, tl - u);
             const int64_t tg = a.tile_t0 + t; // gulp-local sample
-            const int64_t ab = (int64_t)a.gulp_abs + tg;
+            const int64_t ab = (int64_t)gabs + tg;
             // S/N of width w at this thread's sample; false when the
             // window is cold (before delay + W - 1 in absolute samples);
             // windows reaching into the previous gulp use the OR-mask noise.
@@ -781,7 +782,7 @@ __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs 
                 if (i == 0 && (flags || st[0].open)) {
                     OpenRun o = st[0];
                     for (int q = 0; q < clen; ++q) {
-                        const uint64_t abs_t = a.gulp_abs + (uint64_t)(a.tile_t0 + u + q);
+                        const uint64_t abs_t = gabs + (uint64_t)(a.tile_t0 + u + q);
                         const float sn       = s_best[q];
                         track_run(a, o, prow, s_arg[q], abs_t, sn > a.thr, sn);
                     }
@@ -794,7 +795,7 @@ __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs 
                 OpenRun o        = st[w];
                 for (int q = 0; q < clen; ++q) {
                     const int64_t tq  = a.tile_t0 + u + q;
-                    const int64_t abq = (int64_t)a.gulp_abs + tq;
+                    const int64_t abq = (int64_t)gabs + tq;
                     const uint64_t abs_t = (uint64_t)abq;
                     bool above       = false;
                     float sn         = 0.0f;
@@ -893,15 +894,15 @@ __device__ __forceinline__ float trap_back(const float (&D)[9][4], const float (
 
 // Serial run tracking of one width over one chunk (lane 0), the S/N values
 // staged in LDS. Returns the run's open flag afterwards.
-__device__ uint32_t walk_chunk(const SearchArgs& a, const float* snr_lds, const uint8_t* arg_lds,
-                               OpenRun* st, uint32_t prow, int w, int64_t tc0, int clen,
-                               int lane) {
+__device__ uint32_t walk_chunk(const SearchArgs& a, uint64_t gabs, const float* snr_lds,
+                               const uint8_t* arg_lds, OpenRun* st, uint32_t prow, int w,
+                               int64_t tc0, int clen, int lane) {
     // arg_lds: the width index per sample (across_widths), else null.
     if (lane == 0) {
         OpenRun o = *st;
         for (int i = 0; i < clen; ++i) {
             const float sn       = snr_lds[i];
-            const uint64_t abs_t = a.gulp_abs + (uint64_t)(tc0 + i);
+            const uint64_t abs_t = gabs + (uint64_t)(tc0 + i);
             const bool above     = sn > a.thr;
             if (a.mode == FA_PEAK_ALL) {
                 if (above) emit(a, prow, w, abs_t, abs_t, abs_t, sn);
@@ -928,6 +929,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
     const SearchRow row = a.rows[rowi];
     const uint32_t prow = row.prow;
     const int nw        = a.nw;
+    const uint64_t gabs = *a.gulp_abs_p;
     float inv_c[KMAX + 1], inv_o[KMAX + 1];
 #pragma unroll
     for (int k = 0; k <= KMAX; ++k) {
@@ -1069,7 +1071,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
                             sum = (sum - 0.5f * D[k - 2][j]) - 0.5f * e2;
                         }
                         const int64_t t = tc0 + 4 * lane + j;
-                        if (base + j >= tl || (int64_t)a.gulp_abs + t < warm) continue;
+                        if (base + j >= tl || (int64_t)gabs + t < warm) continue;
                         const float sn = sum * (t < warm ? inv_o[k] : inv_c[k]);
                         if (!any[j] || sn > best[j]) {
                             best[j] = sn;
@@ -1090,7 +1092,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
                     wave_sync();
                     const int clen = min(kRegChunk, tl - u);
                     const uint32_t op =
-                        walk_chunk(a, s_snr[wave], s_arg[wave], &s_st[wave][0], prow, 0, tc0, clen, lane);
+                        walk_chunk(a, gabs, s_snr[wave], s_arg[wave], &s_st[wave][0], prow, 0, tc0, clen,
+                                   lane);
                     open_mask = op ? 1U : 0U;
                 }
             }
@@ -1137,7 +1140,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
                     if (base + j < tl) {
                         if (t < (int64_t)row.delay + (int64_t)W - 1) {
                             // Cold until the absolute sample passes the warm-up.
-                            if ((int64_t)a.gulp_abs + t >= (int64_t)row.delay + (int64_t)W - 1)
+                            if ((int64_t)gabs + t >= (int64_t)row.delay + (int64_t)W - 1)
                                 sn = sums[j] * inv_o[k];
                         } else {
                             sn = sums[j] * inv_c[k];
@@ -1153,7 +1156,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
                 wave_sync();
                 const int clen = min(kRegChunk, tl - u);
                 const uint32_t op =
-                    walk_chunk(a, s_snr[wave], nullptr, &s_st[wave][w], prow, w, tc0, clen, lane);
+                    walk_chunk(a, gabs, s_snr[wave], nullptr, &s_st[wave][w], prow, w, tc0, clen, lane);
                 open_mask = op ? (open_mask | (1U << w)) : (open_mask & ~(1U << w));
             }
         }
@@ -1192,6 +1195,15 @@ __global__ void set_u64_kernel(unsigned long long* p, unsigned long long v0,
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         p[0] = v0;
         p[1] = v1;
+    }
+}
+
+// Candidate count 0 and frontier "the sample after this gulp", from the
+// gulp's start in device memory.
+__global__ void begin_gulp_kernel(unsigned long long* ctr, const uint64_t* params, uint64_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctr[0] = 0;
+        ctr[1] = params[0] + n;
     }
 }
 
@@ -1284,8 +1296,8 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     // 256-sample chunk); other widths use the LDS ring ladder.
     // FROST_AMD_SEARCH=ring forces the ring ladder.
     const char* sv = std::getenv("FROST_AMD_SEARCH");
-    reg_search_ = pow2_widths_ && widths_.back() <= 256 &&
-                  !(sv && (std::string(sv) == "ring" || std::string(sv) == "lds"));
+    reg_allowed_ = pow2_widths_ && widths_.back() <= 256;
+    reg_search_  = reg_allowed_ && !(sv && (std::string(sv) == "ring" || std::string(sv) == "lds"));
     {
         // Ring lengths: level b holds one chunk plus the furthest look-back
         // read from it (2^b by level b + 1; each set bit's offset inside a
@@ -1425,6 +1437,9 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     cands_.resize(cand_cap_);
     counters_.resize(2);
     if (!sizing()) FA_HIP(hipMemset(counters_.get(), 0, counters_.bytes()));
+    params_.resize(2);
+    if (!sizing()) FA_HIP(hipMemset(params_.get(), 0, params_.bytes()));
+    if (const char* gv = std::getenv("FROST_AMD_GRAPH")) graph_mode_ = std::string(gv) == "1";
     level_seconds.assign(plan_.nlevels, 0.0);
     level_launches.assign(plan_.nlevels, 0.0);
     if (sizing()) return;
@@ -1481,6 +1496,8 @@ void Engine::bind_arena(void* arena) {
 }
 
 Engine::~Engine() {
+    drop_graphs();
+    if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
     if (search_stream_) {
         (void)hipStreamSynchronize(search_stream_);
         (void)hipStreamDestroy(search_stream_);
@@ -1976,6 +1993,7 @@ hipEvent_t Engine::sync_event(hipStream_t s) {
 }
 
 void Engine::record_pair(int level, hipStream_t s, bool start) {
+    if (capturing_) return; // captured gulps are timed as a whole
     if (start) {
         while (events_.size() < events_used_ + 2) {
             hipEvent_t e;
@@ -2006,22 +2024,38 @@ void Engine::execute(const float* wf, uint64_t stride, uint64_t offset, uint64_t
 }
 
 void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t s) {
+    prepare_gulp(N, kill_mask, s);
+    DeviceGuard dg(device_);
+    FA_HIP(hipEventRecord(ev_begin_, s));
+    if (graph_mode_) {
+        FA_HIP(hipGraphLaunch(gulp_graph(N, s), s));
+    } else {
+        enqueue_gulp(N, s);
+    }
+    FA_HIP(hipEventRecord(ev_end_, s));
+    finish_gulp(N);
+}
+
+void Engine::prepare_gulp(uint64_t N, const uint8_t* kill_mask, hipStream_t s) {
     require(N >= 1 && N <= cap_, "nsamps must be in [1, block_nsamps]");
     require(s != nullptr, "an explicit non-null stream is required");
     require(!pending_collect_,
             "the previous gulp's candidates were not collected (call copy_result first)");
+    require(x_ptr_ != nullptr, "no shared arena is bound to the engine");
     DeviceGuard dg(device_);
     const uint64_t nch = plan_.cfg.nchans;
     const std::vector<uint8_t> m =
         kill_mask ? std::vector<uint8_t>(kill_mask, kill_mask + nch) : ctor_mask_;
-    const bool first   = processed_ == 0;
-    const std::size_t nw = widths_.size();
-    // Noise tables for this gulp's mask (and the OR mask across a change).
+    const bool first = processed_ == 0;
+    // Noise tables for this gulp's mask, and in inv_or_ the tables the
+    // windows reaching into the previous gulp use: those of the OR of the two
+    // masks after a change, else the current ones.
     if (inv_cur_host_.empty() || m != tables_mask_) {
         compute_inv(m, inv_cur_host_);
         tables_mask_ = m;
         FA_HIP(hipMemcpyAsync(inv_cur_.get(), inv_cur_host_.data(), inv_cur_host_.size() * 4,
                               hipMemcpyHostToDevice, s));
+        or_is_cur_ = false;
     }
     const bool use_or = !first && m != prev_mask_;
     if (use_or) {
@@ -2030,16 +2064,48 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
         compute_inv(mor, inv_or_host_);
         FA_HIP(hipMemcpyAsync(inv_or_.get(), inv_or_host_.data(), inv_or_host_.size() * 4,
                               hipMemcpyHostToDevice, s));
+        or_is_cur_ = false;
+    } else if (!or_is_cur_) {
+        FA_HIP(hipMemcpyAsync(inv_or_.get(), inv_cur_.get(), inv_cur_.bytes(),
+                              hipMemcpyDeviceToDevice, s));
+        or_is_cur_ = true;
     }
-    if (keep_planes_) {
+    if (keep_planes_ && keep_.size() < static_cast<uint64_t>(nprows_) * cap_) {
         keep_.reserve(static_cast<uint64_t>(nprows_) * cap_);
+        drop_graphs(); // the plane-row store moved
     }
-    if (shared_) {
-        require(x_ptr_ != nullptr, "no shared arena is bound to the engine");
-        if (hist_in_ > 0)
-            FA_HIP(hipMemcpy2DAsync(x_ptr_, xstride_ * 4, hist_store_.get(), hist_in_ * 4,
-                                    hist_in_ * 4, nch, hipMemcpyDeviceToDevice, s));
-    }
+    // The gulp's absolute start sample, read by the kernels from device
+    // memory (so a captured gulp replays for any gulp).
+    set_u64_kernel<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(params_.get()),
+                                    processed_, 0);
+    FA_HIP(hipGetLastError());
+    pending_mask_ = m;
+}
+
+void Engine::finish_gulp(uint64_t N) {
+    before_mask_ = prev_mask_;
+    last_mask_   = pending_mask_;
+    prev_mask_   = pending_mask_;
+    processed_ += N;
+    last_n_          = N;
+    pending_collect_ = true;
+}
+
+// The gulp's device work, in stream order on s (and the search stream), with
+// no host synchronisation, allocation or memset: it can be captured.
+void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
+    DeviceGuard dg(device_);
+    struct CaptureFlag {
+        bool& f;
+        bool prev;
+        ~CaptureFlag() { f = prev; }
+    } cf{capturing_, capturing_};
+    capturing_ = capturing_ || captured;
+    const uint64_t nch   = plan_.cfg.nchans;
+    const std::size_t nw = widths_.size();
+    if (shared_ && hist_in_ > 0)
+        FA_HIP(hipMemcpy2DAsync(x_ptr_, xstride_ * 4, hist_store_.get(), hist_in_ * 4,
+                                hist_in_ * 4, nch, hipMemcpyDeviceToDevice, s));
     // Open runs before this gulp: a re-run after a candidate overflow starts
     // from them.
     FA_HIP(hipMemcpyAsync(open_saved_.get(), open_.get(), open_.bytes(), hipMemcpyDeviceToDevice,
@@ -2050,9 +2116,9 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
     // last search of the call.
     hipEvent_t reading[2] = {nullptr, nullptr};
     hipEvent_t last_search = nullptr;
-    FA_HIP(hipEventRecord(ev_begin_, s));
-    set_u64_kernel<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(counters_.get()), 0,
-                                    processed_ + N);
+    const int gchunks      = gather_chunks();
+    begin_gulp_kernel<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(counters_.get()),
+                                       params_.get(), N);
     const uint32_t L = plan_.nlevels - 1;
     for (uint64_t t0 = 0; t0 < N; t0 += tile_) {
         const int tl = static_cast<int>(std::min<uint64_t>(tile_, N - t0));
@@ -2086,10 +2152,10 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                 const int xhi   = static_cast<int>(xstride_) - static_cast<int>(hist_in_ + t0) - 1;
                 if (staged_merge_) {
                     switch (ml->radix) {
-                    case 2: launch_gather<2>(*ml, in, out, tl, x0, xlo, xhi, s); break;
-                    case 4: launch_gather<4>(*ml, in, out, tl, x0, xlo, xhi, s); break;
-                    case 8: launch_gather<8>(*ml, in, out, tl, x0, xlo, xhi, s); break;
-                    case 16: launch_gather<16>(*ml, in, out, tl, x0, xlo, xhi, s); break;
+                    case 2: launch_gather<2>(*ml, in, out, tl, x0, xlo, xhi, gchunks, s); break;
+                    case 4: launch_gather<4>(*ml, in, out, tl, x0, xlo, xhi, gchunks, s); break;
+                    case 8: launch_gather<8>(*ml, in, out, tl, x0, xlo, xhi, gchunks, s); break;
+                    case 16: launch_gather<16>(*ml, in, out, tl, x0, xlo, xhi, gchunks, s); break;
                     default: throw std::logic_error("FDMT launch radix must be 2 to 16");
                     }
                 } else {
@@ -2108,7 +2174,7 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
                 // state across tiles needs).
                 FA_HIP(hipStreamWaitEvent(search_stream_, sync_event(s), 0));
                 record_pair(-1, search_stream_, true);
-                launch_search(l, out, tl, t0, use_or, search_stream_);
+                launch_search(l, out, tl, t0, search_stream_);
                 record_pair(-1, search_stream_, false);
                 last_search = sync_event(search_stream_);
                 reading[level_buf_[l]] = last_search;
@@ -2121,31 +2187,103 @@ void Engine::execute_prepared(uint64_t N, const uint8_t* kill_mask, hipStream_t 
     frontier_kernel<<<ceil_div(nopen, 256), 256, 0, s>>>(
         open_.get(), nopen, reinterpret_cast<unsigned long long*>(counters_.get()) + 1);
     FA_HIP(hipGetLastError());
-    FA_HIP(hipEventRecord(ev_end_, s));
-    before_mask_ = prev_mask_;
-    last_mask_   = m;
-    prev_mask_   = m;
-    processed_ += N;
-    last_n_          = N;
-    pending_collect_ = true;
 }
 
-void Engine::launch_search(uint32_t l, const float* buf, int tl, uint64_t t0, bool use_or,
-                           hipStream_t ss) {
+hipGraphExec_t Engine::gulp_graph(uint64_t N, hipStream_t s) {
+    for (const auto& g : graphs_)
+        if (g.arena == x_ptr_ && g.n == N) return g.exec;
+    // Captured on its own stream (forked from and joined to s) so the
+    // capture holds only this gulp's work. Events the enqueue records are
+    // created by an eager gulp first, so the capture creates none.
+    if (sync_events_.size() < 4 * plan_.nlevels * (N / tile_ + 2)) {
+        capturing_ = false;
+        warm_events(N);
+    }
+    if (!capture_stream_) FA_HIP(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking));
+    FA_HIP(hipStreamSynchronize(s));
+    hipGraph_t graph = nullptr;
+    capturing_       = true;
+    FA_HIP(hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_gulp(N, capture_stream_, true);
+    } catch (...) {
+        (void)hipStreamEndCapture(capture_stream_, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        capturing_ = false;
+        throw;
+    }
+    FA_HIP(hipStreamEndCapture(capture_stream_, &graph));
+    capturing_ = false;
+    hipGraphExec_t exec = nullptr;
+    FA_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    graphs_.push_back({x_ptr_, N, graph, exec});
+    return exec;
+}
+
+void Engine::warm_events(uint64_t N) {
+    // Enough timing and ordering events for a gulp of N samples.
+    const std::size_t need = 4 * plan_.nlevels * (N / tile_ + 2);
+    while (sync_events_.size() < need) {
+        hipEvent_t e;
+        FA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        sync_events_.push_back(e);
+    }
+}
+
+void Engine::drop_graphs() {
+    ++graph_version_;
+    for (auto& g : graphs_) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    graphs_.clear();
+}
+
+void Engine::set_graph_mode(bool on) { graph_mode_ = on; }
+
+void Engine::set_gather_chunks(int chunks) {
+    require(chunks >= 0 && chunks <= 64, "gather chunks must be in [0, 64]");
+    if (chunks != chunks_) {
+        chunks_ = chunks;
+        drop_graphs();
+    }
+}
+
+int Engine::gather_chunks() const { return chunks_ ? chunks_ : default_gather_chunks(); }
+
+void Engine::set_search_kernel(int kind) {
+    require(kind >= 0 && kind <= 2, "search kernel must be 0 (automatic), 1 or 2");
+    const bool reg = kind == 2 ? false : reg_allowed_;
+    require(kind != 1 || reg_allowed_, "the register search needs power-of-two widths up to 256");
+    if (!reg) {
+        int lds_max = 0;
+        if (!sizing())
+            FA_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
+        require(sizing() || static_cast<uint64_t>(ring_floats_) * sizeof(float) + 4096 <=
+                                static_cast<uint64_t>(lds_max),
+                "width trials need more LDS than the device offers for the ring-ladder search");
+    }
+    if (reg != reg_search_) {
+        reg_search_ = reg;
+        drop_graphs();
+    }
+}
+
+void Engine::launch_search(uint32_t l, const float* buf, int tl, uint64_t t0, hipStream_t ss) {
     const std::size_t nw = widths_.size();
     SearchArgs a{};
     a.buf      = buf;
     a.rows     = search_rows_[l].get();
     a.nrows    = static_cast<int>(search_count_[l]);
     a.inv_cur  = inv_cur_.get();
-    a.inv_or   = use_or ? inv_or_.get() : inv_cur_.get();
+    a.inv_or   = inv_or_.get();
     a.widths   = widths_dev_.get();
     a.nw       = static_cast<int>(nw);
     a.kmax     = static_cast<int>(kmax_);
     a.wm1      = static_cast<int>(wm1_);
     a.tile_len = tl;
     a.tile_t0  = static_cast<int64_t>(t0);
-    a.gulp_abs = processed_;
+    a.gulp_abs_p = reinterpret_cast<const uint64_t*>(params_.get());
     a.thr      = cfg_.detect_thresh;
     a.mode     = cfg_.peak_mode;
     a.trap     = cfg_.use_trapezoid ? 1 : 0;
@@ -2280,6 +2418,7 @@ void Engine::collect(hipStream_t s) {
                 " the device buffer may hold; raise detect_thresh");
         cand_cap_ = ctr[0];
         cands_.resize(cand_cap_);
+        drop_graphs(); // captured gulps hold the old buffer
         FA_HIP(hipMemcpyAsync(open_.get(), open_saved_.get(), open_.bytes(),
                               hipMemcpyDeviceToDevice, s));
         processed_ -= last_n_;

@@ -145,8 +145,33 @@ public:
     uint64_t xstride() const { return xstride_; }
     uint64_t capacity() const { return cap_; }
 
-    // Searches the `nsamps` samples already written to input_columns().
+    // Searches the `nsamps` samples already written to input_columns():
+    // prepare_gulp (host tables and per-gulp values, on s), the device work
+    // (enqueue_gulp, or its captured graph in graph mode), finish_gulp.
     void execute_prepared(uint64_t nsamps, const uint8_t* kill_mask, hipStream_t s);
+    void prepare_gulp(uint64_t nsamps, const uint8_t* kill_mask, hipStream_t s);
+    // The gulp's device work in stream order on s, capturable into a graph
+    // (no host synchronisation, allocation or memset).
+    // captured: the work is being recorded into a graph (no per-launch
+    // timing events are recorded then).
+    void enqueue_gulp(uint64_t nsamps, hipStream_t s, bool captured = false);
+    void finish_gulp(uint64_t nsamps);
+    // Graph mode: each gulp's device work is captured once per bound arena
+    // and gulp length and replayed (FROST_AMD_GRAPH=1 turns it on).
+    void set_graph_mode(bool on);
+    bool graph_mode() const { return graph_mode_; }
+    // Tuning after construction: consecutive 256-sample chunks per gather
+    // block (0: the default), and the search kernel (0: register ladder when
+    // the widths allow it, else the ring ladder; 2: the ring ladder).
+    void set_gather_chunks(int chunks);
+    void set_search_kernel(int kind);
+    int gather_chunks() const;
+    int search_kernel() const { return reg_search_ ? 1 : 2; }
+    // Bumped whenever a captured gulp goes stale (buffers moved).
+    uint64_t graph_version() const { return graph_version_; }
+    // Creates the ordering events a gulp of n samples records, so that a
+    // capture of enqueue_gulp creates none.
+    void warm_events(uint64_t nsamps);
     // Copies a channel-major waterfall in first.
     void execute(const float* wf, uint64_t stride, uint64_t offset, uint64_t nsamps,
                  const uint8_t* kill_mask, hipStream_t s);
@@ -193,8 +218,9 @@ private:
     void compute_inv(const std::vector<uint8_t>& mask, std::vector<float>& out) const;
     void build_programs();
     void timing_collect();
-    void launch_search(uint32_t level, const float* buf, int tl, uint64_t t0, bool use_or,
-                       hipStream_t ss);
+    void launch_search(uint32_t level, const float* buf, int tl, uint64_t t0, hipStream_t ss);
+    hipGraphExec_t gulp_graph(uint64_t nsamps, hipStream_t s);
+    void drop_graphs();
     void carry_history(hipStream_t s);
 
     FdmtPlan plan_;
@@ -244,6 +270,7 @@ private:
     DeviceBuffer<OpenRun> open_, open_saved_; // runs now / before the last gulp
     DeviceBuffer<CandRecord> cands_;
     DeviceBuffer<uint64_t> counters_; // [0] candidate count, [1] frontier
+    DeviceBuffer<uint64_t> params_;   // [0] the gulp's absolute start sample
     DeviceBuffer<float> keep_;
     DeviceBuffer<float> scratch_;
     uint64_t cand_cap_ = 0;
@@ -255,7 +282,20 @@ private:
     uint64_t last_n_    = 0;
     bool pending_collect_ = false;
     // The last gulp's mask and the mask before it (a re-run restores them).
-    std::vector<uint8_t> last_mask_, before_mask_;
+    std::vector<uint8_t> last_mask_, before_mask_, pending_mask_;
+    bool or_is_cur_  = false; // inv_or_ holds the current tables
+    bool graph_mode_ = false, capturing_ = false;
+    int chunks_      = 0;     // gather chunks per block, 0 = default
+    bool reg_allowed_ = false; // widths the register ladder covers
+    uint64_t graph_version_ = 0;
+    hipStream_t capture_stream_ = nullptr;
+    struct GulpGraph {
+        const void* arena;
+        uint64_t n;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<GulpGraph> graphs_;
     // Ring-ladder search: LDS ring offset and length per ladder level.
     std::vector<int32_t> ring_off_, ring_size_;
     int32_t ring_floats_ = 0;

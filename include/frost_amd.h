@@ -624,6 +624,58 @@ int fa_server_message(fa_server* server, uint64_t ticket, char* out, uint64_t ca
 int fa_server_release(fa_server* server, uint64_t ticket);
 int fa_server_get_stats(fa_server* server, fa_server_stats* out);
 
+/* ---- batched beams and the schedule autotuner -------------------------------
+ *
+ * No frost counterpart as a whole: frost searches its beams one after
+ * another (one FDMTBoxcar per Pipeline, include/frost/config/config.hpp:
+ * 122-123) with dmt's automatic level fusion only (lib/pipeline/
+ * pipeline.cpp:364-366). A batch runs one gulp of many beams of uneven
+ * shape per round: a group's beams clean their gulps into one arena each,
+ * then the group's FDMT launches and plane searches (one branch per beam)
+ * run as one captured graph launch; candidates equal each beam run alone. */
+
+/* Schedule of one engine: FDMT time tile (0: the largest that fits), gather
+ * blocks of gather_chunks 256-sample chunks (0: the default 16), and the
+ * search kernel (0: register ladder when the widths allow it, else the ring
+ * ladder; 2: the ring ladder). */
+typedef struct fa_engine_tuning {
+    uint64_t tile;
+    uint32_t gather_chunks;
+    uint32_t search_kernel;
+} fa_engine_tuning;
+
+/* Times one search shape on the device (synthetic noise, arena_bytes of
+ * arena, median of reps gulps after a warm one) over the three largest
+ * fitting tiles x gather chunks {16, 8, 32} x search kernels, and returns
+ * the fastest schedule and the fixed default (largest fitting tile, 16
+ * chunks, automatic search) with their seconds per gulp. */
+int fa_autotune_engine(const fa_dedisp_config* dedisp, const fa_boxcar_config* boxcar,
+                       uint64_t arena_bytes, int device, uint32_t reps, fa_engine_tuning* best,
+                       double* best_seconds, fa_engine_tuning* fixed, double* fixed_seconds,
+                       uint32_t* tried);
+
+typedef struct fa_batch fa_batch;
+/* group beams per graph round, each with an arena of arena_bytes;
+ * use_graphs 0 runs the same rounds eagerly. */
+int fa_batch_create(int device, uint32_t group, uint64_t arena_bytes, int use_graphs,
+                    fa_batch** batch);
+int fa_batch_destroy(fa_batch* batch);
+/* Pipeline::Impl construction for one beam; tuning may be NULL. */
+int fa_batch_add_beam(fa_batch* batch, const fa_rfi_config* rfi_cfg, const fa_block_layout* layout,
+                      float baseline_length, const fa_dedisp_config* dedisp,
+                      const fa_boxcar_config* boxcar, const fa_engine_tuning* tuning,
+                      uint32_t* beam);
+/* One gulp of every beam (execute_block each): gulps[i] device rows of
+ * nsamps[i] samples for beam i, in add order. Returns with every beam's
+ * candidates on the host. */
+int fa_batch_execute(fa_batch* batch, const void* const* gulps, const uint64_t* nsamps);
+int fa_batch_result(fa_batch* batch, uint32_t beam, fa_raw_candidate* out, uint64_t cap,
+                    uint64_t* n, uint64_t* dropped_global, uint64_t* dropped_per_dm,
+                    uint64_t* frontier);
+/* FDMTBoxcar::drain of one beam; rows as fa_batch_result. */
+int fa_batch_drain(fa_batch* batch, uint32_t beam);
+int fa_batch_beam_tuning(fa_batch* batch, uint32_t beam, fa_engine_tuning* out);
+
 /* ---- candidate post-processing (frost::search) -------------------------- */
 
 /* Giant (include/frost/search/candidate.hpp:24-64). The loc_* fields hold
