@@ -394,6 +394,164 @@ def serving_measure(stream, seconds, nbeams=32, lanes=2, budget=200e9, seed=11):
     }
 
 
+def fdmt_min_bytes(handle, plan_rows, nch, gulp):
+    """Fused minimum FDMT bytes of one gulp of an engine (each launch reads
+    its stored input level's tile + look-back samples once and writes its
+    output level's once; the init reads the channels once)."""
+    from frost_amd import search
+    src = search.engine_level_sources(handle)
+    lsamp = search.engine_level_samples(handle).astype(np.float64)
+    nl, tile = C.c_uint64(), C.c_uint64()
+    from frost_amd import abi
+    abi.check(abi.lib().fa_engine_level_times(handle, None, None, 0, C.byref(tile), C.byref(nl)))
+    tile = tile.value
+    tiles = [min(tile, gulp - t0) for t0 in range(0, gulp, tile)]
+    tot = 0.0
+    for l in range(len(src)):
+        a = int(src[l])
+        if a == -1:
+            tot += sum(4.0 * (nch * tl + lsamp[0] - plan_rows[0] * (tile - tl)) for tl in tiles)
+        elif a >= 0:
+            tot += sum(4.0 * ((lsamp[a] - plan_rows[a] * (tile - tl)) +
+                              (lsamp[l] - plan_rows[l] * (tile - tl))) for tl in tiles)
+    return tot
+
+
+def batched_measure(stream, nbeams=64, group=4, rounds=2, seed=3, reps=1):
+    """64 beams of uneven shape (nch 1024/2048/4096, tsamp 64/128 us, DM max
+    500/1000/2000, gulps 2^18-2^20, 11 planes, widths to 256, default RFI),
+    one gulp each per round, groups of `group` beams replayed as one captured
+    graph; the autotuned schedule per shape against the fixed default."""
+    from frost_amd import abi, batching, search, serving, workloads
+    rng = np.random.default_rng(seed)
+    shapes = [(int(rng.choice([1024, 2048, 4096])), float(rng.choice([64e-6, 128e-6])),
+               float(rng.choice([500.0, 1000.0, 2000.0])), int(rng.choice([1 << 18, 1 << 19, 1 << 20])))
+              for _ in range(nbeams)]
+
+    def cfg(nch, tsamp, dm, gulp):
+        f0, df = 1500.0 - 150.0 / nch, -300.0 / nch
+        lay = abi.BlockLayout(nch, gulp, abi.UINT8, tsamp, f0, df)
+        d = abi.dedisp_config(nch, tsamp, f0, df, dm, block_nsamps=gulp, subband_depth=2)
+        return abi.rfi_config(), lay, 2.0, d, abi.boxcar_config(max_boxcar=256)
+
+    arena = max(serving.beam_requirements(*cfg(*sh), tile=1 << 16)[1] for sh in set(shapes))
+    log(f"batched: {nbeams} beams, {len(set(shapes))} distinct shapes, arena "
+        f"{arena / 2**30:.1f} GiB x {group}")
+    # Autotune per (nch, tsamp, DM) on a 2^18 gulp in the same arena size.
+    tuned, per_shape = {}, []
+    t_tune = time.perf_counter()
+    for key in sorted({sh[:3] for sh in shapes}):
+        _, _, _, d, b = cfg(*key, 1 << 18)
+        best, bs, fixed, fs, n = batching.autotune(d, b, arena, reps=reps)
+        tuned[key] = best
+        per_shape.append({"nch": key[0], "tsamp_us": key[1] * 1e6, "dm_max": key[2],
+                          "best": best.as_dict(), "best_ms": bs * 1e3, "fixed": fixed.as_dict(),
+                          "fixed_ms": fs * 1e3, "schedules": n})
+        log(f"  tuned {key}: best {best.as_dict()} {bs * 1e3:.1f} ms vs fixed {fs * 1e3:.1f} ms")
+    tune_s = time.perf_counter() - t_tune
+    # Device gulps per (nch, tsamp, gulp): 10 pulses each, reused by the beams
+    # of that layout.
+    gulps = {}
+    for nch, tsamp, _, g in sorted({(s[0], s[1], 0, s[3]) for s in shapes}):
+        W = workloads.Workload(f"b{nch}_{g}", nch, tsamp, 1500.0 - 150.0 / nch, 300.0, g, 2000.0,
+                               2, 256, npulses=10, seed=nch + g)
+        dv = search.DeviceArray(W.gulp_bytes())
+        sc = W.synth_config(0)
+        abi.check(abi.lib().fa_synth_filterbank(C.byref(sc), dv.ptr, W.gulp, stream.h))
+        gulps[(nch, tsamp, g)] = dv
+    stream.synchronize()
+    ptrs = [gulps[(s[0], s[1], s[3])].ptr for s in shapes]
+    ns = [s[3] for s in shapes]
+    ndm = {}
+    rows = {}
+    for sh in set(shapes):
+        pl = search.make_fdmt_plan(cfg(*sh)[3])
+        ndm[sh] = int(pl.summary()["ndm"])
+        rows[sh] = pl.level_rows().astype(np.float64)
+    trials = sum(ndm[s] * s[3] for s in shapes)
+
+    def run(tunings):
+        bt = batching.Batch(group=group, arena_bytes=arena, graphs=True)
+        t0 = time.perf_counter()
+        for k, sh in enumerate(shapes):
+            t = None
+            if tunings is not None:
+                best = tunings[sh[:3]]
+                # The probe's tile caps the beam's (the largest fitting tile up
+                # to it); a probe tile equal to the probe gulp caps nothing.
+                t = abi.EngineTuning(best.tile if best.tile < (1 << 18) else 0,
+                                     best.gather_chunks, best.search_kernel)
+            bt.add_beam(*cfg(*sh), tuning=t)
+        setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        bt.execute(ptrs, ns)  # warm-up round: eager, with per-launch timings
+        warm = time.perf_counter() - t0
+        fmin, fsec = 0.0, 0.0
+        for k, sh in enumerate(shapes):
+            h = bt.engine_handle(k)
+            fmin += fdmt_min_bytes(h, rows[sh], sh[0], sh[3])
+            nl, tile = C.c_uint64(), C.c_uint64()
+            abi.check(abi.lib().fa_engine_level_times(h, None, None, 0, C.byref(tile), C.byref(nl)))
+            sec = np.zeros(nl.value, np.float64)
+            cnt = np.zeros(nl.value, np.uint32)
+            abi.check(abi.lib().fa_engine_level_times(h, sec.ctypes.data_as(abi.PF64),
+                                                      cnt.ctypes.data_as(abi.PU32), nl.value,
+                                                      C.byref(tile), C.byref(nl)))
+            fsec += float(sec.sum())
+        times = []
+        for r in range(rounds):
+            t0 = time.perf_counter()
+            bt.execute(ptrs, ns)  # captured group graphs replayed
+            times.append(time.perf_counter() - t0)
+            log(f"  round {r}: {times[-1]:.3f} s")
+        last = [bt.result(k) for k in range(nbeams)]
+        tiles = [bt.tuning(k).tile for k in range(nbeams)]
+        bt.close()
+        return {"setup_s": setup, "warm_round_s": warm, "round_s": float(np.median(times)),
+                "fdmt_min_bytes": fmin, "fdmt_kernel_s": fsec, "last": last, "tiles": tiles}
+
+    tuned_run = run(tuned)
+    fixed_run = run(None)
+    same = tuned_run["last"] == fixed_run["last"]
+    t_round = tuned_run["round_s"]
+    min_gulp_s = min(s[3] * s[1] for s in shapes)
+    realtime_load = sum(t_round / (s[3] * s[1]) for s in shapes) / nbeams
+    return {
+        "metric": "DM-trial·samples/s of 64 batched beams on 1 GPU; real-time factor",
+        "value": trials / t_round,
+        "unit": "DM-trial*samples/s",
+        "n_gpus": 1, "steps": rounds, "warmup": 1, "ms_per_step": 1e3 * t_round,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 in, f32 compute",
+        "data": "synthetic (seeded device generator: Gaussian noise, 10 dispersed pulses per gulp)",
+        "config": {"workload": f"{nbeams} beams: nch 1024/2048/4096, tsamp 64/128 us, DM max "
+                               "500/1000/2000, gulps 2^18/2^19/2^20, 11 planes, widths to 256, "
+                               "default RFI; one gulp of every beam per round",
+                   "shapes": [{"nch": s[0], "tsamp_us": s[1] * 1e6, "dm_max": s[2], "gulp": s[3]}
+                              for s in shapes],
+                   "group": group, "arena_bytes": arena},
+        "realtime": {"round_seconds": t_round, "shortest_gulp_seconds": min_gulp_s,
+                     "realtime_factor": min_gulp_s / t_round,
+                     "beams_sustained_at_shortest_gulp": nbeams * min_gulp_s / t_round},
+        "fdmt_roofline": {"min_bytes_per_round": tuned_run["fdmt_min_bytes"],
+                          "kernel_seconds_per_round": tuned_run["fdmt_kernel_s"],
+                          "gbs": tuned_run["fdmt_min_bytes"] / tuned_run["fdmt_kernel_s"] / 1e9,
+                          "frac": tuned_run["fdmt_min_bytes"] / tuned_run["fdmt_kernel_s"] / 1e9
+                          / HBM_PEAK_GBS,
+                          "note": "summed kernel durations of the eager warm-up round (branches "
+                                  "of a group overlap, so this is per-kernel bandwidth)"},
+        "autotune": {"seconds": tune_s, "per_shape": per_shape,
+                     "tuned_round_seconds": t_round, "fixed_round_seconds": fixed_run["round_s"],
+                     "speedup": fixed_run["round_s"] / t_round,
+                     "tiles_tuned": tuned_run["tiles"], "tiles_fixed": fixed_run["tiles"]},
+        "candidates_identical_tuned_vs_fixed": same,
+        "graph_rounds": "round 1 on replays captured group graphs",
+        "warm_round_seconds": {"tuned": tuned_run["warm_round_s"],
+                               "fixed": fixed_run["warm_round_s"]},
+        "realtime_load": realtime_load,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -409,6 +567,14 @@ def main() -> int:
     a = ap.parse_args()
     if a.copy_probe:
         return copy_probe()
+    if a.workload == "batched_64beams":
+        if a.gpus != 1:
+            print("bench.py: measures one GPU", file=sys.stderr)
+            return 2
+        from frost_amd import search
+        out = batched_measure(search.Stream(0), rounds=max(a.steps, 1))
+        print(json.dumps(out), flush=True)
+        return 0
     if a.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {a.gpus} is not supported (beams are "
               "independent and shard across GPUs as one process per GPU)",

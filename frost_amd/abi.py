@@ -409,6 +409,7 @@ EXPORTS = {
     "fa_batch_result": (C.c_int, [P, U32, C.POINTER(RawCandidate), U64, PU64, PU64, PU64, PU64]),
     "fa_batch_drain": (C.c_int, [P, U32]),
     "fa_batch_beam_tuning": (C.c_int, [P, U32, C.POINTER(EngineTuning)]),
+    "fa_batch_engine": (C.c_int, [P, U32, C.POINTER(P)]),
     "fa_server_beam_requirements": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32,
                                               C.POINTER(DedispConfig), C.POINTER(BoxcarConfig),
                                               U64, U64, PU64, PU64, PU64, PU64]),

@@ -64,6 +64,11 @@ class Batch:
         check(lib().fa_batch_drain(self.h, beam))
         return self.result(beam)
 
+    def engine_handle(self, beam: int):
+        e = C.c_void_p()
+        check(lib().fa_batch_engine(self.h, beam, C.byref(e)))
+        return e
+
     def tuning(self, beam: int) -> abi.EngineTuning:
         t = abi.EngineTuning()
         check(lib().fa_batch_beam_tuning(self.h, beam, C.byref(t)))

@@ -30,6 +30,7 @@
 
 #include "beam.hpp"
 #include "common.hpp"
+#include "handles.hpp"
 
 namespace fa {
 
@@ -322,6 +323,7 @@ void autotune(const fa_dedisp_config& dcfg, const fa_boxcar_config& bcfg, uint64
 
 struct fa_batch {
     std::unique_ptr<fa::Batch> b;
+    std::vector<std::unique_ptr<fa_engine>> views; // borrowed engine handles
 };
 
 using fa::guard;
@@ -393,6 +395,19 @@ int fa_batch_drain(fa_batch* batch, uint32_t beam) {
     return guard([&] {
         require(batch != nullptr, "batch must not be null");
         batch->b->drain(beam);
+    });
+}
+
+int fa_batch_engine(fa_batch* batch, uint32_t beam, fa_engine** engine) {
+    return guard([&] {
+        require(batch && engine, "arguments must not be null");
+        fa::Engine& e = batch->b->beam(beam).engine();
+        if (batch->views.size() <= beam) batch->views.resize(beam + 1);
+        if (!batch->views[beam]) {
+            batch->views[beam]      = std::make_unique<fa_engine>();
+            batch->views[beam]->eng = &e;
+        }
+        *engine = batch->views[beam].get();
     });
 }
 

@@ -1357,14 +1357,15 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     build_programs();
     shared_ = arena_bytes > 0;
     if (shared_) {
-        // The largest power-of-two tile (up to kTileMax and the capacity)
-        // whose input and level buffers, laid out as in the arena, fit it.
-        // The stored levels are re-planned for the tile picked, so its need
-        // is checked again after the programs are rebuilt.
+        // The largest power-of-two tile (up to kTileMax, the capacity and a
+        // requested tile) whose input and level buffers, laid out as in the
+        // arena, fit it. The stored levels are re-planned for the tile
+        // picked, so its need is checked again after the programs are
+        // rebuilt.
         uint64_t pick = 0;
         for (uint64_t t = kTileMax; t >= kTileMin && pick == 0; t >>= 1) {
             const uint64_t tt = std::min<uint64_t>(t, cap_);
-            if (tile_req && tt != tile_) continue;
+            if (tile_req && tt > tile_req) continue;
             if (layout_need(tt) > arena_bytes) continue;
             if (tt != tile_) {
                 tile_ = tt;

@@ -634,7 +634,8 @@ int fa_server_get_stats(fa_server* server, fa_server_stats* out);
  * then the group's FDMT launches and plane searches (one branch per beam)
  * run as one captured graph launch; candidates equal each beam run alone. */
 
-/* Schedule of one engine: FDMT time tile (0: the largest that fits), gather
+/* Schedule of one engine: FDMT time tile (the largest that fits the arena up
+ * to this one; 0: up to the gulp and 2^20), gather
  * blocks of gather_chunks 256-sample chunks (0: the default 16), and the
  * search kernel (0: register ladder when the widths allow it, else the ring
  * ladder; 2: the ring ladder). */
@@ -675,6 +676,9 @@ int fa_batch_result(fa_batch* batch, uint32_t beam, fa_raw_candidate* out, uint6
 /* FDMTBoxcar::drain of one beam; rows as fa_batch_result. */
 int fa_batch_drain(fa_batch* batch, uint32_t beam);
 int fa_batch_beam_tuning(fa_batch* batch, uint32_t beam, fa_engine_tuning* out);
+/* Borrowed handle of a beam's engine (diagnostics: level times, level
+ * samples); valid while the batch lives, never destroyed by the caller. */
+int fa_batch_engine(fa_batch* batch, uint32_t beam, fa_engine** engine);
 
 /* ---- candidate post-processing (frost::search) -------------------------- */
 
