@@ -262,7 +262,7 @@ class MemoryEstimate(C.Structure):
     _fields_ = [("input_bytes", C.c_uint64), ("rfi_bytes", C.c_uint64),
                 ("fdmt_bytes", C.c_uint64), ("postprocess_bytes", C.c_uint64),
                 ("reserve_bytes", C.c_uint64), ("total_bytes", C.c_uint64),
-                ("tile", C.c_uint64)]
+                ("tile", C.c_uint64), ("fdmt_state", C.c_uint32), ("pad_", C.c_uint32)]
 
 
 class MemoryCheck(C.Structure):
@@ -460,6 +460,7 @@ EXPORTS = {
     "fa_engine_level_sources": (C.c_int, [P, PI32, U64, PU64]),
     "fa_engine_level_samples": (C.c_int, [P, PU64, U64, PU64]),
     "fa_engine_host_stats": (C.c_int, [P, PF64, PU64]),
+    "fa_engine_state": (C.c_int, [P, C.POINTER(C.c_int), PU64, PU64]),
     "fa_rfi_create": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32, C.c_int,
                                 C.POINTER(P)]),
     "fa_rfi_destroy": (C.c_int, [P]),

@@ -409,6 +409,16 @@ int fa_engine_level_samples(const fa_engine* engine, uint64_t* samples, uint64_t
     });
 }
 
+int fa_engine_state(const fa_engine* engine, int* carried, uint64_t* tile,
+                    uint64_t* state_bytes) {
+    return guard([&] {
+        require(engine != nullptr, "engine must not be null");
+        if (carried) *carried = engine->eng->carried() ? 1 : 0;
+        if (tile) *tile = engine->eng->tile();
+        if (state_bytes) *state_bytes = engine->eng->arena_need();
+    });
+}
+
 int fa_engine_host_stats(const fa_engine* engine, double* collect_seconds,
                          uint64_t* raw_candidates) {
     return guard([&] {

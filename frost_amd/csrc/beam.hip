@@ -44,9 +44,11 @@ struct BeamSizing {
 // cleaner and the FDMT engine at its smallest automatic tile are built under
 // a SizingScope (sizes recorded, nothing allocated, no HIP call; the beam
 // runs every gulp over the full capacity, so the engine's short-gulp carry
-// scratch is never used); the two ingest slots and the pair-search buffers
-// for max_candidates rows are counted from their sizes; 10 % of the sum is
-// kept in reserve.
+// scratch is never used), the engine in both level-state forms, the smaller
+// counted (the engine carries its levels in rings when the recomputed form
+// does not fit); the device ingest slot (or the caller's device gulp) and
+// the pair-search buffers for max_candidates rows are counted from their
+// sizes; 10 % of the sum is kept in reserve.
 BeamSizing size_beam(const fa_rfi_config& rcfg, fa_block_layout layout, float baseline,
                      fa_dedisp_config dcfg, const fa_boxcar_config& bcfg, uint64_t gulp) {
     layout.max_nsamps = gulp;
@@ -55,21 +57,22 @@ BeamSizing size_beam(const fa_rfi_config& rcfg, fa_block_layout layout, float ba
     BeamSizing r;
     DeviceRfi rfi(rcfg, layout, baseline, 0);
     FdmtPlan plan(dcfg);
-    const uint64_t tile = std::min<uint64_t>(gulp, kAutoTileMin);
-    Engine eng(plan, bcfg, nullptr, 0, tile);
-    const uint64_t bits = layout.dtype == FA_UINT1 ? 1 : layout.dtype == FA_UINT2 ? 2
-                          : layout.dtype == FA_UINT4 ? 4 : layout.dtype == FA_UINT8 ? 8
-                          : layout.dtype == FA_UINT16 ? 16 : 32;
+    const Engine eng(plan, bcfg, nullptr, 0, std::min<uint64_t>(gulp, kAutoTileMin), 0,
+                     LevelState::recompute);
+    const Engine ring(plan, bcfg, nullptr, 0, std::min<uint64_t>(gulp, kCarriedTileMin), 0,
+                      LevelState::carried);
+    const Engine& pick = ring.memory_bytes() < eng.memory_bytes() ? ring : eng;
     fa_memory_estimate& e = r.est;
-    e.input_bytes         = 2 * gulp * (layout.nchans * bits / 8);
+    e.input_bytes         = gulp * (layout.nchans * sample_bits(layout.dtype) / 8);
     e.rfi_bytes           = rfi.memory_bytes();
-    e.fdmt_bytes          = eng.memory_bytes();
+    e.fdmt_bytes          = pick.memory_bytes();
+    e.fdmt_state          = pick.carried() ? 1 : 0;
     e.postprocess_bytes   = postprocess_device_bytes(bcfg.max_candidates);
     const uint64_t comp =
         sat_add(sat_add(e.input_bytes, e.rfi_bytes), sat_add(e.fdmt_bytes, e.postprocess_bytes));
     e.total_bytes   = sat_add(comp, comp / 10);
     e.reserve_bytes = e.total_bytes - comp;
-    e.tile          = eng.tile();
+    e.tile          = pick.tile();
     r.dt_max        = static_cast<uint64_t>(plan.dt_max);
     r.ndm           = plan.dm_list.size();
     return r;

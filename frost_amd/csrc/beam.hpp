@@ -62,14 +62,15 @@ public:
         DeviceGuard dg(device_);
         const uint8_t* dev = static_cast<const uint8_t*>(gulp);
         if (!is_device) {
-            staging_.reserve(capacity_ * row_bytes_);
-            FA_HIP(hipMemcpyAsync(staging_.get(), gulp, nsamps * row_bytes_, hipMemcpyHostToDevice, s));
-            dev = staging_.get();
+            // The host gulp goes through the device slot the ingest uses.
+            require(!slots_[0].submitted && !slots_[1].submitted,
+                    "a host gulp cannot be executed while submitted ingest gulps wait for the "
+                    "device slot; execute them first");
+            dev_slot_.reserve(capacity_ * row_bytes_);
+            FA_HIP(hipMemcpyAsync(dev_slot_.get(), gulp, nsamps * row_bytes_, hipMemcpyHostToDevice, s));
+            dev = dev_slot_.get();
         }
-        const uint8_t* km = clean(dev, nsamps, s);
-        engine_->execute_prepared(capacity_, km, s);
-        engine_->collect(s);
-        samples_read_ += nsamps;
+        run(dev, nsamps, s);
     }
 
     // Clean + calibrate a device gulp into the engine's input columns
@@ -86,21 +87,24 @@ public:
     }
 
     // Ingest with overlap (the counterpart of frost's GulpPrefetch,
-    // lib/pipeline/pipeline.cpp:109-262): two pinned host slots and two
-    // device slots. The caller fills the slot ingest_slot() returns and
-    // submits it; its copy to the device runs on the beam's copy stream
-    // while the previous gulp computes; execute_next() waits for the oldest
-    // submitted copy on the compute stream and runs that gulp.
+    // lib/pipeline/pipeline.cpp:109-262): two pinned host slots and one
+    // device slot. The caller fills the host slot ingest_slot() returns and
+    // submits it; its copy to the device slot runs on the beam's copy stream
+    // as soon as the device slot's previous gulp has been cleaned (the RFI
+    // pass is the only reader of the raw gulp), so the copy of gulp k + 1
+    // overlaps the FDMT search of gulp k. execute_next() waits for the
+    // oldest submitted copy on the compute stream and runs that gulp.
     uint8_t* ingest_slot(uint64_t* bytes) {
         DeviceGuard dg(device_);
         Slot& sl = slots_[fill_];
         require(!sl.submitted, "both ingest slots hold submitted gulps; execute one first");
         if (!copy_stream_) FA_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
+        if (!dev_free_) FA_HIP(hipEventCreateWithFlags(&dev_free_, hipEventDisableTiming));
         if (!sl.copied) FA_HIP(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
         sl.host.reserve(capacity_ * row_bytes_);
-        sl.dev.reserve(capacity_ * row_bytes_);
-        // The slot's previous copy finished before its gulp ran; wait anyway
-        // so a slot is never refilled under an outstanding copy.
+        dev_slot_.reserve(capacity_ * row_bytes_);
+        // The host slot's previous copy finished before its gulp ran; wait
+        // anyway so a slot is never refilled under an outstanding copy.
         FA_HIP(hipEventSynchronize(sl.copied));
         if (bytes) *bytes = capacity_ * row_bytes_;
         return sl.host.get();
@@ -112,11 +116,10 @@ public:
         require(sl.host.get() != nullptr && !sl.submitted,
                 "no ingest slot to submit (call fa_beam_ingest_slot first)");
         DeviceGuard dg(device_);
-        FA_HIP(hipMemcpyAsync(sl.dev.get(), sl.host.get(), nsamps * row_bytes_,
-                              hipMemcpyHostToDevice, copy_stream_));
-        FA_HIP(hipEventRecord(sl.copied, copy_stream_));
         sl.nsamps    = nsamps;
         sl.submitted = true;
+        sl.queued    = false;
+        if (dev_owner_ < 0) start_copy(fill_);
         fill_ ^= 1;
     }
 
@@ -125,15 +128,25 @@ public:
         Slot& sl = slots_[next_];
         require(sl.submitted, "no submitted gulp to execute");
         DeviceGuard dg(device_);
+        if (!sl.queued) start_copy(next_); // the device slot is free (the previous gulp ran)
         FA_HIP(hipStreamWaitEvent(s, sl.copied, 0));
-        execute(sl.dev.get(), sl.nsamps, true, s);
+        const int other = next_ ^ 1;
+        run(dev_slot_.get(), sl.nsamps, s, [&] {
+            // The raw gulp has been read: the next submitted gulp may land.
+            FA_HIP(hipEventRecord(dev_free_, s));
+            dev_free_recorded_ = true;
+            dev_owner_         = -1;
+            if (slots_[other].submitted && !slots_[other].queued) start_copy(other);
+        });
         sl.submitted = false;
         next_ ^= 1;
     }
 
     ~Beam() {
+        if (copy_stream_) (void)hipStreamSynchronize(copy_stream_);
         for (auto& sl : slots_)
             if (sl.copied) (void)hipEventDestroy(sl.copied);
+        if (dev_free_) (void)hipEventDestroy(dev_free_);
         if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
     }
 
@@ -148,7 +161,7 @@ public:
     // Device bytes held now, by owner.
     fa_memory_estimate memory() const {
         fa_memory_estimate m{};
-        m.input_bytes = staging_.bytes() + slots_[0].dev.bytes() + slots_[1].dev.bytes();
+        m.input_bytes = dev_slot_.bytes();
         m.rfi_bytes   = rfi_->memory_bytes();
         m.fdmt_bytes  = engine_->memory_bytes();
         m.total_bytes = m.input_bytes + m.rfi_bytes + m.fdmt_bytes;
@@ -157,22 +170,49 @@ public:
     }
 
 private:
+    // Clean, search and collect one device gulp; after_clean runs once the
+    // RFI pass (the raw gulp's only reader) is enqueued.
+    template <class F = void (*)()>
+    void run(const uint8_t* dev, uint64_t nsamps, hipStream_t s, F after_clean = [] {}) {
+        const uint8_t* km = clean(dev, nsamps, s);
+        after_clean();
+        engine_->execute_prepared(capacity_, km, s);
+        engine_->collect(s);
+        samples_read_ += nsamps;
+    }
+
+    // Copy host slot i into the device slot, after the device slot's last
+    // gulp was cleaned.
+    void start_copy(int i) {
+        Slot& sl = slots_[i];
+        if (dev_free_recorded_) FA_HIP(hipStreamWaitEvent(copy_stream_, dev_free_, 0));
+        FA_HIP(hipMemcpyAsync(dev_slot_.get(), sl.host.get(), sl.nsamps * row_bytes_,
+                              hipMemcpyHostToDevice, copy_stream_));
+        FA_HIP(hipEventRecord(sl.copied, copy_stream_));
+        sl.queued  = true;
+        dev_owner_ = i;
+    }
+
     int device_ = 0;
     bool descending_ = false;
     uint64_t row_bytes_ = 0, capacity_ = 0, samples_read_ = 0;
     std::unique_ptr<DeviceRfi> rfi_;
     std::unique_ptr<Engine> engine_;
     std::vector<uint8_t> static_mask_, fdmt_mask_;
-    DeviceBuffer<uint8_t> staging_;
+    // The raw gulp on the device: a host gulp's copy, or an ingest slot's.
+    DeviceBuffer<uint8_t> dev_slot_;
     struct Slot {
         PinnedBuffer<uint8_t> host;
-        DeviceBuffer<uint8_t> dev;
         hipEvent_t copied = nullptr;
         uint64_t nsamps   = 0;
         bool submitted    = false;
+        bool queued       = false; // its copy to the device slot is enqueued
     };
     Slot slots_[2];
     int fill_ = 0, next_ = 0;
+    int dev_owner_ = -1; // host slot whose gulp holds the device slot, not yet cleaned
+    hipEvent_t dev_free_    = nullptr; // recorded after the device slot's gulp was cleaned
+    bool dev_free_recorded_ = false;
     hipStream_t copy_stream_ = nullptr;
 };
 

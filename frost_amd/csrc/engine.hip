@@ -24,30 +24,50 @@ constexpr uint64_t kTileMin = kAutoTileMin;
 constexpr uint32_t kMaxBox = 4096; // widest filter (frost's boxcar_max bound)
 constexpr uint32_t kMaxWidths = 32; // width trials per plane row
 constexpr uint64_t kCandCapMax = uint64_t{1} << 28; // candidate buffer growth bound
+constexpr uint64_t kSpillCap   = uint64_t{1} << 22; // carried form: host spill records
+constexpr uint64_t kRingTileMin = kCarriedTileMin; // carried form: automatic tile bounds
+constexpr uint64_t kRingTileMax = kCarriedTileMax;
 
 // ---- kernels ---------------------------------------------------------------
+
+// Ring position of a sample p samples after ring position q, p in [-R, 2R - q).
+__device__ __forceinline__ int wrap_pos(int p, int R) {
+    p = p < 0 ? p + R : p;
+    return p >= R ? p - R : p;
+}
+
+// Ring position of absolute sample t in a ring of R samples.
+__device__ __forceinline__ int ring_of(uint64_t t, int R) {
+    return static_cast<int>(t % static_cast<uint64_t>(static_cast<uint32_t>(R)));
+}
 
 // Level 0: per channel, the left-fold sums x(t) + x(t-1) + ... + x(t-D) for
 // each of its row delays D (ascending), over [t0 - hc, t0 + tile). The
 // channel's row table is staged in LDS; each thread carries kInitPer running
-// sums through the rows in delay order.
+// sums through the rows in delay order. Carried state (ring != 0): hc is 0
+// and row r's sample t goes to ring position t mod R_r (the row table holds
+// R_r in place of h).
 constexpr int kInitRows = 64; // level-0 rows per channel held in LDS
 
 __global__ void __launch_bounds__(kBlock)
 fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
                  const int32_t* __restrict__ chan, const int64_t* __restrict__ rows,
-                 float* __restrict__ out, int tile_len, int nchans) {
+                 float* __restrict__ out, int tile_len, int nchans,
+                 const uint64_t* __restrict__ gulp_abs_p, int64_t tile_t0, int ring) {
     extern __shared__ float sx[];
     __shared__ int64_t s_base[kInitRows];
-    __shared__ int32_t s_d[kInitRows], s_h[kInitRows];
+    __shared__ int32_t s_d[kInitRows], s_h[kInitRows], s_q[kInitRows];
+    const uint64_t t0abs = ring ? *gulp_abs_p + static_cast<uint64_t>(tile_t0) : 0;
     for (int c = blockIdx.y; c < nchans; c += gridDim.y) {
         const int rf = chan[4 * c], nr = chan[4 * c + 1], hc = chan[4 * c + 2],
                   dm = chan[4 * c + 3];
         const int total = tile_len + hc;
         // Blocks start at indices congruent to hc mod 64, so each wave's
         // stores cover whole 256-byte lines of the rows (sample 0 of a row is
-        // 64-float aligned); block 0 starts up to 64 indices early.
-        const int i0    = blockIdx.x * (kBlock * kInitPer) + (hc & 63) - 64;
+        // 64-float aligned); block 0 starts up to 64 indices early. Rings are
+        // multiples of 64 long, so sample t0 sits at t0 mod 64 in every ring.
+        const int align = ring ? static_cast<int>((64 - (t0abs & 63)) & 63) : (hc & 63);
+        const int i0    = blockIdx.x * (kBlock * kInitPer) + align - 64;
         if (i0 >= total) continue;
         // xc[i] is the sample at t0 - hc + i; indices from -dm are held.
         const float* xc = x + (uint64_t)c * xstride + (xcol0 - hc);
@@ -68,6 +88,7 @@ fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
                 s_d[j]            = (int32_t)rw[0];
                 s_base[j]         = rw[1];
                 s_h[j]            = (int32_t)rw[2];
+                s_q[j]            = ring ? ring_of(t0abs, (int32_t)rw[2]) : 0;
             }
             __syncthreads();
             // Thread tid holds samples i0 + 4 tid + 4 kBlock g + (0..3),
@@ -87,6 +108,29 @@ fdmt_init_kernel(const float* __restrict__ x, uint64_t xstride, int64_t xcol0,
                     for (int q = 0; q < kInitPer; ++q)
                         acc[q] = acc[q] +
                                  sx[dm + 4 * threadIdx.x + 4 * kBlock * (q / 4) + (q % 4) - (k + 1)];
+                }
+                if (ring) {
+                    // Ring row: sample t0 + i at (q + i) mod R; a group of
+                    // four starts on a multiple of 4 and never straddles the
+                    // wrap (R is a multiple of 64).
+                    const int R = s_h[jr], q = s_q[jr];
+                    float* o    = out + s_base[jr];
+#pragma unroll
+                    for (int gq = 0; gq < kGroups; ++gq) {
+                        const int ib = i0 + 4 * threadIdx.x + 4 * kBlock * gq;
+                        if (ib >= 0 && ib + 3 < total) {
+                            typedef float f4 __attribute__((ext_vector_type(4)));
+                            const f4 v4 = {acc[4 * gq], acc[4 * gq + 1], acc[4 * gq + 2], acc[4 * gq + 3]};
+                            __builtin_nontemporal_store(v4, reinterpret_cast<f4*>(o + wrap_pos(q + ib, R)));
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const int i = ib + j;
+                                if (i >= 0 && i < total) o[wrap_pos(q + i, R)] = acc[4 * gq + j];
+                            }
+                        }
+                    }
+                    continue;
                 }
                 const int skip = hc - s_h[jr];
                 float* o       = out + s_base[jr] - skip;
@@ -185,7 +229,14 @@ struct GatherDims {
     // samples exist for tile samples [xlo, xhi].
     const float* x;
     int xlo, xhi;
+    // Carried state: rows are rings (GatherLeaf::ring, row record [3]);
+    // tile sample 0 is absolute sample *gulp_abs_p + tile_t0.
+    int ring;
+    const uint64_t* gulp_abs_p;
+    int64_t tile_t0;
 };
+
+constexpr int kGatherRowsMax = 128; // output rows per row block
 
 constexpr int kX0Grid = 384; // grid elements per channel window (6 per lane)
 constexpr int kX0Win  = 448; // window samples (grid + delays, 7 per lane)
@@ -245,6 +296,7 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                    GatherDims g) {
     extern __shared__ float lds[];
     __shared__ float xwin[X0 ? 4 : 1][X0 ? kX0Win : 1]; // per producer wave
+    __shared__ int s_rq[kGatherRowsMax]; // carried: ring position of tile sample 0 per output row
     constexpr int M     = (kChunkG + kSpreadG) / 64;
     const uint32_t id   = blockIdx.x;
     const uint32_t k    = id >> 3;
@@ -386,12 +438,35 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
         // the value staged for samples outside the row. The load count is
         // static, so waiting for set A leaves set B in flight.
         float va[UMAX][M], vb[UMAX][M];
+        // Carried state: ring position of tile sample 0 in each input row.
+        int rq[UMAX];
+        if (g.ring) {
+            const uint64_t t0abs = *g.gulp_abs_p + static_cast<uint64_t>(g.tile_t0);
+#pragma unroll
+            for (int u = 0; u < UMAX; ++u)
+                rq[u] = ring_of(t0abs, leaves[B.leaf_first + min(wave + 4 * u, B.nleaves - 1)].ring);
+        }
         auto load = [&](float (&v)[UMAX][M], int c) {
             const int tau0 = tau_first + c * kChunkG;
 #pragma unroll
             for (int u = 0; u < UMAX; ++u) {
                 const int sidx      = min(wave + 4 * u, B.nleaves - 1);
                 const GatherLeaf lf = leaves[B.leaf_first + sidx];
+                if (g.ring) {
+                    // Ring row: tile sample t at (rq + t) mod R; samples at
+                    // or past the tile end read 0 (they feed only dropped
+                    // outputs); every read sample is at most the row's
+                    // look-back before the tile.
+                    const auto rs = row_rsrc(in + lf.src, lf.ring);
+                    const int tb  = tau0 - lf.shmax + lane;
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        const int t   = tb + 64 * m;
+                        const bool ok = t < g.tile_len && (m < M - 1 || lane + 64 * m < lf.len);
+                        v[u][m]       = buffer_load(rs, ok ? wrap_pos(rq[u] + t, lf.ring) * 4 : kBufNone);
+                    }
+                    continue;
+                }
                 const auto rs       = row_rsrc(in + (lf.src + lf.shmax - lf.hc), lf.hc + g.tile_len);
                 const int off0      = (tau0 - lf.shmax + lf.hc + lane) * 4;
 #pragma unroll
@@ -440,6 +515,15 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
         constexpr int RS = 4 + K;
         const int q      = wave - 4;
         const int r_end  = B.row_first + B.nrows;
+        if (g.ring) {
+            // Ring position of tile sample 0 in each of this wave's rows
+            // (rows q, q + 4, ...: lane i takes row q + 4 i).
+            const uint64_t t0abs = *g.gulp_abs_p + static_cast<uint64_t>(g.tile_t0);
+            const int rr         = B.row_first + q + 4 * lane;
+            if (lane < kGatherRowsMax / 4 && rr < r_end)
+                s_rq[rr - B.row_first] = ring_of(t0abs, rows[(int64_t)rr * RS + 3]);
+            wave_sync();
+        }
         for (int c = lo_c; c < hi_c; ++c) {
             lds_barrier();
             const float* buf = lds + (c & 1) * g.half + lane;
@@ -464,10 +548,23 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                 }
                 const int64_t o = (int64_t)(uint32_t)rec[0] | ((int64_t)rec[1] << 32);
                 const int h     = rec[2];
-                const auto rs   = row_rsrc(out + (o - h), h + g.tile_len);
-                const int off0  = (tau0 + h) * 4;
+                if (g.ring) {
+                    // Ring row: tile samples [0, tile) only (h is 0).
+                    const int R   = rec[3];
+                    const int q0  = s_rq[r - B.row_first];
+                    const auto rs = row_rsrc(out + o, R);
 #pragma unroll
-                for (int s4 = 0; s4 < 4; ++s4) buffer_store(rs, off0 + 256 * s4, tree_sum<K>(w[s4]));
+                    for (int s4 = 0; s4 < 4; ++s4) {
+                        const int t = tau0 + 64 * s4;
+                        buffer_store(rs, t < g.tile_len ? wrap_pos(q0 + t, R) * 4 : kBufNone,
+                                     tree_sum<K>(w[s4]));
+                    }
+                } else {
+                    const auto rs  = row_rsrc(out + (o - h), h + g.tile_len);
+                    const int off0 = (tau0 + h) * 4;
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) buffer_store(rs, off0 + 256 * s4, tree_sum<K>(w[s4]));
+                }
                 if (more) {
 #pragma unroll
                     for (int j = 0; j < RS; ++j) rec[j] = nxt[j];
@@ -502,9 +599,13 @@ int default_gather_chunks() {
 // held by the input buffer (used only by launches reading level 0 from x).
 template <int K>
 void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, const float* x0,
-                   int xlo, int xhi, int chunks, hipStream_t s) {
+                   int xlo, int xhi, int chunks, bool ring, const uint64_t* gulp_abs_p,
+                   int64_t tile_t0, hipStream_t s) {
     static_assert(kChunkG == 256, "four consumer waves cover one chunk");
     GatherDims g{};
+    g.ring       = ring ? 1 : 0;
+    g.gulp_abs_p = gulp_abs_p;
+    g.tile_t0    = tile_t0;
     g.nblocks    = static_cast<int>(ml.nblocks);
     g.tile_len   = tl;
     g.tau_origin = -((ml.hmax + 63) & ~63); // chunks start on 64-sample boundaries
@@ -587,6 +688,8 @@ struct SearchArgs {
     CandRecord* cands;
     unsigned long long* counter;
     uint64_t cap;
+    CandRecord* spill;   // records cap .. cap + spill_cap - 1 (pinned host memory), or null
+    uint64_t spill_cap;
     float* keep;       // optional plane-row store [prow][keep_stride]
     uint64_t keep_stride;
 };
@@ -594,7 +697,7 @@ struct SearchArgs {
 __device__ void emit(const SearchArgs& a, uint32_t prow, uint32_t w, uint64_t start,
                      uint64_t peak, uint64_t end, float snr) {
     const unsigned long long idx = atomicAdd(a.counter, 1ULL);
-    if (idx < a.cap) {
+    if (idx < a.cap + a.spill_cap) {
         CandRecord r;
         r.peak  = peak;
         r.begin = start;
@@ -603,7 +706,8 @@ __device__ void emit(const SearchArgs& a, uint32_t prow, uint32_t w, uint64_t st
         r.prow  = prow;
         r.w     = w;
         r.pad   = 0;
-        a.cands[idx] = r;
+        if (idx < a.cap) a.cands[idx] = r;
+        else a.spill[idx - a.cap] = r;
     }
 }
 
@@ -631,6 +735,19 @@ __device__ __forceinline__ void track_run(const SearchArgs& a, OpenRun& o, uint3
         o.open = 0;
     }
 }
+
+// A searched plane row: sample t0 + t of the tile, t in [-(W_max - 1), tile).
+// Linear rows are read at in + (W_max - 1) + t; carried rows at ring
+// position (t0 + t) mod R.
+struct RowRead {
+    const float* p;
+    int q, R;
+    __device__ __forceinline__ RowRead(const SearchArgs& a, const SearchRow& row, uint64_t gabs)
+        : p(a.buf + row.in), q(a.wm1), R(row.ring) {
+        if (R) q = ring_of(gabs + static_cast<uint64_t>(a.tile_t0), R);
+    }
+    __device__ __forceinline__ float at(int t) const { return p[R ? wrap_pos(q + t, R) : q + t]; }
+};
 
 // ---- ring-ladder search (any widths up to 4096) ------------------------------
 //
@@ -726,11 +843,11 @@ __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs 
     for (int j = i; j < ra.lds_floats; j += kRingChunk) lds[j] = 0.0f;
     __syncthreads();
     const Ring R{lds, s_off, s_size, s_base};
-    const float* src = a.buf + row.in + a.wm1; // tile sample 0
+    const RowRead src(a, row, gabs);
     const int tl     = a.tile_len;
     for (int u = ra.u0; u < tl; u += kRingChunk) {
         const int t   = u + i;
-        const float v = (t >= -a.wm1 && t < tl) ? src[t] : 0.0f;
+        const float v = (t >= -a.wm1 && t < tl) ? src.at(t) : 0.0f;
         lds[s_off[0] + ring_pos(s_base[0] + i, s_size[0])] = v;
         if (a.keep && t >= 0 && t < tl)
             a.keep[(uint64_t)prow * a.keep_stride + a.tile_t0 + t] = v;
@@ -950,7 +1067,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
     }
     wave_sync();
     const int64_t warm_max = (int64_t)row.delay + (1 << KMAX) - 1;
-    const float* src       = a.buf + row.in + a.wm1; // sample t0
+    const RowRead src(a, row, gabs); // sample t0 + t
     const int tl           = a.tile_len;
     float D[9][4], P[9][4], nx[4];
 #pragma unroll
@@ -960,7 +1077,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int i = -kRegChunk + 4 * lane + j;
-        nx[j]       = (i >= -a.wm1 && i < tl) ? src[i] : 0.0f;
+        nx[j]       = (i >= -a.wm1 && i < tl) ? src.at(i) : 0.0f;
     }
     for (int u = -kRegChunk; u < tl; u += kRegChunk) {
         const int base = u + 4 * lane;
@@ -969,12 +1086,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
         // Prefetch the next chunk.
         if (u + 2 * kRegChunk <= tl) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) nx[j] = src[base + kRegChunk + j];
+            for (int j = 0; j < 4; ++j) nx[j] = src.at(base + kRegChunk + j);
         } else {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int i = base + kRegChunk + j;
-                nx[j]       = i < tl ? src[i] : 0.0f;
+                nx[j]       = i < tl ? src.at(i) : 0.0f;
             }
         }
         if (a.keep && u >= 0) {
@@ -1273,7 +1390,7 @@ uint64_t row_span(int32_t h, uint64_t tile) {
 // ---- Engine ------------------------------------------------------------------
 
 Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t* kill_mask,
-               int device, uint64_t tile, uint64_t arena_bytes)
+               int device, uint64_t tile, uint64_t arena_bytes, LevelState state)
     : plan_(plan), cfg_(cfg), device_(device) {
     require(cfg.max_boxcar >= 1, "max_boxcar must be >= 1");
     require(cfg.max_boxcar <= kMaxBox, "max_boxcar must be in [1, 4096]");
@@ -1354,9 +1471,46 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     }
 
     DeviceGuard dg(device_);
-    build_programs();
+    if (state == LevelState::automatic) {
+        if (const char* sv2 = std::getenv("FROST_AMD_FDMT_STATE")) {
+            const std::string v(sv2);
+            if (v == "carried") state = LevelState::carried;
+            if (v == "recompute") state = LevelState::recompute;
+        }
+    }
     shared_ = arena_bytes > 0;
-    if (shared_) {
+    require(!(shared_ && state == LevelState::carried),
+            "the carried FDMT level state keeps per-beam rings; it does not run in a shared arena");
+    // Carried rings are addressed by the staged gather only.
+    auto go_carried = [&](uint64_t t) {
+        carried_      = true;
+        staged_merge_ = true;
+        tile_         = t;
+        build_programs();
+    };
+    // The largest power-of-two carried tile whose input and rings fit
+    // `budget` bytes (0: none).
+    auto carried_pick = [&](uint64_t budget) -> uint64_t {
+        for (uint64_t t = kRingTileMax; t >= kRingTileMin; t >>= 1) {
+            const uint64_t tt = std::min<uint64_t>(t, cap_);
+            if (carried_need(tt) <= budget) return tt;
+        }
+        return 0;
+    };
+    if (state == LevelState::carried) {
+        uint64_t t = std::min<uint64_t>(cap_, tile_req ? tile_req : kRingTileMin);
+        if (!tile_req && !sizing()) {
+            size_t free_b = 0, total_b = 0;
+            FA_HIP(hipMemGetInfo(&free_b, &total_b));
+            if (const uint64_t pick = carried_pick(free_b / 10 * 9)) t = pick;
+        }
+        go_carried(t);
+    } else {
+        build_programs();
+    }
+    if (carried_) {
+        // Tile chosen above.
+    } else if (shared_) {
         // The largest power-of-two tile (up to kTileMax, the capacity and a
         // requested tile) whose input and level buffers, laid out as in the
         // arena, fit it. The stored levels are re-planned for the tile
@@ -1386,7 +1540,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
         size_t free_b = 0, total_b = 0;
         FA_HIP(hipMemGetInfo(&free_b, &total_b));
         const uint64_t budget = free_b / 4 * 3;
-        uint64_t pick         = std::min<uint64_t>(cap_, kTileMin);
+        uint64_t pick         = 0;
         for (uint64_t t = kTileMax; t >= kTileMin; t >>= 1) {
             const uint64_t tt = std::min<uint64_t>(t, cap_);
             if (buffers_bound(tt) <= budget) {
@@ -1394,14 +1548,23 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
                 break;
             }
         }
-        if (pick != tile_) {
+        // Nothing fits at the smallest tile: a large beam. Its look-back
+        // would be recomputed many times over per tile; carry the levels in
+        // rings instead, at the largest tile that fits 90 % of free memory.
+        uint64_t ring_tile = 0;
+        if (pick == 0 && state == LevelState::automatic) ring_tile = carried_pick(free_b / 10 * 9);
+        if (ring_tile) {
+            go_carried(ring_tile);
+        } else {
+            if (pick == 0) pick = std::min<uint64_t>(cap_, kTileMin);
             tile_ = pick;
             build_programs();
         }
     }
 
-    // Extended input (zero history) and level buffers: owned, or laid out
-    // in the shared arena as [x | level buffer 0 | level buffer 1].
+    // Extended input (zero history) and level buffers: owned (two ping-pong
+    // buffers, or the rings of the carried form), or laid out in the shared
+    // arena as [x | level buffer 0 | level buffer 1].
     xstride_ = round64(hist_in_ + cap_);
     uint64_t len0 = 0, len1 = 0;
     for (uint32_t l = 0; l < plan_.nlevels; ++l) {
@@ -1409,7 +1572,15 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
         if (level_buf_[l] == 1) len1 = std::max(len1, level_len_[l]);
     }
     len1 = std::max<uint64_t>(len1, 1);
-    if (shared_) {
+    if (carried_) {
+        x_.resize(nch * xstride_);
+        if (!sizing()) FA_HIP(hipMemset(x_.get(), 0, x_.bytes()));
+        rings_.resize(std::max<uint64_t>(rings_floats_, 1));
+        if (!sizing()) FA_HIP(hipMemset(rings_.get(), 0, rings_.bytes())); // samples before the stream: 0
+        x_ptr_      = x_.get();
+        buf_ptr_[0] = buf_ptr_[1] = rings_.get();
+        arena_need_ = x_.bytes() + rings_.bytes();
+    } else if (shared_) {
         auto up256 = [](uint64_t b) { return (b + 255) & ~uint64_t{255}; };
         buf_off_[0] = up256(nch * xstride_ * sizeof(float));
         buf_off_[1] = up256(buf_off_[0] + len0 * sizeof(float));
@@ -1436,6 +1607,14 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     if (!sizing()) FA_HIP(hipMemset(open_.get(), 0, open_.bytes()));
     cand_cap_ = std::max<uint64_t>(4 * cfg.max_candidates, uint64_t{1} << 20);
     cands_.resize(cand_cap_);
+    if (carried_ && !sizing()) {
+        // A carried gulp cannot be searched again (its rings have moved on):
+        // records past the device buffer go to pinned host memory.
+        spill_cap_ = kSpillCap;
+        FA_HIP(hipHostMalloc(reinterpret_cast<void**>(&spill_), spill_cap_ * sizeof(CandRecord),
+                             hipHostMallocMapped));
+        FA_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&spill_dev_), spill_, 0));
+    }
     counters_.resize(2);
     if (!sizing()) FA_HIP(hipMemset(counters_.get(), 0, counters_.bytes()));
     params_.resize(2);
@@ -1507,19 +1686,94 @@ Engine::~Engine() {
     for (auto e : events_) (void)hipEventDestroy(e);
     if (ev_begin_) (void)hipEventDestroy(ev_begin_);
     if (ev_end_) (void)hipEventDestroy(ev_end_);
+    if (spill_) (void)hipHostFree(spill_);
 }
 
-void Engine::build_programs() {
-    const uint32_t L = plan_.nlevels - 1;
-    hist_.assign(plan_.nlevels, {});
-    for (uint32_t l = 0; l <= L; ++l) hist_[l].assign(plan_.levels[l].size(), -1);
-    // Plane bookkeeping.
+void Engine::plane_setup() {
     plane_first_.clear();
     nprows_ = 0;
     for (const auto& p : plan_.planes) {
         plane_first_.push_back(nprows_);
         nprows_ += static_cast<uint32_t>(p.delay.size());
     }
+    dmax0_ = 0;
+    for (const int32_t d : plan_.levels[0].delay) dmax0_ = std::max(dmax0_, d);
+    if (dmax0_ > kX0Dmax) x0_fused_ = false; // level-0 delays too long for the fused form
+}
+
+// Level-0 program: per channel [first row, rows, computed look-back, largest
+// delay]; per row [D, offset, h] (carried rings: [D, ring start, R]). The
+// input history H covers the earliest sample a level-0 value reads: its
+// computed look-back (when level 0 is formed inside the first launch, the
+// look-back that launch reads) plus its delay.
+void Engine::build_init() {
+    const auto& l0     = plan_.levels[0];
+    const uint64_t nch = plan_.cfg.nchans;
+    const bool ring0   = carried_ && !x0_fused_;
+    std::vector<int32_t> chan(4 * nch, 0);
+    std::vector<int64_t> rows(3 * l0.size());
+    for (std::size_t r = 0; r < l0.size(); ++r) {
+        const int32_t c = l0.chan_or_lo[r];
+        const int32_t h = ring0 ? 0 : hist_[0][r];
+        if (r == 0 || l0.chan_or_lo[r - 1] != c) chan[4 * c] = static_cast<int32_t>(r);
+        chan[4 * c + 1] += 1;
+        chan[4 * c + 2] = std::max(chan[4 * c + 2], h);
+        chan[4 * c + 3] = std::max(chan[4 * c + 3], l0.delay[r]);
+        rows[3 * r]     = l0.delay[r];
+        rows[3 * r + 1] = base_[0][r];
+        rows[3 * r + 2] = ring0 ? ring_len_[0][r] : h;
+    }
+    hist_in_ = 0;
+    for (uint64_t c = 0; c < nch; ++c)
+        hist_in_ = std::max<uint64_t>(hist_in_, static_cast<uint64_t>(chan[4 * c + 2]) +
+                                                    static_cast<uint64_t>(chan[4 * c + 3]));
+    xstride_ = round64(hist_in_ + cap_); // extended input row: H history + capacity
+    init_chan_.resize(chan.size());
+    upload(init_chan_.get(), chan.data(), chan.size() * 4);
+    init_rows_.resize(rows.size());
+    upload(init_rows_.get(), rows.data(), rows.size() * 8);
+}
+
+void Engine::build_searches() {
+    search_rows_.clear();
+    search_rows_.resize(plan_.nlevels);
+    search_count_.assign(plan_.nlevels, 0);
+    for (uint32_t l = 0; l < plan_.nlevels; ++l) {
+        std::vector<SearchRow> sr;
+        for (std::size_t p = 0; p < plan_.planes.size(); ++p) {
+            const auto& pd = plan_.planes[p];
+            if (pd.level != l) continue;
+            for (std::size_t i = 0; i < pd.delay.size(); ++i) {
+                const int32_t r = pd.level_row[i];
+                SearchRow s{};
+                if (carried_) {
+                    s.in   = base_[l][r];
+                    s.ring = ring_len_[l][r];
+                } else {
+                    s.in   = base_[l][r] + (hist_[l][r] - static_cast<int32_t>(wm1_));
+                    s.ring = 0;
+                }
+                s.delay = pd.delay[i];
+                s.prow  = plane_first_[p] + static_cast<uint32_t>(i);
+                sr.push_back(s);
+            }
+        }
+        search_count_[l] = static_cast<uint32_t>(sr.size());
+        if (sr.empty()) continue;
+        search_rows_[l].resize(sr.size());
+        upload(search_rows_[l].get(), sr.data(), sr.size() * sizeof(SearchRow));
+    }
+}
+
+void Engine::build_programs() {
+    if (carried_) {
+        build_carried();
+        return;
+    }
+    const uint32_t L = plan_.nlevels - 1;
+    plane_setup();
+    hist_.assign(plan_.nlevels, {});
+    for (uint32_t l = 0; l <= L; ++l) hist_[l].assign(plan_.levels[l].size(), -1);
     // Look-back per row: planes need W_max - 1, upper halves their shift.
     for (int l = static_cast<int>(L); l >= 0; --l) {
         for (const auto& p : plan_.planes)
@@ -1539,6 +1793,7 @@ void Engine::build_programs() {
     }
     // Buffer layout per level.
     base_.assign(plan_.nlevels, {});
+    ring_len_.assign(plan_.nlevels, {});
     level_len_.assign(plan_.nlevels, 0);
     for (uint32_t l = 0; l <= L; ++l) {
         uint64_t off = 0;
@@ -1550,33 +1805,7 @@ void Engine::build_programs() {
         }
         level_len_[l] = off;
     }
-    // Level 0 program.
-    const auto& l0    = plan_.levels[0];
-    const uint64_t nch = plan_.cfg.nchans;
-    std::vector<int32_t> chan(4 * nch, 0);
-    std::vector<int64_t> rows(3 * l0.size());
-    hist_in_ = 0;
-    dmax0_   = 0;
-    for (std::size_t r = 0; r < l0.size(); ++r) {
-        const int32_t c = l0.chan_or_lo[r];
-        if (r == 0 || l0.chan_or_lo[r - 1] != c) chan[4 * c] = static_cast<int32_t>(r);
-        chan[4 * c + 1] += 1;
-        chan[4 * c + 2] = std::max(chan[4 * c + 2], hist_[0][r]);
-        chan[4 * c + 3] = std::max(chan[4 * c + 3], l0.delay[r]);
-        rows[3 * r]     = l0.delay[r];
-        rows[3 * r + 1] = base_[0][r];
-        rows[3 * r + 2] = hist_[0][r];
-    }
-    for (uint64_t c = 0; c < nch; ++c) {
-        hist_in_ = std::max<uint64_t>(hist_in_, static_cast<uint64_t>(chan[4 * c + 2]) +
-                                                    static_cast<uint64_t>(chan[4 * c + 3]));
-        dmax0_   = std::max(dmax0_, chan[4 * c + 3]);
-    }
-    xstride_ = round64(hist_in_ + cap_); // extended input row: H history + capacity
-    init_chan_.resize(chan.size());
-    upload(init_chan_.get(), chan.data(), chan.size() * 4);
-    init_rows_.resize(rows.size());
-    upload(init_rows_.get(), rows.data(), rows.size() * 8);
+    build_init();
     // Stored levels: level 0, every level holding a plane, and enough others
     // that no launch spans more than max_fuse_ levels; the fewest stored
     // floats win (each stored level is written and read once per tile).
@@ -1587,7 +1816,6 @@ void Engine::build_programs() {
     constexpr uint64_t kInf = ~uint64_t{0};
     std::vector<uint64_t> best(plan_.nlevels, kInf);
     std::vector<int32_t> prev(plan_.nlevels, -1);
-    if (dmax0_ > kX0Dmax) x0_fused_ = false; // level-0 delays too long for the fused form
     best[0] = x0_fused_ ? 0 : level_len_[0]; // fused: level 0 is formed from the input
     for (uint32_t l = 1; l <= L; ++l) {
         for (uint32_t j = l; j-- > 0 && l - j <= F;) {
@@ -1613,96 +1841,204 @@ void Engine::build_programs() {
         level_buf_[stored[i]] = static_cast<int>(k++ & 1);
     }
     launches_.clear();
-    for (std::size_t i = 1; i < stored.size(); ++i) {
-        const uint32_t a = stored[i - 1], b = stored[i];
-        level_source_[b] = static_cast<int32_t>(a);
-        MergeLaunch ml;
-        ml.in_level  = a;
-        ml.out_level = b;
-        ml.radix     = 1 << (b - a);
-        ml.in_buf     = level_buf_[a];
-        ml.out_buf    = level_buf_[b];
-        ml.from_input = a == 0 && x0_fused_;
-        const auto& lv = plan_.levels[b];
-        ml.nrows       = static_cast<uint32_t>(lv.size());
-        const int K    = ml.radix;
-        std::vector<int64_t> prog(static_cast<std::size_t>(ml.nrows) * (K + 2));
-        std::vector<std::pair<int32_t, int64_t>> leaves;
-        // Descendants of (level, row) at level a in tree order, with the
-        // summed shift of the upper branches taken on the way down.
-        auto expand = [&](auto&& self, uint32_t l, int32_t r, int64_t sh) -> void {
-            if (l == a) {
-                leaves.emplace_back(r, sh);
-                return;
-            }
-            const auto& v = plan_.levels[l];
-            self(self, l - 1, v.chan_or_lo[r], sh);
-            self(self, l - 1, v.hi[r], sh + v.shift[r]);
-        };
-        for (uint32_t r = 0; r < ml.nrows; ++r) {
-            const int32_t h = hist_[b][r];
-            ml.hmax         = std::max(ml.hmax, h);
-            leaves.clear();
-            expand(expand, b, static_cast<int32_t>(r), 0);
-            int64_t* row = prog.data() + static_cast<std::size_t>(r) * (K + 2);
-            row[0]       = base_[b][r];
-            for (int k = 0; k < K; ++k) {
-                const auto [c, sh] = leaves[k];
-                if (hist_[a][c] < h + sh) throw std::logic_error("FDMT look-back too short");
-                row[1 + k] = base_[a][c] + (hist_[a][c] - h - sh);
-            }
-            row[K + 1] = h;
+    for (std::size_t i = 1; i < stored.size(); ++i)
+        add_launch(stored[i - 1], stored[i]);
+    build_searches();
+}
+
+// Descendants of (level l, row r) at level a in tree order, with the summed
+// shift of the upper branches taken on the way down.
+static void tree_leaves(const FdmtPlan& plan, uint32_t a, uint32_t l, int32_t r, int64_t sh,
+                        std::vector<std::pair<int32_t, int64_t>>& out) {
+    if (l == a) {
+        out.emplace_back(r, sh);
+        return;
+    }
+    const auto& v = plan.levels[l];
+    tree_leaves(plan, a, l - 1, v.chan_or_lo[r], sh, out);
+    tree_leaves(plan, a, l - 1, v.hi[r], sh + v.shift[r], out);
+}
+
+// The launch computing level b from stored level a: its direct program
+// [out, src_0 .. src_{K-1}, h] (linear rows only) and its staged program.
+void Engine::add_launch(uint32_t a, uint32_t b) {
+    level_source_[b] = static_cast<int32_t>(a);
+    MergeLaunch ml;
+    ml.in_level   = a;
+    ml.out_level  = b;
+    ml.radix      = 1 << (b - a);
+    ml.in_buf     = level_buf_[a];
+    ml.out_buf    = level_buf_[b];
+    ml.from_input = a == 0 && x0_fused_;
+    const auto& lv = plan_.levels[b];
+    ml.nrows       = static_cast<uint32_t>(lv.size());
+    const int K    = ml.radix;
+    std::vector<GatherRow> orow(ml.nrows);
+    std::vector<std::pair<int32_t, int32_t>> rl(static_cast<std::size_t>(ml.nrows) * K);
+    std::vector<int64_t> prog;
+    if (!carried_) prog.resize(static_cast<std::size_t>(ml.nrows) * (K + 2));
+    std::vector<std::pair<int32_t, int64_t>> leaves;
+    for (uint32_t r = 0; r < ml.nrows; ++r) {
+        // Carried rows compute only the tile; linear rows their look-back too.
+        const int32_t h = carried_ ? 0 : hist_[b][r];
+        ml.hmax         = std::max(ml.hmax, h);
+        leaves.clear();
+        tree_leaves(plan_, a, b, static_cast<int32_t>(r), 0, leaves);
+        orow[r] = GatherRow{carried_ ? base_[b][r] : base_[b][r] + h, h,
+                            carried_ ? ring_len_[b][r] : 0};
+        for (int k = 0; k < K; ++k) {
+            const auto [c, sh] = leaves[k];
+            if (hist_[a][c] < h + sh) throw std::logic_error("FDMT look-back too short");
+            rl[static_cast<std::size_t>(r) * K + k] = {c, static_cast<int32_t>(sh)};
+            if (!carried_)
+                prog[static_cast<std::size_t>(r) * (K + 2) + 1 + k] =
+                    base_[a][c] + (hist_[a][c] - h - sh);
         }
+        if (!carried_) {
+            prog[static_cast<std::size_t>(r) * (K + 2)]         = base_[b][r];
+            prog[static_cast<std::size_t>(r) * (K + 2) + K + 1] = h;
+        }
+    }
+    if (!prog.empty()) {
         ml.prog.resize(prog.size());
         upload(ml.prog.get(), prog.data(), prog.size() * sizeof(int64_t));
-        build_gather(ml, prog);
-        launches_.push_back(std::move(ml));
     }
-    // Search programs per level.
-    search_rows_.clear();
-    search_rows_.resize(plan_.nlevels);
-    search_count_.assign(plan_.nlevels, 0);
-    for (uint32_t l = 0; l <= L; ++l) {
-        std::vector<SearchRow> sr;
-        for (std::size_t p = 0; p < plan_.planes.size(); ++p) {
-            const auto& pd = plan_.planes[p];
-            if (pd.level != l) continue;
-            for (std::size_t i = 0; i < pd.delay.size(); ++i) {
-                const int32_t r = pd.level_row[i];
-                SearchRow s;
-                s.in    = base_[l][r] + (hist_[l][r] - static_cast<int32_t>(wm1_));
-                s.delay = pd.delay[i];
-                s.prow  = plane_first_[p] + static_cast<uint32_t>(i);
-                sr.push_back(s);
+    build_gather(ml, orow, rl);
+    launches_.push_back(std::move(ml));
+}
+
+// Carried form. Stored level j's rows are rings of R = round64(c + T)
+// samples, c being the look-back the launch reading j applies (its largest
+// summed upper-branch shift) and W_max - 1 for searched rows; the stored
+// levels are those with the fewest ring floats, launches spanning at most
+// max_fuse_ levels and every plane level stored.
+void Engine::carried_layout() {
+    const uint32_t L = plan_.nlevels - 1;
+    plane_setup();
+    std::vector<std::vector<uint8_t>> searched(plan_.nlevels);
+    for (uint32_t l = 0; l <= L; ++l) searched[l].assign(plan_.levels[l].size(), 0);
+    for (const auto& p : plan_.planes)
+        for (const int32_t r : p.level_row) searched[p.level][r] = 1;
+    auto need_of = [&](uint32_t j, uint32_t l) {
+        std::vector<int32_t> cur(plan_.levels[l].size(), 0), nxt;
+        for (uint32_t m = l; m > j; --m) {
+            const auto& v = plan_.levels[m];
+            nxt.assign(plan_.levels[m - 1].size(), 0);
+            for (std::size_t r = 0; r < v.size(); ++r) {
+                int32_t& lo = nxt[v.chan_or_lo[r]];
+                int32_t& hi = nxt[v.hi[r]];
+                lo          = std::max(lo, cur[r]);
+                hi          = std::max(hi, cur[r] + v.shift[r]);
             }
+            cur.swap(nxt);
         }
-        search_count_[l] = static_cast<uint32_t>(sr.size());
-        if (sr.empty()) continue;
-        search_rows_[l].resize(sr.size());
-        upload(search_rows_[l].get(), sr.data(), sr.size() * sizeof(SearchRow));
+        for (std::size_t r = 0; r < cur.size(); ++r)
+            if (searched[j][r]) cur[r] = std::max<int32_t>(cur[r], static_cast<int32_t>(wm1_));
+        return cur;
+    };
+    auto ring_floats = [&](const std::vector<int32_t>& need) {
+        uint64_t n = 0;
+        for (const int32_t c : need) n += round64(static_cast<uint64_t>(c) + tile_);
+        return n;
+    };
+    std::vector<bool> must(plan_.nlevels, false);
+    must[0] = must[L] = true;
+    for (const auto& p : plan_.planes) must[p.level] = true;
+    const uint32_t F = std::max<uint32_t>(1, std::min<uint32_t>(max_fuse_, 4));
+    constexpr uint64_t kInf = ~uint64_t{0};
+    std::vector<uint64_t> best(plan_.nlevels, kInf);
+    std::vector<int32_t> prev(plan_.nlevels, -1);
+    best[0] = 0; // level 0's rings are charged to the launch that reads them
+    for (uint32_t l = 1; l <= L; ++l) {
+        for (uint32_t j = l; j-- > 0 && l - j <= F;) {
+            if (best[j] != kInf) {
+                const uint64_t c = (j == 0 && x0_fused_) ? 0 : ring_floats(need_of(j, l));
+                if (best[j] + c < best[l]) {
+                    best[l] = best[j] + c;
+                    prev[l] = static_cast<int32_t>(j);
+                }
+            }
+            if (must[j]) break; // a plane level cannot be skipped
+        }
     }
+    carried_stored_.clear();
+    for (int32_t l = static_cast<int32_t>(L); l > 0; l = prev[l]) {
+        if (l < 0 || prev[l] < 0) throw std::logic_error("FDMT level fusion has no solution");
+        carried_stored_.push_back(static_cast<uint32_t>(l));
+    }
+    carried_stored_.push_back(0);
+    std::reverse(carried_stored_.begin(), carried_stored_.end());
+    // Ring geometry, one region per stored level in one buffer.
+    hist_.assign(plan_.nlevels, {});
+    base_.assign(plan_.nlevels, {});
+    ring_len_.assign(plan_.nlevels, {});
+    level_len_.assign(plan_.nlevels, 0);
+    for (uint32_t l = 0; l <= L; ++l) {
+        hist_[l].assign(plan_.levels[l].size(), 0);
+        base_[l].assign(plan_.levels[l].size(), 0);
+        ring_len_[l].assign(plan_.levels[l].size(), 0);
+    }
+    uint64_t off = 0;
+    for (std::size_t i = 0; i < carried_stored_.size(); ++i) {
+        const uint32_t j = carried_stored_[i];
+        hist_[j] = i + 1 < carried_stored_.size() ? need_of(j, carried_stored_[i + 1]) : need_of(j, j);
+        if (j == 0 && x0_fused_) continue; // formed from the input: no rings
+        for (std::size_t r = 0; r < hist_[j].size(); ++r) {
+            const uint64_t R = round64(static_cast<uint64_t>(hist_[j][r]) + tile_);
+            if (R >= (uint64_t{1} << 29))
+                throw std::invalid_argument("FDMT ring longer than 2^29 samples; lower the time tile");
+            base_[j][r]     = static_cast<int64_t>(off);
+            ring_len_[j][r] = static_cast<int32_t>(R);
+            off += R;
+            level_len_[j] += R;
+        }
+    }
+    rings_floats_ = off;
+    // The input history (as build_init computes it): per channel, the
+    // largest level-0 look-back read from the input plus its largest delay.
+    const auto& l0 = plan_.levels[0];
+    std::vector<uint64_t> hmax(plan_.cfg.nchans, 0), dmax(plan_.cfg.nchans, 0);
+    for (std::size_t r = 0; r < l0.size(); ++r) {
+        const int32_t c = l0.chan_or_lo[r];
+        if (x0_fused_) hmax[c] = std::max<uint64_t>(hmax[c], static_cast<uint64_t>(hist_[0][r]));
+        dmax[c] = std::max<uint64_t>(dmax[c], static_cast<uint64_t>(l0.delay[r]));
+    }
+    hist_in_ = 0;
+    for (uint64_t c = 0; c < plan_.cfg.nchans; ++c) hist_in_ = std::max(hist_in_, hmax[c] + dmax[c]);
+    xstride_ = round64(hist_in_ + cap_);
+}
+
+void Engine::build_carried() {
+    carried_layout();
+    const auto& st = carried_stored_;
+    level_source_.assign(plan_.nlevels, -2);
+    level_buf_.assign(plan_.nlevels, -1);
+    level_source_[0] = x0_fused_ ? -2 : -1;
+    for (const uint32_t j : st)
+        if (!(j == 0 && x0_fused_)) level_buf_[j] = 0; // every ring is in one buffer
+    build_init();
+    launches_.clear();
+    for (std::size_t i = 1; i < st.size(); ++i) add_launch(st[i - 1], st[i]);
+    build_searches();
+}
+
+uint64_t Engine::carried_need(uint64_t tile) {
+    tile_ = tile;
+    carried_layout();
+    const uint64_t nch = plan_.cfg.nchans;
+    return (nch * xstride_ + rings_floats_) * sizeof(float);
 }
 
 // Splits a launch's output rows into runs whose distinct input rows (with
 // the spread of shifts their consumers apply) fit the LDS budget, and emits
 // the staged program. prog holds the direct program [out, src_0..K-1, h].
-void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) const {
+void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
+                          const std::vector<std::pair<int32_t, int32_t>>& rl) const {
     const int K        = ml.radix;
     const uint32_t a   = ml.in_level;
     const auto& base_a = base_[a];
     const auto& hist_a = hist_[a];
-    // Input row and shift of each source: src = base_a[c] + h_a[c] - h - sh.
-    // Recover (c, sh) by matching base offsets (rows' regions are disjoint).
-    std::vector<std::pair<int64_t, int32_t>> starts; // (base, row)
-    starts.reserve(base_a.size());
-    for (std::size_t c = 0; c < base_a.size(); ++c)
-        starts.emplace_back(base_a[c], static_cast<int32_t>(c));
-    std::sort(starts.begin(), starts.end());
-    auto row_of = [&](int64_t off) {
-        auto it = std::upper_bound(starts.begin(), starts.end(),
-                                   std::make_pair(off, std::numeric_limits<int32_t>::max()));
-        return (--it)->second;
-    };
+    const auto& ring_a = ring_len_[a];
     struct Lf {
         int32_t c;
         int32_t shmin, shmax;
@@ -1718,7 +2054,6 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
     if (2 * static_cast<uint64_t>(gather_lds_) * sizeof(float) > 160 * 1024)
         throw std::invalid_argument("FROST_AMD_GATHER_LDS too large");
     ml.max_leaves = static_cast<uint32_t>(max_leaves);
-    std::vector<std::pair<int32_t, int32_t>> rl(static_cast<std::size_t>(ml.nrows) * K);
     uint32_t first = 0, lds_max = 0;
     int32_t hmax = 0;
     // Level-0 rows formed from the input: rows of one channel share a
@@ -1803,12 +2138,14 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
         for (std::size_t i = 0; i < cur.size(); ++i) {
             const auto& x = cur[i];
             GatherLeaf f{};
-            f.src   = base_a[x.c] + hist_a[x.c] - x.shmax;
+            // Linear rows: offset of sample t0 - shmax; rings: the ring start.
+            f.src   = carried_ ? base_a[x.c] : base_a[x.c] + hist_a[x.c] - x.shmax;
             f.slot  = off;
             f.len   = kChunkG + (x.shmax - x.shmin);
             f.shmax = x.shmax;
             f.hc    = hist_a[x.c];
             f.delay = x0 ? l0.delay[x.c] : 0;
+            f.ring  = carried_ && !x0 ? ring_a[x.c] : 0;
             slot[i] = off;
             off += f.len;
             if (x0) {
@@ -1842,19 +2179,18 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
         gb.push_back(b);
     };
     for (uint32_t r = 0; r < ml.nrows; ++r) {
-        const int64_t* row = prog.data() + static_cast<std::size_t>(r) * (K + 2);
-        const int32_t h    = static_cast<int32_t>(row[K + 1]);
-        const int64_t o    = row[0] + h;
+        const int32_t h    = orow[r].h;
+        const int64_t o    = orow[r].out;
         rec[r * RS + 0]    = static_cast<int32_t>(static_cast<uint32_t>(o & 0xffffffff));
         rec[r * RS + 1]    = static_cast<int32_t>(o >> 32);
         rec[r * RS + 2]    = h;
+        rec[r * RS + 3]    = orow[r].ring;
         tent               = cur;
         for (int k = 0; k < K; ++k) {
-            const int32_t c  = row_of(row[1 + k]);
-            const int64_t sh = base_a[c] + hist_a[c] - h - row[1 + k];
-            if (sh < 0 || sh > std::numeric_limits<int32_t>::max() / 2)
+            const auto [c, sh32] = rl[static_cast<std::size_t>(r) * K + k];
+            const int64_t sh     = sh32;
+            if (sh < 0 || sh > std::numeric_limits<int32_t>::max() / 2 || h + sh > hist_a[c])
                 throw std::logic_error("FDMT gather shift out of range");
-            rl[static_cast<std::size_t>(r) * K + k] = {c, static_cast<int32_t>(sh)};
             bool found = false;
             for (auto& x : tent)
                 if (x.c == c) {
@@ -1867,7 +2203,7 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) con
         bool ok          = true;
         const uint64_t n = need(tent, ok);
         if (r > first && (!ok || n > gather_lds_ || (!x0 && tent.size() > max_leaves) ||
-                          r - first >= 128)) {
+                          r - first >= static_cast<uint32_t>(kGatherRowsMax))) {
             close(r);
             first = r;
             hmax  = 0;
@@ -2113,9 +2449,10 @@ void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
                           s));
     events_used_ = 0;
     sync_used_   = 0;
-    // Searches still reading each level buffer (their last event), and the
-    // last search of the call.
-    hipEvent_t reading[2] = {nullptr, nullptr};
+    // Searches still reading each level buffer (their last event; carried:
+    // each level's rings), and the last search of the call.
+    std::vector<hipEvent_t> reading(plan_.nlevels, nullptr);
+    auto buf_id = [&](uint32_t l) { return carried_ ? static_cast<int>(l) : level_buf_[l]; };
     hipEvent_t last_search = nullptr;
     const int gchunks      = gather_chunks();
     begin_gulp_kernel<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(counters_.get()),
@@ -2126,20 +2463,23 @@ void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
         for (uint32_t l = 0; l <= L; ++l) {
             if (level_source_[l] == -2) continue; // computed inside a fused launch
             float* out = buf_ptr_[level_buf_[l]];
-            if (hipEvent_t& r = reading[level_buf_[l]]) {
+            if (hipEvent_t& r = reading[buf_id(l)]) {
                 FA_HIP(hipStreamWaitEvent(s, r, 0)); // searches of the old contents done
                 r = nullptr;
             }
             record_pair(static_cast<int>(l), s, true);
             if (l == 0) {
-                int hcmax = 0;
-                for (const int32_t h : hist_[0]) hcmax = std::max(hcmax, h);
+                int hcmax = 0; // computed look-back (none for rings)
+                if (!carried_)
+                    for (const int32_t h : hist_[0]) hcmax = std::max(hcmax, h);
                 const unsigned gx = ceil_div(tl + hcmax + 64, kBlock * kInitPer);
                 const unsigned gy = static_cast<unsigned>(std::min<uint64_t>(nch, 65535));
                 const size_t lds  = (kBlock * kInitPer + dmax0_) * sizeof(float);
                 fdmt_init_kernel<<<dim3(gx, gy), kBlock, lds, s>>>(
                     x_ptr_, xstride_, static_cast<int64_t>(hist_in_ + t0), init_chan_.get(),
-                    init_rows_.get(), out, tl, static_cast<int>(nch));
+                    init_rows_.get(), out, tl, static_cast<int>(nch),
+                    reinterpret_cast<const uint64_t*>(params_.get()), static_cast<int64_t>(t0),
+                    carried_ ? 1 : 0);
             } else {
                 const MergeLaunch* ml = nullptr;
                 for (const auto& mlc : launches_)
@@ -2151,12 +2491,13 @@ void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
                 const float* x0 = x_ptr_ + hist_in_ + t0;
                 const int xlo   = -static_cast<int>(hist_in_ + t0);
                 const int xhi   = static_cast<int>(xstride_) - static_cast<int>(hist_in_ + t0) - 1;
+                const auto* gabs_p = reinterpret_cast<const uint64_t*>(params_.get());
                 if (staged_merge_) {
                     switch (ml->radix) {
-                    case 2: launch_gather<2>(*ml, in, out, tl, x0, xlo, xhi, gchunks, s); break;
-                    case 4: launch_gather<4>(*ml, in, out, tl, x0, xlo, xhi, gchunks, s); break;
-                    case 8: launch_gather<8>(*ml, in, out, tl, x0, xlo, xhi, gchunks, s); break;
-                    case 16: launch_gather<16>(*ml, in, out, tl, x0, xlo, xhi, gchunks, s); break;
+                    case 2: launch_gather<2>(*ml, in, out, tl, x0, xlo, xhi, gchunks, carried_, gabs_p, static_cast<int64_t>(t0), s); break;
+                    case 4: launch_gather<4>(*ml, in, out, tl, x0, xlo, xhi, gchunks, carried_, gabs_p, static_cast<int64_t>(t0), s); break;
+                    case 8: launch_gather<8>(*ml, in, out, tl, x0, xlo, xhi, gchunks, carried_, gabs_p, static_cast<int64_t>(t0), s); break;
+                    case 16: launch_gather<16>(*ml, in, out, tl, x0, xlo, xhi, gchunks, carried_, gabs_p, static_cast<int64_t>(t0), s); break;
                     default: throw std::logic_error("FDMT launch radix must be 2 to 16");
                     }
                 } else {
@@ -2178,7 +2519,7 @@ void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
                 launch_search(l, out, tl, t0, search_stream_);
                 record_pair(-1, search_stream_, false);
                 last_search = sync_event(search_stream_);
-                reading[level_buf_[l]] = last_search;
+                reading[buf_id(l)] = last_search;
             }
         }
     }
@@ -2292,6 +2633,8 @@ void Engine::launch_search(uint32_t l, const float* buf, int tl, uint64_t t0, hi
     a.cands    = cands_.get();
     a.counter  = reinterpret_cast<unsigned long long*>(counters_.get());
     a.cap      = cand_cap_;
+    a.spill    = spill_dev_;
+    a.spill_cap = spill_cap_;
     a.keep     = keep_planes_ ? keep_.get() : nullptr;
     a.keep_stride = cap_;
     if (reg_search_) {
@@ -2407,7 +2750,7 @@ void Engine::collect(hipStream_t s) {
     uint64_t ctr[2];
     FA_HIP(hipMemcpyAsync(ctr, counters_.get(), sizeof(ctr), hipMemcpyDeviceToHost, s));
     FA_HIP(hipStreamSynchronize(s));
-    if (ctr[0] > cand_cap_) {
+    if (ctr[0] > cand_cap_ && !carried_) {
         // More records than the buffer holds: which ones were stored depends
         // on the order of the atomic appends. Grow the buffer and search the
         // gulp again from the runs open before it (its input is still in
@@ -2435,12 +2778,23 @@ void Engine::collect(hipStream_t s) {
     carry_history(s);
     frontier_             = ctr[1];
     raw_candidates_       = ctr[0];
-    const uint64_t stored = ctr[0];
+    // Carried form: records past the device buffer are in the host spill;
+    // past both, some are lost (reported below, after this gulp's rows).
+    const uint64_t stored = std::min<uint64_t>(ctr[0], cand_cap_ + spill_cap_);
+    const uint64_t on_dev = std::min<uint64_t>(stored, cand_cap_);
     recs_.resize(stored);
-    if (stored) {
-        FA_HIP(hipMemcpyAsync(recs_.data(), cands_.get(), stored * sizeof(CandRecord),
+    if (on_dev) {
+        FA_HIP(hipMemcpyAsync(recs_.data(), cands_.get(), on_dev * sizeof(CandRecord),
                               hipMemcpyDeviceToHost, s));
         FA_HIP(hipStreamSynchronize(s));
+    }
+    if (stored > on_dev) std::memcpy(recs_.data() + on_dev, spill_, (stored - on_dev) * sizeof(CandRecord));
+    const bool lost = ctr[0] > stored;
+    if (carried_ && ctr[0] > cand_cap_) {
+        // Make room on the device for the next gulps.
+        cand_cap_ = std::min<uint64_t>(std::max<uint64_t>(ctr[0], 2 * cand_cap_), kCandCapMax);
+        cands_.resize(cand_cap_);
+        drop_graphs();
     }
     // Order of frost's caps: (subband level, subband idx, dm idx, filter idx,
     // peak). Planes are numbered by (depth, idx) and a plane row is
@@ -2503,6 +2857,11 @@ void Engine::collect(hipStream_t s) {
     collect_seconds_ =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
     timing_collect();
+    if (lost)
+        throw std::runtime_error(
+            "the search emitted " + std::to_string(ctr[0]) + " raw candidates in one gulp, more " +
+            "than the " + std::to_string(stored) + " the carried FDMT state can hold; the " +
+            "gulp's rows are incomplete (raise detect_thresh)");
 }
 
 void Engine::drain(hipStream_t s) {
@@ -2558,16 +2917,17 @@ uint64_t Engine::memory_bytes() const {
         tables += ml.prog.bytes() + ml.blocks.bytes() + ml.leaves.bytes() + ml.chans.bytes() +
                   ml.rows.bytes();
     for (const auto& sr : search_rows_) tables += sr.bytes();
-    return x_.bytes() + buf_[0].bytes() + buf_[1].bytes() + hist_store_.bytes() +
+    return x_.bytes() + buf_[0].bytes() + buf_[1].bytes() + rings_.bytes() + hist_store_.bytes() +
            inv_cur_.bytes() + inv_or_.bytes() + open_.bytes() + open_saved_.bytes() + cands_.bytes() + keep_.bytes() + scratch_.bytes() +
            init_chan_.bytes() + init_rows_.bytes() + widths_dev_.bytes() + counters_.bytes() +
            tables;
 }
 
 std::vector<uint64_t> Engine::level_samples() const {
+    // Carried rows compute only the tile; linear rows their look-back too.
     std::vector<uint64_t> out(plan_.nlevels, 0);
     for (uint32_t l = 0; l < plan_.nlevels; ++l)
-        for (const int32_t h : hist_[l]) out[l] += tile_ + static_cast<uint64_t>(h);
+        for (const int32_t h : hist_[l]) out[l] += tile_ + (carried_ ? 0 : static_cast<uint64_t>(h));
     return out;
 }
 

@@ -21,10 +21,19 @@
 //
 // Device layout: the input is an extended channel-major buffer
 // x[nchans][H + capacity] whose first H columns carry the previous gulp's
-// last H samples (H = dt_max + W_max - 1 at most). Each gulp is processed in
-// time tiles of T samples; a level's rows live in one buffer, row r holding
-// the T + h_r samples [t0 - h_r, t0 + T), where h_r is the look-back its
-// consumers need.
+// last H samples. Each gulp is processed in time tiles of T samples. Two
+// forms of the level state:
+//  * recomputed (default while it fits): a level's rows live in one of two
+//    ping-pong buffers, row r holding the T + h_r samples [t0 - h_r, t0 + T),
+//    where h_r is the look-back every later launch and search needs (summed
+//    down the tree), recomputed from x each tile (H = dt_max + W_max - 1 at
+//    most);
+//  * carried (large beams): every stored level row is a ring of
+//    R_r = c_r + T samples (rounded to 64), c_r being the look-back its own
+//    consumers read (the shifts of the launch that reads it, W_max - 1 for a
+//    searched row). Sample t sits at ring position t mod R_r, each tile
+//    computes only its T new samples and the rings keep the rest across
+//    tiles and gulps (H = the level-0 delay reach only).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,6 +49,9 @@ namespace fa {
 // Bounds of the automatically chosen FDMT time tile (samples per pass).
 constexpr uint64_t kAutoTileMin = uint64_t{1} << 14;
 constexpr uint64_t kAutoTileMax = uint64_t{1} << 20;
+// Bounds of the automatically chosen tile of the carried level state.
+constexpr uint64_t kCarriedTileMin = uint64_t{1} << 11;
+constexpr uint64_t kCarriedTileMax = uint64_t{1} << 16;
 
 struct CandRecord {
     uint64_t peak, begin, end;
@@ -69,10 +81,10 @@ struct GatherBlock {
 // An input row staged for one row block: LDS [slot, slot + len) holds the
 // row's samples t0 + tau0 - shmax + j, j < len (zero outside [t0 - hc, t0 + T)).
 struct GatherLeaf {
-    int64_t src; // offset of sample t0 - shmax in the input level buffer
+    int64_t src; // offset of sample t0 - shmax in the input level buffer (carried: ring start)
     int32_t slot, len, shmax, hc;
     int32_t delay; // level-0 input rows computed from the input: the row's delay D
-    int32_t pad;
+    int32_t ring;  // carried state: the input row's ring length (0: linear row)
 };
 
 // Level-0 input rows formed from the input waterfall by the first launch:
@@ -110,16 +122,22 @@ struct MergeLaunch {
     DeviceBuffer<GatherBlock> blocks;
     DeviceBuffer<GatherLeaf> leaves;
     DeviceBuffer<GatherChan> chans;
-    // Per output row: [out offset of sample t0 (lo, hi), h, 0, K LDS offsets
-    // of the descendants at chunk sample 0].
+    // Per output row: [out offset of sample t0 (lo, hi), h, ring length (0:
+    // linear row), K LDS offsets of the descendants at chunk sample 0]; a
+    // carried row's offset is its ring start.
     DeviceBuffer<int32_t> rows;
 };
 
 struct SearchRow {
-    int64_t in;     // offset of sample t0 - (W_max - 1) in the level buffer
+    int64_t in;     // offset of sample t0 - (W_max - 1) in the level buffer (carried: ring start)
     int32_t delay;  // plane row delay (cold prefix / OR-mask window)
     uint32_t prow;  // global plane row
+    int32_t ring;   // carried state: ring length (0: linear row)
+    int32_t pad;
 };
+
+// Level state forms (FROST_AMD_FDMT_STATE=recompute|carried forces one).
+enum class LevelState { automatic, recompute, carried };
 
 class Engine {
 public:
@@ -128,8 +146,12 @@ public:
     // each gulp); the engine keeps only its carried input history, open runs,
     // noise tables and candidate buffer, and picks the largest time tile
     // whose buffers fit the arena.
+    // state: the level-state form; automatic keeps the recomputed form when
+    // its buffers fit (75 % of the free device memory at the smallest tile)
+    // and otherwise carries the levels in rings.
     Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t* kill_mask,
-           int device, uint64_t tile = 0, uint64_t arena_bytes = 0);
+           int device, uint64_t tile = 0, uint64_t arena_bytes = 0,
+           LevelState state = LevelState::automatic);
     ~Engine();
     Engine(const Engine&)            = delete;
     Engine& operator=(const Engine&) = delete;
@@ -203,6 +225,7 @@ public:
     std::vector<double> level_seconds, level_launches;
     double search_seconds = 0, total_seconds = 0;
     uint64_t tile() const { return tile_; }
+    bool carried() const { return carried_; }
     // Host side of the last collect: seconds, and candidates the search emitted.
     double collect_seconds() const { return collect_seconds_; }
     uint64_t raw_candidates() const { return raw_candidates_; }
@@ -217,6 +240,13 @@ public:
 private:
     void compute_inv(const std::vector<uint8_t>& mask, std::vector<float>& out) const;
     void build_programs();
+    void build_carried();   // programs and ring layout of the carried form
+    void plane_setup();     // plane bookkeeping shared by both forms
+    void build_init();      // level-0 program from hist_ / base_ / ring_len_
+    void build_searches();  // search programs from hist_ / base_ / ring_len_
+    uint64_t carried_need(uint64_t tile); // device bytes of the carried form at a tile
+    void carried_layout();  // carried form: stored levels and ring geometry at tile_
+    void add_launch(uint32_t a, uint32_t b);
     void timing_collect();
     void launch_search(uint32_t level, const float* buf, int tl, uint64_t t0, hipStream_t ss);
     hipGraphExec_t gulp_graph(uint64_t nsamps, hipStream_t s);
@@ -239,11 +269,30 @@ private:
     std::vector<MergeLaunch> launches_;
     std::vector<int32_t> level_source_; // level read to make l; -1 input; -2 not stored
     std::vector<int> level_buf_;        // buffer holding a stored level
+    // Carried form: ring length per stored row (0 elsewhere), and the rings.
+    bool carried_ = false;
+    std::vector<uint32_t> carried_stored_;
+    std::vector<std::vector<int32_t>> ring_len_;
+    DeviceBuffer<float> rings_;
+    uint64_t rings_floats_ = 0;
+    // Candidate records past the device buffer (carried form: a gulp cannot
+    // be searched again, its level state has moved on), in pinned host memory.
+    CandRecord* spill_     = nullptr; // host address
+    CandRecord* spill_dev_ = nullptr; // its device address
+    uint64_t spill_cap_    = 0;
     uint32_t max_fuse_   = 4;
     bool staged_merge_   = true;
     bool x0_fused_       = false; // level 0 formed inside the first gather launch
     uint32_t gather_lds_ = 8192; // LDS floats per staged row block (one half)
-    void build_gather(MergeLaunch& ml, const std::vector<int64_t>& prog) const;
+    // Staged program of a launch: per output row its offset (sample t0, or
+    // ring start), computed look-back h, ring length (0: linear) and its K
+    // descendants (input row, summed shift) in tree order.
+    struct GatherRow {
+        int64_t out;
+        int32_t h, ring;
+    };
+    void build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
+                      const std::vector<std::pair<int32_t, int32_t>>& rl) const;
     // Level 0 program: per channel first row, row count, hc, Dmax; per row D,
     // out base, h.
     DeviceBuffer<int32_t> init_chan_;  // [nch][4]

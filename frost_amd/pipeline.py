@@ -101,6 +101,9 @@ class Beam:
     def level_samples(self):
         return search.engine_level_samples(self.engine_handle())
 
+    def engine_state(self) -> dict:
+        return search.engine_state(self.engine_handle())
+
     def host_stats(self):
         """(collect seconds, candidates emitted before caps) of the last gulp."""
         sec, n = C.c_double(), C.c_uint64()

@@ -146,6 +146,12 @@ def engine_level_sources(handle) -> np.ndarray:
     return src
 
 
+def engine_state(handle) -> dict:
+    c, t, b = C.c_int(), C.c_uint64(), C.c_uint64()
+    check(lib().fa_engine_state(handle, C.byref(c), C.byref(t), C.byref(b)))
+    return dict(carried=bool(c.value), tile=t.value, state_bytes=b.value)
+
+
 def engine_level_samples(handle) -> np.ndarray:
     """Per FDMT level: samples one full time tile holds over its rows."""
     nl = C.c_uint64()
@@ -261,6 +267,14 @@ class FDMTBoxcar:
         b = C.c_uint64()
         check(lib().fa_engine_memory_bytes(self.h, C.byref(b)))
         return b.value
+
+    def state(self) -> dict:
+        """Level-state form: carried rings or recomputed look-back, the time
+        tile, and the device bytes of the input and level buffers."""
+        return engine_state(self.h)
+
+    def carried(self) -> bool:
+        return self.state()["carried"]
 
     def host_stats(self):
         """(collect seconds, raw candidates the last gulp's search emitted)."""

@@ -354,6 +354,16 @@ int fa_engine_level_sources(const fa_engine* engine, int32_t* source, uint64_t c
 int fa_engine_level_samples(const fa_engine* engine, uint64_t* samples, uint64_t cap,
                             uint64_t* nlevels);
 
+/* Level-state form: *carried 1 when the stored levels are kept in per-row
+ * rings across tiles and gulps (each tile computes only its new samples),
+ * 0 when each tile recomputes its look-back from the input history; *tile
+ * the time tile; *state_bytes the device bytes of the extended input and
+ * level buffers (rings). The automatic form is the recomputed one when its
+ * buffers fit; FROST_AMD_FDMT_STATE=carried|recompute forces one. No
+ * reference counterpart (dmt's own tree state is internal). */
+int fa_engine_state(const fa_engine* engine, int* carried, uint64_t* tile,
+                    uint64_t* state_bytes);
+
 /* Host side of the last copy_result: wall seconds spent sorting, capping and
  * converting candidates, and the candidates the search emitted before caps.
  * No reference counterpart (diagnostics). */
@@ -468,11 +478,14 @@ int fa_beam_execute(fa_beam* beam, const void* gulp, uint64_t nsamps,
 /* Ingest with overlap (replaces GulpPrefetch, lib/pipeline/pipeline.cpp:
  * 109-262, and its pageable, non-overlapped copies): fa_beam_ingest_slot
  * returns the next of two pinned host slots (*bytes = capacity x row bytes)
- * for the caller to fill with a gulp's rows; fa_beam_ingest_submit starts
- * its copy to the device on the beam's copy stream; fa_beam_execute_next runs
- * the oldest submitted gulp (as fa_beam_execute with that slot) once its copy
- * is done. Submitting gulp k + 1 before executing gulp k overlaps the copy
- * with the compute. At most two gulps are submitted and not yet executed. */
+ * for the caller to fill with a gulp's rows; fa_beam_ingest_submit queues
+ * its copy into the beam's one device slot on the beam's copy stream, which
+ * starts as soon as the slot's previous gulp has been cleaned;
+ * fa_beam_execute_next runs the oldest submitted gulp (as fa_beam_execute
+ * with that slot) once its copy is done. Submitting gulp k + 1 before
+ * executing gulp k overlaps its copy with gulp k's search. At most two gulps
+ * are submitted and not yet executed; a host gulp passed to fa_beam_execute
+ * uses the same device slot and is refused while ingest gulps are pending. */
 int fa_beam_ingest_slot(fa_beam* beam, void** host_slot, uint64_t* bytes);
 int fa_beam_ingest_submit(fa_beam* beam, uint64_t nsamps);
 int fa_beam_execute_next(fa_beam* beam, void* stream);
@@ -492,15 +505,22 @@ int fa_beam_rfi(fa_beam* beam, fa_rfi** rfi);
 /* ---- device memory admission (lib/pipeline/memory.cpp) ------------------ */
 
 /* MemoryEstimate (include/frost/pipeline/memory.hpp:17-26) for this
- * library's beam: device bytes it allocates, by owner. */
+ * library's beam: device bytes it allocates, by owner. The estimate is the
+ * smallest footprint the beam runs in: the FDMT engine at its smallest
+ * automatic tile, in whichever level-state form is smaller (fdmt_state 0:
+ * levels recomputed per tile from the input history; 1: levels carried in
+ * per-row rings, the form large beams take when the first does not fit). */
 typedef struct fa_memory_estimate {
-    uint64_t input_bytes;       /* two device ingest slots of the raw gulp */
+    uint64_t input_bytes;       /* the raw gulp on the device (the ingest slot,
+                                   or the caller's device gulp) */
     uint64_t rfi_bytes;         /* device RFI cleaner */
     uint64_t fdmt_bytes;        /* FDMT engine at `tile` */
     uint64_t postprocess_bytes; /* device pair-search buffers for max_candidates rows */
     uint64_t reserve_bytes;     /* 10 % of the above */
     uint64_t total_bytes;
     uint64_t tile;              /* FDMT time tile the figure is for */
+    uint32_t fdmt_state;        /* 0 recomputed, 1 carried */
+    uint32_t pad_;
 } fa_memory_estimate;
 
 /* MemoryCheck (include/frost/pipeline/memory.hpp:54-62). */

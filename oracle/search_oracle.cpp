@@ -615,25 +615,34 @@ void Engine::execute(const float* wf, SizeType N, const uint8_t* kill_mask) {
             const Node& nd = plan.nodes[id];
             if (nd.level != l) continue;
             cur_rows[id].resize(nd.delays.size());
-            for (std::size_t r = 0; r < nd.delays.size(); ++r) work.push_back({id, r});
+            if (l == 0) work.push_back({id, 0}); // one channel's rows in one pass
+            else
+                for (std::size_t r = 0; r < nd.delays.size(); ++r) work.push_back({id, r});
         }
-#pragma omp parallel for schedule(dynamic, 8)
+#pragma omp parallel for schedule(dynamic, 1)
         for (std::size_t k = 0; k < work.size(); ++k) {
             const std::size_t id = work[k].first, r = work[k].second;
             const Node& nd       = plan.nodes[id];
-            auto& out            = cur_rows[id][r];
-            out.assign(E, 0.0F);
-            const int32_t D = nd.delays[r];
             if (l == 0) {
+                // x(t) + x(t-1) + ... + x(t-D), added in that order: the rows
+                // of a channel (delays ascending) extend one running sum, so
+                // row D continues the additions of the row before it.
                 const float* x = &X[nd.c0 * E];
-                for (SizeType t = 0; t < E; ++t) {
-                    float acc = x[t];
-                    for (int32_t q = 1; q <= D; ++q)
-                        acc = acc + ((int64_t)t - q >= 0 ? x[t - q] : 0.0F);
-                    out[t] = acc;
+                std::vector<float> acc(x, x + E);
+                int32_t q = 0;
+                for (std::size_t rr = 0; rr < nd.delays.size(); ++rr) {
+                    for (; q < nd.delays[rr];) {
+                        ++q;
+                        for (SizeType t = 0; t < E; ++t)
+                            acc[t] = acc[t] + ((int64_t)t - q >= 0 ? x[t - q] : 0.0F);
+                    }
+                    cur_rows[id][rr] = acc;
                 }
                 continue;
             }
+            auto& out       = cur_rows[id][r];
+            out.assign(E, 0.0F);
+            const int32_t D = nd.delays[r];
             const Node& lo    = plan.nodes[nd.lo];
             const Node& hi    = plan.nodes[nd.hi];
             const uint32_t cm = (nd.c0 + nd.c1) / 2;

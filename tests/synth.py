@@ -24,12 +24,14 @@ def noise_byte_block(nsamps=200, nchans=64):
 
 def filterbank(nsamps, nchans, *, seed=1, mean=128.0, sigma=16.0, dtype="u8", tsamp=64e-6,
                f0=1500.0, df=-1.0, pulses=(), rfi_chans=(), rfi_amp=20.0, impulses=(),
-               impulse_amp=8.0, dead=(), gain=None, steps=(), bands=()):
+               impulse_amp=8.0, dead=(), gain=None, steps=(), bands=(), scattered=()):
     """Time-major [nsamps][nchans] block: Gaussian noise (mean, sigma), optional
     per-channel gain, dispersed boxcar pulses (dm, t0, width, amplitude in
-    sigma), sinusoidal narrow-band RFI channels, DM-0 impulses, weak steps
-    (chan, t0, t1, amp), partial-band bursts (t, c0, c1, amp) and dead
-    (constant) channels. uint8 rounds and clips to [0, 255]."""
+    sigma), dispersed scattered pulses (dm, t0, tau, peak amplitude: a
+    one-sided exponential tail of tau samples, cut at 6 tau), sinusoidal
+    narrow-band RFI channels, DM-0 impulses, weak steps (chan, t0, t1, amp),
+    partial-band bursts (t, c0, c1, amp) and dead (constant) channels. uint8
+    rounds and clips to [0, 255]."""
     rng = np.random.default_rng(seed)
     x = rng.standard_normal((nsamps, nchans)).astype(np.float32)
     f = f0 + df * np.arange(nchans)
@@ -40,6 +42,15 @@ def filterbank(nsamps, nchans, *, seed=1, mean=128.0, sigma=16.0, dtype="u8", ts
             a = t0 + d[c]
             if a < nsamps:
                 x[a:min(a + w, nsamps), c] += amp
+    for dm, t0, tau, amp in scattered:
+        d = np.rint(K_DM * dm * (f ** -2.0 - ftop ** -2.0) / tsamp).astype(np.int64)
+        n = max(int(np.ceil(6 * tau)), 1)
+        prof = (amp * np.exp(-np.arange(n, dtype=np.float64) / tau)).astype(np.float32)
+        for c in range(nchans):
+            a = t0 + d[c]
+            if a < nsamps:
+                b = min(a + n, nsamps)
+                x[a:b, c] += prof[:b - a]
     tt = np.arange(nsamps, dtype=np.float32)
     for k, c in enumerate(rfi_chans):
         x[:, c] += rfi_amp * np.sin(2 * np.pi * tt / (37.0 + 11 * k) + k)

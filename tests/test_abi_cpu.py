@@ -32,8 +32,8 @@ def test_struct_layouts():
     assert C.sizeof(abi.Giant) == 160
     assert C.sizeof(abi.SearchConfig) == 72
     assert C.sizeof(abi.ClusterTolerances) == 48
-    assert C.sizeof(abi.MemoryEstimate) == 56
-    assert C.sizeof(abi.MemoryCheck) == 96
+    assert C.sizeof(abi.MemoryEstimate) == 64
+    assert C.sizeof(abi.MemoryCheck) == 104
 
 
 def test_search_config_defaults():

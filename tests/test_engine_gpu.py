@@ -23,14 +23,16 @@ def stream():
 
 
 def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None,
-             search_path=None, fuse=None, merge=None, init=None):
+             search_path=None, fuse=None, merge=None, init=None, state=None):
     """Runs the GPU engine and the oracle on the same gulps; returns both
     result lists (per gulp + drain) and, if keep, the plane rows. `fuse` caps
     the FDMT levels summed per launch (default 4); merge="direct" gathers
     from global memory instead of staging through LDS; init="fused" forms
-    level 0 from the input inside the first launch instead of storing it."""
+    level 0 from the input inside the first launch instead of storing it;
+    state="carried" keeps the stored levels in per-row rings across tiles
+    and gulps instead of recomputing each tile's look-back."""
     env = {"FROST_AMD_TILE": tile, "FROST_AMD_SEARCH": search_path, "FROST_AMD_FUSE": fuse,
-           "FROST_AMD_MERGE": merge, "FROST_AMD_INIT": init}
+           "FROST_AMD_MERGE": merge, "FROST_AMD_INIT": init, "FROST_AMD_FDMT_STATE": state}
     for k, v in env.items():
         if v:
             os.environ[k] = str(v)
@@ -43,7 +45,8 @@ def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None
     src = gpu.level_sources()
     stored = [0] + [l for l in range(1, len(src)) if src[l] >= 0]
     gaps = [b - a for a, b in zip(stored, stored[1:])]
-    say(f"  stored levels {stored}, levels per launch {gaps}")
+    say(f"  stored levels {stored}, levels per launch {gaps}, carried {gpu.carried()}")
+    assert gpu.carried() == (state == "carried")
     assert max(gaps) <= int(fuse or 4)
     if keep:
         gpu.keep_planes(True)

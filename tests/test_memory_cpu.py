@@ -36,8 +36,9 @@ def test_components_sum_to_total():
     comp = e.input_bytes + e.rfi_bytes + e.fdmt_bytes + e.postprocess_bytes
     assert e.reserve_bytes == comp // 10
     assert e.total_bytes == comp + e.reserve_bytes
-    assert e.input_bytes == 2 * 65536 * 512  # two u8 ingest slots
-    assert e.tile == 16384  # the smallest automatic tile
+    assert e.input_bytes == 65536 * 512  # the u8 gulp in the device ingest slot
+    # The smaller level-state form at its smallest automatic tile.
+    assert (e.tile, e.fdmt_state) in ((16384, 0), (2048, 1))
     assert e.fdmt_bytes > 4 * 512 * 65536  # at least the engine's input columns
 
 
@@ -51,7 +52,7 @@ def test_lowered_default_gulp_and_refusal():
     that fits and stays above max_delay + boxcar_max; budgets too small for
     even that are refused naming max_delay."""
     lowered = refused = False
-    for mib in range(256, 4097, 256):
+    for mib in [16, 32, 64, 128] + list(range(256, 4097, 256)):
         try:
             c = check(262144, True, mib * MiB)
         except abi.FrostAmdError as e:
@@ -73,10 +74,10 @@ def test_lowered_default_gulp_and_refusal():
 
 
 def test_explicit_gulp_is_refused_with_the_largest_that_fits():
-    r = check(262144, True, 1 * GiB)
+    r = check(262144, True, 768 * MiB)
     assert r.lowered
     with pytest.raises(abi.FrostAmdError, match=f"largest nsamps_gulp that fits is {r.gulp_used} "):
-        check(262144, False, 1 * GiB)
+        check(262144, False, 768 * MiB)
 
 
 def test_memory_limit_caps_the_available_bytes():
@@ -99,3 +100,26 @@ def test_rejects_mismatched_gulp():
     dc.block_nsamps = 1000
     with pytest.raises(abi.InvalidArgument):
         pipeline.estimate_memory(r, lay, base, dc, bc)
+
+
+def test_large_beam_fits_with_carried_levels():
+    """The large single-beam shape (float32, 8192 channels, 700-1500 MHz,
+    32 us, gulps of 2^21, DM 0-5000, trapezoid widths to 1024, two sub-band
+    levels) fits one 288 GB device: its levels are carried in per-row rings
+    (each stored row keeps only the look-back its own consumers read), where
+    recomputing each tile's look-back from the input history would need
+    terabytes. frost's dmt keeps the same per-level tree history
+    (lib/pipeline/memory.cpp:318-319,374-382)."""
+    nch, gulp, tsamp = 8192, 1 << 21, 32e-6
+    df = -800.0 / nch
+    f0 = 1500.0 + df / 2
+    lay = abi.BlockLayout(nch, gulp, abi.FLOAT32, tsamp, f0, df)
+    dc = abi.dedisp_config(nch, tsamp, f0, df, 5000.0, block_nsamps=gulp, subband_depth=2)
+    bc = abi.boxcar_config(max_boxcar=1024, use_trapezoid=1)
+    e = pipeline.estimate_memory(abi.rfi_config(), lay, 2.0, dc, bc)
+    assert e.fdmt_state == 1 and e.tile == 2048
+    assert e.input_bytes == 4 * nch * gulp
+    assert e.total_bytes < 270 * 10**9  # under 288 GB with room to spare
+    # The engine's own share: the float input with its short level-0 history,
+    # and the rings.
+    assert 4 * nch * gulp < e.fdmt_bytes < 4 * nch * gulp + 110 * 10**9

@@ -26,15 +26,17 @@ def configs(nch=128, cap=65536, tsamp=256e-6, f0=1500.0, df=-300.0 / 128, dm_max
 
 
 def test_estimate_equals_a_live_beam(monkeypatch):
-    """At the planner's tile, a beam that has run a full gulp and a short one
-    and filled both ingest slots holds exactly the estimate's RFI, FDMT and
-    input bytes."""
+    """At the planner's tile and level-state form, a beam that has run a
+    full gulp and a short one and ingested two gulps holds exactly the
+    estimate's RFI, FDMT and input bytes."""
     say("test_estimate_equals_a_live_beam")
     rcfg, lay, base, dcfg, bcfg = configs()
     est = pipeline.estimate_memory(rcfg, lay, base, dcfg, bcfg)
     monkeypatch.setenv("FROST_AMD_TILE", str(est.tile))
+    monkeypatch.setenv("FROST_AMD_FDMT_STATE", "carried" if est.fdmt_state else "recompute")
     beam = pipeline.Beam(rcfg, lay, base, dcfg, bcfg)
     monkeypatch.delenv("FROST_AMD_TILE")
+    monkeypatch.delenv("FROST_AMD_FDMT_STATE")
     m = beam.memory()
     assert m.tile == est.tile
     assert m.rfi_bytes == est.rfi_bytes
@@ -65,8 +67,10 @@ def test_check_memory_reads_the_device():
     assert c.device_memory == 1 and c.available_bytes > 0
     assert not c.lowered and c.gulp_used == lay.max_nsamps
     assert c.requested_bytes == c.estimate.total_bytes
-    # A memory limit below the estimate lowers the default gulp.
-    lim = (c.estimate.total_bytes * 3 // 4) >> 20
+    # A memory limit below the estimate (90 % of it between the estimates at
+    # half and at the full gulp) lowers the default gulp.
+    half = pipeline.estimate_memory(*configs(cap=lay.max_nsamps // 2)).total_bytes
+    lim = int((half + c.estimate.total_bytes) / 2 / 0.9) >> 20
     r2, lay2, _, d2, b2 = configs()
     low = pipeline.check_memory(r2, lay2, base, d2, b2, True, 0, lim)
     assert low.lowered and low.available_bytes == lim << 20
