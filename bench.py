@@ -397,23 +397,29 @@ def serving_measure(stream, seconds, nbeams=32, lanes=2, budget=200e9, seed=11):
 def fdmt_min_bytes(handle, plan_rows, nch, gulp):
     """Fused minimum FDMT bytes of one gulp of an engine (each launch reads
     its stored input level's tile + look-back samples once and writes its
-    output level's once; the init reads the channels once)."""
+    output level's once: tile + recomputed look-back, or the tile alone when
+    the levels are carried in rings; the init reads the channels once)."""
     from frost_amd import search
     src = search.engine_level_sources(handle)
     lsamp = search.engine_level_samples(handle).astype(np.float64)
+    carried = search.engine_state(handle)["carried"]
     nl, tile = C.c_uint64(), C.c_uint64()
     from frost_amd import abi
     abi.check(abi.lib().fa_engine_level_times(handle, None, None, 0, C.byref(tile), C.byref(nl)))
     tile = tile.value
     tiles = [min(tile, gulp - t0) for t0 in range(0, gulp, tile)]
+
+    def written(l, tl):
+        return plan_rows[l] * tl if carried else lsamp[l] - plan_rows[l] * (tile - tl)
+
     tot = 0.0
     for l in range(len(src)):
         a = int(src[l])
         if a == -1:
-            tot += sum(4.0 * (nch * tl + lsamp[0] - plan_rows[0] * (tile - tl)) for tl in tiles)
+            tot += sum(4.0 * (nch * tl + written(0, tl)) for tl in tiles)
         elif a >= 0:
-            tot += sum(4.0 * ((lsamp[a] - plan_rows[a] * (tile - tl)) +
-                              (lsamp[l] - plan_rows[l] * (tile - tl))) for tl in tiles)
+            tot += sum(4.0 * ((lsamp[a] - plan_rows[a] * (tile - tl)) + written(l, tl))
+                       for tl in tiles)
     return tot
 
 
@@ -552,6 +558,100 @@ def batched_measure(stream, nbeams=64, group=4, rounds=2, seed=3, reps=1):
     }
 
 
+def workloads_large_name() -> str:
+    from frost_amd import workloads
+    return workloads.LARGE_8192.name
+
+
+def large_measure(steps, warmup):
+    """One large beam (float32, 8192 ch, 700-1500 MHz, 32 us, gulps of 2^21,
+    DM 0-5000, trapezoids to 1024, 11 planes): the engine carries its FDMT
+    levels in per-row rings (the recomputed form would need terabytes).
+    A step is one gulp: RFI + FDMT + search + candidate rows on the host,
+    with the gulp resident in HBM."""
+    from frost_amd import abi, pipeline, search, workloads
+    W = workloads.LARGE_8192
+    stream = search.Stream(0)
+    gulp = search.DeviceArray(W.gulp_bytes())
+    sc = W.synth_config(0)
+    abi.check(abi.lib().fa_synth_filterbank(C.byref(sc), gulp.ptr, W.gulp, stream.h))
+    stream.synchronize()
+    log(f"workload {W.name}: gulp {W.gulp_bytes() / 2**30:.1f} GiB generated on device")
+    est = pipeline.estimate_memory(W.rfi_config(), W.layout(), W.baseline, W.dedisp_config(),
+                                   W.boxcar_config())
+    t0 = time.perf_counter()
+    beam = pipeline.Beam(W.rfi_config(), W.layout(), W.baseline, W.dedisp_config(),
+                         W.boxcar_config())
+    setup_s = time.perf_counter() - t0
+    st = beam.engine_state()
+    mem = beam.memory()
+    log(f"beam built in {setup_s:.1f} s: {st}, beam device bytes {mem.total_bytes / 1e9:.1f} GB, "
+        f"estimate {est.total_bytes / 1e9:.1f} GB")
+    plan = search.make_fdmt_plan(W.dedisp_config())
+    summ = plan.summary()
+    rows = plan.level_rows().astype(np.float64)
+    ndm = summ["ndm"]
+    for i in range(warmup):
+        t0 = time.perf_counter()
+        beam.execute_device(gulp, W.gulp, stream)
+        log(f"warmup {i}: {time.perf_counter() - t0:.2f} s, {beam.candidate_count()} candidates")
+    lat, fdmt_s, ncand = [], 0.0, 0
+    stage = {"rfi": 0.0, "init": 0.0, "merge": 0.0, "search": 0.0}
+    level_s = np.zeros(len(rows))
+    for i in range(steps):
+        t0 = time.perf_counter()
+        beam.execute_device(gulp, W.gulp, stream)
+        stream.synchronize()
+        lat.append(time.perf_counter() - t0)
+        sec, cnt, tile = beam.level_times()
+        fdmt_s += float(sec.sum())
+        level_s += sec
+        tm = beam.engine_timings()
+        rep = beam.rfi_report()
+        stage["rfi"] += rep.t_bandpass + rep.t_moments + rep.t_scan + rep.t_flags + rep.t_apply
+        for k in ("init", "merge", "search"):
+            stage[k] += tm[k]
+        ncand += beam.candidate_count()
+        log(f"step {i}: {lat[-1]:.2f} s, {beam.candidate_count()} candidates, masked "
+            f"{rep.chans_masked} ch")
+    total = float(sum(lat))
+    fmin = fdmt_min_bytes(beam.engine_handle(), rows, W.nchans, W.gulp)
+    fsec = fdmt_s / steps
+    return {
+        "metric": "DM-trial·samples/s of one large beam on 1 GPU; real-time factor",
+        "value": ndm * W.gulp * steps / total,
+        "unit": "DM-trial*samples/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * total / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 in, f32 compute",
+        "data": "synthetic (seeded device generator: Gaussian noise, sloped bandpass, 100 "
+                "scattered pulses; 10% of the channels zapped)",
+        "config": {"workload": W.name, "nchans": W.nchans, "tsamp_us": W.tsamp * 1e6,
+                   "gulp": W.gulp, "dm_max": W.dm_max, "dm_trials": ndm,
+                   "planes": summ["nplanes"], "fdmt_rows": summ["nrows_total"],
+                   "boxcar_max": W.boxcar_max, "trapezoid": True, "rfi": "narrow+broad",
+                   "zapped_channels": sum(b - a for a, b in W.zap_chans()),
+                   "candidates_per_gulp": ncand / steps},
+        "fdmt_state": {"form": "carried" if st["carried"] else "recomputed",
+                       "tile": st["tile"], "input_and_level_bytes": st["state_bytes"]},
+        "memory": {"beam_device_bytes": mem.total_bytes, "gulp_bytes": W.gulp_bytes(),
+                   "estimate_total_bytes": est.total_bytes, "estimate_fdmt_state": est.fdmt_state,
+                   "estimate_tile": est.tile, "hbm_bytes": 288e9},
+        "setup_seconds": setup_s,
+        "realtime": {"gulp_seconds": W.gulp_seconds(),
+                     "fraction_of_realtime": (total / steps) / W.gulp_seconds(),
+                     "realtime_factor": W.gulp_seconds() / (total / steps)},
+        "stage_seconds_per_gulp": {k: v / steps for k, v in stage.items()},
+        "fdmt_level_seconds_per_gulp": {
+            f"{int(src)}->{l}": float(level_s[l] / steps)
+            for l, src in enumerate(beam.level_sources()) if src >= -1},
+        "fdmt_roofline": {"min_bytes_per_gulp": fmin, "kernel_seconds_per_gulp": fsec,
+                          "gbs": fmin / fsec / 1e9, "frac": fmin / fsec / 1e9 / HBM_PEAK_GBS,
+                          "note": "every FDMT kernel: each launch reads its input rings' tile "
+                                  "+ carried look-back once and writes its output tile once"},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -567,6 +667,14 @@ def main() -> int:
     a = ap.parse_args()
     if a.copy_probe:
         return copy_probe()
+    if a.workload == workloads_large_name():
+        if a.gpus != 1:
+            print("bench.py: measures one GPU (one large beam runs on one device)",
+                  file=sys.stderr)
+            return 2
+        out = large_measure(max(a.steps, 1), a.warmup)
+        print(json.dumps(out), flush=True)
+        return 0
     if a.workload == "batched_64beams":
         if a.gpus != 1:
             print("bench.py: measures one GPU", file=sys.stderr)

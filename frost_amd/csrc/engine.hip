@@ -221,6 +221,8 @@ constexpr int kGatherThreads = 512;
 
 struct GatherDims {
     int nblocks;
+    int slices; // row-block slices spread over the XCDs (8 / time groups, at least 1)
+    int nbs;    // row blocks per slice
     int tile_len;
     int tau_origin;
     int half;   // floats per LDS half
@@ -229,9 +231,9 @@ struct GatherDims {
     // samples exist for tile samples [xlo, xhi].
     const float* x;
     int xlo, xhi;
-    // Carried state: rows are rings (GatherLeaf::ring, row record [3]);
-    // tile sample 0 is absolute sample *gulp_abs_p + tile_t0.
-    int ring;
+    // Carried state (the kernel's RING form: rows are rings, GatherLeaf::ring
+    // and row record [3]): tile sample 0 is absolute sample
+    // *gulp_abs_p + tile_t0.
     const uint64_t* gulp_abs_p;
     int64_t tile_t0;
 };
@@ -288,7 +290,7 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int K, int UMAX, bool X0>
+template <int K, int UMAX, bool X0, bool RING>
 __global__ void __launch_bounds__(kGatherThreads)
 fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                    const GatherBlock* __restrict__ blocks, const GatherLeaf* __restrict__ leaves,
@@ -298,10 +300,17 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
     __shared__ float xwin[X0 ? 4 : 1][X0 ? kX0Win : 1]; // per producer wave
     __shared__ int s_rq[kGatherRowsMax]; // carried: ring position of tile sample 0 per output row
     constexpr int M     = (kChunkG + kSpreadG) / 64;
+    // Work item (row block, time group): the row blocks are cut into
+    // g.slices contiguous slices of g.nbs blocks (one slice unless the tile
+    // has fewer than 8 time groups); virtual group vg = (time group) *
+    // slices + slice. Block id goes to XCD id % 8, which sweeps the blocks
+    // of virtual groups xcd, xcd + 8, ... in row order.
     const uint32_t id   = blockIdx.x;
     const uint32_t k    = id >> 3;
-    const uint32_t blk  = k % (uint32_t)g.nblocks;
-    const uint32_t grp  = (id & 7U) + 8U * (k / (uint32_t)g.nblocks);
+    const uint32_t vg   = (id & 7U) + 8U * (k / (uint32_t)g.nbs);
+    const uint32_t blk  = (vg % (uint32_t)g.slices) * (uint32_t)g.nbs + k % (uint32_t)g.nbs;
+    const uint32_t grp  = vg / (uint32_t)g.slices;
+    if (blk >= (uint32_t)g.nblocks) return; // uniform: past the last row block
     const int tau_first = g.tau_origin + (int)grp * (kChunkG * g.chunks);
     const GatherBlock B = blocks[blk];
     const int lo_c      = max(0, (-B.hmax - tau_first) / kChunkG); // first chunk in range
@@ -440,7 +449,7 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
         float va[UMAX][M], vb[UMAX][M];
         // Carried state: ring position of tile sample 0 in each input row.
         int rq[UMAX];
-        if (g.ring) {
+        if constexpr (RING) {
             const uint64_t t0abs = *g.gulp_abs_p + static_cast<uint64_t>(g.tile_t0);
 #pragma unroll
             for (int u = 0; u < UMAX; ++u)
@@ -452,7 +461,7 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
             for (int u = 0; u < UMAX; ++u) {
                 const int sidx      = min(wave + 4 * u, B.nleaves - 1);
                 const GatherLeaf lf = leaves[B.leaf_first + sidx];
-                if (g.ring) {
+                if constexpr (RING) {
                     // Ring row: tile sample t at (rq + t) mod R; samples at
                     // or past the tile end read 0 (they feed only dropped
                     // outputs); every read sample is at most the row's
@@ -515,7 +524,7 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
         constexpr int RS = 4 + K;
         const int q      = wave - 4;
         const int r_end  = B.row_first + B.nrows;
-        if (g.ring) {
+        if constexpr (RING) {
             // Ring position of tile sample 0 in each of this wave's rows
             // (rows q, q + 4, ...: lane i takes row q + 4 i).
             const uint64_t t0abs = *g.gulp_abs_p + static_cast<uint64_t>(g.tile_t0);
@@ -548,7 +557,7 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                 }
                 const int64_t o = (int64_t)(uint32_t)rec[0] | ((int64_t)rec[1] << 32);
                 const int h     = rec[2];
-                if (g.ring) {
+                if constexpr (RING) {
                     // Ring row: tile samples [0, tile) only (h is 0).
                     const int R   = rec[3];
                     const int q0  = s_rq[r - B.row_first];
@@ -574,16 +583,20 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
     }
 }
 
-template <int K, int UMAX, bool X0>
+template <int K, int UMAX, bool X0, bool RING>
 void launch_gather_u(const MergeLaunch& ml, const float* in, float* out, const GatherDims& g,
                      int tl, hipStream_t s) {
     const unsigned ngrp   = ceil_div(tl - g.tau_origin, kChunkG * g.chunks);
-    const uint64_t blocks = (uint64_t)((ngrp + 7) / 8) * 8 * ml.nblocks;
+    GatherDims gd = g;
+    gd.slices = static_cast<int>(std::max(1U, std::min<unsigned>(8U / std::max(ngrp, 1U), ml.nblocks)));
+    gd.nbs    = static_cast<int>(ceil_div(ml.nblocks, static_cast<unsigned>(gd.slices)));
+    const uint64_t nvg    = static_cast<uint64_t>(ngrp) * gd.slices;
+    const uint64_t blocks = (nvg + 7) / 8 * 8 * gd.nbs;
     if (blocks * kGatherThreads >= (uint64_t{1} << 32))
         throw std::invalid_argument("FDMT merge grid too large; reduce the time tile");
-    fdmt_gather_kernel<K, UMAX, X0><<<static_cast<unsigned>(blocks), kGatherThreads,
+    fdmt_gather_kernel<K, UMAX, X0, RING><<<static_cast<unsigned>(blocks), kGatherThreads,
                                       2 * g.half * sizeof(float), s>>>(
-        in, out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), ml.chans.get(), g);
+        in, out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), ml.chans.get(), gd);
 }
 
 int default_gather_chunks() {
@@ -595,6 +608,25 @@ int default_gather_chunks() {
     return n;
 }
 
+template <int K, bool RING>
+void launch_gather_r(const MergeLaunch& ml, const float* in, float* out, const GatherDims& g,
+                     int tl, hipStream_t s) {
+    if (ml.from_input) {
+        switch (ml.max_leaves) {
+        case 16: return launch_gather_u<K, 4, true, RING>(ml, in, out, g, tl, s);
+        case 32: return launch_gather_u<K, 8, true, RING>(ml, in, out, g, tl, s);
+        default: throw std::logic_error("staged FDMT gather: unsupported channels per block");
+        }
+    }
+    switch (ml.max_leaves) {
+    case 16: return launch_gather_u<K, 4, false, RING>(ml, in, out, g, tl, s);
+    case 32: return launch_gather_u<K, 8, false, RING>(ml, in, out, g, tl, s);
+    case 48: return launch_gather_u<K, 12, false, RING>(ml, in, out, g, tl, s);
+    case 64: return launch_gather_u<K, 16, false, RING>(ml, in, out, g, tl, s);
+    default: throw std::logic_error("staged FDMT gather: unsupported input rows per block");
+    }
+}
+
 // x0: tile sample 0 of channel 0 of the input; [xlo, xhi] the tile samples
 // held by the input buffer (used only by launches reading level 0 from x).
 template <int K>
@@ -603,7 +635,6 @@ void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, c
                    int64_t tile_t0, hipStream_t s) {
     static_assert(kChunkG == 256, "four consumer waves cover one chunk");
     GatherDims g{};
-    g.ring       = ring ? 1 : 0;
     g.gulp_abs_p = gulp_abs_p;
     g.tile_t0    = tile_t0;
     g.nblocks    = static_cast<int>(ml.nblocks);
@@ -614,20 +645,8 @@ void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, c
     g.x          = x0;
     g.xlo        = xlo;
     g.xhi        = xhi;
-    if (ml.from_input) {
-        switch (ml.max_leaves) {
-        case 16: return launch_gather_u<K, 4, true>(ml, in, out, g, tl, s);
-        case 32: return launch_gather_u<K, 8, true>(ml, in, out, g, tl, s);
-        default: throw std::logic_error("staged FDMT gather: unsupported channels per block");
-        }
-    }
-    switch (ml.max_leaves) {
-    case 16: return launch_gather_u<K, 4, false>(ml, in, out, g, tl, s);
-    case 32: return launch_gather_u<K, 8, false>(ml, in, out, g, tl, s);
-    case 48: return launch_gather_u<K, 12, false>(ml, in, out, g, tl, s);
-    case 64: return launch_gather_u<K, 16, false>(ml, in, out, g, tl, s);
-    default: throw std::logic_error("staged FDMT gather: unsupported input rows per block");
-    }
+    if (ring) return launch_gather_r<K, true>(ml, in, out, g, tl, s);
+    return launch_gather_r<K, false>(ml, in, out, g, tl, s);
 }
 
 struct MergeShape {
@@ -739,14 +758,22 @@ __device__ __forceinline__ void track_run(const SearchArgs& a, OpenRun& o, uint3
 // A searched plane row: sample t0 + t of the tile, t in [-(W_max - 1), tile).
 // Linear rows are read at in + (W_max - 1) + t; carried rows at ring
 // position (t0 + t) mod R.
-struct RowRead {
-    const float* p;
-    int q, R;
-    __device__ __forceinline__ RowRead(const SearchArgs& a, const SearchRow& row, uint64_t gabs)
-        : p(a.buf + row.in), q(a.wm1), R(row.ring) {
-        if (R) q = ring_of(gabs + static_cast<uint64_t>(a.tile_t0), R);
+template <bool RING> struct RowRead {
+    const float* p; // linear: sample t0; ring: the ring start
+    int q = 0, R = 0;
+    __device__ __forceinline__ RowRead(const SearchArgs& a, const SearchRow& row, uint64_t gabs) {
+        if constexpr (RING) {
+            p = a.buf + row.in;
+            R = row.ring;
+            q = ring_of(gabs + static_cast<uint64_t>(a.tile_t0), R);
+        } else {
+            p = a.buf + row.in + a.wm1;
+        }
     }
-    __device__ __forceinline__ float at(int t) const { return p[R ? wrap_pos(q + t, R) : q + t]; }
+    __device__ __forceinline__ float at(int t) const {
+        if constexpr (RING) return p[wrap_pos(q + t, R)];
+        else return p[t];
+    }
 };
 
 // ---- ring-ladder search (any widths up to 4096) ------------------------------
@@ -810,6 +837,7 @@ struct Ring {
     }
 };
 
+template <bool RING>
 __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs ra) {
     extern __shared__ float lds[];
     const SearchArgs& a = ra.a;
@@ -843,7 +871,7 @@ __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs 
     for (int j = i; j < ra.lds_floats; j += kRingChunk) lds[j] = 0.0f;
     __syncthreads();
     const Ring R{lds, s_off, s_size, s_base};
-    const RowRead src(a, row, gabs);
+    const RowRead<RING> src(a, row, gabs);
     const int tl     = a.tile_len;
     for (int u = ra.u0; u < tl; u += kRingChunk) {
         const int t   = u + i;
@@ -973,6 +1001,7 @@ template <int K, int KMAX, bool TRAP> struct LadderStep;
 
 struct RegSearchArgs {
     SearchArgs a;
+    bool ring;        // carried rows (ring addressing)
     uint32_t present; // bit k set when width 2^k is searched
     int widx[16];     // width index of level k
 };
@@ -1033,7 +1062,7 @@ __device__ uint32_t walk_chunk(const SearchArgs& a, uint64_t gabs, const float* 
     return __builtin_amdgcn_readfirstlane(st->open);
 }
 
-template <int KMAX, bool TRAP>
+template <int KMAX, bool TRAP, bool RING>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRAP ? 1 : 6))) search_reg_kernel(RegSearchArgs ra) {
     const SearchArgs& a = ra.a;
     __shared__ float s_snr[kBlock / 64][kRegChunk];
@@ -1067,7 +1096,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
     }
     wave_sync();
     const int64_t warm_max = (int64_t)row.delay + (1 << KMAX) - 1;
-    const RowRead src(a, row, gabs); // sample t0 + t
+    const RowRead<RING> src(a, row, gabs); // sample t0 + t
     const int tl           = a.tile_len;
     float D[9][4], P[9][4], nx[4];
 #pragma unroll
@@ -1288,7 +1317,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
 
 template <int KMAX, bool TRAP>
 void launch_reg(const RegSearchArgs& ra, unsigned blocks, hipStream_t s) {
-    search_reg_kernel<KMAX, TRAP><<<blocks, kBlock, 0, s>>>(ra);
+    if (ra.ring) search_reg_kernel<KMAX, TRAP, true><<<blocks, kBlock, 0, s>>>(ra);
+    else search_reg_kernel<KMAX, TRAP, false><<<blocks, kBlock, 0, s>>>(ra);
 }
 
 using RegLauncher = void (*)(const RegSearchArgs&, unsigned, hipStream_t);
@@ -2640,6 +2670,7 @@ void Engine::launch_search(uint32_t l, const float* buf, int tl, uint64_t t0, hi
     if (reg_search_) {
         RegSearchArgs ra{};
         ra.a       = a;
+        ra.ring    = carried_;
         ra.present = 0;
         for (int k = 0; k < 16; ++k) ra.widx[k] = -1;
         for (std::size_t w = 0; w < nw; ++w) {
@@ -2660,7 +2691,10 @@ void Engine::launch_search(uint32_t l, const float* buf, int tl, uint64_t t0, hi
     }
     rs.u0         = -static_cast<int>((wm1_ + kRingChunk - 1) / kRingChunk * kRingChunk);
     rs.lds_floats = ring_floats_;
-    search_ring_kernel<<<search_count_[l], kRingChunk, ring_floats_ * sizeof(float), ss>>>(rs);
+    if (carried_)
+        search_ring_kernel<true><<<search_count_[l], kRingChunk, ring_floats_ * sizeof(float), ss>>>(rs);
+    else
+        search_ring_kernel<false><<<search_count_[l], kRingChunk, ring_floats_ * sizeof(float), ss>>>(rs);
 }
 
 // Carries the last H input samples of each channel to the front of the
@@ -2924,10 +2958,11 @@ uint64_t Engine::memory_bytes() const {
 }
 
 std::vector<uint64_t> Engine::level_samples() const {
-    // Carried rows compute only the tile; linear rows their look-back too.
+    // Linear rows: the tile and its recomputed look-back; carried rows: the
+    // ring's tile and carried look-back (only the tile is computed).
     std::vector<uint64_t> out(plan_.nlevels, 0);
     for (uint32_t l = 0; l < plan_.nlevels; ++l)
-        for (const int32_t h : hist_[l]) out[l] += tile_ + (carried_ ? 0 : static_cast<uint64_t>(h));
+        for (const int32_t h : hist_[l]) out[l] += tile_ + static_cast<uint64_t>(h);
     return out;
 }
 

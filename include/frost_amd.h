@@ -349,8 +349,9 @@ int fa_engine_level_sources(const fa_engine* engine, int32_t* source, uint64_t c
 
 /* Per FDMT level: samples one full time tile holds over the level's rows,
  * the sum over rows of (tile + look-back). A launch reading stored level a
- * and writing level b reads at least 4 x samples[a] bytes and writes
- * 4 x samples[b] per tile. No reference counterpart (diagnostics). */
+ * and writing level b reads at least 4 x samples[a] bytes per tile and
+ * writes 4 x samples[b] (recomputed form) or 4 x rows[b] x tile (carried
+ * form: only the tile is computed). No reference counterpart (diagnostics). */
 int fa_engine_level_samples(const fa_engine* engine, uint64_t* samples, uint64_t cap,
                             uint64_t* nlevels);
 
