@@ -227,6 +227,7 @@ struct GatherDims {
     int tau_origin;
     int half;   // floats per LDS half
     int chunks; // consecutive chunks per block
+    int zero_slot; // offset in each half of a chunk of zeros (zero-channel leaves), or -1
     // Level-0 runs (X0): x points at tile sample 0 of channel 0; input
     // samples exist for tile samples [xlo, xhi].
     const float* x;
@@ -318,6 +319,14 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
     if (tau_first + kChunkG * g.chunks <= -B.hmax || lo_c >= hi_c) return; // uniform
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (g.zero_slot >= 0) {
+        // Zero-channel leaves read this chunk of zeros in either half (no
+        // producer writes there); the first hand-over barrier orders it.
+        for (int i = threadIdx.x; i < kChunkG; i += kGatherThreads) {
+            lds[g.zero_slot + i]          = 0.0f;
+            lds[g.half + g.zero_slot + i] = 0.0f;
+        }
+    }
     if (X0 && wave < 4) {
         // Producer of level-0 rows: per channel group, a window of input
         // samples (two chunks of loads in flight, register sets A and B) is
@@ -641,6 +650,7 @@ void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, c
     g.tile_len   = tl;
     g.tau_origin = -((ml.hmax + 63) & ~63); // chunks start on 64-sample boundaries
     g.half       = static_cast<int>(ml.lds_floats);
+    g.zero_slot  = ml.zero_slot;
     g.chunks     = chunks;
     g.x          = x0;
     g.xlo        = xlo;
@@ -1359,6 +1369,11 @@ __global__ void frontier_kernel(const OpenRun* open, uint64_t n, unsigned long l
     if (i < n && open[i].open) atomicMin(out, (unsigned long long)open[i].start);
 }
 
+__global__ void zero_floats_kernel(float* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
 __global__ void zero_open_kernel(OpenRun* open, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) open[i] = OpenRun{0, 0, 0, 0.0f, 0, 0};
@@ -1596,6 +1611,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     // buffers, or the rings of the carried form), or laid out in the shared
     // arena as [x | level buffer 0 | level buffer 1].
     xstride_ = round64(hist_in_ + cap_);
+    const uint64_t nx = plan_.nchans_fdmt; // input rows: the channels and the zero channel
     uint64_t len0 = 0, len1 = 0;
     for (uint32_t l = 0; l < plan_.nlevels; ++l) {
         if (level_buf_[l] == 0) len0 = std::max(len0, level_len_[l]);
@@ -1603,7 +1619,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     }
     len1 = std::max<uint64_t>(len1, 1);
     if (carried_) {
-        x_.resize(nch * xstride_);
+        x_.resize(nx * xstride_);
         if (!sizing()) FA_HIP(hipMemset(x_.get(), 0, x_.bytes()));
         rings_.resize(std::max<uint64_t>(rings_floats_, 1));
         if (!sizing()) FA_HIP(hipMemset(rings_.get(), 0, rings_.bytes())); // samples before the stream: 0
@@ -1612,13 +1628,13 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
         arena_need_ = x_.bytes() + rings_.bytes();
     } else if (shared_) {
         auto up256 = [](uint64_t b) { return (b + 255) & ~uint64_t{255}; };
-        buf_off_[0] = up256(nch * xstride_ * sizeof(float));
+        buf_off_[0] = up256(nx * xstride_ * sizeof(float));
         buf_off_[1] = up256(buf_off_[0] + len0 * sizeof(float));
         arena_need_ = buf_off_[1] + len1 * sizeof(float);
         hist_store_.resize(std::max<uint64_t>(nch * hist_in_, 1));
         if (!sizing()) FA_HIP(hipMemset(hist_store_.get(), 0, hist_store_.bytes()));
     } else {
-        x_.resize(nch * xstride_);
+        x_.resize(nx * xstride_);
         if (!sizing()) FA_HIP(hipMemset(x_.get(), 0, x_.bytes()));
         buf_[0].resize(len0);
         buf_[1].resize(len1);
@@ -1668,7 +1684,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
 uint64_t Engine::buffers_bound(uint64_t tile) const {
     // x plus two level buffers each as long as the longest level, with the
     // arena's alignment slack.
-    const uint64_t nch    = plan_.cfg.nchans;
+    const uint64_t nch    = plan_.nchans_fdmt; // input rows (with the zero channel)
     const uint64_t xbytes = nch * round64(hist_in_ + cap_) * sizeof(float);
     uint64_t lmax         = 0;
     for (uint32_t l = 0; l < plan_.nlevels; ++l) {
@@ -1683,7 +1699,7 @@ uint64_t Engine::layout_need(uint64_t tile) const {
     // [x | level buffer 0 | level buffer 1], each buffer on a 256-byte
     // boundary, with the levels stored in each buffer as planned now.
     auto up256        = [](uint64_t b) { return (b + 255) & ~uint64_t{255}; };
-    const uint64_t nch = plan_.cfg.nchans;
+    const uint64_t nch = plan_.nchans_fdmt;
     uint64_t len[2]    = {0, 1};
     for (uint32_t l = 0; l < plan_.nlevels; ++l) {
         if (level_buf_[l] < 0) continue;
@@ -1738,7 +1754,7 @@ void Engine::plane_setup() {
 // look-back that launch reads) plus its delay.
 void Engine::build_init() {
     const auto& l0     = plan_.levels[0];
-    const uint64_t nch = plan_.cfg.nchans;
+    const uint64_t nch = plan_.nchans_fdmt;
     const bool ring0   = carried_ && !x0_fused_;
     std::vector<int32_t> chan(4 * nch, 0);
     std::vector<int64_t> rows(3 * l0.size());
@@ -2027,14 +2043,14 @@ void Engine::carried_layout() {
     // The input history (as build_init computes it): per channel, the
     // largest level-0 look-back read from the input plus its largest delay.
     const auto& l0 = plan_.levels[0];
-    std::vector<uint64_t> hmax(plan_.cfg.nchans, 0), dmax(plan_.cfg.nchans, 0);
+    std::vector<uint64_t> hmax(plan_.nchans_fdmt, 0), dmax(plan_.nchans_fdmt, 0);
     for (std::size_t r = 0; r < l0.size(); ++r) {
         const int32_t c = l0.chan_or_lo[r];
         if (x0_fused_) hmax[c] = std::max<uint64_t>(hmax[c], static_cast<uint64_t>(hist_[0][r]));
         dmax[c] = std::max<uint64_t>(dmax[c], static_cast<uint64_t>(l0.delay[r]));
     }
     hist_in_ = 0;
-    for (uint64_t c = 0; c < plan_.cfg.nchans; ++c) hist_in_ = std::max(hist_in_, hmax[c] + dmax[c]);
+    for (uint64_t c = 0; c < plan_.nchans_fdmt; ++c) hist_in_ = std::max(hist_in_, hmax[c] + dmax[c]);
     xstride_ = round64(hist_in_ + cap_);
 }
 
@@ -2055,7 +2071,7 @@ void Engine::build_carried() {
 uint64_t Engine::carried_need(uint64_t tile) {
     tile_ = tile;
     carried_layout();
-    const uint64_t nch = plan_.cfg.nchans;
+    const uint64_t nch = plan_.nchans_fdmt;
     return (nch * xstride_ + rings_floats_) * sizeof(float);
 }
 
@@ -2069,6 +2085,16 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
     const auto& base_a = base_[a];
     const auto& hist_a = hist_[a];
     const auto& ring_a = ring_len_[a];
+    // The zero channel's row (level 0, plans with pass-through nodes): its
+    // leaves are never staged; they read a zero slot placed after the
+    // block's staged rows (one output row may hold it twice, at shifts far
+    // apart).
+    int32_t zero_row = -1;
+    if (a == 0 && plan_.nchans_fdmt > plan_.cfg.nchans)
+        for (std::size_t c = 0; c < plan_.levels[0].size(); ++c)
+            if (plan_.levels[0].chan_or_lo[c] == static_cast<int32_t>(plan_.cfg.nchans))
+                zero_row = static_cast<int32_t>(c);
+    constexpr int32_t kZeroLeaf = -1; // record offset resolved to the zero slot below
     struct Lf {
         int32_t c;
         int32_t shmin, shmax;
@@ -2202,6 +2228,10 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
         for (uint32_t r = first; r < end; ++r)
             for (int k = 0; k < K; ++k) {
                 const auto [c, sh] = rl[static_cast<std::size_t>(r) * K + k];
+                if (c == zero_row) {
+                    rec[r * RS + 4 + k] = kZeroLeaf;
+                    continue;
+                }
                 std::size_t i      = 0;
                 while (cur[i].c != c) ++i;
                 rec[r * RS + 4 + k] = slot[i] + (cur[i].shmax - sh);
@@ -2221,6 +2251,7 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
             const int64_t sh     = sh32;
             if (sh < 0 || sh > std::numeric_limits<int32_t>::max() / 2 || h + sh > hist_a[c])
                 throw std::logic_error("FDMT gather shift out of range");
+            if (c == zero_row) continue;
             bool found = false;
             for (auto& x : tent)
                 if (x.c == c) {
@@ -2240,6 +2271,7 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
             cur.clear();
             for (int k = 0; k < K; ++k) {
                 const auto [c, sh] = rl[static_cast<std::size_t>(r) * K + k];
+                if (c == zero_row) continue;
                 bool found         = false;
                 for (auto& x : cur)
                     if (x.c == c) {
@@ -2279,6 +2311,17 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
     }
     ml.nblocks    = static_cast<uint32_t>(gb.size());
     ml.lds_floats = lds_max;
+    ml.zero_slot  = -1;
+    if (zero_row >= 0) {
+        // One chunk of zeros after the largest block's staged rows.
+        ml.zero_slot = static_cast<int32_t>((lds_max + 63) & ~63U);
+        ml.lds_floats = static_cast<uint32_t>(ml.zero_slot) + kChunkG;
+        for (std::size_t r = 0; r < ml.nrows; ++r)
+            for (int k = 0; k < K; ++k) {
+                int32_t& v = rec[r * RS + 4 + static_cast<std::size_t>(k)];
+                if (v == kZeroLeaf) v = ml.zero_slot;
+            }
+    }
     ml.blocks.resize(gb.size());
     ml.leaves.resize(gl.size());
     ml.rows.resize(rec.size());
@@ -2304,7 +2347,10 @@ void Engine::compute_inv(const std::vector<uint8_t>& mask, std::vector<float>& o
     std::vector<double> prev(l0.size() * nw), cur;
     for (std::size_t r = 0; r < l0.size(); ++r)
         for (std::size_t w = 0; w < nw; ++w)
-            prev[r * nw + w] = mask[l0.chan_or_lo[r]] ? vtab[(l0.delay[r] + 1) * nw + w] : 0.0;
+            prev[r * nw + w] = static_cast<std::size_t>(l0.chan_or_lo[r]) < mask.size() &&
+                                       mask[l0.chan_or_lo[r]]
+                                   ? vtab[(l0.delay[r] + 1) * nw + w]
+                                   : 0.0; // masked, or the zero channel
     out.assign(static_cast<std::size_t>(nprows_) * nw, 0.0f);
     auto emit_level = [&](uint32_t l, const std::vector<double>& var) {
         for (std::size_t p = 0; p < plan_.planes.size(); ++p) {
@@ -2473,6 +2519,12 @@ void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
     if (shared_ && hist_in_ > 0)
         FA_HIP(hipMemcpy2DAsync(x_ptr_, xstride_ * 4, hist_store_.get(), hist_in_ * 4,
                                 hist_in_ * 4, nch, hipMemcpyDeviceToDevice, s));
+    if (shared_ && plan_.nchans_fdmt > nch) {
+        // The zero channel's row of the shared arena (other engines write
+        // there between this engine's gulps).
+        zero_floats_kernel<<<ceil_div(xstride_, 256), 256, 0, s>>>(x_ptr_ + nch * xstride_, xstride_);
+        FA_HIP(hipGetLastError());
+    }
     // Open runs before this gulp: a re-run after a candidate overflow starts
     // from them.
     FA_HIP(hipMemcpyAsync(open_saved_.get(), open_.get(), open_.bytes(), hipMemcpyDeviceToDevice,
@@ -2503,11 +2555,12 @@ void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
                 if (!carried_)
                     for (const int32_t h : hist_[0]) hcmax = std::max(hcmax, h);
                 const unsigned gx = ceil_div(tl + hcmax + 64, kBlock * kInitPer);
-                const unsigned gy = static_cast<unsigned>(std::min<uint64_t>(nch, 65535));
+                const uint64_t nx = plan_.nchans_fdmt;
+                const unsigned gy = static_cast<unsigned>(std::min<uint64_t>(nx, 65535));
                 const size_t lds  = (kBlock * kInitPer + dmax0_) * sizeof(float);
                 fdmt_init_kernel<<<dim3(gx, gy), kBlock, lds, s>>>(
                     x_ptr_, xstride_, static_cast<int64_t>(hist_in_ + t0), init_chan_.get(),
-                    init_rows_.get(), out, tl, static_cast<int>(nch),
+                    init_rows_.get(), out, tl, static_cast<int>(nx),
                     reinterpret_cast<const uint64_t*>(params_.get()), static_cast<int64_t>(t0),
                     carried_ ? 1 : 0);
             } else {

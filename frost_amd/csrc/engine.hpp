@@ -118,6 +118,7 @@ struct MergeLaunch {
     uint32_t nblocks    = 0;
     uint32_t lds_floats = 0;
     uint32_t max_leaves = 0; // input rows per block bound (16, 32, 48 or 64)
+    int32_t zero_slot   = -1; // LDS floats offset of a chunk of zeros (zero-channel leaves), or -1
     bool from_input     = false; // in_level 0 is not stored: producers form it from x
     DeviceBuffer<GatherBlock> blocks;
     DeviceBuffer<GatherLeaf> leaves;

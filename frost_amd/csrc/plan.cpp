@@ -16,6 +16,8 @@ int32_t nearest(double v) { return static_cast<int32_t>(std::floor(v + 0.5)); }
 
 struct NodeInfo {
     uint32_t level = 0, c0 = 0, c1 = 0;
+    uint32_t cm    = 0;     // split: lower child [c0, cm), upper [cm, c1)
+    bool pass      = false; // one channel at level 1: lower child = the channel, upper = zero
     int32_t lo = -1, hi = -1;
     std::vector<int32_t> need; // delays required by parents / planes
 };
@@ -120,8 +122,6 @@ FdmtPlan::FdmtPlan(const fa_dedisp_config& c) : cfg(c) {
     f_max            = f_min + static_cast<float>(c.nchans) * std::abs(c.df);
     if (!(f_min > 0.0F) || !finite(f_max) || !(f_max > f_min))
         throw std::invalid_argument("FDMT requires positive ascending frequency edges");
-    if ((c.nchans & (c.nchans - 1)) != 0)
-        throw std::invalid_argument("FDMT: nchans must be a power of two");
     if (c.subband_depth > 2)
         throw std::invalid_argument("FDMT: subband_depth must be 0, 1 or 2");
     uint32_t L = 0;
@@ -132,38 +132,99 @@ FdmtPlan::FdmtPlan(const fa_dedisp_config& c) : cfg(c) {
               static_cast<double>(c.nchans);
     dm_list = generate_dm_list(c);
 
-    // Tree nodes then staggered nodes, per level.
+    // Tree nodes then staggered nodes, per level. The tree is built top
+    // down from the whole band at level L = ceil(log2 nchans): a node of n
+    // channels splits into balanced halves [c0, c0 + n/2) and
+    // [c0 + n/2, c1) one level down (frost's decision for non-power-of-two
+    // counts, docs/decisions.md:12-13). A node of n channels sits at a level
+    // l with n <= 2^l, so only level-1 nodes can hold one channel: such a
+    // node passes its channel through, its upper child being a zero channel
+    // (index nchans, one row of delay 0, zero at every sample), so that
+    // every level is a uniform binary merge. Power-of-two counts have no
+    // such node.
     std::vector<NodeInfo> nodes;
     std::vector<std::vector<int32_t>> at(nlevels);
-    for (uint32_t l = 0; l < nlevels; ++l) {
-        const uint32_t span = 1U << l;
-        const uint32_t n    = static_cast<uint32_t>(c.nchans >> l);
-        for (uint32_t j = 0; j < n; ++j) {
-            NodeInfo nd;
-            nd.level = l;
-            nd.c0    = j * span;
-            nd.c1    = nd.c0 + span;
-            if (l > 0) {
-                nd.lo = at[l - 1][2 * j];
-                nd.hi = at[l - 1][2 * j + 1];
+    int32_t zero_node = -1;
+    {
+        NodeInfo root;
+        root.level = L;
+        root.c0    = 0;
+        root.c1    = static_cast<uint32_t>(c.nchans);
+        at[L].push_back(0);
+        nodes.push_back(root);
+    }
+    std::vector<std::vector<int32_t>> tree(nlevels); // tree nodes per level, ascending
+    tree[L] = at[L];
+    for (uint32_t l = L; l >= 1; --l) {
+        for (const int32_t id : tree[l]) {
+            const uint32_t c0 = nodes[id].c0, c1 = nodes[id].c1, n = c1 - c0;
+            if (n == 1) {
+                // Pass-through (level 1 only): the channel, and the zero channel.
+                if (zero_node < 0) {
+                    NodeInfo z;
+                    z.level   = 0;
+                    z.c0      = static_cast<uint32_t>(c.nchans);
+                    z.c1      = z.c0 + 1;
+                    zero_node = static_cast<int32_t>(nodes.size());
+                    nodes.push_back(z);
+                }
+                NodeInfo ch;
+                ch.level = l - 1;
+                ch.c0    = c0;
+                ch.c1    = c1;
+                const int32_t cid = static_cast<int32_t>(nodes.size());
+                nodes.push_back(ch);
+                tree[l - 1].push_back(cid);
+                nodes[id].pass = true;
+                nodes[id].cm   = c1;
+                nodes[id].lo   = cid;
+                nodes[id].hi   = zero_node;
+                continue;
             }
-            at[l].push_back(static_cast<int32_t>(nodes.size()));
-            nodes.push_back(std::move(nd));
+            const uint32_t cm = c0 + n / 2;
+            NodeInfo a, b;
+            a.level = b.level = l - 1;
+            a.c0 = c0;
+            a.c1 = cm;
+            b.c0 = cm;
+            b.c1 = c1;
+            const int32_t ia = static_cast<int32_t>(nodes.size());
+            nodes.push_back(a);
+            nodes.push_back(b);
+            tree[l - 1].push_back(ia);
+            tree[l - 1].push_back(ia + 1);
+            nodes[id].cm = cm;
+            nodes[id].lo = ia;
+            nodes[id].hi = ia + 1;
         }
+    }
+    if (zero_node >= 0 && L < 1) throw std::logic_error("FDMT plan: zero channel without a tree");
+    // Staggered nodes at depth 1..subband_depth straddle two adjacent tree
+    // nodes: the upper child of one and the lower child of the next.
+    for (uint32_t l = 0; l <= L; ++l) {
+        at[l] = tree[l];
         const uint32_t depth = L - l;
         if (depth >= 1 && depth <= c.subband_depth) {
-            for (uint32_t s = 0; s + 1 < n; ++s) {
+            for (std::size_t s = 0; s + 1 < tree[l].size(); ++s) {
+                const NodeInfo& left  = nodes[tree[l][s]];
+                const NodeInfo& right = nodes[tree[l][s + 1]];
+                if (left.pass || right.pass)
+                    throw std::invalid_argument(
+                        "FDMT: subband_depth too deep for the channel count");
                 NodeInfo nd;
                 nd.level = l;
-                nd.c0    = s * span + span / 2;
-                nd.c1    = nd.c0 + span;
-                nd.lo    = at[l - 1][2 * s + 1];
-                nd.hi    = at[l - 1][2 * s + 2];
+                nd.lo    = left.hi;
+                nd.hi    = right.lo;
+                nd.c0    = nodes[nd.lo].c0;
+                nd.c1    = nodes[nd.hi].c1;
+                nd.cm    = right.c0;
                 at[l].push_back(static_cast<int32_t>(nodes.size()));
                 nodes.push_back(std::move(nd));
             }
         }
     }
+    if (zero_node >= 0) at[0].push_back(zero_node); // the zero channel's row last
+    nchans_fdmt = static_cast<uint32_t>(c.nchans) + (zero_node >= 0 ? 1U : 0U);
 
     // Planes: depth 0..subband_depth, tree nodes first, then staggered.
     std::vector<int32_t> plane_node;
@@ -176,7 +237,7 @@ FdmtPlan::FdmtPlan(const fa_dedisp_config& c) : cfg(c) {
             p.level     = l;
             p.depth     = d;
             p.idx       = idx++;
-            p.staggered = (nd.c0 % (1U << l)) != 0 ? 1U : 0U;
+            p.staggered = idx > tree[l].size() ? 1U : 0U;
             p.c0        = nd.c0;
             p.c1        = nd.c1;
             p.f_low     = static_cast<float>(f_min + nd.c0 * chan_bw);
@@ -196,15 +257,15 @@ FdmtPlan::FdmtPlan(const fa_dedisp_config& c) : cfg(c) {
         }
     }
 
-    // Requirements flow from each node to its halves.
+    // Requirements flow from each node to its halves (a pass-through node's
+    // whole delay to its channel, 0 to the zero channel).
     for (uint32_t l = L; l >= 1; --l) {
         for (const int32_t id : at[l]) {
             NodeInfo& nd = nodes[id];
             sort_unique(nd.need);
-            const uint32_t cm = (nd.c0 + nd.c1) / 2;
-            const double r    = lower_fraction(nd.c0, cm, nd.c1);
+            const double r = nd.pass ? 1.0 : lower_fraction(nd.c0, nd.cm, nd.c1);
             for (const int32_t D : nd.need) {
-                const int32_t dl = nearest(static_cast<double>(D) * r);
+                const int32_t dl = nd.pass ? D : nearest(static_cast<double>(D) * r);
                 nodes[nd.lo].need.push_back(dl);
                 nodes[nd.hi].need.push_back(D - dl);
             }
@@ -220,8 +281,7 @@ FdmtPlan::FdmtPlan(const fa_dedisp_config& c) : cfg(c) {
         for (const int32_t id : at[l]) {
             const NodeInfo& nd = nodes[id];
             first_row[id]      = static_cast<int32_t>(lv.size());
-            const uint32_t cm  = (nd.c0 + nd.c1) / 2;
-            const double r     = l > 0 ? lower_fraction(nd.c0, cm, nd.c1) : 0.0;
+            const double r     = l > 0 && !nd.pass ? lower_fraction(nd.c0, nd.cm, nd.c1) : 1.0;
             for (const int32_t D : nd.need) {
                 lv.delay.push_back(D);
                 lv.node.push_back(id);
@@ -231,14 +291,14 @@ FdmtPlan::FdmtPlan(const fa_dedisp_config& c) : cfg(c) {
                     lv.shift.push_back(0);
                     continue;
                 }
-                const int32_t dl = nearest(static_cast<double>(D) * r);
+                const int32_t dl = nd.pass ? D : nearest(static_cast<double>(D) * r);
                 const auto& lo   = nodes[nd.lo].need;
                 const auto& hi   = nodes[nd.hi].need;
                 const auto il = std::lower_bound(lo.begin(), lo.end(), dl) - lo.begin();
                 const auto ih = std::lower_bound(hi.begin(), hi.end(), D - dl) - hi.begin();
                 lv.chan_or_lo.push_back(first_row[nd.lo] + static_cast<int32_t>(il));
                 lv.hi.push_back(first_row[nd.hi] + static_cast<int32_t>(ih));
-                lv.shift.push_back(dl);
+                lv.shift.push_back(nd.pass ? 0 : dl); // the zero row needs no shift
             }
         }
     }

@@ -3,9 +3,9 @@
 //
 // Replaces dedisp::make_dmt_fdmt_plan (lib/dedisp/dedisp.cpp:71-89), whose
 // dmt::plans::FDMTPlan is not available. The rules (band edges, rounded
-// nu^-2 delays, lower/upper half split with the lower-half delay
+// nu^-2 delays, balanced lower/upper half splits with the lower-half delay
 // round(D * r), pruning to the delays some plane needs, staggered planes)
-// are stated in the engine header fdmt.hpp.
+// are stated in the engine header engine.hpp and in plan.cpp.
 #pragma once
 
 #include <cstdint>
@@ -34,7 +34,11 @@ public:
     fa_dedisp_config cfg{};
     float f_min = 0, f_max = 0;
     double chan_bw  = 0;   // (f_max - f_min) / nchans
-    uint32_t nlevels = 0;  // log2(nchans) + 1
+    uint32_t nlevels = 0;  // ceil(log2(nchans)) + 1
+    // Rows of the level-0 input: nchans, plus one zero channel (index
+    // nchans, all samples 0) when a one-channel level-1 node passes its
+    // channel through (non-power-of-two counts).
+    uint32_t nchans_fdmt = 0;
     std::vector<float> dm_list;
     int32_t dt_max = 0;
 

@@ -194,6 +194,8 @@ struct PlanePlan {
 
 struct Node {
     uint32_t level, c0, c1;
+    uint32_t cm = 0;          // split: lower child [c0, cm), upper [cm, c1)
+    bool pass   = false;      // one channel at level 1: the channel, plus the zero channel
     int32_t lo = -1, hi = -1; // child node ids (level-1), -1 at level 0
     std::vector<int32_t> delays; // sorted distinct row delays
 };
@@ -201,7 +203,8 @@ struct Node {
 struct Plan {
     fa_dedisp_config cfg;
     double f_min = 0, f_max = 0, df = 0;
-    uint32_t nlevels = 0; // = log2(nchans) + 1
+    uint32_t nlevels = 0; // = ceil(log2(nchans)) + 1
+    uint32_t nchans_fdmt = 0; // level-0 input rows: nchans (+1 zero channel for pass-through nodes)
     std::vector<float> dm_list;
     std::vector<Node> nodes;
     std::vector<PlanePlan> planes;
@@ -209,6 +212,9 @@ struct Plan {
     explicit Plan(const fa_dedisp_config& cfg);
     double band_delay(uint32_t c0, uint32_t c1, double dm) const;
     double lo_fraction(uint32_t c0, uint32_t cm, uint32_t c1) const;
+    // Lower child's delay of a node row of delay D (all of it through a
+    // pass-through node).
+    int32_t split_delay(const Node& nd, int32_t D) const;
     // Per-channel (offset, length) of a node row: channels [c0, c1).
     void channel_terms(int32_t node, int32_t delay, std::vector<int32_t>& off,
                        std::vector<int32_t>& len) const;

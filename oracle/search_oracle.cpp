@@ -16,7 +16,13 @@
 //    floor(delay + 0.5). Distinct delays are the plane's rows; the row's DM
 //    is the first DM of the list giving that delay.
 //  * FDMT tree (Zackay & Ofek 2017): node = channel range, children = lower
-//    and upper halves. Row of delay D: lower child row round(D*r) with
+//    and upper halves, split top down into balanced halves [c0, c0 + n/2)
+//    and [c0 + n/2, c1) from the whole band at level ceil(log2 nchans)
+//    (frost's rule for non-power-of-two counts, docs/decisions.md:12-13).
+//    A one-channel node (only at level 1) passes its channel through: its
+//    row of delay D is the channel's row of delay D plus the row of a zero
+//    channel (index nchans, all zero), out(t) = lo(t) + 0.
+//    Row of delay D: lower child row round(D*r) with
 //    r = (f_lo^-2 - f_mid^-2)/(f_lo^-2 - f_hi^-2), upper child row
 //    D - round(D*r) read round(D*r) samples earlier: out(t) = lo(t) + hi(t-s).
 //    Level 0 (one channel, delay D): x(t) + x(t-1) + ... + x(t-D), summed in
@@ -139,8 +145,6 @@ std::vector<uint32_t> box_width_trials(SizeType max_boxcar, float wtsp) {
 Plan::Plan(const fa_dedisp_config& c) : cfg(c) {
     float fmn = 0, fmx = 0;
     add_band_edges(cfg, fmn, fmx);
-    if ((cfg.nchans & (cfg.nchans - 1)) != 0)
-        throw std::invalid_argument("FDMT: nchans must be a power of two");
     if (cfg.subband_depth > 2)
         throw std::invalid_argument("FDMT: subband_depth must be 0, 1 or 2");
     f_min  = fmn;
@@ -151,37 +155,60 @@ Plan::Plan(const fa_dedisp_config& c) : cfg(c) {
     while ((SizeType{1} << L) < cfg.nchans) ++L;
     if (cfg.subband_depth > L) throw std::invalid_argument("FDMT: depth too deep");
     nlevels = L + 1;
-    // Nodes: per level, tree nodes (ascending) then staggered ones.
-    std::vector<std::vector<int32_t>> level_nodes(nlevels);
-    for (uint32_t l = 0; l <= L; ++l) {
-        const uint32_t w = 1U << l;
-        for (uint32_t j = 0; j < cfg.nchans / w; ++j) {
-            Node nd;
-            nd.level = l;
-            nd.c0    = j * w;
-            nd.c1    = (j + 1) * w;
-            if (l > 0) {
-                nd.lo = level_nodes[l - 1][2 * j];
-                nd.hi = level_nodes[l - 1][2 * j + 1];
+    // Tree nodes per level (ascending), built top down by balanced halves.
+    std::vector<std::vector<int32_t>> tree(nlevels), level_nodes(nlevels);
+    nodes.push_back(Node{L, 0, (uint32_t)cfg.nchans});
+    tree[L].push_back(0);
+    int32_t zero = -1;
+    for (uint32_t l = L; l >= 1; --l) {
+        for (const int32_t id : tree[l]) {
+            const uint32_t c0 = nodes[id].c0, c1 = nodes[id].c1;
+            if (c1 - c0 == 1) {
+                if (zero < 0) {
+                    zero = (int32_t)nodes.size();
+                    nodes.push_back(Node{0, (uint32_t)cfg.nchans, (uint32_t)cfg.nchans + 1});
+                }
+                const int32_t ch = (int32_t)nodes.size();
+                nodes.push_back(Node{l - 1, c0, c1});
+                tree[l - 1].push_back(ch);
+                nodes[id].pass = true;
+                nodes[id].cm   = c1;
+                nodes[id].lo   = ch;
+                nodes[id].hi   = zero;
+                continue;
             }
-            level_nodes[l].push_back((int32_t)nodes.size());
-            nodes.push_back(nd);
+            const uint32_t cm = c0 + (c1 - c0) / 2;
+            const int32_t a   = (int32_t)nodes.size();
+            nodes.push_back(Node{l - 1, c0, cm});
+            nodes.push_back(Node{l - 1, cm, c1});
+            tree[l - 1].push_back(a);
+            tree[l - 1].push_back(a + 1);
+            nodes[id].cm = cm;
+            nodes[id].lo = a;
+            nodes[id].hi = a + 1;
         }
+    }
+    // Staggered nodes: the upper child of a tree node and the lower child
+    // of the next, at depths 1..subband_depth.
+    for (uint32_t l = 0; l <= L; ++l) {
+        level_nodes[l] = tree[l];
         const uint32_t depth = L - l;
         if (depth >= 1 && depth <= cfg.subband_depth) {
-            const uint32_t ntree = cfg.nchans / w;
-            for (uint32_t s = 0; s + 1 < ntree; ++s) {
-                Node nd;
-                nd.level = l;
-                nd.c0    = s * w + w / 2;
-                nd.c1    = (s + 1) * w + w / 2;
-                nd.lo    = level_nodes[l - 1][2 * s + 1];
-                nd.hi    = level_nodes[l - 1][2 * s + 2];
+            for (std::size_t s = 0; s + 1 < tree[l].size(); ++s) {
+                const Node& left  = nodes[tree[l][s]];
+                const Node& right = nodes[tree[l][s + 1]];
+                if (left.pass || right.pass) throw std::invalid_argument("FDMT: depth too deep");
+                Node nd{l, nodes[left.hi].c0, nodes[right.lo].c1};
+                nd.cm = right.c0;
+                nd.lo = left.hi;
+                nd.hi = right.lo;
                 level_nodes[l].push_back((int32_t)nodes.size());
                 nodes.push_back(nd);
             }
         }
     }
+    if (zero >= 0) level_nodes[0].push_back(zero);
+    nchans_fdmt = (uint32_t)cfg.nchans + (zero >= 0 ? 1U : 0U);
     // Planes, by depth: tree nodes then staggered.
     std::vector<std::vector<int32_t>> req(nodes.size());
     for (uint32_t d = 0; d <= cfg.subband_depth; ++d) {
@@ -193,9 +220,7 @@ Plan::Plan(const fa_dedisp_config& c) : cfg(c) {
             pp.level     = l;
             pp.depth     = d;
             pp.idx       = idx++;
-            pp.staggered = (nd.c1 - nd.c0 == (1U << l)) && (nd.c0 % (1U << l) == 0)
-                               ? 0
-                               : 1;
+            pp.staggered = idx > tree[l].size() ? 1 : 0;
             pp.c0     = nd.c0;
             pp.c1     = nd.c1;
             pp.f_low  = (float)(f_min + nd.c0 * df);
@@ -221,10 +246,8 @@ Plan::Plan(const fa_dedisp_config& c) : cfg(c) {
             std::sort(r.begin(), r.end());
             r.erase(std::unique(r.begin(), r.end()), r.end());
             const Node& nd     = nodes[id];
-            const uint32_t cm  = (nd.c0 + nd.c1) / 2;
-            const double frac  = lo_fraction(nd.c0, cm, nd.c1);
             for (const int32_t D : r) {
-                const int32_t dl = round_delay((double)D * frac);
+                const int32_t dl = split_delay(nd, D);
                 req[nd.lo].push_back(dl);
                 req[nd.hi].push_back(D - dl);
             }
@@ -247,6 +270,10 @@ double Plan::band_delay(uint32_t c0, uint32_t c1, double dm) const {
     return kDispersion * dm * (ilo - ihi) / (double)cfg.dt;
 }
 
+int32_t Plan::split_delay(const Node& nd, int32_t D) const {
+    return nd.pass ? D : round_delay((double)D * lo_fraction(nd.c0, nd.cm, nd.c1));
+}
+
 double Plan::lo_fraction(uint32_t c0, uint32_t cm, uint32_t c1) const {
     const double f0 = f_min + c0 * df, fm = f_min + cm * df, f1 = f_min + c1 * df;
     const double i0 = 1.0 / (f0 * f0), im = 1.0 / (fm * fm), i1 = 1.0 / (f1 * f1);
@@ -265,11 +292,10 @@ void Plan::channel_terms(int32_t node, int32_t delay, std::vector<int32_t>& off,
         stack.pop_back();
         const Node& nd = nodes[it.node];
         if (nd.level == 0) {
-            out.push_back({nd.c0, {it.shift, it.delay + 1}});
+            if (nd.c0 < cfg.nchans) out.push_back({nd.c0, {it.shift, it.delay + 1}}); // not the zero channel
             continue;
         }
-        const uint32_t cm = (nd.c0 + nd.c1) / 2;
-        const int32_t dl  = round_delay((double)it.delay * lo_fraction(nd.c0, cm, nd.c1));
+        const int32_t dl = split_delay(nd, it.delay);
         stack.push_back({nd.lo, dl, it.shift});
         stack.push_back({nd.hi, it.delay - dl, it.shift + dl});
     }
@@ -593,9 +619,10 @@ void Engine::execute(const float* wf, SizeType N, const uint8_t* kill_mask) {
         inv_or[p] = changed ? compute_inv(mor, p) : inv_cur[p];
     }
     inv_mask = m;
-    // Extended input [nch][hist + N]: carried history, then this gulp.
+    // Extended input [nch][hist + N]: carried history, then this gulp (and a
+    // zero channel's row after the channels when the plan has one).
     const SizeType E = hist + N;
-    std::vector<float> X(nch * E);
+    std::vector<float> X((SizeType)plan.nchans_fdmt * E, 0.0F);
     for (SizeType c = 0; c < nch; ++c) {
         std::memcpy(&X[c * E], &carry[c * hist], hist * sizeof(float));
         std::memcpy(&X[c * E + hist], wf + c * N, N * sizeof(float));
@@ -645,8 +672,8 @@ void Engine::execute(const float* wf, SizeType N, const uint8_t* kill_mask) {
             const int32_t D = nd.delays[r];
             const Node& lo    = plan.nodes[nd.lo];
             const Node& hi    = plan.nodes[nd.hi];
-            const uint32_t cm = (nd.c0 + nd.c1) / 2;
-            const int32_t dl  = round_delay((double)D * plan.lo_fraction(nd.c0, cm, nd.c1));
+            const int32_t dl  = plan.split_delay(nd, D);
+            const int32_t sh  = nd.pass ? 0 : dl; // the zero channel's row needs no shift
             const auto il =
                 std::lower_bound(lo.delays.begin(), lo.delays.end(), dl) - lo.delays.begin();
             const auto ih = std::lower_bound(hi.delays.begin(), hi.delays.end(), D - dl) -
@@ -654,7 +681,7 @@ void Engine::execute(const float* wf, SizeType N, const uint8_t* kill_mask) {
             const auto& a = prev_rows[nd.lo][il];
             const auto& b = prev_rows[nd.hi][ih];
             for (SizeType t = 0; t < E; ++t)
-                out[t] = a[t] + ((int64_t)t - dl >= 0 ? b[t - dl] : 0.0F);
+                out[t] = a[t] + ((int64_t)t - sh >= 0 ? b[t - sh] : 0.0F);
         }
         const auto tl1 = clock::now();
         for (uint32_t p = 0; p < plan.planes.size(); ++p)

@@ -67,6 +67,13 @@ CONFIGS = [
     dict(nchans=256, dt=64e-6, f0=1500.0, df=-300 / 256, dm_max=1000.0, block_nsamps=65536),
     dict(nchans=1024, dt=128e-6, f0=1500.0, df=-300 / 1024, dm_max=500.0, subband_depth=1,
          block_nsamps=1 << 18),
+    # channel counts that are not powers of two (balanced halves)
+    dict(nchans=24, dt=1e-3, f0=1400.0, df=-1.0, dm_max=60.0, dm_pulse_width=1000.0,
+         subband_depth=2, block_nsamps=1024),
+    dict(nchans=336, dt=64e-6, f0=1500.0, df=-300 / 336, dm_max=800.0, subband_depth=2,
+         block_nsamps=65536),
+    dict(nchans=3, dt=1e-3, f0=1400.0, df=-4.0, dm_max=100.0, dm_pulse_width=1000.0,
+         block_nsamps=1024),
 ]
 
 
@@ -109,7 +116,7 @@ def test_invalid_arguments_raise():
     with pytest.raises(abi.InvalidArgument):
         search.make_fdmt_plan(abi.dedisp_config(16, 1e-3, 1400.0, 0.0, 50.0))
     with pytest.raises(abi.InvalidArgument):
-        search.make_fdmt_plan(abi.dedisp_config(12, 1e-3, 1400.0, -1.0, 50.0))
+        search.make_fdmt_plan(abi.dedisp_config(1, 1e-3, 1400.0, -1.0, 50.0))  # frost: nchans >= 2
     with pytest.raises(abi.InvalidArgument):
         search.make_fdmt_plan(abi.dedisp_config(16, 1e-3, 1400.0, -1.0, 50.0, dm_tol=1.0))
     with pytest.raises(abi.InvalidArgument):

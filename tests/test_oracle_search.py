@@ -11,19 +11,22 @@ def small_cfg(nchans=16, depth=0, dm_max=60.0, dt=1e-3, f0=1400.0, df=-1.0, N=51
                              block_nsamps=N, subband_depth=depth)
 
 
-@pytest.mark.parametrize("depth", [0, 1, 2])
-def test_fdmt_tree_equals_brute_force_on_integers(oracle, depth):
+@pytest.mark.parametrize("nch,depth", [(16, 0), (16, 1), (16, 2), (12, 2), (13, 1), (24, 2),
+                                       (5, 0), (3, 0), (6, 1), (7, 1), (10, 2)])
+def test_fdmt_tree_equals_brute_force_on_integers(oracle, nch, depth):
     """Integer-valued inputs make every FP32 sum exact, so the FDMT tree and a
     direct sum over the plan's per-channel (offset, length) terms must agree
-    bit for bit, for every plane and row, across two gulps (carry)."""
-    cfg = small_cfg(depth=depth)
+    bit for bit, for every plane and row, across two gulps (carry); also for
+    channel counts that are not powers of two (balanced halves, one-channel
+    level-1 nodes passing their channel through)."""
+    cfg = small_cfg(nchans=nch, depth=depth, df=-16.0 / nch)
     N = 512
     eng = oracle.Engine(cfg, abi.boxcar_config(max_boxcar=8, detect_thresh=50.0),
                         keep_planes=True)
     hist = eng.hist
-    rng = np.random.default_rng(depth)
-    gulps = [rng.integers(-8, 9, (16, N)).astype(np.float32) for _ in range(2)]
-    stream = np.concatenate([np.zeros((16, hist), np.float32)] + gulps, axis=1)
+    rng = np.random.default_rng(depth + 100 * nch)
+    gulps = [rng.integers(-8, 9, (nch, N)).astype(np.float32) for _ in range(2)]
+    stream = np.concatenate([np.zeros((nch, hist), np.float32)] + gulps, axis=1)
     plan = eng.plan
     for g, wf in enumerate(gulps):
         eng.execute(wf)
@@ -36,15 +39,19 @@ def test_fdmt_tree_equals_brute_force_on_integers(oracle, depth):
                 assert (bf == rows[r]).all(), (g, p, r)
 
 
-def test_channel_terms_cover_band_delay(oracle):
+@pytest.mark.parametrize("nch", [64, 100, 24, 10])
+def test_channel_terms_cover_band_delay(oracle, nch):
     """Each row's channel terms tile the row's delay: offsets are
     non-decreasing with frequency, the lowest channel starts at 0 and the last
-    term ends at the row delay (off + len - 1 == delay)."""
-    plan = oracle.Plan(small_cfg(nchans=64, depth=2, dm_max=500.0))
+    term ends at the row delay (off + len - 1 == delay); one term per channel
+    of the plane."""
+    plan = oracle.Plan(small_cfg(nchans=nch, depth=2, dm_max=500.0, df=-64.0 / nch))
     for p in range(plan.summary()["nplanes"]):
         dm, delay = plan.plane_rows(p)
         for r in range(dm.size):
             off, ln = plan.channel_terms(p, r)
+            info = plan.plane(p)
+            assert len(off) == info.chan_end - info.chan_begin
             assert off[0] == 0
             assert (np.diff(off) >= 0).all()
             assert off[-1] + ln[-1] - 1 == delay[r]
@@ -230,3 +237,27 @@ def test_search_row_equals_engine_rows(oracle, depth):
                 assert sorted(fresh.search_row(p, r, bad)) != mine
                 planted = True
     assert total > 0 and planted
+
+
+@pytest.mark.parametrize("nch", [24, 100, 1000])
+def test_balanced_halves_plane_layout(oracle, nch):
+    """Non-power-of-two channel counts split top down into balanced halves
+    (frost, docs/decisions.md:12-13): at two sub-band levels the 11 planes are
+    the band, its halves [0, n/2) and [n/2, n), the staggered half across
+    their boundary, the four quarters and the three staggered quarters, each
+    staggered band the upper child of one node and the lower child of the
+    next."""
+    cfg = small_cfg(nchans=nch, depth=2, dm_max=200.0, df=-300.0 / nch)
+    plan = oracle.Plan(cfg)
+    spans = [(plan.plane(p).chan_begin, plan.plane(p).chan_end, plan.plane(p).staggered)
+             for p in range(plan.summary()["nplanes"])]
+    h = nch // 2
+    q = [0, h // 2, h, h + (nch - h) // 2, nch]
+    assert spans[0] == (0, nch, 0)
+    assert spans[1:4] == [(0, h, 0), (h, nch, 0), (q[1], q[3], 1)]
+    assert spans[4:8] == [(q[k], q[k + 1], 0) for k in range(4)]
+    # staggered quarters: upper eighth of one quarter to lower eighth of the next
+    for k, (c0, c1, st) in enumerate(spans[8:11]):
+        a0, a1 = q[k], q[k + 1]
+        b0, b1 = q[k + 1], q[k + 2]
+        assert st == 1 and c0 == a0 + (a1 - a0) // 2 and c1 == b0 + (b1 - b0) // 2
