@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
@@ -261,51 +262,98 @@ bandpass_kernel(Geometry g, float* __restrict__ loc_out, float* __restrict__ sca
 
 // ---- moments -------------------------------------------------------------------
 
-// One lane per (window, channel): FP64 raw and clipped moments of
-// z = (x - loc) * inv in time order (lib/cpu/rfi_cpu_measure.cpp:91-138).
+// FP64 raw and clipped moments of z = (x - loc) * inv per (window, channel)
+// (lib/cpu/rfi_cpu_measure.cpp:91-138), in a blocked order the oracle
+// restates: the window's rows in segments of kMomentSeg rows, each summed
+// in time order, and the segment sums folded in segment order (frost's
+// -ffast-math build fixes no summation order). One lane per (window,
+// segment, channel); the kernel is bound by its FP64 instruction issue.
+constexpr uint64_t kMomentSeg   = 4096;
+
 template <int DT>
 __global__ void __launch_bounds__(64)
-moments_kernel(Geometry g, const float* __restrict__ loc, const float* __restrict__ inv,
-               const uint8_t* __restrict__ static_mask, float clip,
-               MomentSumsDev* __restrict__ out) {
-    const uint64_t w = blockIdx.x;
-    const uint64_t c = (uint64_t)blockIdx.y * 64 + threadIdx.x;
+moments_seg_kernel(Geometry g, const float* __restrict__ loc, const float* __restrict__ inv,
+                   const uint8_t* __restrict__ static_mask, float clip, uint64_t segmax,
+                   MomentSumsDev* __restrict__ part) {
+    const uint64_t w   = blockIdx.x;
+    const uint64_t seg = blockIdx.z;
+    const uint64_t b = g.window_begin(w), e = g.window_end(w);
+    const uint64_t sb = b + seg * kMomentSeg;
+    if (sb >= e) return; // uniform: past the window's last segment
+    const uint64_t se = sb + kMomentSeg < e ? sb + kMomentSeg : e;
+    const uint64_t c  = (uint64_t)blockIdx.y * 64 + threadIdx.x;
     if (c >= g.nch) return;
-    MomentSumsDev s{};
+    MomentSumsDev* out = part + (w * segmax + seg) * g.nch + c;
     const float lc = loc[w * g.nch + c], ic = inv[w * g.nch + c];
-    if (static_mask[c] != 0 && ic != 0.0f) {
-        const uint64_t b = g.window_begin(w), e = g.window_end(w);
-        double prev      = 0.0;
-        const uint8_t* p = g.block + b * g.row_bytes;
-        for (uint64_t t0 = b; t0 < e; t0 += kRowBatch) {
-            const int nq = (int)min((uint64_t)kRowBatch, e - t0);
-            float xs[kRowBatch];
-#pragma unroll
-            for (int q = 0; q < kRowBatch; ++q)
-                xs[q] = xat<DT>(p + (uint64_t)min(q, nq - 1) * g.row_bytes, c, g.dtype);
-            p += kRowBatch * g.row_bytes;
-#pragma unroll
-            for (int q = 0; q < kRowBatch; ++q) {
-                if (q >= nq) break;
-                const float zf  = (xs[q] - lc) * ic;
-                const double z  = zf;
-                const double z2 = z * z;
-                s.n += 1.0;
-                s.s1 += z;
-                s.s2 += z2;
-                s.s3 += z2 * z;
-                s.s4 += z2 * z2;
-                if (t0 + q > b) s.lag += z * prev;
-                prev = z;
-                if (fabs(z) <= (double)clip) {
-                    s.nc += 1.0;
-                    s.c1 += z;
-                    s.c2 += z2;
-                }
-            }
-        }
+    if (static_mask[c] == 0 || ic == 0.0f) { // not measured: zero sums
+        *out = MomentSumsDev{};
+        return;
     }
-    out[w * g.nch + c] = s;
+    // The clipped sums add +0.0 for a clipped sample, which leaves them
+    // unchanged (they start at +0.0 and never become -0.0).
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, lag = 0.0, c1 = 0.0, c2 = 0.0;
+    int32_t cc = 0;
+    double prev = 0.0;
+    const double dclip = (double)clip;
+    auto add = [&](float x, bool lagged) {
+        const float zf  = (x - lc) * ic;
+        const double z  = zf;
+        const double z2 = z * z;
+        s1 += z;
+        s2 += z2;
+        s3 += z2 * z;
+        s4 += z2 * z2;
+        if (lagged) lag += z * prev;
+        prev              = z;
+        const bool kept   = fabs(z) <= dclip;
+        cc += kept ? 1 : 0;
+        c1 += kept ? z : 0.0;
+        c2 += kept ? z2 : 0.0;
+    };
+    uint64_t t = sb;
+    if (sb == b) { // the window's first row has no lag pair
+        add(xat<DT>(g.block + b * g.row_bytes, c, g.dtype), false);
+        ++t;
+    } else {       // the lag pair across the segment boundary
+        prev = (double)((xat<DT>(g.block + (sb - 1) * g.row_bytes, c, g.dtype) - lc) * ic);
+    }
+    constexpr int NB = 16; // rows loaded ahead
+    const uint8_t* p = g.block + t * g.row_bytes;
+    for (; t + NB <= se; t += NB) {
+        float xs[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) xs[q] = xat<DT>(p + (uint64_t)q * g.row_bytes, c, g.dtype);
+        p += NB * g.row_bytes;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) add(xs[q], true);
+    }
+    for (; t < se; ++t, p += g.row_bytes) add(xat<DT>(p, c, g.dtype), true);
+    *out = MomentSumsDev{(double)(se - sb), s1, s2, s3, s4, lag, (double)cc, c1, c2};
+}
+
+// Per (window, channel): the segment sums folded in segment order.
+__global__ void moments_fold_kernel(Geometry g, uint64_t segmax,
+                                    const MomentSumsDev* __restrict__ part,
+                                    MomentSumsDev* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nwin * g.nch) return;
+    const uint64_t w = i / g.nch, c = i % g.nch;
+    const uint64_t len  = g.window_end(w) - g.window_begin(w);
+    const uint64_t nseg = (len + kMomentSeg - 1) / kMomentSeg;
+    MomentSumsDev t{};
+    for (uint64_t k = 0; k < nseg; ++k) {
+        const MomentSumsDev& q = part[(w * segmax + k) * g.nch + c];
+        t.n += q.n;
+        t.s1 += q.s1;
+        t.s2 += q.s2;
+        t.s3 += q.s3;
+        t.s4 += q.s4;
+        t.lag += q.lag;
+        t.nc += q.nc;
+        t.c1 += q.c1;
+        t.c2 += q.c2;
+    }
+    out[i] = t;
 }
 
 // ---- Kadane (lib/cpu/rfi_kernels.hpp:65-133) ------------------------------------
@@ -1562,6 +1610,9 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
     measured_.resize(maxwin * nch_);
     if (need_moments_) {
         moments_.resize(maxwin * nch_);
+        // Segment sums: the last window (up to two window lengths) bounds the count.
+        moment_segmax_ = (2 * window_len_ + kMomentSeg - 1) / kMomentSeg;
+        moments_part_.resize(maxwin * moment_segmax_ * nch_);
         clip_mean_.resize(maxwin * nch_);
         clip_inv_.resize(maxwin * nch_);
     }
@@ -1626,6 +1677,7 @@ uint64_t DeviceRfi::chunk_count() const {
 
 uint64_t DeviceRfi::memory_bytes() const {
     return loc_.bytes() + scale_.bytes() + inv_.bytes() + measured_.bytes() + moments_.bytes() +
+           moments_part_.bytes() +
            clip_mean_.bytes() + clip_inv_.bytes() + mask_dev_.bytes() + flags_dev_.bytes() +
            kwhole_.bytes() + good_prefix_.bytes() + kspec_.bytes() + s_det_.bytes() +
            s_keep_.bytes() + row_std_.bytes() + cmin_.bytes() + cmax_.bytes() + kpos_.bytes() +
@@ -1898,12 +1950,21 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
     FA_HIP(hipMemcpyAsync(mask_dev_.get(), static_mask_.data(), n, hipMemcpyHostToDevice, s));
     FA_HIP(hipEventRecord(ev_[1], s));
     if (need_moments_) {
+        // Segments of the longest window this gulp has: its last, which runs
+        // to nsamps (under two window lengths).
+        const uint64_t last_len = nsamps - (nwin_ - 1) * window_len_;
+        const uint64_t nseg     = (last_len + kMomentSeg - 1) / kMomentSeg;
+        if (nseg > moment_segmax_) throw std::logic_error("moments: more segments than planned");
+        const dim3 grid(static_cast<unsigned>(nwin_), cblk, static_cast<unsigned>(nseg));
         if (layout_.dtype == FA_UINT8)
-            moments_kernel<FA_UINT8><<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
-                g, loc_.get(), inv_.get(), mask_dev_.get(), clip_, moments_.get());
+            moments_seg_kernel<FA_UINT8><<<grid, 64, 0, s>>>(g, loc_.get(), inv_.get(), mask_dev_.get(),
+                                                             clip_, moment_segmax_, moments_part_.get());
         else
-            moments_kernel<-1><<<dim3(static_cast<unsigned>(nwin_), cblk), 64, 0, s>>>(
-                g, loc_.get(), inv_.get(), mask_dev_.get(), clip_, moments_.get());
+            moments_seg_kernel<-1><<<grid, 64, 0, s>>>(g, loc_.get(), inv_.get(), mask_dev_.get(),
+                                                       clip_, moment_segmax_, moments_part_.get());
+        FA_HIP(hipGetLastError());
+        moments_fold_kernel<<<ceil_div(nwin_ * n, 256), 256, 0, s>>>(g, moment_segmax_,
+                                                                      moments_part_.get(), moments_.get());
         FA_HIP(hipGetLastError());
     }
     FA_HIP(hipStreamSynchronize(s));

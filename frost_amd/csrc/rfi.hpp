@@ -113,6 +113,8 @@ private:
     DeviceBuffer<float> loc_, scale_, inv_;
     DeviceBuffer<uint8_t> measured_;
     DeviceBuffer<MomentSumsDev> moments_;
+    DeviceBuffer<MomentSumsDev> moments_part_; // per (window, segment, channel) sums
+    uint64_t moment_segmax_ = 0;
     DeviceBuffer<float> clip_mean_, clip_inv_;
     DeviceBuffer<uint8_t> mask_dev_, flags_dev_, kwhole_;
     DeviceBuffer<int32_t> good_prefix_, kspec_;

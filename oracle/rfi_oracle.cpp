@@ -19,6 +19,7 @@ constexpr SizeType kMinBlockSamps   = 64;
 constexpr int kMaxDraws             = 10;
 constexpr SizeType kMaxChunks       = 64;
 constexpr SizeType kMinChunkRows    = 1024;
+constexpr SizeType kMomentSegRows   = 4096; // moments: rows per blocked partial sum
 constexpr uint64_t kStreamCell = 1, kStreamTime = 2, kStreamNoise = 3,
                    kStreamMasked = 4;
 enum CellState : uint8_t {
@@ -612,7 +613,12 @@ void Cleaner::measure_bandpass() {
 }
 
 void Cleaner::measure_moments() {
-    // lib/cpu/rfi_cpu_measure.cpp:91-138 (per window/channel, sequential t)
+    // lib/cpu/rfi_cpu_measure.cpp:91-138, in a blocked order: the window's
+    // rows in segments of kMomentSegRows, each summed in time order (the lag
+    // pair at a segment start uses the row before it), then the segment sums
+    // added in segment order onto zero. frost's -ffast-math build fixes no
+    // summation order; this one lets the device sum the segments in
+    // parallel. Integer counts are exact in any order.
     const SizeType n = nchans;
     moments.assign(nwin * n, MomentSums{});
     for (SizeType w = 0; w < nwin; ++w) {
@@ -621,24 +627,37 @@ void Cleaner::measure_moments() {
         for (SizeType c = 0; c < n; ++c) {
             const float lc = loc[w * n + c], ic = inv_scale[w * n + c];
             if (static_mask[c] == 0 || ic == 0.0F) continue;
-            MomentSums& s = moments[w * n + c];
-            double prev   = 0.0;
-            for (SizeType t = b; t < e; ++t) {
-                const float zf  = (row(t)[c] - lc) * ic;
-                const double z  = zf;
-                const double z2 = z * z;
-                s.n += 1.0;
-                s.s1 += z;
-                s.s2 += z2;
-                s.s3 += z2 * z;
-                s.s4 += z2 * z2;
-                if (t > b) s.lag += z * prev;
-                prev = z;
-                if (std::abs(z) <= clip) {
-                    s.nc += 1.0;
-                    s.c1 += z;
-                    s.c2 += z2;
+            MomentSums& tot = moments[w * n + c];
+            for (SizeType sb = b; sb < e; sb += kMomentSegRows) {
+                const SizeType se = std::min(sb + kMomentSegRows, e);
+                MomentSums s;
+                double prev = sb > b ? (double)((row(sb - 1)[c] - lc) * ic) : 0.0;
+                for (SizeType t = sb; t < se; ++t) {
+                    const float zf  = (row(t)[c] - lc) * ic;
+                    const double z  = zf;
+                    const double z2 = z * z;
+                    s.n += 1.0;
+                    s.s1 += z;
+                    s.s2 += z2;
+                    s.s3 += z2 * z;
+                    s.s4 += z2 * z2;
+                    if (t > b) s.lag += z * prev;
+                    prev = z;
+                    if (std::abs(z) <= clip) {
+                        s.nc += 1.0;
+                        s.c1 += z;
+                        s.c2 += z2;
+                    }
                 }
+                tot.n += s.n;
+                tot.s1 += s.s1;
+                tot.s2 += s.s2;
+                tot.s3 += s.s3;
+                tot.s4 += s.s4;
+                tot.lag += s.lag;
+                tot.nc += s.nc;
+                tot.c1 += s.c1;
+                tot.c2 += s.c2;
             }
         }
     }
