@@ -761,47 +761,62 @@ __global__ void __launch_bounds__(64) scan_cols_kernel(ColArgs a) {
             kreg_start(kp, (int32_t)t0);
             kreg_start(kn, (int32_t)t0);
         }
-        const uint8_t* p = g.block + t0 * g.row_bytes;
-        for (uint64_t tb = t0; tb < t1; tb += kRowBatch) {
-            const int nq = (int)min((uint64_t)kRowBatch, t1 - tb);
-            float xs[kRowBatch];
-#pragma unroll
-            for (int q = 0; q < kRowBatch; ++q)
-                xs[q] = xat<DT>(p + (uint64_t)min(q, nq - 1) * g.row_bytes, c, g.dtype);
-            p += kRowBatch * g.row_bytes;
-#pragma unroll
-            for (int q = 0; q < kRowBatch; ++q) {
-                if (q >= nq) break;
-                const uint64_t t = tb + q;
-                if (t >= wend) {
-                    ++w;
-                    wend = g.window_end(w);
-                    load_w();
-                }
-                const float x = xs[q];
-                float z       = 0.0f;
-                if (ic == 0.0f) {
-                    lo = fmin_(lo, x);
-                    hi = fmax_(hi, x);
-                } else {
-                    z = (x - lc) * ic;
-                }
-                if (a.zsums && ic != 0.0f && !(fabsf(z) > a.clip)) {
-                    const double sk = a.s_keep64[t];
-                    zn += 1.0;
-                    zsz += z;
-                    zss += sk;
-                    zsss += sk * sk;
-                    zszs += z * sk;
-                }
-                if (a.kchan) {
-                    const bool outlier = fabsf(z) > a.clip;
-                    const double u     = outlier ? 0.0 : (double)((z - cm) * ci);
-                    kreg_extend(kp, (int32_t)t, u);
-                    kreg_extend(kn, (int32_t)t, -u);
-                }
+        auto step = [&](uint64_t t, float x) {
+            if (t >= wend) {
+                ++w;
+                wend = g.window_end(w);
+                load_w();
             }
+            float z = 0.0f;
+            if (ic == 0.0f) {
+                lo = fmin_(lo, x);
+                hi = fmax_(hi, x);
+            } else {
+                z = (x - lc) * ic;
+            }
+            if (a.zsums && ic != 0.0f && !(fabsf(z) > a.clip)) {
+                const double sk = a.s_keep64[t];
+                zn += 1.0;
+                zsz += z;
+                zss += sk;
+                zsss += sk * sk;
+                zszs += z * sk;
+            }
+            if (a.kchan) {
+                const bool outlier = fabsf(z) > a.clip;
+                const double u     = outlier ? 0.0 : (double)((z - cm) * ci);
+                kreg_extend(kp, (int32_t)t, u);
+                kreg_extend(kn, (int32_t)t, -u);
+            }
+        };
+        // Rows in batches of kRowBatch, the next batch's loads issued before
+        // this batch's arithmetic (two register sets), then the tail.
+        const uint8_t* p = g.block + t0 * g.row_bytes;
+        float xa[kRowBatch], xb[kRowBatch];
+        auto load = [&](float (&x)[kRowBatch], const uint8_t* q0) {
+#pragma unroll
+            for (int q = 0; q < kRowBatch; ++q) x[q] = xat<DT>(q0 + (uint64_t)q * g.row_bytes, c, g.dtype);
+        };
+        uint64_t t = t0;
+        const uint64_t full = t0 + (t1 - t0) / kRowBatch * kRowBatch;
+        if (t < full) load(xa, p);
+        while (t < full) {
+            const bool more_b = t + kRowBatch < full;
+            if (more_b) load(xb, p + (uint64_t)kRowBatch * g.row_bytes);
+#pragma unroll
+            for (int q = 0; q < kRowBatch; ++q) step(t + q, xa[q]);
+            t += kRowBatch;
+            p += (uint64_t)kRowBatch * g.row_bytes;
+            if (!more_b) break;
+            const bool more_a = t + kRowBatch < full;
+            if (more_a) load(xa, p + (uint64_t)kRowBatch * g.row_bytes);
+#pragma unroll
+            for (int q = 0; q < kRowBatch; ++q) step(t + q, xb[q]);
+            t += kRowBatch;
+            p += (uint64_t)kRowBatch * g.row_bytes;
+            if (!more_a) break;
         }
+        for (; t < t1; ++t, p += g.row_bytes) step(t, xat<DT>(p, c, g.dtype));
         last = (int64_t)t1 - 1;
     }
     a.cmin[o] = lo;
