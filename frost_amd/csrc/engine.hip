@@ -1072,7 +1072,7 @@ __device__ uint32_t walk_chunk(const SearchArgs& a, uint64_t gabs, const float* 
     return __builtin_amdgcn_readfirstlane(st->open);
 }
 
-template <int KMAX, bool TRAP, bool RING>
+template <int KMAX, bool TRAP, bool RING, bool ACROSS>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRAP ? 1 : 6))) search_reg_kernel(RegSearchArgs ra) {
     const SearchArgs& a = ra.a;
     __shared__ float s_snr[kBlock / 64][kRegChunk];
@@ -1192,7 +1192,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
             }
             quiet = __ballot(anyp) == 0;
         }
-        if (a.mode == FA_PEAK_ACROSS_WIDTHS) {
+        if constexpr (ACROSS) {
             // One run per row: per sample the largest S/N over the widths
             // searched at it (the lowest width index on ties).
             if (u >= 0 && (!quiet || open_mask != 0)) {
@@ -1257,7 +1257,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
 #pragma unroll
         for (int k = 0; k <= KMAX; ++k) {
             const int w = ra.widx[k];
-            if (a.mode == FA_PEAK_ACROSS_WIDTHS || quiet || w < 0) continue;
+            if (ACROSS || quiet || w < 0) continue;
             const uint32_t W = 1U << k;
             float sv[4], sums[4];
 #pragma unroll
@@ -1327,8 +1327,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
 
 template <int KMAX, bool TRAP>
 void launch_reg(const RegSearchArgs& ra, unsigned blocks, hipStream_t s) {
-    if (ra.ring) search_reg_kernel<KMAX, TRAP, true><<<blocks, kBlock, 0, s>>>(ra);
-    else search_reg_kernel<KMAX, TRAP, false><<<blocks, kBlock, 0, s>>>(ra);
+    const bool across = ra.a.mode == FA_PEAK_ACROSS_WIDTHS;
+    if (ra.ring) {
+        if (across) search_reg_kernel<KMAX, TRAP, true, true><<<blocks, kBlock, 0, s>>>(ra);
+        else search_reg_kernel<KMAX, TRAP, true, false><<<blocks, kBlock, 0, s>>>(ra);
+    } else {
+        if (across) search_reg_kernel<KMAX, TRAP, false, true><<<blocks, kBlock, 0, s>>>(ra);
+        else search_reg_kernel<KMAX, TRAP, false, false><<<blocks, kBlock, 0, s>>>(ra);
+    }
 }
 
 using RegLauncher = void (*)(const RegSearchArgs&, unsigned, hipStream_t);
