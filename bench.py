@@ -230,6 +230,24 @@ def ingest_measure(W, beam, gulp, stream, ndm, steps=3):
             "unit": "DM-trial*samples/s"}
 
 
+def pipelined_measure(W, beam, gulp, stream, ndm, steps):
+    """Pipelined gulps (the RFI measurement of gulp k + 1 runs beside gulp
+    k's FDMT and search): K whole gulps timed from the first call to the
+    flush that collects the last, so every gulp's work is inside the
+    interval."""
+    beam.execute_pipelined(gulp, stream, W.gulp, rows=False)
+    beam.flush(stream, rows=False)  # warm: the measurement stream exists
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        beam.execute_pipelined(gulp, stream, W.gulp, rows=False)
+    beam.flush(stream, rows=False)
+    stream.synchronize()
+    dt = time.perf_counter() - t0
+    return {"steps": steps, "ms_per_gulp": 1e3 * dt / steps, "value": ndm * W.gulp * steps / dt,
+            "note": "whole gulps: the first call's measurement and the last flush are timed"}
+
+
 def rfi_transform_measure(W, gulp, stream, steps=2):
     """The RFI stage alone on one gulp, with the workload's strategies, and
     with zero-DM (zdot) and equalisation added: ms per gulp of each, into one
@@ -755,6 +773,9 @@ def main() -> int:
     ingest = ingest_measure(W, beam, gulp, stream, ndm)
     log(f"ingest from pinned host slots: serial {ingest['serial_ms_per_gulp']:.1f} ms, "
         f"overlapped {ingest['overlapped_ms_per_gulp']:.1f} ms per gulp")
+    piped = pipelined_measure(W, beam, gulp, stream, ndm, a.steps)
+    log(f"pipelined gulps: {piped['ms_per_gulp']:.1f} ms per gulp "
+        f"(sequential {1e3 * total / a.steps:.1f})")
     rfi_tf = rfi_transform_measure(W, gulp, stream)
     log(f"RFI stage alone: {rfi_tf['ms_per_gulp']}")
     # Fused minimum bytes of a launch reading stored level a and writing level
@@ -829,6 +850,7 @@ def main() -> int:
                        "host_collect_seconds_per_gulp": collect_s / a.steps},
         "postprocess": post,
         "ingest_from_host": ingest,
+        "pipelined": piped,
         "rfi_transforms": rfi_tf,
         "roofline": {
             "kernel": "fdmt_gather_kernel<16> (FDMT levels l-4 -> l in one pass)",

@@ -477,6 +477,8 @@ EXPORTS = {
     "fa_beam_execute": (C.c_int, [P, P, U64, C.c_int, P]),
     "fa_beam_result": (C.c_int, [P, C.POINTER(RawCandidate), U64, PU64, PU64, PU64, PU64]),
     "fa_beam_drain": (C.c_int, [P, P]),
+    "fa_beam_execute_pipelined": (C.c_int, [P, P, U64, C.c_int, P, C.POINTER(C.c_int)]),
+    "fa_beam_flush": (C.c_int, [P, P, C.POINTER(C.c_int)]),
     "fa_beam_ingest_slot": (C.c_int, [P, C.POINTER(P), PU64]),
     "fa_beam_ingest_submit": (C.c_int, [P, U64]),
     "fa_beam_execute_next": (C.c_int, [P, P]),

@@ -218,6 +218,24 @@ int fa_beam_execute(fa_beam* beam, const void* gulp, uint64_t nsamps, int is_dev
     });
 }
 
+int fa_beam_execute_pipelined(fa_beam* beam, const void* gulp, uint64_t nsamps, int is_device,
+                              void* stream, int* collected) {
+    return guard([&] {
+        require(beam != nullptr, "beam must not be null");
+        const bool c = beam->beam->execute_pipelined(gulp, nsamps, is_device != 0, fa::as_stream(stream));
+        if (collected) *collected = c ? 1 : 0;
+    });
+}
+
+int fa_beam_flush(fa_beam* beam, void* stream, int* collected) {
+    return guard([&] {
+        require(beam != nullptr, "beam must not be null");
+        require(stream != nullptr, "an explicit non-null stream is required");
+        const bool c = beam->beam->flush(fa::as_stream(stream));
+        if (collected) *collected = c ? 1 : 0;
+    });
+}
+
 int fa_beam_ingest_slot(fa_beam* beam, void** host_slot, uint64_t* bytes) {
     return guard([&] {
         require(beam != nullptr && host_slot != nullptr, "arguments must not be null");
@@ -264,7 +282,7 @@ int fa_beam_drain(fa_beam* beam, void* stream) {
 int fa_beam_channel_mask(const fa_beam* beam, uint8_t* out, uint64_t cap) {
     return guard([&] {
         require(beam && out, "beam and out must not be null");
-        const auto& m = beam->beam->rfi().channel_mask();
+        const auto& m = beam->beam->result_mask();
         require(cap >= m.size(), "output too small");
         std::memcpy(out, m.data(), m.size());
     });
@@ -273,7 +291,7 @@ int fa_beam_channel_mask(const fa_beam* beam, uint8_t* out, uint64_t cap) {
 int fa_beam_rfi_report(const fa_beam* beam, fa_rfi_report* out) {
     return guard([&] {
         require(beam && out, "beam and out must not be null");
-        *out = beam->beam->rfi().report();
+        *out = beam->beam->result_report();
     });
 }
 

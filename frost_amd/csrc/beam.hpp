@@ -39,6 +39,7 @@ public:
         // mask (lib/pipeline/pipeline.cpp:334-344,392-399).
         descending_ = layout.df < 0.0;
         static_mask_ = rfi_->channel_mask();
+        result_mask_ = static_mask_;
         std::vector<uint8_t> fdmt = static_mask_;
         if (descending_) std::reverse(fdmt.begin(), fdmt.end());
         FdmtPlan plan(dcfg);
@@ -58,6 +59,7 @@ public:
     void execute(const void* gulp, uint64_t nsamps, bool is_device, hipStream_t s) {
         require(gulp != nullptr, "gulp must not be null");
         require(s != nullptr, "an explicit non-null stream is required");
+        require(!pending_, "a pipelined gulp is pending (flush it first)");
         require(nsamps >= 1 && nsamps <= capacity_, "nsamps must be in [1, capacity]");
         DeviceGuard dg(device_);
         const uint8_t* dev = static_cast<const uint8_t*>(gulp);
@@ -79,12 +81,61 @@ public:
     const uint8_t* clean(const uint8_t* dev, uint64_t nsamps, hipStream_t s) {
         rfi_->execute_normalized(dev, capacity_, nsamps, engine_->input_columns(),
                                  engine_->xstride(), 0, s);
-        const auto& mask = rfi_->channel_mask();
-        if (mask == static_mask_) return nullptr;
-        fdmt_mask_ = mask;
-        if (descending_) std::reverse(fdmt_mask_.begin(), fdmt_mask_.end());
-        return fdmt_mask_.data();
+        return cleaned();
     }
+
+    // Pipelined gulps: this gulp's RFI measurements (bound by FP64 issue)
+    // run on the beam's measurement stream beside the previous gulp's FDMT
+    // and search (bound by memory), then the previous gulp is collected
+    // (its rows, mask and report become the beam's result) and this gulp is
+    // calibrated into the engine's input and its search enqueued on s.
+    // Returns true when a previous gulp was collected; flush() collects the
+    // last one. Rows are those of execute(), one call later.
+    bool execute_pipelined(const void* gulp, uint64_t nsamps, bool is_device, hipStream_t s) {
+        require(gulp != nullptr, "gulp must not be null");
+        require(s != nullptr, "an explicit non-null stream is required");
+        require(nsamps >= 1 && nsamps <= capacity_, "nsamps must be in [1, capacity]");
+        require(!slots_[0].submitted && !slots_[1].submitted,
+                "pipelined gulps cannot run while submitted ingest gulps wait for the device slot");
+        DeviceGuard dg(device_);
+        if (!measure_stream_) FA_HIP(hipStreamCreateWithFlags(&measure_stream_, hipStreamNonBlocking));
+        const uint8_t* dev = static_cast<const uint8_t*>(gulp);
+        if (!is_device) {
+            // The device slot's last gulp was calibrated before its search began.
+            dev_slot_.reserve(capacity_ * row_bytes_);
+            FA_HIP(hipMemcpyAsync(dev_slot_.get(), gulp, nsamps * row_bytes_, hipMemcpyHostToDevice,
+                                  measure_stream_));
+            dev = dev_slot_.get();
+        }
+        rfi_->measure(dev, capacity_, nsamps, measure_stream_);
+        const bool collected = flush(s);
+        rfi_->apply(engine_->input_columns(), engine_->xstride(), 0, s);
+        inflight_mask_    = rfi_->channel_mask();
+        inflight_report_  = rfi_->report();
+        engine_->execute_prepared(capacity_, kill_mask(inflight_mask_), s);
+        pending_        = true;
+        pending_stream_ = s;
+        samples_read_ += nsamps;
+        return collected;
+    }
+
+    // Collects the pending pipelined gulp, if any.
+    bool flush(hipStream_t s) {
+        if (!pending_) return false;
+        require(s != nullptr, "an explicit non-null stream is required");
+        DeviceGuard dg(device_);
+        if (pending_stream_ != s) FA_HIP(hipStreamSynchronize(pending_stream_));
+        engine_->collect(pending_stream_);
+        pending_       = false;
+        result_mask_   = inflight_mask_;
+        result_report_ = inflight_report_;
+        return true;
+    }
+    bool pipelined_pending() const { return pending_; }
+
+    // The channel mask and RFI report of the gulp whose rows are the result.
+    const std::vector<uint8_t>& result_mask() const { return result_mask_; }
+    const fa_rfi_report& result_report() const { return result_report_; }
 
     // Ingest with overlap (the counterpart of frost's GulpPrefetch,
     // lib/pipeline/pipeline.cpp:109-262): two pinned host slots and one
@@ -125,6 +176,7 @@ public:
 
     void execute_next(hipStream_t s) {
         require(s != nullptr, "an explicit non-null stream is required");
+        require(!pending_, "a pipelined gulp is pending (flush it first)");
         Slot& sl = slots_[next_];
         require(sl.submitted, "no submitted gulp to execute");
         DeviceGuard dg(device_);
@@ -143,6 +195,11 @@ public:
     }
 
     ~Beam() {
+        if (pending_ && pending_stream_) (void)hipStreamSynchronize(pending_stream_);
+        if (measure_stream_) {
+            (void)hipStreamSynchronize(measure_stream_);
+            (void)hipStreamDestroy(measure_stream_);
+        }
         if (copy_stream_) (void)hipStreamSynchronize(copy_stream_);
         for (auto& sl : slots_)
             if (sl.copied) (void)hipEventDestroy(sl.copied);
@@ -150,7 +207,10 @@ public:
         if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
     }
 
-    void drain(hipStream_t s) { engine_->drain(s); }
+    void drain(hipStream_t s) {
+        require(!pending_, "a pipelined gulp is pending (flush it first)");
+        engine_->drain(s);
+    }
     void bind_arena(void* arena) { engine_->bind_arena(arena); }
     uint64_t row_bytes() const { return row_bytes_; }
     uint64_t capacity() const { return capacity_; }
@@ -170,6 +230,22 @@ public:
     }
 
 private:
+    // After the RFI pass: the result mask and report, and the gulp's FDMT
+    // kill mask (null when it is the static one).
+    const uint8_t* cleaned() {
+        result_mask_   = rfi_->channel_mask();
+        result_report_ = rfi_->report();
+        return kill_mask(result_mask_);
+    }
+    // The FDMT (ascending-frequency) kill mask of a channel mask, or null
+    // when it is the static one.
+    const uint8_t* kill_mask(const std::vector<uint8_t>& mask) {
+        if (mask == static_mask_) return nullptr;
+        fdmt_mask_ = mask;
+        if (descending_) std::reverse(fdmt_mask_.begin(), fdmt_mask_.end());
+        return fdmt_mask_.data();
+    }
+
     // Clean, search and collect one device gulp; after_clean runs once the
     // RFI pass (the raw gulp's only reader) is enqueued.
     template <class F = void (*)()>
@@ -214,6 +290,11 @@ private:
     hipEvent_t dev_free_    = nullptr; // recorded after the device slot's gulp was cleaned
     bool dev_free_recorded_ = false;
     hipStream_t copy_stream_ = nullptr;
+    // Pipelined gulps.
+    hipStream_t measure_stream_ = nullptr, pending_stream_ = nullptr;
+    bool pending_ = false;
+    std::vector<uint8_t> result_mask_, inflight_mask_;
+    fa_rfi_report result_report_{}, inflight_report_{};
 };
 
 } // namespace fa

@@ -1648,6 +1648,7 @@ DeviceRfi::DeviceRfi(const fa_rfi_config& cfg, const fa_block_layout& layout, fl
         kneg_.resize(kMaxChunks * nch_);
     }
     kseg_.resize(4 * nch_);
+    unmeasured_dev_.resize(nch_);
     counters_.resize(8);
     moments_pin_.reserve(maxwin * nch_);
     series_pin_.reserve(2 * layout.max_nsamps);
@@ -1696,7 +1697,7 @@ uint64_t DeviceRfi::memory_bytes() const {
            clip_mean_.bytes() + clip_inv_.bytes() + mask_dev_.bytes() + flags_dev_.bytes() +
            kwhole_.bytes() + good_prefix_.bytes() + kspec_.bytes() + s_det_.bytes() +
            s_keep_.bytes() + row_std_.bytes() + cmin_.bytes() + cmax_.bytes() + kpos_.bytes() +
-           kneg_.bytes() + kseg_.bytes() + counters_.bytes() + slow_tiles_.bytes() + tail_loc_.bytes() +
+           kneg_.bytes() + kseg_.bytes() + unmeasured_dev_.bytes() + counters_.bytes() + slow_tiles_.bytes() + tail_loc_.bytes() +
            tail_scale_.bytes() + skeep64_.bytes() + zsums_.bytes() + zdot_a_.bytes() +
            zdot_b_.bytes() + loc2_.bytes() + scale2_.bytes() + tail_out_loc_.bytes() +
            tail_out_scale_.bytes() + rowbuf_.bytes() + flag_rows_.bytes() + inv2_.bytes() +
@@ -1718,13 +1719,20 @@ void DeviceRfi::calibration(float* loc, float* scale, float* inv) const {
 void DeviceRfi::execute_normalized(const uint8_t* block, uint64_t capacity, uint64_t nsamps,
                                    float* out, uint64_t stride, uint64_t offset,
                                    hipStream_t s) {
-    require(block != nullptr && out != nullptr, "block and out must not be null");
+    require(out != nullptr, "block and out must not be null");
+    require(stride >= offset + capacity, "output stride too small");
+    measure(block, capacity, nsamps, s);
+    apply(out, stride, offset, s);
+}
+
+void DeviceRfi::measure(const uint8_t* block, uint64_t capacity, uint64_t nsamps, hipStream_t s) {
+    require(block != nullptr, "block and out must not be null");
     require(s != nullptr, "an explicit non-null stream is required");
     require(capacity >= 1 && capacity <= layout_.max_nsamps, "RFICleaner: invalid normalized block size");
     if (nsamps == 0) nsamps = capacity;
     require(nsamps <= capacity, "RFICleaner: nsamps exceeds block");
-    require(stride >= offset + capacity, "output stride too small");
     DeviceGuard dg(device_);
+    pend_ = Pending{};
     if (layout_.dtype == FA_FLOAT32) {
         const unsigned long long none = ~0ULL;
         FA_HIP(hipMemcpyAsync(bad_index_.get(), &none, 8, hipMemcpyHostToDevice, s));
@@ -1738,7 +1746,24 @@ void DeviceRfi::execute_normalized(const uint8_t* block, uint64_t capacity, uint
                 "RFI: non-finite float32 input at sample " + std::to_string(bad / nch_) +
                 ", channel " + std::to_string(bad % nch_));
     }
-    const bool short_block = nsamps < std::max(kMinWindow, kMinBlockSamps);
+    pend_.block    = block;
+    pend_.capacity = capacity;
+    pend_.nsamps   = nsamps;
+    pend_.short_block = nsamps < std::max(kMinWindow, kMinBlockSamps);
+    pend_.ready    = true;
+    if (!pend_.short_block) full_block_measure(block, capacity, nsamps, s);
+}
+
+void DeviceRfi::apply(float* out, uint64_t stride, uint64_t offset, hipStream_t s) {
+    require(pend_.ready, "RFI: apply without a measured block");
+    require(out != nullptr, "block and out must not be null");
+    require(s != nullptr, "an explicit non-null stream is required");
+    require(stride >= offset + pend_.capacity, "output stride too small");
+    DeviceGuard dg(device_);
+    const uint8_t* block   = pend_.block;
+    const uint64_t capacity = pend_.capacity, nsamps = pend_.nsamps;
+    pend_.ready            = false;
+    const bool short_block = pend_.short_block;
     const int flip         = layout_.df < 0.0 ? 1 : 0;
     Geometry g{block, row_bytes_, nch_, nsamps, window_len_, 1, layout_.dtype};
     if (short_block) {
@@ -1802,7 +1827,7 @@ void DeviceRfi::execute_normalized(const uint8_t* block, uint64_t capacity, uint
         ++block_index_;
         return;
     }
-    full_block(block, capacity, nsamps, out, stride, offset, s);
+    full_block_apply(capacity, out, stride, offset, s);
     ++block_index_;
     tail_ready_ = true;
 }
@@ -1937,8 +1962,9 @@ void DeviceRfi::build_channel_mask(hipStream_t s) {
     FA_HIP(hipStreamSynchronize(s));
 }
 
-void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsamps, float* out,
-                           uint64_t stride, uint64_t offset, hipStream_t s) {
+void DeviceRfi::full_block_measure(const uint8_t* block, uint64_t capacity, uint64_t nsamps,
+                                   hipStream_t s) {
+    (void)capacity;
     const uint64_t n = nch_;
     nsamps_          = nsamps;
     nwin_            = std::max<uint64_t>(nsamps / window_len_, 1);
@@ -2055,7 +2081,15 @@ void DeviceRfi::full_block(const uint8_t* block, uint64_t capacity, uint64_t nsa
     FA_HIP(hipEventRecord(ev_[3], s));
     build_flags(s);
     FA_HIP(hipEventRecord(ev_[4], s));
+}
 
+void DeviceRfi::full_block_apply(uint64_t capacity, float* out, uint64_t stride, uint64_t offset,
+                                 hipStream_t s) {
+    const uint64_t n      = nch_;
+    const uint64_t nsamps = nsamps_;
+    Geometry g{pend_.block, row_bytes_, n, nsamps, window_len_, nwin_, layout_.dtype};
+    const unsigned cblk = ceil_div(n, 64);
+    const bool kspec = cfg_.kadane_spectrum_enabled, kchan = cfg_.kadane_channel_enabled;
     // Apply + normalise.
     zero_u64_kernel<<<1, 64, 0, s>>>(counters_.get(), 8);
     ApplyArgs ap{};
@@ -2161,13 +2195,14 @@ void DeviceRfi::build_flags(hipStream_t s) {
     const uint64_t n = nch_, nsamps = nsamps_;
     const uint64_t nchunks = chunk_count();
     const bool good_any    = std::count(mask_.begin(), mask_.end(), 1) > 0;
-    DeviceBuffer<uint64_t> unm(n);
-    FA_HIP(hipMemcpyAsync(unm.get(), unmeasured_.data(), n * 8, hipMemcpyHostToDevice, s));
+    // A member buffer: an allocation (and its free) here would wait on
+    // work of other streams, such as a pipelined gulp's FDMT.
+    FA_HIP(hipMemcpyAsync(unmeasured_dev_.get(), unmeasured_.data(), n * 8, hipMemcpyHostToDevice, s));
     zero_u64_kernel<<<1, 64, 0, s>>>(counters_.get() + 4, 1);
     if (good_any) {
         const double min_len = static_cast<double>(cfg_.kadane_channel_min_duration) / layout_.tsamp;
         combine_cols_kernel<<<ceil_div(n, 256), 256, 0, s>>>(
-            n, nchunks, nwin_, unm.get(), mask_dev_.get(), cmin_.get(), cmax_.get(),
+            n, nchunks, nwin_, unmeasured_dev_.get(), mask_dev_.get(), cmin_.get(), cmax_.get(),
             cfg_.kadane_channel_enabled ? 1 : 0, kpos_.get(), kneg_.get(), min_len,
             cfg_.kadane_channel_threshold, kseg_.get(), counters_.get() + 4);
         FA_HIP(hipGetLastError());

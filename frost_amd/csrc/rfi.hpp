@@ -64,8 +64,17 @@ public:
     DeviceRfi& operator=(const DeviceRfi&) = delete;
 
     // block: device, capacity rows; writes out[c * stride + offset + t].
+    // The same as measure() then apply() on one stream.
     void execute_normalized(const uint8_t* block, uint64_t capacity, uint64_t nsamps,
                             float* out, uint64_t stride, uint64_t offset, hipStream_t s);
+    // The block's measurements and decisions (bandpass, moments, masks,
+    // scans, flags; returns once they are on the host) ...
+    void measure(const uint8_t* block, uint64_t capacity, uint64_t nsamps, hipStream_t s);
+    // ... and its calibrated, filled output (the block must stay valid until
+    // this returns). A measured block must be applied before the next
+    // measure, and its apply must have returned before that measure starts
+    // (they share the calibration buffers).
+    void apply(float* out, uint64_t stride, uint64_t offset, hipStream_t s);
 
     const std::vector<uint8_t>& channel_mask() const { return mask_; }
     const fa_rfi_report& report() const { return report_; }
@@ -77,8 +86,16 @@ public:
     uint64_t memory_bytes() const;
 
 private:
-    void full_block(const uint8_t* block, uint64_t capacity, uint64_t nsamps, float* out,
-                    uint64_t stride, uint64_t offset, hipStream_t s);
+    void full_block_measure(const uint8_t* block, uint64_t capacity, uint64_t nsamps,
+                            hipStream_t s);
+    void full_block_apply(uint64_t capacity, float* out, uint64_t stride, uint64_t offset,
+                          hipStream_t s);
+    struct Pending {
+        const uint8_t* block = nullptr;
+        uint64_t capacity = 0, nsamps = 0;
+        bool short_block = false, ready = false;
+    };
+    Pending pend_; // the block measure() took, until apply()
     void build_channel_mask(hipStream_t s);
     void build_flags(hipStream_t s);
     void copy_series(hipStream_t s);
@@ -122,6 +139,7 @@ private:
     DeviceBuffer<float> cmin_, cmax_;
     DeviceBuffer<KadaneDev> kpos_, kneg_;
     DeviceBuffer<int64_t> kseg_; // [nch][4]: pos start/end, neg start/end
+    DeviceBuffer<uint64_t> unmeasured_dev_; // per channel: windows not measured
     DeviceBuffer<unsigned long long> counters_;
     DeviceBuffer<uint32_t> slow_tiles_; // (time tile, channel tile) pairs needing fills
     DeviceBuffer<float> tail_loc_, tail_scale_;

@@ -37,6 +37,33 @@ class Beam:
         check(lib().fa_beam_execute(self.h, g.ctypes.data_as(C.c_void_p), n, 0, stream.h))
         return self.result()
 
+    def execute_pipelined(self, gulp, stream: Stream, nsamps: int = 0, rows: bool = True):
+        """This gulp's RFI measurement beside the previous gulp's search,
+        then the previous gulp collected. gulp: host rows (ndarray) or a
+        DeviceArray (then nsamps is required). Returns the previous gulp's
+        result (rows as tuples unless rows is False, then True), or None for
+        the first gulp."""
+        done = C.c_int()
+        if isinstance(gulp, DeviceArray):
+            check(lib().fa_beam_execute_pipelined(self.h, gulp.ptr, nsamps, 1, stream.h,
+                                                  C.byref(done)))
+        else:
+            g = np.ascontiguousarray(gulp).view(np.uint8).reshape(-1)
+            n = nsamps or g.size // self.row_bytes
+            check(lib().fa_beam_execute_pipelined(self.h, g.ctypes.data_as(C.c_void_p), n, 0,
+                                                  stream.h, C.byref(done)))
+        if not done.value:
+            return None
+        return self.result() if rows else True
+
+    def flush(self, stream: Stream, rows: bool = True):
+        """Collects the pending pipelined gulp: its result, or None."""
+        done = C.c_int()
+        check(lib().fa_beam_flush(self.h, stream.h, C.byref(done)))
+        if not done.value:
+            return None
+        return self.result() if rows else True
+
     def ingest_slot(self) -> np.ndarray:
         """The next pinned host slot (uint8, capacity x row bytes) to fill
         with a gulp's rows before ingest_submit()."""

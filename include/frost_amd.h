@@ -491,6 +491,20 @@ int fa_beam_ingest_slot(fa_beam* beam, void** host_slot, uint64_t* bytes);
 int fa_beam_ingest_submit(fa_beam* beam, uint64_t nsamps);
 int fa_beam_execute_next(fa_beam* beam, void* stream);
 
+/* Pipelined gulps (no reference counterpart: frost's driver runs the
+ * stages of a gulp in turn, lib/pipeline/pipeline.cpp:529-619). This gulp's
+ * RFI measurements run on a stream of the beam's own beside the previous
+ * gulp's FDMT and search; then the previous gulp is collected and this gulp
+ * is calibrated and its search enqueued on `stream`. *collected is 1 when a
+ * previous gulp was collected: fa_beam_result, fa_beam_channel_mask and
+ * fa_beam_rfi_report then describe that gulp, with the same rows, mask and
+ * report fa_beam_execute gives for it. A device gulp must stay valid until
+ * the call returns. fa_beam_flush collects the last gulp; fa_beam_execute,
+ * fa_beam_execute_next and fa_beam_drain are refused while one is pending. */
+int fa_beam_execute_pipelined(fa_beam* beam, const void* gulp, uint64_t nsamps,
+                              int is_device, void* stream, int* collected);
+int fa_beam_flush(fa_beam* beam, void* stream, int* collected);
+
 /* Results of the last fa_beam_execute / fa_beam_drain. */
 int fa_beam_result(fa_beam* beam, fa_raw_candidate* out, uint64_t cap,
                    uint64_t* n, uint64_t* dropped_global,

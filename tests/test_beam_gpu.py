@@ -114,3 +114,53 @@ def test_ingest_overlap_matches_direct_execute():
     with pytest.raises(abi.InvalidArgument, match="no submitted gulp"):
         fed.execute_next(stream)
     assert fed.drain(stream) == direct.drain(stream)
+
+
+def test_pipelined_gulps_match_sequential():
+    """Pipelined gulps (RFI measurement of gulp k + 1 beside gulp k's FDMT,
+    results one call late): rows, channel masks, reports and frontier equal
+    those of execute() on the same gulps, host and device gulps mixed, with a
+    partial last gulp; execute and drain are refused while a gulp is pending."""
+    say("test_pipelined_gulps_match_sequential")
+    nch, cap = 128, 16384
+    tsamp, f0, df = 256e-6, 1500.0, -300.0 / 128
+    rcfg = abi.rfi_config(moments_enabled=1, iqrm_enabled=1, kadane_channel_enabled=1)
+    lay = abi.BlockLayout(nch, cap, abi.UINT8, tsamp, f0, df)
+    dcfg = abi.dedisp_config(nch, tsamp, f0, df, 500.0, block_nsamps=cap, subband_depth=2)
+    bcfg = abi.boxcar_config(max_boxcar=64)
+    stream = search.Stream(0)
+    seq = pipeline.Beam(rcfg, lay, 0.5, dcfg, bcfg)
+    pip = pipeline.Beam(rcfg, lay, 0.5, dcfg, bcfg)
+    sizes = [cap, cap, cap, cap, 9000]
+    gulps = [synth.filterbank(cap, nch, seed=70 + k, tsamp=tsamp, f0=f0, df=df,
+                              pulses=[(120.0 + 80 * k, 2000 + 2500 * k, 4, 1.5)],
+                              rfi_chans=(7 + 20 * k,), impulses=(500 + 3000 * k,))
+             for k in range(len(sizes))]
+    want = []
+    for k, g in enumerate(gulps):
+        want.append((seq.execute(g, stream, sizes[k]), seq.channel_mask().copy(),
+                     seq.rfi_report().counts()))
+    dev = {k: search.DeviceArray.from_host(np.ascontiguousarray(gulps[k]), stream) for k in (1, 3)}
+    stream.synchronize()
+    got = []
+    for k, g in enumerate(gulps):
+        say(f"  gulp {k} nsamps={sizes[k]} {'device' if k in dev else 'host'}")
+        r = pip.execute_pipelined(dev[k] if k in dev else g, stream, sizes[k])
+        if k == 0:
+            assert r is None
+            with pytest.raises(abi.InvalidArgument, match="pending"):
+                pip.execute(g, stream, sizes[k])
+            with pytest.raises(abi.InvalidArgument, match="pending"):
+                pip.drain(stream)
+        else:
+            got.append((r, pip.channel_mask().copy(), pip.rfi_report().counts()))
+    got.append((pip.flush(stream), pip.channel_mask().copy(), pip.rfi_report().counts()))
+    assert pip.flush(stream) is None
+    for k, (a, b) in enumerate(zip(got, want)):
+        assert a[0] == b[0], k
+        assert (a[1] == b[1]).all(), k
+        assert a[2] == b[2], k
+    assert sum(len(w[0][0]) for w in want) > 0
+    assert pip.drain(stream) == seq.drain(stream)
+    # sequential calls resume after a flush
+    assert pip.execute(gulps[0], stream) == seq.execute(gulps[0], stream)
