@@ -703,6 +703,7 @@ struct SearchArgs {
     int nrows;
     const float* inv_cur;
     const float* inv_or;
+    const float* floor_cur; // smallest window sum whose S/N exceeds thr, per (row, width)
     const uint32_t* widths;
     int nw;
     int kmax;
@@ -723,46 +724,212 @@ struct SearchArgs {
     uint64_t keep_stride;
 };
 
-__device__ void emit(const SearchArgs& a, uint32_t prow, uint32_t w, uint64_t start,
-                     uint64_t peak, uint64_t end, float snr) {
-    const unsigned long long idx = atomicAdd(a.counter, 1ULL);
-    if (idx < a.cap + a.spill_cap) {
-        CandRecord r;
-        r.peak  = peak;
-        r.begin = start;
-        r.end   = end;
-        r.snr   = snr;
-        r.prow  = prow;
-        r.w     = w;
-        r.pad   = 0;
-        if (idx < a.cap) a.cands[idx] = r;
-        else a.spill[idx - a.cap] = r;
+// ---- parallel run tracking and candidate compaction -------------------------
+//
+// One wave, one chunk of up to 256 samples, lane l holding chunk samples
+// 4l .. 4l + 3 (those at or past clen take no part). A run is a maximal
+// stretch of above-threshold samples, reported at the first sample below
+// the threshold after it, with its first maximum as the peak. The chunk's
+// runs come from one segmented scan over the samples: a sample below the
+// threshold, or one above it right after a sample below, starts a segment;
+// a segment carries its largest S/N (the first of equal ones), the sample
+// and width index of that maximum, and its first sample. The run open
+// before the chunk is the element ahead of sample 0. Each lane closes at
+// most two runs in its four samples; the wave counts them with ballots,
+// takes one block of record slots with one atomic add and places each
+// lane's records at its mbcnt offset. The run still open at the chunk's
+// last sample is the carried state afterwards.
+
+struct ScanElem {
+    float v;    // largest S/N of the segment so far
+    uint32_t p; // (arg + 1) | (first + 1) << 9 | width << 18 | head << 24;
+                // arg / first = -1: the carried run's peak / start
+};
+constexpr uint32_t kScanArg   = 0x1FFu;
+constexpr uint32_t kScanFirst = 0x1FFu << 9;
+constexpr uint32_t kScanW     = 0x3Fu << 18;
+constexpr uint32_t kScanHead  = 1u << 24;
+
+__device__ __forceinline__ ScanElem scan_comb(ScanElem L, ScanElem R) {
+    // R follows L: a head in R starts a new segment; otherwise R's maximum
+    // replaces L's only when strictly larger (the first maximum is kept).
+    ScanElem o = L;
+    if (R.p & kScanHead) {
+        o = R;
+    } else if (R.v > L.v) {
+        o.v = R.v;
+        o.p = (L.p & (kScanFirst | kScanHead)) | (R.p & (kScanArg | kScanW));
     }
+    return o;
 }
 
-// Run tracking of one sample (per_width and across_widths peaks): an
-// above-threshold sample opens or extends the run (the first maximum is
-// its peak, w the width index reported with it); a sample below closes it.
-__device__ __forceinline__ void track_run(const SearchArgs& a, OpenRun& o, uint32_t prow,
-                                          uint32_t w, uint64_t abs_t, bool above, float snr) {
-    if (above) {
-        if (!o.open) {
-            o.open  = 1;
-            o.w     = static_cast<uint16_t>(w);
-            o.start = o.peak = o.end = abs_t;
-            o.snr   = snr;
-        } else {
-            if (snr > o.snr) {
-                o.snr  = snr;
-                o.peak = abs_t;
-                o.w    = static_cast<uint16_t>(w);
+__device__ __forceinline__ ScanElem scan_shfl_up(ScanElem e, int d) {
+    return ScanElem{__shfl_up(e.v, d), static_cast<uint32_t>(__shfl_up(static_cast<int>(e.p), d))};
+}
+
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+}
+
+// Where a wave's candidate records go: the device buffer, then (carried
+// state) the pinned host spill; the counter counts every record.
+struct CandSink {
+    CandRecord* cands;
+    CandRecord* spill;
+    unsigned long long* counter;
+    uint64_t cap, spill_cap;
+};
+
+__device__ __forceinline__ void put_record(const CandSink& k, unsigned long long idx, const CandRecord& r) {
+    if (idx < k.cap) k.cands[idx] = r;
+    else if (idx < k.cap + k.spill_cap) k.spill[idx - k.cap] = r;
+}
+
+// First record slot of this wave's n records (n uniform): one atomic add by
+// lane 0, broadcast to the wave.
+__device__ __forceinline__ unsigned long long wave_slots(const CandSink& k, uint32_t n, int lane) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(k.counter, static_cast<unsigned long long>(n));
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base >> 32));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Runs of one width (or of the per-sample best width, across_widths) over
+// one chunk: sn the S/N per sample, ab whether it is above the threshold
+// (false for cold samples and those at or past clen), aw the width index
+// reported with it. abs0 is the absolute sample of chunk sample 0. all:
+// every above-threshold sample is a record of its own and no run stays open.
+// Returns whether a run is open after the chunk (uniform).
+__device__ __forceinline__ uint32_t run_chunk(const CandSink& a, OpenRun* st, uint32_t prow, const float (&sn)[4],
+                              const bool (&ab)[4], const uint32_t (&aw)[4], bool all, int clen,
+                              uint64_t abs0, int lane) {
+    const uint64_t b0 = __ballot(ab[0]), b1 = __ballot(ab[1]), b2 = __ballot(ab[2]),
+                   b3 = __ballot(ab[3]);
+    if (all) {
+        const int cnt     = (int)ab[0] + (int)ab[1] + (int)ab[2] + (int)ab[3];
+        const uint64_t c1 = __ballot(cnt >= 1), c2 = __ballot(cnt >= 2), c3 = __ballot(cnt >= 3),
+                       c4 = __ballot(cnt >= 4);
+        const uint32_t tot = __popcll(c1) + __popcll(c2) + __popcll(c3) + __popcll(c4);
+        const unsigned long long base = wave_slots(a, tot, lane);
+        unsigned long long idx = base + lanes_below(c1) + lanes_below(c2) + lanes_below(c3) +
+                                 lanes_below(c4);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (ab[j]) {
+                const uint64_t t = abs0 + (uint64_t)(4 * lane + j);
+                put_record(a, idx++, CandRecord{t, t, t, sn[j], prow, aw[j], 0});
             }
-            o.end = abs_t;
         }
-    } else if (o.open) {
-        emit(a, prow, o.w, o.start, o.peak, o.end, o.snr);
-        o.open = 0;
+        return 0;
     }
+    const OpenRun o  = *st;
+    const bool prev0 = lane == 0 ? (o.open != 0) : ((b3 >> (lane - 1)) & 1ULL) != 0;
+    const bool prev[4] = {prev0, ab[0], ab[1], ab[2]};
+    ScanElem e[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t idx = (uint32_t)(4 * lane + j) + 1U;
+        if (4 * lane + j >= clen) e[j] = ScanElem{-INFINITY, 0U};                // takes no part
+        else if (!ab[j]) e[j] = ScanElem{-INFINITY, kScanHead};                 // below: a break
+        else if (!prev[j]) e[j] = ScanElem{sn[j], idx | (idx << 9) | (aw[j] << 18) | kScanHead};
+        else e[j] = ScanElem{sn[j], idx | (aw[j] << 18)};
+    }
+    // Inclusive scan of the lane aggregates, then the prefix ahead of each lane.
+    ScanElem A = scan_comb(scan_comb(e[0], e[1]), scan_comb(e[2], e[3]));
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const ScanElem B = scan_shfl_up(A, d);
+        if (lane >= d) A = scan_comb(B, A);
+    }
+    ScanElem X = scan_shfl_up(A, 1);
+    if (lane == 0) X = ScanElem{-INFINITY, 0U};
+    const ScanElem C = o.open ? ScanElem{o.snr, ((uint32_t)o.w << 18) | kScanHead}
+                              : ScanElem{-INFINITY, kScanHead};
+    ScanElem cur = scan_comb(C, X);
+    // Runs closing at this lane's samples: the state just before the close.
+    ScanElem r[2] = {cur, cur};
+    int rc[2]     = {0, 0};
+    int cnt       = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int idx = 4 * lane + j;
+        if (idx < clen && !ab[j] && prev[j]) {
+            if (cnt == 0) {
+                r[0]  = cur;
+                rc[0] = idx;
+            } else {
+                r[1]  = cur;
+                rc[1] = idx;
+            }
+            ++cnt;
+        }
+        cur = scan_comb(cur, e[j]);
+    }
+    auto record = [&](const ScanElem& s, int64_t last) {
+        // last: chunk index of the run's last sample (-1: the previous chunk's).
+        const int arg   = (int)(s.p & kScanArg) - 1;
+        const int first = (int)((s.p & kScanFirst) >> 9) - 1;
+        CandRecord rec;
+        rec.begin = first < 0 ? o.start : abs0 + (uint64_t)first;
+        rec.peak  = arg < 0 ? o.peak : abs0 + (uint64_t)arg;
+        rec.end   = last < 0 ? o.end : abs0 + (uint64_t)last;
+        rec.snr   = s.v;
+        rec.prow  = prow;
+        rec.w     = (s.p & kScanW) >> 18;
+        rec.pad   = 0;
+        return rec;
+    };
+    const uint64_t c1 = __ballot(cnt >= 1), c2 = __ballot(cnt >= 2);
+    const uint32_t tot = __popcll(c1) + __popcll(c2);
+    if (tot != 0) {
+        const unsigned long long base = wave_slots(a, tot, lane);
+        const unsigned long long idx  = base + lanes_below(c1) + lanes_below(c2);
+        if (cnt >= 1) put_record(a, idx, record(r[0], rc[0] - 1));
+        if (cnt >= 2) put_record(a, idx + 1, record(r[1], rc[1] - 1));
+    }
+    // The state after the chunk: lane 63's (samples past clen take no part).
+    const ScanElem fin{__shfl(cur.v, 63), static_cast<uint32_t>(__shfl(static_cast<int>(cur.p), 63))};
+    const int last       = clen - 1;
+    const uint64_t blast = (last & 3) == 0 ? b0 : (last & 3) == 1 ? b1 : (last & 3) == 2 ? b2 : b3;
+    const uint32_t open  = (uint32_t)((blast >> (last >> 2)) & 1ULL);
+    wave_sync();
+    if (lane == 0) {
+        OpenRun n = o;
+        n.open    = static_cast<uint16_t>(open);
+        if (open) {
+            const CandRecord rec = record(fin, last);
+            n.start              = rec.begin;
+            n.peak               = rec.peak;
+            n.end                = rec.end;
+            n.snr                = rec.snr;
+            n.w                  = static_cast<uint16_t>(rec.w);
+        }
+        *st = n;
+    }
+    wave_sync();
+    return open;
+}
+
+// run_chunk on S/N values staged in LDS, four consecutive ones per lane,
+// with the width index per sample (across_widths) or one width for all.
+__device__ __forceinline__ uint32_t run_chunk_staged(const CandSink& k, float thr, OpenRun* st,
+                                                     uint32_t prow, const float* s_sn,
+                                                     const uint8_t* s_aw, uint32_t wfix, bool all,
+                                                     int clen, uint64_t abs0) {
+    const int lane = threadIdx.x & 63;
+    float sn[4];
+    bool ab[4];
+    uint32_t aw[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int i = 4 * lane + j;
+        sn[j]       = s_sn[i];
+        ab[j]       = i < clen && sn[j] > thr;
+        aw[j]       = s_aw ? s_aw[i] : wfix;
+    }
+    return run_chunk(k, st, prow, sn, ab, aw, all, clen, abs0, lane);
 }
 
 // A searched plane row: sample t0 + t of the tile, t in [-(W_max - 1), tile).
@@ -858,7 +1025,7 @@ __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs 
     __shared__ float s_best[kRingChunk];
     __shared__ uint8_t s_arg[kRingChunk];
     __shared__ int s_off[kRingLevels], s_size[kRingLevels], s_base[kRingLevels];
-    __shared__ uint32_t flags;
+    __shared__ uint32_t flags, s_open; // widths with a sample above the threshold / a run open
     const int i         = threadIdx.x;
     const SearchRow row = a.rows[blockIdx.x];
     const uint32_t prow = row.prow;
@@ -877,10 +1044,16 @@ __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs 
         s_size[i] = ra.size[i];
         s_base[i] = 0;
     }
-    if (i == 0) flags = 0;
+    if (i == 0) {
+        flags  = 0;
+        s_open = 0;
+    }
+    __syncthreads();
+    if (i < nw && st[i].open) atomicOr(&s_open, 1U << i);
     for (int j = i; j < ra.lds_floats; j += kRingChunk) lds[j] = 0.0f;
     __syncthreads();
     const Ring R{lds, s_off, s_size, s_base};
+    const CandSink sink{a.cands, a.spill, a.counter, a.cap, a.spill_cap};
     const RowRead<RING> src(a, row, gabs);
     const int tl     = a.tile_len;
     for (int u = ra.u0; u < tl; u += kRingChunk) {
@@ -933,39 +1106,32 @@ __global__ void __launch_bounds__(kRingChunk) search_ring_kernel(RingSearchArgs 
                 if (bits) atomicOr(&flags, bits);
             }
             __syncthreads();
+            // Run tracking on wave 0 (a parallel scan over the chunk, four
+            // samples per lane), one width at a time: those with a sample
+            // above the threshold or a run open.
+            const uint64_t abs0 = gabs + (uint64_t)(a.tile_t0 + u);
+            uint32_t need       = flags | s_open;
             if (across) {
-                if (i == 0 && (flags || st[0].open)) {
-                    OpenRun o = st[0];
-                    for (int q = 0; q < clen; ++q) {
-                        const uint64_t abs_t = gabs + (uint64_t)(a.tile_t0 + u + q);
-                        const float sn       = s_best[q];
-                        track_run(a, o, prow, s_arg[q], abs_t, sn > a.thr, sn);
-                    }
-                    st[0] = o;
+                if (need && i < 64) {
+                    const uint32_t op = run_chunk_staged(sink, a.thr, &st[0], prow, s_best, s_arg, 0,
+                                                         false, clen, abs0);
+                    if (i == 0) s_open = op;
                 }
-            } else if (i < nw && (((flags >> i) & 1U) || st[i].open)) {
-                // Thread w walks width w over the chunk in time order.
-                const int w      = i;
-                const uint32_t W = s_w[w];
-                OpenRun o        = st[w];
-                for (int q = 0; q < clen; ++q) {
-                    const int64_t tq  = a.tile_t0 + u + q;
-                    const int64_t abq = (int64_t)gabs + tq;
-                    const uint64_t abs_t = (uint64_t)abq;
-                    bool above       = false;
-                    float sn         = 0.0f;
-                    if (abq >= s_warm[w]) {
-                        const float inv = tq < s_warm[w] ? s_invor[w] : s_inv[w];
-                        sn              = R.value(W, q, a.trap) * inv;
-                        above           = sn > a.thr;
+            } else {
+                while (need) {
+                    const int w = __builtin_ctz(need);
+                    need &= need - 1U;
+                    float sn;
+                    s_best[i] = snr_of(w, sn) ? sn : -INFINITY;
+                    __syncthreads();
+                    if (i < 64) {
+                        const uint32_t op = run_chunk_staged(sink, a.thr, &st[w], prow, s_best, nullptr,
+                                                             (uint32_t)w, a.mode == FA_PEAK_ALL, clen,
+                                                             abs0);
+                        if (i == 0) s_open = op ? (s_open | (1U << w)) : (s_open & ~(1U << w));
                     }
-                    if (a.mode == FA_PEAK_ALL) {
-                        if (above) emit(a, prow, w, abs_t, abs_t, abs_t, sn);
-                        continue;
-                    }
-                    track_run(a, o, prow, w, abs_t, above, sn);
+                    __syncthreads();
                 }
-                st[w] = o;
             }
             __syncthreads();
             if (i == 0) flags = 0;
@@ -1048,36 +1214,15 @@ __device__ __forceinline__ float trap_back(const float (&D)[9][4], const float (
 }
 
 
-// Serial run tracking of one width over one chunk (lane 0), the S/N values
-// staged in LDS. Returns the run's open flag afterwards.
-__device__ uint32_t walk_chunk(const SearchArgs& a, uint64_t gabs, const float* snr_lds,
-                               const uint8_t* arg_lds, OpenRun* st, uint32_t prow, int w,
-                               int64_t tc0, int clen, int lane) {
-    // arg_lds: the width index per sample (across_widths), else null.
-    if (lane == 0) {
-        OpenRun o = *st;
-        for (int i = 0; i < clen; ++i) {
-            const float sn       = snr_lds[i];
-            const uint64_t abs_t = gabs + (uint64_t)(tc0 + i);
-            const bool above     = sn > a.thr;
-            if (a.mode == FA_PEAK_ALL) {
-                if (above) emit(a, prow, w, abs_t, abs_t, abs_t, sn);
-                continue;
-            }
-            track_run(a, o, prow, arg_lds ? arg_lds[i] : w, abs_t, above, sn);
-        }
-        *st = o;
-    }
-    wave_sync();
-    return __builtin_amdgcn_readfirstlane(st->open);
-}
 
 template <int KMAX, bool TRAP, bool RING, bool ACROSS>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRAP ? 1 : 6))) search_reg_kernel(RegSearchArgs ra) {
     const SearchArgs& a = ra.a;
-    __shared__ float s_snr[kBlock / 64][kRegChunk];
-    __shared__ uint8_t s_arg[kBlock / 64][kRegChunk];
     __shared__ OpenRun s_st[kBlock / 64][16];
+    __shared__ float s_inv[kBlock / 64][2][KMAX + 1]; // [current, OR-mask][ladder level]
+    // A chunk's S/N values per ladder level, staged for run tracking.
+    __shared__ float s_sn[kBlock / 64][ACROSS ? 1 : KMAX + 1][kRegChunk];
+    __shared__ uint8_t s_aw[kBlock / 64][kRegChunk];  // across_widths: the width index per sample
     const int lane = threadIdx.x % 64;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int rowi = blockIdx.x * (kBlock / 64) + wave;
@@ -1086,12 +1231,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
     const uint32_t prow = row.prow;
     const int nw        = a.nw;
     const uint64_t gabs = *a.gulp_abs_p;
-    float inv_c[KMAX + 1], inv_o[KMAX + 1];
+    // Quick-test floors per ladder level (+inf: width not searched); the
+    // noise scales, needed only where a test passes, wait in LDS.
+    float flo[KMAX + 1];
 #pragma unroll
     for (int k = 0; k <= KMAX; ++k) {
         const int w = ra.widx[k];
-        inv_c[k]    = w >= 0 ? a.inv_cur[(uint64_t)prow * nw + w] : 0.0f;
-        inv_o[k]    = w >= 0 ? a.inv_or[(uint64_t)prow * nw + w] : 0.0f;
+        flo[k]      = w >= 0 ? a.floor_cur[(uint64_t)prow * nw + w] : INFINITY;
+        if (lane == 0) {
+            s_inv[wave][0][k] = w >= 0 ? a.inv_cur[(uint64_t)prow * nw + w] : 0.0f;
+            s_inv[wave][1][k] = w >= 0 ? a.inv_or[(uint64_t)prow * nw + w] : 0.0f;
+        }
     }
     uint32_t open_mask = 0; // bit w: width index w has an open run
     {
@@ -1108,21 +1258,25 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
     const int64_t warm_max = (int64_t)row.delay + (1 << KMAX) - 1;
     const RowRead<RING> src(a, row, gabs); // sample t0 + t
     const int tl           = a.tile_len;
-    float D[9][4], P[9][4], nx[4];
+    const CandSink sink{a.cands, a.spill, a.counter, a.cap, a.spill_cap};
+    // Ladder levels of the current chunk (L0) and of the previous one (L1).
+    float L0[9][4], L1[9][4], nx[4];
 #pragma unroll
     for (int k = 0; k < 9; ++k)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) P[k][j] = D[k][j] = 0.0f;
+        for (int j = 0; j < 4; ++j) L0[k][j] = L1[k][j] = 0.0f;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int i = -kRegChunk + 4 * lane + j;
         nx[j]       = (i >= -a.wm1 && i < tl) ? src.at(i) : 0.0f;
     }
-    for (int u = -kRegChunk; u < tl; u += kRegChunk) {
+    // One chunk starting at tile sample u: its levels D from the samples in
+    // nx (the next chunk's are requested meanwhile) and the previous chunk's
+    // levels P; then the threshold tests and run tracking.
+    auto chunk = [&](float (&D)[9][4], const float (&P)[9][4], int u) {
         const int base = u + 4 * lane;
 #pragma unroll
         for (int j = 0; j < 4; ++j) D[0][j] = nx[j];
-        // Prefetch the next chunk.
         if (u + 2 * kRegChunk <= tl) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) nx[j] = src.at(base + kRegChunk + j);
@@ -1141,7 +1295,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
         }
         const int64_t tc0 = a.tile_t0 + u;
         const bool fast   = tc0 >= warm_max && u + kRegChunk <= tl;
-        // The ladder levels of this chunk.
 #pragma unroll
         for (int k = 1; k <= KMAX; ++k) {
 #pragma unroll
@@ -1160,44 +1313,47 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
                 D[k][j] = D[k - 1][j] + bj;
             }
         }
-        // One test for the whole chunk first: with no run open and no S/N
-        // above the threshold at any searched width, nothing below can emit
-        // or change state (the per-width tests are the same comparisons).
+        // Window sum of width 2^k ending at sample j (the trapezoid takes off
+        // half of each edge of (2^k + 1) / 4 samples).
+        auto wsum = [&](int k, int j) -> float {
+            float sum = D[k][j];
+            if (TRAP && k >= 2) {
+                float e2;
+                switch (k) {
+                case 2: e2 = trap_back<2>(D, P, j, lane); break;
+                case 3: e2 = trap_back<3>(D, P, j, lane); break;
+                case 4: e2 = trap_back<4>(D, P, j, lane); break;
+                case 5: e2 = trap_back<5>(D, P, j, lane); break;
+                case 6: e2 = trap_back<6>(D, P, j, lane); break;
+                case 7: e2 = trap_back<7>(D, P, j, lane); break;
+                default: e2 = trap_back<8>(D, P, j, lane); break;
+                }
+                sum = (sum - 0.5f * D[k - 2][j]) - 0.5f * e2;
+            }
+            return sum;
+        };
+        // One test for the whole chunk first: with no run open and no window
+        // sum at its width's floor (the smallest sum whose S/N exceeds the
+        // threshold), nothing below can emit or change state.
         bool quiet = u < 0; // the warm-up chunk before the tile is never searched
         if (!quiet && fast && open_mask == 0) {
-            bool anyp = false;
+            bool hit = false;
 #pragma unroll
             for (int k = 0; k <= KMAX; ++k) {
-                if (ra.widx[k] < 0) continue;
-                float m = -INFINITY;
+                if (!((ra.present >> k) & 1U)) continue;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    float sum = D[k][j];
-                    if (TRAP && k >= 2) {
-                        float e2;
-                        switch (k) {
-                        case 2: e2 = trap_back<2>(D, P, j, lane); break;
-                        case 3: e2 = trap_back<3>(D, P, j, lane); break;
-                        case 4: e2 = trap_back<4>(D, P, j, lane); break;
-                        case 5: e2 = trap_back<5>(D, P, j, lane); break;
-                        case 6: e2 = trap_back<6>(D, P, j, lane); break;
-                        case 7: e2 = trap_back<7>(D, P, j, lane); break;
-                        default: e2 = trap_back<8>(D, P, j, lane); break;
-                        }
-                        sum = (sum - 0.5f * D[k - 2][j]) - 0.5f * e2;
-                    }
-                    m = j == 0 ? sum : fmaxf(m, sum);
-                }
-                anyp = anyp || m * inv_c[k] > a.thr;
+                for (int j = 0; j < 4; ++j) hit = hit | (wsum(k, j) >= flo[k]);
             }
-            quiet = __ballot(anyp) == 0;
+            quiet = __ballot(hit) == 0;
         }
+        const int clen      = min(kRegChunk, tl - u);
+        const uint64_t abs0 = gabs + (uint64_t)tc0;
         if constexpr (ACROSS) {
             // One run per row: per sample the largest S/N over the widths
             // searched at it (the lowest width index on ties).
             if (u >= 0 && (!quiet || open_mask != 0)) {
                 float best[4];
-                int arg[4];
+                uint32_t arg[4];
                 bool any[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -1207,31 +1363,20 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
                 }
 #pragma unroll
                 for (int k = 0; k <= KMAX; ++k) {
-                    const int w = ra.widx[k];
-                    if (w < 0) continue;
+                    if (!((ra.present >> k) & 1U)) continue;
+                    const int w        = ra.widx[k];
                     const int64_t warm = (int64_t)row.delay + (int64_t)(1 << k) - 1;
+                    const float invc   = s_inv[wave][0][k];
+                    const float invo   = s_inv[wave][1][k];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        float sum = D[k][j];
-                        if (TRAP && k >= 2) {
-                            float e2;
-                            switch (k) {
-                            case 2: e2 = trap_back<2>(D, P, j, lane); break;
-                            case 3: e2 = trap_back<3>(D, P, j, lane); break;
-                            case 4: e2 = trap_back<4>(D, P, j, lane); break;
-                            case 5: e2 = trap_back<5>(D, P, j, lane); break;
-                            case 6: e2 = trap_back<6>(D, P, j, lane); break;
-                            case 7: e2 = trap_back<7>(D, P, j, lane); break;
-                            default: e2 = trap_back<8>(D, P, j, lane); break;
-                            }
-                            sum = (sum - 0.5f * D[k - 2][j]) - 0.5f * e2;
-                        }
+                        const float sum = wsum(k, j);
                         const int64_t t = tc0 + 4 * lane + j;
                         if (base + j >= tl || (int64_t)gabs + t < warm) continue;
-                        const float sn = sum * (t < warm ? inv_o[k] : inv_c[k]);
+                        const float sn = sum * (t < warm ? invo : invc);
                         if (!any[j] || sn > best[j]) {
                             best[j] = sn;
-                            arg[j]  = w;
+                            arg[j]  = (uint32_t)w;
                         }
                         any[j] = true;
                     }
@@ -1239,87 +1384,90 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TRA
                 bool pred = false;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const float sn             = any[j] ? best[j] : -INFINITY;
-                    pred                       = pred || sn > a.thr;
-                    s_snr[wave][4 * lane + j] = sn;
-                    s_arg[wave][4 * lane + j] = static_cast<uint8_t>(arg[j]);
+                    const float sn = any[j] ? best[j] : -INFINITY;
+                    pred           = pred | (sn > a.thr);
+                    s_sn[wave][0][4 * lane + j] = sn;
+                    s_aw[wave][4 * lane + j]    = static_cast<uint8_t>(arg[j]);
                 }
                 if (__ballot(pred) != 0 || open_mask != 0) {
                     wave_sync();
-                    const int clen = min(kRegChunk, tl - u);
-                    const uint32_t op =
-                        walk_chunk(a, gabs, s_snr[wave], s_arg[wave], &s_st[wave][0], prow, 0, tc0, clen,
-                                   lane);
-                    open_mask = op ? 1U : 0U;
+                    open_mask = run_chunk_staged(sink, a.thr, &s_st[wave][0], prow, s_sn[wave][0],
+                                                 s_aw[wave], 0, false, clen, abs0)
+                                    ? 1U : 0U;
+                    wave_sync();
                 }
             }
-        }
+        } else if (!quiet) {
+            uint32_t need = 0; // ladder levels whose S/N values wait in LDS
 #pragma unroll
-        for (int k = 0; k <= KMAX; ++k) {
-            const int w = ra.widx[k];
-            if (ACROSS || quiet || w < 0) continue;
-            const uint32_t W = 1U << k;
-            float sv[4], sums[4];
+            for (int k = 0; k <= KMAX; ++k) {
+                if (!((ra.present >> k) & 1U)) continue;
+                const int w      = ra.widx[k];
+                const uint32_t W = 1U << k;
+                float sums[4], sv[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                float sum = D[k][j];
-                if (TRAP && k >= 2) {
-                    float e2;
-                    switch (k) {
-                    case 2: e2 = trap_back<2>(D, P, j, lane); break;
-                    case 3: e2 = trap_back<3>(D, P, j, lane); break;
-                    case 4: e2 = trap_back<4>(D, P, j, lane); break;
-                    case 5: e2 = trap_back<5>(D, P, j, lane); break;
-                    case 6: e2 = trap_back<6>(D, P, j, lane); break;
-                    case 7: e2 = trap_back<7>(D, P, j, lane); break;
-                    default: e2 = trap_back<8>(D, P, j, lane); break;
+                for (int j = 0; j < 4; ++j) sums[j] = wsum(k, j);
+                bool pred = false;
+                if (fast) {
+                    // The floor test first; the S/N values are formed only
+                    // where it passes or a run is open.
+                    bool pre = false;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) pre = pre | (sums[j] >= flo[k]);
+                    if (__ballot(pre) == 0 && !((open_mask >> w) & 1U)) continue;
+                    const float inv = s_inv[wave][0][k];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        sv[j] = sums[j] * inv;
+                        pred  = pred | (sv[j] > a.thr);
                     }
-                    sum = (sum - 0.5f * D[k - 2][j]) - 0.5f * e2;
-                }
-                sums[j] = sum;
-            }
-            bool pred = false;
-            if (fast) {
-                // Multiplying by inv > 0 is monotone under rounding, so the
-                // largest of the four sums decides whether any S/N exceeds
-                // the threshold; the S/N values are formed only then.
-                const float m = fmaxf(fmaxf(sums[0], sums[1]), fmaxf(sums[2], sums[3]));
-                pred          = m * inv_c[k] > a.thr;
-                if (__ballot(pred) == 0 && !((open_mask >> w) & 1U)) continue;
+                } else {
+                    const float invc = s_inv[wave][0][k];
+                    const float invo = s_inv[wave][1][k];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) sv[j] = sums[j] * inv_c[k];
-            } else {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int64_t t = tc0 + 4 * lane + j;
-                    float sn        = -INFINITY;
-                    if (base + j < tl) {
-                        if (t < (int64_t)row.delay + (int64_t)W - 1) {
-                            // Cold until the absolute sample passes the warm-up.
-                            if ((int64_t)gabs + t >= (int64_t)row.delay + (int64_t)W - 1)
-                                sn = sums[j] * inv_o[k];
-                        } else {
-                            sn = sums[j] * inv_c[k];
+                    for (int j = 0; j < 4; ++j) {
+                        const int64_t t = tc0 + 4 * lane + j;
+                        float sn        = -INFINITY;
+                        if (base + j < tl) {
+                            if (t < (int64_t)row.delay + (int64_t)W - 1) {
+                                // Cold until the absolute sample passes the warm-up.
+                                if ((int64_t)gabs + t >= (int64_t)row.delay + (int64_t)W - 1)
+                                    sn = sums[j] * invo;
+                            } else {
+                                sn = sums[j] * invc;
+                            }
                         }
+                        sv[j] = sn;
+                        pred  = pred | (sn > a.thr);
                     }
-                    sv[j] = sn;
-                    pred  = pred || sn > a.thr;
+                }
+                if (__ballot(pred) != 0 || ((open_mask >> w) & 1U)) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) s_sn[wave][k][4 * lane + j] = sv[j];
+                    need |= 1U << k;
                 }
             }
-            if (__ballot(pred) != 0 || ((open_mask >> w) & 1U)) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) s_snr[wave][4 * lane + j] = sv[j];
-                wave_sync();
-                const int clen = min(kRegChunk, tl - u);
-                const uint32_t op =
-                    walk_chunk(a, gabs, s_snr[wave], nullptr, &s_st[wave][w], prow, w, tc0, clen, lane);
+            // Run tracking of the staged levels, one at a time (one copy of
+            // the tracker in the code).
+            if (need) wave_sync();
+            while (need) {
+                const int k = __builtin_ctz(need);
+                need &= need - 1U;
+                const int w       = __popc(ra.present & ((1U << k) - 1U)); // width index of 2^k
+                const uint32_t op = run_chunk_staged(sink, a.thr, &s_st[wave][w], prow, s_sn[wave][k],
+                                                     nullptr, (uint32_t)w, a.mode == FA_PEAK_ALL,
+                                                     clen, abs0);
                 open_mask = op ? (open_mask | (1U << w)) : (open_mask & ~(1U << w));
             }
+            wave_sync();
         }
+    };
+    for (int u = -kRegChunk; u < tl; u += kRegChunk) {
+        chunk(L0, L1, u);
 #pragma unroll
         for (int k = 0; k <= KMAX; ++k)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) P[k][j] = D[k][j];
+            for (int j = 0; j < 4; ++j) L1[k][j] = L0[k][j];
     }
     wave_sync();
     if (lane < nw) a.open[(uint64_t)prow * nw + lane] = s_st[wave][lane];
@@ -1395,6 +1543,35 @@ __global__ void copy_cols_kernel(const float* __restrict__ src, uint64_t sstride
 }
 
 // ---- host helpers ------------------------------------------------------------
+
+// The smallest float s with fl(s * inv) > thr (float multiply, round to
+// nearest), one step lower still, or -inf where no such bound exists. For
+// a finite inv > 0, fl(s * inv) is non-decreasing in s, so a window sum
+// below the returned value cannot give an S/N above thr: the search's quick
+// test "sum >= floor" never skips a window that the exact test
+// "sum * inv > thr" passes (the step of margin keeps it so even if the two
+// products differed in their last bit).
+float detect_floor(float inv, float thr) {
+    if (!(inv > 0.0f) || !std::isfinite(inv) || !std::isfinite(thr)) return -INFINITY;
+    auto pass = [&](float s) { return s * inv > thr; };
+    float s = thr / inv;
+    if (!std::isfinite(s)) return -INFINITY;
+    int steps = 0;
+    if (pass(s)) {
+        for (;;) {
+            const float d = std::nextafter(s, -INFINITY);
+            if (!pass(d)) break;
+            s = d;
+            if (++steps > 64) return -INFINITY;
+        }
+    } else {
+        do {
+            s = std::nextafter(s, INFINITY);
+            if (++steps > 64 || !std::isfinite(s)) return -INFINITY;
+        } while (!pass(s));
+    }
+    return std::nextafter(s, -INFINITY);
+}
 
 double boxcar_window_variance(int64_t a, int64_t W) {
     const int64_t lo = std::min(a, W);
@@ -1652,6 +1829,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     const uint64_t nw = widths_.size();
     inv_cur_.resize((uint64_t)nprows_ * nw);
     inv_or_.resize((uint64_t)nprows_ * nw);
+    floor_cur_.resize((uint64_t)nprows_ * nw);
     widths_dev_.resize(nw);
     upload(widths_dev_.get(), widths_.data(), nw * 4);
     open_.resize((uint64_t)nprows_ * nw);
@@ -2474,6 +2652,11 @@ void Engine::prepare_gulp(uint64_t N, const uint8_t* kill_mask, hipStream_t s) {
         tables_mask_ = m;
         FA_HIP(hipMemcpyAsync(inv_cur_.get(), inv_cur_host_.data(), inv_cur_host_.size() * 4,
                               hipMemcpyHostToDevice, s));
+        floor_cur_host_.resize(inv_cur_host_.size());
+        for (std::size_t i = 0; i < inv_cur_host_.size(); ++i)
+            floor_cur_host_[i] = detect_floor(inv_cur_host_[i], cfg_.detect_thresh);
+        FA_HIP(hipMemcpyAsync(floor_cur_.get(), floor_cur_host_.data(), floor_cur_host_.size() * 4,
+                              hipMemcpyHostToDevice, s));
         or_is_cur_ = false;
     }
     const bool use_or = !first && m != prev_mask_;
@@ -2708,6 +2891,7 @@ void Engine::launch_search(uint32_t l, const float* buf, int tl, uint64_t t0, hi
     a.nrows    = static_cast<int>(search_count_[l]);
     a.inv_cur  = inv_cur_.get();
     a.inv_or   = inv_or_.get();
+    a.floor_cur = floor_cur_.get();
     a.widths   = widths_dev_.get();
     a.nw       = static_cast<int>(nw);
     a.kmax     = static_cast<int>(kmax_);
@@ -3011,7 +3195,7 @@ uint64_t Engine::memory_bytes() const {
                   ml.rows.bytes();
     for (const auto& sr : search_rows_) tables += sr.bytes();
     return x_.bytes() + buf_[0].bytes() + buf_[1].bytes() + rings_.bytes() + hist_store_.bytes() +
-           inv_cur_.bytes() + inv_or_.bytes() + open_.bytes() + open_saved_.bytes() + cands_.bytes() + keep_.bytes() + scratch_.bytes() +
+           inv_cur_.bytes() + inv_or_.bytes() + floor_cur_.bytes() + open_.bytes() + open_saved_.bytes() + cands_.bytes() + keep_.bytes() + scratch_.bytes() +
            init_chan_.bytes() + init_rows_.bytes() + widths_dev_.bytes() + counters_.bytes() +
            tables;
 }

@@ -316,6 +316,9 @@ private:
     uint64_t buffers_bound(uint64_t tile) const; // x + two level buffers, upper bound
     uint64_t layout_need(uint64_t tile) const;   // arena bytes at a tile, as planned now
     DeviceBuffer<float> inv_cur_, inv_or_;
+    // Per (plane row, width): the smallest window sum whose S/N exceeds the
+    // threshold under the current tables (the search's quick test).
+    DeviceBuffer<float> floor_cur_;
     DeviceBuffer<uint32_t> widths_dev_;
     DeviceBuffer<OpenRun> open_, open_saved_; // runs now / before the last gulp
     DeviceBuffer<CandRecord> cands_;
@@ -327,7 +330,7 @@ private:
     bool keep_planes_  = false;
 
     std::vector<uint8_t> ctor_mask_, prev_mask_, tables_mask_;
-    std::vector<float> inv_cur_host_, inv_or_host_;
+    std::vector<float> inv_cur_host_, inv_or_host_, floor_cur_host_;
     uint64_t processed_ = 0;
     uint64_t last_n_    = 0;
     bool pending_collect_ = false;
