@@ -470,6 +470,7 @@ EXPORTS = {
     "fa_rfi_calibration": (C.c_int, [P, PU64, PF32, PF32, PF32, U64]),
     "fa_rfi_time_flags": (C.c_int, [P, PU8, U64]),
     "fa_synth_filterbank": (C.c_int, [C.POINTER(SynthConfig), P, U64, P]),
+    "fa_hbm_probe": (C.c_int, [C.c_int, P, U64, C.c_int, P, PF64]),
     "fa_beam_create": (C.c_int, [C.POINTER(RfiConfig), C.POINTER(BlockLayout), F32,
                                  C.POINTER(DedispConfig), C.POINTER(BoxcarConfig), C.c_int,
                                  C.POINTER(P)]),

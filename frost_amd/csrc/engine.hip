@@ -244,6 +244,7 @@ constexpr int kGatherRowsMax = 128; // output rows per row block
 constexpr int kX0Grid = 384; // grid elements per channel window (6 per lane)
 constexpr int kX0Win  = 448; // window samples (grid + delays, 7 per lane)
 constexpr int kX0Dmax = 16;  // largest level-0 delay formed from the input
+constexpr int kX0LeafMax = 64; // input rows of one level-0 run (LDS slots of >= 256 floats)
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -327,6 +328,15 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
             lds[g.half + g.zero_slot + i] = 0.0f;
         }
     }
+    // Level-0 runs: the run's channel groups and input rows, read by the
+    // producers at every chunk, wait in LDS.
+    __shared__ GatherChan s_chan[X0 ? 16 : 1];
+    __shared__ GatherLeaf s_leaf[X0 ? kX0LeafMax : 1];
+    if constexpr (X0) {
+        for (int j = threadIdx.x; j < B.nchan; j += kGatherThreads) s_chan[j] = chans[B.chan_first + j];
+        for (int j = threadIdx.x; j < B.nleaves; j += kGatherThreads) s_leaf[j] = leaves[B.leaf_first + j];
+        lds_barrier();
+    }
     if (X0 && wave < 4) {
         // Producer of level-0 rows: per channel group, a window of input
         // samples (two chunks of loads in flight, register sets A and B) is
@@ -341,7 +351,7 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
 #pragma unroll
             for (int u = 0; u < UMAX; ++u) {
                 const int gi        = min(wave + 4 * u, B.nchan - 1);
-                const GatherChan ch = chans[B.chan_first + gi];
+                const GatherChan ch = s_chan[gi];
                 const int tw        = tau0 - ch.xs - ch.dmax; // tile sample of window element 0
                 const float* xr     = g.x + ch.xoff;
                 if (tw >= g.xlo && tw + kX0Win - 1 <= g.xhi) {
@@ -363,10 +373,18 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
             for (int u = 0; u < UMAX; ++u) {
                 const int gi = wave + 4 * u;
                 if (gi < B.nchan) {
-                    const GatherChan ch = chans[B.chan_first + gi];
+                    const GatherChan ch  = s_chan[gi];
+                    const GatherLeaf* lv = s_leaf + (ch.leaf_first - B.leaf_first);
                     wave_sync();
+                    // v[u] through uniform branches (u is not a constant
+                    // here: the channel loop is not unrolled).
 #pragma unroll
-                    for (int m = 0; m < MX; ++m) xw[lane + 64 * m] = v[u][m];
+                    for (int q = 0; q < UMAX; ++q) {
+                        if (q == u) {
+#pragma unroll
+                            for (int m = 0; m < MX; ++m) xw[lane + 64 * m] = v[q][m];
+                        }
+                    }
                     wave_sync();
                     // Lane l holds grid elements 6l .. 6l + 5 and the kX0Dmax
                     // samples before them: xl[q] = x(tile sample of grid 6l,
@@ -384,7 +402,7 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                     int fslot[4], flen[4], fsh[4], fhc[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        const GatherLeaf& q = leaves[ch.leaf_first + min(k, ch.nleaf - 1)];
+                        const GatherLeaf& q = lv[min(k, ch.nleaf - 1)];
                         fslot[k]            = q.slot;
                         flen[k]             = q.len;
                         fsh[k]              = q.shmax;
@@ -404,7 +422,7 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                             lf.shmax = pick(fsh);
                             lf.hc    = pick(fhc);
                         } else {
-                            lf = leaves[ch.leaf_first + lk];
+                            lf = lv[lk];
                         }
                         ++lk;
                         const int j0  = MG * lane - (ch.xs - lf.shmax); // slot index of grid 6l
@@ -427,12 +445,15 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
                         }
                     };
                     if (ch.dmask & 1U) emit();
+                    // Fully unrolled (a uniform test per delay instead of a
+                    // break), so every xl index is a constant register.
 #pragma unroll
                     for (int d = 1; d <= kX0Dmax; ++d) {
-                        if (d > ch.dmax) break;
+                        if (d <= ch.dmax) {
 #pragma unroll
-                        for (int m = 0; m < MG; ++m) acc[m] = acc[m] + xl[m + kX0Dmax - d];
-                        if ((ch.dmask >> d) & 1U) emit();
+                            for (int m = 0; m < MG; ++m) acc[m] = acc[m] + xl[m + kX0Dmax - d];
+                            if ((ch.dmask >> d) & 1U) emit();
+                        }
                     }
                 }
             }
@@ -2363,6 +2384,8 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
         GatherBlock b{};
         b.leaf_first = static_cast<int32_t>(gl.size());
         b.nleaves    = static_cast<int32_t>(cur.size());
+        if (x0 && cur.size() > static_cast<std::size_t>(kX0LeafMax))
+            throw std::logic_error("level-0 run holds more input rows than the kernel stages");
         b.row_first  = static_cast<int32_t>(first);
         b.nrows      = static_cast<int32_t>(end - first);
         b.hmax       = hmax;

@@ -95,8 +95,121 @@ __global__ void synth_kernel(SynthArgs a) {
     }
 }
 
+// ---- HBM probes: what a plain stream of 16-byte accesses achieves ----------
+//
+// Grid-stride loops over n 16-byte elements, eight accesses in flight per
+// thread: a read-only sum (one value per block written out so the loads are
+// not dropped), a write-only fill and a copy, stores streamed (non-temporal)
+// like the FDMT level stores.
+typedef float probe4 __attribute__((ext_vector_type(4)));
+constexpr int kProbeUnroll = 8;
+
+__global__ void __launch_bounds__(256) probe_read_kernel(const probe4* __restrict__ p, uint64_t n,
+                                                         float* __restrict__ sink) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i            = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float acc             = 0.0f;
+    for (; i + (kProbeUnroll - 1) * stride < n; i += kProbeUnroll * stride) {
+        probe4 v[kProbeUnroll];
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) v[k] = __builtin_nontemporal_load(p + i + k * stride);
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) acc += (v[k].x + v[k].y) + (v[k].z + v[k].w);
+    }
+    for (; i < n; i += stride) {
+        const probe4 v = p[i];
+        acc += (v.x + v.y) + (v.z + v.w);
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sink[blockIdx.x] = red[0];
+}
+
+template <bool NT> __device__ __forceinline__ void probe_store(probe4* p, probe4 v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <bool NT>
+__global__ void __launch_bounds__(256) probe_write_kernel(probe4* __restrict__ p, uint64_t n, float v0) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const probe4 v        = {v0, v0, v0, v0};
+    uint64_t i            = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (kProbeUnroll - 1) * stride < n; i += kProbeUnroll * stride) {
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) probe_store<NT>(p + i + k * stride, v);
+    }
+    for (; i < n; i += stride) probe_store<NT>(p + i, v);
+}
+
+template <bool NT>
+__global__ void __launch_bounds__(256) probe_copy_kernel(const probe4* __restrict__ a, probe4* __restrict__ b,
+                                                         uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i            = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (kProbeUnroll - 1) * stride < n; i += kProbeUnroll * stride) {
+        probe4 v[kProbeUnroll];
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) v[k] = __builtin_nontemporal_load(a + i + k * stride);
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k) probe_store<NT>(b + i + k * stride, v[k]);
+    }
+    for (; i < n; i += stride) probe_store<NT>(b + i, a[i]);
+}
+
 } // namespace
 } // namespace fa
+
+extern "C" int fa_hbm_probe(int kind, void* buf, uint64_t bytes, int iters, void* stream,
+                            double* gbs) {
+    using namespace fa;
+    return guard([&] {
+        require(buf && gbs, "buf and gbs must not be null");
+        require(kind >= 0 && kind <= 4,
+                "kind must be 0 (read), 1 (streamed write), 2 (copy, streamed stores), "
+                "3 (write) or 4 (copy)");
+        require(iters >= 1, "iters must be >= 1");
+        require(reinterpret_cast<uintptr_t>(buf) % 16 == 0, "buf must be 16-byte aligned");
+        // Copy: the first half is read, the second half written.
+        const bool copy  = kind == 2 || kind == 4;
+        const uint64_t n = (copy ? bytes / 2 : bytes) / 16;
+        require(n >= 1, "buffer too small");
+        hipStream_t s = as_stream(stream);
+        int dev = 0, ncu = 0;
+        FA_HIP(hipGetDevice(&dev));
+        FA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const unsigned grid = static_cast<unsigned>(std::max(ncu, 1) * 16);
+        DeviceBuffer<float> sink(grid);
+        auto* p = static_cast<fa::probe4*>(buf);
+        auto launch = [&] {
+            if (kind == 0) probe_read_kernel<<<grid, 256, 0, s>>>(p, n, sink.get());
+            else if (kind == 1) probe_write_kernel<true><<<grid, 256, 0, s>>>(p, n, 1.0f);
+            else if (kind == 2) probe_copy_kernel<true><<<grid, 256, 0, s>>>(p, p + n, n);
+            else if (kind == 3) probe_write_kernel<false><<<grid, 256, 0, s>>>(p, n, 1.0f);
+            else probe_copy_kernel<false><<<grid, 256, 0, s>>>(p, p + n, n);
+        };
+        launch(); // warm-up
+        hipEvent_t e0, e1;
+        FA_HIP(hipEventCreate(&e0));
+        FA_HIP(hipEventCreate(&e1));
+        FA_HIP(hipEventRecord(e0, s));
+        for (int k = 0; k < iters; ++k) launch();
+        FA_HIP(hipEventRecord(e1, s));
+        FA_HIP(hipGetLastError());
+        FA_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        FA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        FA_HIP(hipEventDestroy(e0));
+        FA_HIP(hipEventDestroy(e1));
+        const double moved = static_cast<double>(n) * 16.0 * (copy ? 2.0 : 1.0) * iters;
+        *gbs               = moved / (static_cast<double>(ms) * 1e-3) / 1e9;
+    });
+}
 
 extern "C" int fa_synth_filterbank(const fa_synth_config* cfg, void* out, uint64_t out_stride,
                                    void* stream) {

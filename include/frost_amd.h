@@ -460,6 +460,15 @@ typedef struct fa_synth_config {
 int fa_synth_filterbank(const fa_synth_config* cfg, void* out, uint64_t out_stride,
                         void* stream);
 
+/* Measurement support, no counterpart in frost: the HBM bandwidth a plain
+ * stream of 16-byte accesses reaches on this device (GB/s over `iters`
+ * passes after one warm-up). kind 0: read all `bytes` of `buf`; 1: write
+ * them (streamed stores); 2: copy the first half to the second (streamed
+ * stores); 3 and 4: write and copy with ordinary stores. The
+ * practical ceiling the FDMT and search kernels are compared with, beside
+ * the published peak. */
+int fa_hbm_probe(int kind, void* buf, uint64_t bytes, int iters, void* stream, double* gbs);
+
 /* ---- one beam: the per-gulp driver (Pipeline::Impl::execute_block) ------- */
 
 /* Pipeline::Impl (lib/pipeline/pipeline.cpp:378-438): RFI cleaner + FDMT
