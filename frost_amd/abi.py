@@ -465,6 +465,8 @@ EXPORTS = {
                                 C.POINTER(P)]),
     "fa_rfi_destroy": (C.c_int, [P]),
     "fa_rfi_execute_normalized": (C.c_int, [P, P, U64, U64, P, U64, U64, P]),
+    "fa_rfi_execute": (C.c_int, [P, P, U64, P]),
+    "fa_rfi_execute_copy": (C.c_int, [P, P, U64, P, U64, P]),
     "fa_rfi_channel_mask": (C.c_int, [P, PU8, U64]),
     "fa_rfi_get_report": (C.c_int, [P, C.POINTER(RfiReport)]),
     "fa_rfi_calibration": (C.c_int, [P, PU64, PF32, PF32, PF32, U64]),

@@ -1302,6 +1302,235 @@ rezero_rows_kernel(ApplyArgs a, const uint32_t* __restrict__ rows, uint32_t nrow
     }
 }
 
+// ---- byte-domain apply (RFICleaner::execute(std::span<uint8_t>)) -----------
+//
+// lib/cpu/rfi_cpu.cpp:526-720 with the block's own codec: a written cell is
+// quantised (nearest, ties to even, clamped to the sample range; float32 is
+// written as it is) and packed in place. Cell states and fills are decided
+// from a snapshot of the input (ApplyArgs::g reads it), so the order in which
+// threads write does not matter; a flagged row's time fill takes its source
+// cell's final value, computed from the snapshot by the same rules.
+
+__device__ __forceinline__ int dtype_bits_dev(int dtype) {
+    switch (dtype) {
+    case FA_UINT1: return 1;
+    case FA_UINT2: return 2;
+    case FA_UINT4: return 4;
+    case FA_UINT8: return 8;
+    case FA_UINT16: return 16;
+    default: return 32;
+    }
+}
+
+// Codec<D>::quantise (lib/prepare/sample_codec.hpp:102-108): std::clamp of
+// the nearest integer, as (v < lo) ? lo : (hi < v) ? hi : v.
+__device__ __forceinline__ float quantise_dev(float v, int dtype) {
+    if (dtype == FA_FLOAT32) return v;
+    const int bits   = dtype_bits_dev(dtype);
+    const float maxv = bits < 32 ? (float)((1u << bits) - 1u) : 4294967040.0f; // below 2^32
+    const float r    = rintf(v);
+    return r < 0.0f ? 0.0f : (maxv < r ? maxv : r);
+}
+
+// A fill of the given stream in noise units (0 for the mean fill).
+__device__ __forceinline__ float fill_units(const ApplyArgs& a, uint64_t stream, uint64_t t,
+                                            uint64_t c) {
+    return a.fill == FA_FILL_NOISE
+               ? normal_draw(hash5(a.seed, a.block_index, (stream << 56) ^ t, c, 0))
+               : 0.0f;
+}
+
+// Final value of cell (t, c) of an unflagged row, and whether the byte
+// cleaner writes it (clean cells without a transform and unmeasured cells
+// keep their sample).
+__device__ float byte_unflagged(const ApplyArgs& a, uint64_t t, uint64_t c, bool& write,
+                                unsigned long long (&local)[4]) {
+    const uint64_t w = a.g.window_of(t);
+    const uint64_t i = w * a.g.nch + c;
+    if (a.mask[c] == 0) {
+        write = true;
+        return quantise_dev(level(a, i, fill_units(a, kStreamMasked, t, c)), a.g.dtype);
+    }
+    int st  = kClean;
+    bool fb = false;
+    const float v = cleaned_unflagged(a, t, c, &st, &fb);
+    write         = (st == kClean && a.transform) || st >= kNarrow;
+    if (st == kNarrow) ++local[0];
+    else if (st == kKSpec) ++local[1];
+    else if (st == kKChan) ++local[2];
+    if (fb) ++local[3];
+    return write ? quantise_dev(v, a.g.dtype) : a.g.x(t, c);
+}
+
+// Final value of an unmasked cell of a flagged row without zero-DM.
+__device__ float byte_flagged(const ApplyArgs& a, uint64_t t, uint64_t c,
+                              unsigned long long (&local)[4]) {
+    const uint64_t w = a.g.window_of(t);
+    const uint64_t i = w * a.g.nch + c;
+    if (a.fill != FA_FILL_NEIGHBOUR)
+        return quantise_dev(level(a, i, fill_units(a, kStreamNoise, t, c)), a.g.dtype);
+    const uint64_t lo = t > a.time_r ? t - a.time_r : 0;
+    const uint64_t hi = t + a.time_r < a.g.nsamps - 1 ? t + a.time_r : a.g.nsamps - 1;
+    for (int k = 0; k < kMaxDraws; ++k) {
+        const uint64_t h   = hash5(a.seed, a.block_index, (kStreamTime << 56) ^ t, c, (uint64_t)k);
+        const uint64_t src = lo + h % (hi - lo + 1);
+        if (a.flags[src] != 0) continue;
+        bool wr;
+        unsigned long long none[4] = {0, 0, 0, 0};
+        return byte_unflagged(a, src, c, wr, none); // the source's final sample, copied
+    }
+    ++local[3];
+    return quantise_dev(level(a, i, 0.0f), a.g.dtype);
+}
+
+__device__ __forceinline__ void put_sample(uint8_t* row, uint64_t c, int dtype, float q) {
+    switch (dtype) {
+    case FA_UINT8: row[c] = (uint8_t)q; return;
+    case FA_UINT16: reinterpret_cast<uint16_t*>(row)[c] = (uint16_t)q; return;
+    case FA_UINT32: reinterpret_cast<uint32_t*>(row)[c] = (uint32_t)q; return;
+    default: reinterpret_cast<float*>(row)[c] = q; return;
+    }
+}
+
+// Every row except those rebuilt with zero-DM: one thread per output byte
+// (sub-byte samples: the cells packed in it) or per sample, grid-stride over
+// the block; masked cells of flagged rows get their masked-stream fill.
+__global__ void __launch_bounds__(256) bytes_apply_kernel(ApplyArgs a, uint8_t* __restrict__ out) {
+    const int bits     = dtype_bits_dev(a.g.dtype);
+    const uint64_t per = bits < 8 ? (uint64_t)(8 / bits) : 1; // cells per unit
+    const uint64_t units_per_row = (a.g.nch + per - 1) / per;
+    const uint64_t total         = units_per_row * a.g.nsamps;
+    unsigned long long local[4]  = {0, 0, 0, 0};
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < total;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t  = u / units_per_row;
+        const uint64_t c0 = (u % units_per_row) * per;
+        const bool flagged = a.flags[t] != 0;
+        if (flagged && a.rezero) continue; // bytes_rezero_kernel
+        uint8_t* row = out + t * a.g.row_bytes;
+        if (bits >= 8) {
+            float q;
+            bool write = true;
+            if (!flagged) {
+                q = byte_unflagged(a, t, c0, write, local);
+            } else if (a.mask[c0] == 0) {
+                const uint64_t i = a.g.window_of(t) * a.g.nch + c0;
+                q = quantise_dev(level(a, i, fill_units(a, kStreamMasked, t, c0)), a.g.dtype);
+            } else {
+                q = byte_flagged(a, t, c0, local);
+            }
+            if (write) put_sample(row, c0, a.g.dtype, q);
+            continue;
+        }
+        const unsigned mk = (1u << bits) - 1u;
+        unsigned byte     = a.g.block[t * a.g.row_bytes + (c0 * bits >> 3)];
+        for (uint64_t k = 0; k < per && c0 + k < a.g.nch; ++k) {
+            const uint64_t c = c0 + k;
+            float q;
+            bool write = true;
+            if (!flagged) {
+                q = byte_unflagged(a, t, c, write, local);
+            } else if (a.mask[c] == 0) {
+                const uint64_t i = a.g.window_of(t) * a.g.nch + c;
+                q = quantise_dev(level(a, i, fill_units(a, kStreamMasked, t, c)), a.g.dtype);
+            } else {
+                q = byte_flagged(a, t, c, local);
+            }
+            if (!write) continue;
+            const unsigned sh = (unsigned)(k * bits);
+            byte = (byte & ~(mk << sh)) | (((unsigned)q & mk) << sh);
+        }
+        row[c0 * bits >> 3] = (uint8_t)byte;
+    }
+    for (int q = 0; q < 4; ++q) {
+        unsigned long long v = local[q];
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&a.counters[q], v);
+    }
+}
+
+// Flagged rows with zero-DM on (lib/cpu/rfi_cpu.cpp:631-712): masked cells
+// get their fill; each other cell's value in noise units (a time draw's
+// source sample as the unflagged pass writes it, or the fill), the band mean
+// over the cells measured in the row's window summed in channel order, then
+// level(v - mean) quantised with the rounding error carried across channels
+// in channel order (one lane per row). One block per row.
+__global__ void __launch_bounds__(256)
+bytes_rezero_kernel(ApplyArgs a, const uint32_t* __restrict__ rows, uint32_t nrows,
+                    float* __restrict__ rowbuf, uint8_t* __restrict__ out) {
+    float* vbuf = rowbuf + (uint64_t)blockIdx.x * a.g.nch;
+    __shared__ unsigned long long s_fb;
+    const uint64_t n = a.g.nch;
+    const int dtype  = a.g.dtype;
+    const int bits   = dtype_bits_dev(dtype);
+    for (uint32_t k = blockIdx.x; k < nrows; k += gridDim.x) {
+        const uint64_t t  = rows[k];
+        const uint64_t w  = a.g.window_of(t);
+        const uint64_t lo = t > a.time_r ? t - a.time_r : 0;
+        const uint64_t hi = t + a.time_r < a.g.nsamps - 1 ? t + a.time_r : a.g.nsamps - 1;
+        if (threadIdx.x == 0) s_fb = 0;
+        __syncthreads();
+        unsigned long long fb = 0;
+        for (uint64_t c = threadIdx.x; c < n; c += blockDim.x) {
+            float v = 0.0f;
+            if (a.mask[c] == 0) {
+                v = quantise_dev(level(a, w * n + c, fill_units(a, kStreamMasked, t, c)), dtype);
+            } else if (a.fill != FA_FILL_NEIGHBOUR) {
+                v = fill_units(a, kStreamNoise, t, c);
+            } else {
+                bool found = false;
+                for (int d = 0; d < kMaxDraws && !found; ++d) {
+                    const uint64_t h =
+                        hash5(a.seed, a.block_index, (kStreamTime << 56) ^ t, c, (uint64_t)d);
+                    const uint64_t src = lo + h % (hi - lo + 1);
+                    if (a.flags[src] != 0) continue;
+                    found = true;
+                    bool wr;
+                    unsigned long long none[4] = {0, 0, 0, 0};
+                    const float s      = byte_unflagged(a, src, c, wr, none);
+                    const uint64_t is  = a.g.window_of(src) * n + c;
+                    v = a.equalize ? (s - a.eq_loc) / a.eq_scale : (s - a.loc[is]) * a.inv[is];
+                }
+                if (!found) ++fb;
+            }
+            vbuf[c] = v;
+        }
+        if (fb) atomicAdd(&s_fb, fb);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sum = 0.0, count = 0.0;
+            for (uint64_t c = 0; c < n; ++c)
+                if (a.mask[c] != 0 && a.inv[w * n + c] != 0.0f) {
+                    sum += vbuf[c];
+                    count += 1.0;
+                }
+            const float mean = (float)(count > 0.0 ? sum / count : 0.0);
+            uint8_t* row     = out + t * a.g.row_bytes;
+            float carry      = 0.0f;
+            for (uint64_t c = 0; c < n; ++c) {
+                float q;
+                if (a.mask[c] == 0) {
+                    q = vbuf[c];
+                } else {
+                    const float want = level(a, w * n + c, vbuf[c] - mean) + carry;
+                    q                = quantise_dev(want, dtype);
+                    const float e    = want - q;
+                    carry            = e < -0.5f ? -0.5f : (0.5f < e ? 0.5f : e);
+                }
+                if (bits >= 8) {
+                    put_sample(row, c, dtype, q);
+                } else {
+                    const uint64_t bit = c * (uint64_t)bits;
+                    const unsigned mk  = (1u << bits) - 1u, sh = (unsigned)(bit & 7);
+                    row[bit >> 3] = (uint8_t)((row[bit >> 3] & ~(mk << sh)) | (((unsigned)q & mk) << sh));
+                }
+            }
+            if (s_fb) atomicAdd(&a.counters[3], s_fb);
+        }
+        __syncthreads();
+    }
+}
+
 // Mean and population std of each (window, channel) of the cleaned values
 // (lib/cpu/rfi_cpu.cpp:217-239): Welford in time order, FP64. One wave per
 // (window, 64 channels): tiles of 32 rows x 64 channels are read from the
@@ -1701,7 +1930,7 @@ uint64_t DeviceRfi::memory_bytes() const {
            tail_scale_.bytes() + skeep64_.bytes() + zsums_.bytes() + zdot_a_.bytes() +
            zdot_b_.bytes() + loc2_.bytes() + scale2_.bytes() + tail_out_loc_.bytes() +
            tail_out_scale_.bytes() + rowbuf_.bytes() + flag_rows_.bytes() + inv2_.bytes() +
-           bad_index_.bytes();
+           bad_index_.bytes() + snapshot_.bytes();
 }
 
 void DeviceRfi::calibration(float* loc, float* scale, float* inv) const {
@@ -1830,6 +2059,103 @@ void DeviceRfi::apply(float* out, uint64_t stride, uint64_t offset, hipStream_t 
     full_block_apply(capacity, out, stride, offset, s);
     ++block_index_;
     tail_ready_ = true;
+}
+
+void DeviceRfi::execute_bytes(uint8_t* block, uint64_t nsamps, hipStream_t s) {
+    require(block != nullptr, "block must not be null");
+    require(s != nullptr, "an explicit non-null stream is required");
+    if (nsamps > layout_.max_nsamps)
+        throw std::invalid_argument("RFICleaner: block exceeds the layout's max_nsamps (" +
+                                    std::to_string(layout_.max_nsamps) + ")");
+    DeviceGuard dg(device_);
+    // The byte API can change masks and calibration apart from the
+    // normalised one; only a normalised full block re-establishes a tail.
+    tail_ready_ = false;
+    if (nsamps == 0 || nsamps < std::max(kMinWindow, kMinBlockSamps)) {
+        if (layout_.dtype == FA_FLOAT32 && nsamps > 0) measure(block, nsamps, nsamps, s);
+        pend_                = Pending{};
+        report_              = fa_rfi_report{};
+        report_.nsamps       = nsamps;
+        mask_                = static_mask_;
+        report_.chans_zapped = static_cast<uint64_t>(std::count(static_mask_.begin(), static_mask_.end(), 0));
+        report_.chans_masked = report_.chans_zapped;
+        time_flags_host_.assign(nsamps, 0);
+        ++block_index_;
+        return;
+    }
+    // Statistics and decisions from a snapshot of the input, then every
+    // written cell packed into the caller's block.
+    snapshot_.reserve(nsamps * row_bytes_);
+    FA_HIP(hipMemcpyAsync(snapshot_.get(), block, nsamps * row_bytes_, hipMemcpyDeviceToDevice, s));
+    measure(snapshot_.get(), nsamps, nsamps, s);
+    pend_.ready = false;
+    apply_bytes(block, s);
+    ++block_index_;
+}
+
+void DeviceRfi::apply_bytes(uint8_t* out, hipStream_t s) {
+    const uint64_t n      = nch_;
+    const uint64_t nsamps = nsamps_;
+    Geometry g{snapshot_.get(), row_bytes_, n, nsamps, window_len_, nwin_, layout_.dtype};
+    const bool kspec = cfg_.kadane_spectrum_enabled, kchan = cfg_.kadane_channel_enabled;
+    zero_u64_kernel<<<1, 64, 0, s>>>(counters_.get(), 8);
+    ApplyArgs ap{};
+    ap.g           = g;
+    ap.loc         = loc_.get();
+    ap.scale       = scale_.get();
+    ap.inv         = inv_.get();
+    ap.mask        = mask_dev_.get();
+    ap.flags       = flags_dev_.get();
+    ap.kspec       = kspec ? kspec_.get() : nullptr;
+    ap.kseg        = kchan ? kseg_.get() : nullptr;
+    ap.clip        = clip_;
+    ap.narrow      = narrow_active_ ? 1 : 0;
+    ap.fill        = cfg_.fill;
+    ap.seed        = cfg_.seed;
+    ap.block_index = block_index_;
+    ap.chan_r      = chan_fill_radius_;
+    ap.time_r      = time_fill_radius_;
+    ap.capacity    = nsamps;
+    ap.counters    = counters_.get();
+    ap.transform   = transform_ ? 1 : 0;
+    ap.zero_dm     = cfg_.zero_dm;
+    ap.equalize    = cfg_.equalize ? 1 : 0;
+    ap.rezero      = cfg_.zero_dm != FA_ZERO_DM_OFF ? 1 : 0;
+    ap.eq_loc      = eq_loc_;
+    ap.eq_scale    = eq_scale_;
+    ap.s_keep      = s_keep_.get();
+    ap.zdot_a      = zdot_a_.get();
+    ap.zdot_b      = zdot_b_.get();
+    bytes_apply_kernel<<<8192, 256, 0, s>>>(ap, out);
+    FA_HIP(hipGetLastError());
+    if (ap.rezero) {
+        flag_rows_host_.clear();
+        for (uint64_t t = 0; t < nsamps; ++t)
+            if (time_flags_host_[t]) flag_rows_host_.push_back(static_cast<uint32_t>(t));
+        const uint64_t nf = flag_rows_host_.size();
+        if (nf > 0) {
+            FA_HIP(hipMemcpyAsync(flag_rows_.get(), flag_rows_host_.data(), nf * 4,
+                                  hipMemcpyHostToDevice, s));
+            const unsigned nb = static_cast<unsigned>(std::min<uint64_t>(nf, kRowBlocks));
+            bytes_rezero_kernel<<<nb, 256, 0, s>>>(ap, flag_rows_.get(), static_cast<uint32_t>(nf),
+                                                   rowbuf_.get(), out);
+            FA_HIP(hipGetLastError());
+        }
+    }
+    remeasured_ = false;
+    unsigned long long cnt[8];
+    FA_HIP(hipMemcpyAsync(cnt, counters_.get(), sizeof(cnt), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipEventRecord(ev_[5], s));
+    FA_HIP(hipStreamSynchronize(s));
+    report_.cells_narrow          = cnt[0];
+    report_.cells_kadane_spectrum = cnt[1];
+    report_.cells_kadane_channel  = cnt[2];
+    report_.fill_fallbacks        = cnt[3];
+    report_.t_bandpass = event_seconds(ev_[0], ev_[1]);
+    report_.t_moments  = event_seconds(ev_[1], ev_[2]);
+    report_.t_scan     = event_seconds(ev_[2], ev_[3]);
+    report_.t_flags    = event_seconds(ev_[3], ev_[4]);
+    report_.t_apply    = event_seconds(ev_[4], ev_[5]);
 }
 
 void DeviceRfi::build_channel_mask(hipStream_t s) {
@@ -2315,6 +2641,29 @@ int fa_rfi_execute_normalized(fa_rfi* rfi, const void* block, uint64_t capacity,
         require(rfi != nullptr, "rfi must not be null");
         rfi->rfi->execute_normalized(static_cast<const uint8_t*>(block), capacity, nsamps, out,
                                      out_stride, out_offset, fa::as_stream(stream));
+    });
+}
+
+int fa_rfi_execute(fa_rfi* rfi, void* block, uint64_t nbytes, void* stream) {
+    return guard([&] {
+        require(rfi != nullptr, "rfi must not be null");
+        const uint64_t rb = rfi->rfi->row_bytes();
+        require(nbytes % rb == 0, "RFICleaner: block must hold a whole number of time samples");
+        rfi->rfi->execute_bytes(static_cast<uint8_t*>(block), nbytes / rb, fa::as_stream(stream));
+    });
+}
+
+int fa_rfi_execute_copy(fa_rfi* rfi, const void* in, uint64_t in_bytes, void* out,
+                        uint64_t out_bytes, void* stream) {
+    return guard([&] {
+        require(rfi != nullptr, "rfi must not be null");
+        require(out_bytes >= in_bytes, "RFICleaner: out smaller than in");
+        const uint64_t rb = rfi->rfi->row_bytes();
+        require(in_bytes % rb == 0, "RFICleaner: block must hold a whole number of time samples");
+        hipStream_t s = fa::as_stream(stream);
+        if (in != out && in_bytes > 0)
+            FA_HIP(hipMemcpyAsync(out, in, in_bytes, hipMemcpyDeviceToDevice, s));
+        rfi->rfi->execute_bytes(static_cast<uint8_t*>(out), in_bytes / rb, s);
     });
 }
 

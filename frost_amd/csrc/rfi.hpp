@@ -75,6 +75,10 @@ public:
     // measure, and its apply must have returned before that measure starts
     // (they share the calibration buffers).
     void apply(float* out, uint64_t stride, uint64_t offset, hipStream_t s);
+    // RFICleaner::execute(std::span<uint8_t>) (lib/cpu/rfi_cpu.cpp:62-124):
+    // cleans nsamps packed rows of `block` (device) in place in their own
+    // sample type; drops the retained calibration of the normalised API.
+    void execute_bytes(uint8_t* block, uint64_t nsamps, hipStream_t s);
 
     const std::vector<uint8_t>& channel_mask() const { return mask_; }
     const fa_rfi_report& report() const { return report_; }
@@ -82,6 +86,7 @@ public:
     void calibration(float* loc, float* scale, float* inv) const;
     const std::vector<uint8_t>& time_flags() const { return time_flags_host_; }
     const fa_block_layout& layout() const { return layout_; }
+    uint64_t row_bytes() const { return row_bytes_; }
     int device() const { return device_; }
     uint64_t memory_bytes() const;
 
@@ -90,6 +95,7 @@ private:
                             hipStream_t s);
     void full_block_apply(uint64_t capacity, float* out, uint64_t stride, uint64_t offset,
                           hipStream_t s);
+    void apply_bytes(uint8_t* out, hipStream_t s);
     struct Pending {
         const uint8_t* block = nullptr;
         uint64_t capacity = 0, nsamps = 0;
@@ -152,6 +158,7 @@ private:
     DeviceBuffer<float> rowbuf_; // [kRowBlocks][nch] per-block row values
     std::vector<uint32_t> flag_rows_host_;
     DeviceBuffer<unsigned long long> bad_index_;
+    DeviceBuffer<uint8_t> snapshot_; // byte API: the input the cell states are decided from
 
     // Host mirrors.
     std::vector<float> scale_host_;

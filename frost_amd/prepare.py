@@ -53,6 +53,25 @@ class RFICleaner:
         out = np.zeros((self.layout.nchans, cap), np.float32)
         return d_out.to_host(out, stream)
 
+    def execute_device(self, block: DeviceArray, nbytes: int, stream: Stream):
+        """RFICleaner::execute(std::span<uint8_t>): cleans nbytes of packed
+        rows of a device block in place."""
+        check(lib().fa_rfi_execute(self.h, block.ptr, nbytes, stream.h))
+
+    def execute(self, block: np.ndarray, stream: Stream) -> np.ndarray:
+        """Host packed block -> the cleaned packed block (a new host array of
+        the same bytes)."""
+        raw = np.ascontiguousarray(block).view(np.uint8).reshape(-1)
+        d = DeviceArray.from_host(raw, stream)
+        self.execute_device(d, raw.size, stream)
+        return d.to_host(np.zeros_like(raw), stream)
+
+    def execute_copy_device(self, src: DeviceArray, nbytes: int, out: DeviceArray,
+                            out_bytes: int, stream: Stream):
+        """RFICleaner::execute(in, out): copies in to the front of out and
+        cleans that copy."""
+        check(lib().fa_rfi_execute_copy(self.h, src.ptr, nbytes, out.ptr, out_bytes, stream.h))
+
     def channel_mask(self) -> np.ndarray:
         out = np.zeros(self.layout.nchans, np.uint8)
         check(lib().fa_rfi_channel_mask(self.h, out.ctypes.data_as(abi.PU8), out.size))

@@ -391,6 +391,22 @@ int fa_rfi_execute_normalized(fa_rfi* rfi, const void* block,
                               uint64_t out_stride, uint64_t out_offset,
                               void* stream);
 
+/* RFICleaner::execute(std::span<uint8_t> block)
+ * (include/frost/prepare/rfi.hpp:256-258): cleans the packed, time-major
+ * block (device, nbytes = a whole number of rows, at most max_nsamps) in
+ * place in its own sample type: replaced and transformed cells are written
+ * quantised (nearest, clamped to the type's range), everything else is left
+ * bit-exact. Blocks shorter than frost's statistics minimum are left as they
+ * are. Drops the retained calibration of fa_rfi_execute_normalized's short
+ * tails. Synchronises the stream. */
+int fa_rfi_execute(fa_rfi* rfi, void* block, uint64_t nbytes, void* stream);
+
+/* RFICleaner::execute(std::span<const uint8_t> in, std::span<uint8_t> out)
+ * (include/frost/prepare/rfi.hpp:259-260): copies in (device, in_bytes) to
+ * the front of out (device, out_bytes >= in_bytes) and cleans that copy. */
+int fa_rfi_execute_copy(fa_rfi* rfi, const void* in, uint64_t in_bytes, void* out,
+                        uint64_t out_bytes, void* stream);
+
 /* RFICleaner::channel_mask() (input channel order, 1 = good) and report(). */
 int fa_rfi_channel_mask(const fa_rfi* rfi, uint8_t* out, uint64_t cap);
 int fa_rfi_get_report(const fa_rfi* rfi, fa_rfi_report* out);

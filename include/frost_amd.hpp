@@ -184,6 +184,15 @@ public:
         check(fa_rfi_execute_normalized(r_, device_block, capacity, nsamps, device_out, stride,
                                         offset, s.get()));
     }
+    /** execute(std::span<uint8_t>): the packed device block cleaned in place. */
+    void execute(void* device_block, uint64_t nbytes, const Stream& s) {
+        check(fa_rfi_execute(r_, device_block, nbytes, s.get()));
+    }
+    /** execute(in, out): in copied to the front of out, then cleaned there. */
+    void execute(const void* device_in, uint64_t in_bytes, void* device_out, uint64_t out_bytes,
+                 const Stream& s) {
+        check(fa_rfi_execute_copy(r_, device_in, in_bytes, device_out, out_bytes, s.get()));
+    }
     std::vector<uint8_t> channel_mask() const {
         std::vector<uint8_t> m(nchans_);
         check(fa_rfi_channel_mask(r_, m.data(), m.size()));

@@ -67,6 +67,9 @@ struct Segment {
 };
 
 float unpack_sample(const uint8_t* row, SizeType c, int dtype);
+float quantise_sample(float v, int dtype);
+void write_sample(uint8_t* row, SizeType c, int dtype, float v);
+void copy_sample(const uint8_t* src, uint8_t* dst, SizeType c, int dtype);
 SizeType largest_power_of_five_leq(SizeType count);
 float median5(float a, float b, float c, float d, float e);
 bool refine_bandpass(const float* v, SizeType m, SizeType stride, float& loc,
@@ -109,6 +112,9 @@ public:
     // RFICleaner::execute_normalized (lib/cpu/rfi_cpu.cpp:161-260).
     void execute_normalized(const uint8_t* block, SizeType capacity,
                             SizeType valid, float* out);
+    // RFICleaner::execute(std::span<uint8_t>) (lib/cpu/rfi_cpu.cpp:62-124):
+    // cleans `nbytes` of packed rows in place.
+    void execute_bytes(uint8_t* block, SizeType nbytes);
 
     // Inputs
     fa_rfi_config cfg;
@@ -165,7 +171,9 @@ private:
     float* row(SizeType t) { return work.data() + t * nchans; }
     SizeType chunk_count() const;
 
+    bool measure_all();
     void execute_float();
+    void apply_bytes(uint8_t* block);
     void measure_bandpass();
     void measure_moments();
     void build_channel_mask();

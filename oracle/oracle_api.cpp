@@ -138,6 +138,9 @@ int oracle_rfi_execute_normalized(void* h, const uint8_t* block,
         static_cast<Cleaner*>(h)->execute_normalized(block, capacity, nsamps, out);
     });
 }
+int oracle_rfi_execute_bytes(void* h, uint8_t* block, uint64_t nbytes) {
+    return guard([&] { static_cast<Cleaner*>(h)->execute_bytes(block, nbytes); });
+}
 int oracle_rfi_mask(void* h, uint8_t* out) {
     return guard([&] {
         const auto& m = static_cast<Cleaner*>(h)->mask;

@@ -50,6 +50,7 @@ def lib() -> C.CDLL:
                                             C.POINTER(P)]),
             "oracle_rfi_destroy": (None, [P]),
             "oracle_rfi_execute_normalized": (C.c_int, [P, P, C.c_uint64, C.c_uint64, P]),
+            "oracle_rfi_execute_bytes": (C.c_int, [P, P, C.c_uint64]),
             "oracle_rfi_mask": (C.c_int, [P, P]),
             "oracle_rfi_report": (C.c_int, [P, C.POINTER(abi.RfiReport)]),
             "oracle_rfi_calibration": (C.c_int, [P, P, P, P, P, C.c_uint64]),
@@ -156,6 +157,13 @@ class Cleaner:
         out = np.zeros((self.layout.nchans, capacity), np.float32)
         check(lib().oracle_rfi_execute_normalized(self.h, ptr(block), capacity, nsamps,
                                                   ptr(out)))
+        return out
+
+    def execute_bytes(self, block: np.ndarray) -> np.ndarray:
+        """RFICleaner::execute(span<uint8_t>): the packed block cleaned in
+        place; returns the cleaned copy (the argument is not modified)."""
+        out = np.ascontiguousarray(block).view(np.uint8).reshape(-1).copy()
+        check(lib().oracle_rfi_execute_bytes(self.h, ptr(out), out.size))
         return out
 
     def mask(self) -> np.ndarray:

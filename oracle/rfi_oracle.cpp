@@ -212,6 +212,54 @@ float unpack_sample(const uint8_t* row, SizeType c, int dtype) {
     }
 }
 
+// Sample codec output (lib/prepare/sample_codec.hpp:42-113): quantise to
+// the nearest representable sample (round half to even, clamped to the
+// type's range; float32 unchanged) and write it packed, sub-byte samples
+// LSB-first within their byte.
+float quantise_sample(float v, int dtype) {
+    if (dtype == FA_FLOAT32) return v;
+    const int bits = bits_of(dtype);
+    const float maxv =
+        bits < 32 ? (float)((1U << bits) - 1U)
+                  : std::nextafter((float)((double)((uint64_t{1} << 32) - 1U) + 1.0), 0.0F);
+    return std::clamp(std::nearbyint(v), 0.0F, maxv);
+}
+
+void write_sample(uint8_t* row, SizeType c, int dtype, float v) {
+    switch (dtype) {
+    case FA_UINT8: row[c] = (uint8_t)v; return;
+    case FA_UINT16: {
+        const uint16_t raw = (uint16_t)v;
+        std::memcpy(row + c * 2, &raw, 2);
+        return;
+    }
+    case FA_UINT32: {
+        const uint32_t raw = (uint32_t)v;
+        std::memcpy(row + c * 4, &raw, 4);
+        return;
+    }
+    case FA_FLOAT32: std::memcpy(row + c * 4, &v, 4); return;
+    default: {
+        const int bits       = bits_of(dtype);
+        const SizeType bit   = c * (SizeType)bits;
+        const unsigned maskv = (1U << bits) - 1U;
+        const unsigned shift = bit & 7U;
+        uint8_t& byte        = row[bit >> 3U];
+        byte = (uint8_t)((byte & ~(maskv << shift)) | (((unsigned)v & maskv) << shift));
+    }
+    }
+}
+
+void copy_sample(const uint8_t* src, uint8_t* dst, SizeType c, int dtype) {
+    const int bits = dtype == FA_FLOAT32 ? 32 : bits_of(dtype);
+    if (bits >= 8) {
+        std::memcpy(dst + c * (SizeType)(bits / 8), src + c * (SizeType)(bits / 8),
+                    (SizeType)(bits / 8));
+        return;
+    }
+    write_sample(dst, c, dtype, unpack_sample(src, c, dtype));
+}
+
 SizeType largest_power_of_five_leq(SizeType count) {
     // lib/prepare/bandpass_sampling.hpp:8-15
     if (count == 0) return 0;
@@ -561,8 +609,10 @@ SizeType Cleaner::chunk_count() const {
     return std::clamp<SizeType>(by, 1, kMaxChunks);
 }
 
-void Cleaner::execute_float() {
-    // lib/cpu/rfi_cpu.cpp:62-124 on the float work copy
+bool Cleaner::measure_all() {
+    // lib/cpu/rfi_cpu.cpp:62-118 on the float work copy: the statistics and
+    // decisions of a block; false (and the block counted) when it is too
+    // short for them.
     report              = fa_rfi_report{};
     report.nsamps       = nsamps;
     mask                = static_mask;
@@ -571,7 +621,7 @@ void Cleaner::execute_float() {
     report.chans_masked = report.chans_zapped;
     if (nsamps < std::max(kMinWindow, kMinBlockSamps)) {
         ++block_index;
-        return;
+        return false;
     }
     nwin = std::max<SizeType>(nsamps / window_len, 1);
     measure_bandpass();
@@ -579,8 +629,193 @@ void Cleaner::execute_float() {
     build_channel_mask();
     scan_rows();
     build_flags();
+    return true;
+}
+
+void Cleaner::execute_float() {
+    // lib/cpu/rfi_cpu.cpp:62-124 on the float work copy
+    if (!measure_all()) return;
     apply();
     ++block_index;
+}
+
+void Cleaner::execute_bytes(uint8_t* block, SizeType nbytes) {
+    // RFICleaner::Impl::execute (lib/cpu/rfi_cpu.cpp:62-124): the block is
+    // cleaned in place in its own sample type; the retained calibration of
+    // the normalised API is dropped.
+    tail_ready = false;
+    if (nbytes % row_bytes != 0)
+        throw std::invalid_argument("RFICleaner: block must hold a whole number of time samples");
+    const SizeType ns = nbytes / row_bytes;
+    if (ns > layout.max_nsamps)
+        throw std::invalid_argument("RFICleaner: block exceeds the layout's max_nsamps (" +
+                                    std::to_string(layout.max_nsamps) + ")");
+    if (layout.dtype == FA_FLOAT32) {
+        for (SizeType i = 0; i < ns * nchans; ++i) {
+            uint32_t bits;
+            std::memcpy(&bits, block + i * 4, 4);
+            if ((bits & 0x7F800000U) == 0x7F800000U)
+                throw std::invalid_argument("RFI: non-finite float32 input at sample " +
+                                            std::to_string(i / nchans) + ", channel " +
+                                            std::to_string(i % nchans));
+        }
+    }
+    work.resize(ns * nchans);
+#pragma omp parallel for schedule(static)
+    for (SizeType t = 0; t < ns; ++t)
+        for (SizeType c = 0; c < nchans; ++c)
+            work[t * nchans + c] = unpack_sample(block + t * row_bytes, c, layout.dtype);
+    nsamps = ns;
+    if (!measure_all()) return;
+    apply_bytes(block);
+    ++block_index;
+}
+
+void Cleaner::apply_bytes(uint8_t* block) {
+    // lib/cpu/rfi_cpu.cpp:526-720 with the block's own codec: written cells
+    // are quantised and packed in place; the work copy keeps the input
+    // values the cell states are decided from.
+    const SizeType n = nchans;
+    const int dt     = layout.dtype;
+    const bool kchan = cfg.kadane_channel_enabled, kspec = cfg.kadane_spectrum_enabled;
+    const bool nbr   = cfg.fill == FA_FILL_NEIGHBOUR;
+    auto prow        = [&](SizeType t) { return block + t * row_bytes; };
+    auto put         = [&](uint8_t* out, SizeType c, float v) {
+        write_sample(out, c, dt, quantise_sample(v, dt));
+    };
+    uint64_t cn = 0, cks = 0, ckc = 0, fb = 0;
+    // Unflagged rows first: each reads and writes only itself.
+#pragma omp parallel reduction(+ : cn, cks, ckc, fb)
+    {
+    std::vector<float> z(n), v(n);
+    std::vector<uint8_t> state(n);
+#pragma omp for schedule(static)
+    for (SizeType t = 0; t < nsamps; ++t) {
+        if (time_flags[t]) continue;
+        const SizeType w = window_of(t);
+        const float* lc  = loc.data() + w * n;
+        const float* ic  = inv_scale.data() + w * n;
+        const float* x   = row(t);
+        uint8_t* out     = prow(t);
+        for (SizeType c = 0; c < n; ++c) {
+            if (mask[c] == 0) {
+                state[c] = kMasked;
+                continue;
+            }
+            if (ic[c] == 0.0F) {
+                state[c] = kUnmeasured;
+                continue;
+            }
+            z[c] = (x[c] - lc[c]) * ic[c];
+            if (narrow_active && std::abs(z[c]) > clip)
+                state[c] = kNarrow;
+            else if (kspec && (kspec_pos[t].contains(c) || kspec_neg[t].contains(c)))
+                state[c] = kKadaneSpectrum;
+            else if (kchan && (kchan_pos[c].contains(t) || kchan_neg[c].contains(t)))
+                state[c] = kKadaneChannel;
+            else {
+                state[c] = kClean;
+                v[c]     = z[c] - zero_dm_term(c, t);
+            }
+        }
+        for (SizeType c = 0; c < n; ++c) {
+            switch (state[c]) {
+            case kClean:
+                if (transform) put(out, c, level(w, c, v[c]));
+                continue;
+            case kUnmeasured: continue;
+            case kMasked: put(out, c, fill_value(kStreamMasked, w, t, c)); continue;
+            case kNarrow: ++cn; break;
+            case kKadaneSpectrum: ++cks; break;
+            default: ++ckc; break;
+            }
+            if (!nbr) {
+                put(out, c, fill_value(kStreamNoise, w, t, c));
+                continue;
+            }
+            const SizeType r  = chan_fill_radius;
+            const SizeType lo = c > r ? c - r : 0;
+            const SizeType hi = std::min(c + r, n - 1);
+            float value = 0.0F;
+            bool found  = false;
+            for (int a = 0; a < kMaxDraws && !found; ++a) {
+                const SizeType src = uniform_index(draw(kStreamCell, t, c, a), lo, hi);
+                if (state[src] == kClean) {
+                    value = v[src];
+                    found = true;
+                }
+            }
+            fb += found ? 0 : 1;
+            put(out, c, level(w, c, value));
+        }
+    }
+    } // omp parallel
+    // Then flagged rows, which copy from the now-final unflagged rows.
+    std::vector<SizeType> flagged;
+    for (SizeType t = 0; t < nsamps; ++t)
+        if (time_flags[t]) flagged.push_back(t);
+    const bool rezero = cfg.zero_dm != FA_ZERO_DM_OFF;
+#pragma omp parallel reduction(+ : fb)
+    {
+    std::vector<float> v(n);
+#pragma omp for schedule(static)
+    for (std::size_t ft = 0; ft < flagged.size(); ++ft) {
+        const SizeType t  = flagged[ft];
+        const SizeType w  = window_of(t);
+        const SizeType r  = time_fill_radius;
+        const SizeType lo = t > r ? t - r : 0;
+        const SizeType hi = std::min(t + r, nsamps - 1);
+        uint8_t* out      = prow(t);
+        double sum = 0.0, count = 0.0;
+        for (SizeType c = 0; c < n; ++c) {
+            if (mask[c] == 0) {
+                put(out, c, fill_value(kStreamMasked, w, t, c));
+                continue;
+            }
+            v[c] = 0.0F;
+            if (!nbr) {
+                v[c] = cfg.fill == FA_FILL_NOISE
+                           ? normal_draw(draw(kStreamNoise, t, c, 0))
+                           : 0.0F;
+            } else {
+                bool found = false;
+                for (int a = 0; a < kMaxDraws && !found; ++a) {
+                    const SizeType src =
+                        uniform_index(draw(kStreamTime, t, c, a), lo, hi);
+                    if (time_flags[src]) continue;
+                    found = true;
+                    if (rezero)
+                        v[c] = to_noise_units(window_of(src), c, unpack_sample(prow(src), c, dt));
+                    else
+                        copy_sample(prow(src), out, c, dt);
+                }
+                if (!found) ++fb;
+                if (found && !rezero) continue;
+            }
+            if (rezero && inv_scale[w * n + c] != 0.0F) {
+                sum += v[c];
+                count += 1.0;
+            }
+            if (!rezero) put(out, c, level(w, c, v[c]));
+        }
+        if (!rezero) continue;
+        const float mean = (float)(count > 0.0 ? sum / count : 0.0);
+        // The rounding error carried across channels keeps the band sum
+        // within half a count (lib/cpu/rfi_cpu.cpp:700-711).
+        float carry = 0.0F;
+        for (SizeType c = 0; c < n; ++c) {
+            if (mask[c] == 0) continue;
+            const float want = level(w, c, v[c] - mean) + carry;
+            const float q    = quantise_sample(want, dt);
+            carry            = std::clamp(want - q, -0.5F, 0.5F);
+            write_sample(out, c, dt, q);
+        }
+    }
+    } // omp parallel
+    report.cells_narrow          = cn;
+    report.cells_kadane_spectrum = cks;
+    report.cells_kadane_channel  = ckc;
+    report.fill_fallbacks        = fb;
 }
 
 void Cleaner::measure_bandpass() {
