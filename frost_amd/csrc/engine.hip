@@ -613,6 +613,263 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
     }
 }
 
+// Node form of the first launch when level 0 is formed from the input: one
+// block holds a whole sub-band's rows (up to kNodeChans channels, every
+// level-0 row they read and all output rows over them) over a run of short
+// chunks, so each channel window is loaded and folded once per chunk for the
+// whole sub-band. Waves 0-7 (producers) own two channels each: they load the
+// channel windows of chunk c + 1 while folding chunk c into its level-0 rows,
+// x(t) + x(t-1) + ... + x(t-D) in that order, four grid positions per lane
+// in registers, and write each row's slot in one half of a double-buffered
+// LDS area as aligned quads (zero outside the row's samples
+// [t0 - hc, t0 + T)). Waves 8-15 (consumers) sum each output row's K rows
+// from the other half in tree order, one sample per lane, and store it.
+// Producers only load and consumers only store, so neither waits on the
+// other's memory traffic; one barrier per chunk hands the halves over.
+constexpr int kNodeChunk   = 64;   // samples per chunk: one per lane
+constexpr int kNodeChans   = 16;   // channels per block: two per producer wave
+constexpr int kNodeGrid    = 256;  // grid positions per channel (chunk + shift spread): four per lane
+constexpr int kNodeWin     = kNodeGrid + kX0Dmax; // window floats per channel
+constexpr int kNodeRows    = 256;  // output rows per block
+constexpr int kNodeThreads = 1024; // 8 producer and 8 consumer waves
+
+template <int K, bool RING>
+__global__ void __launch_bounds__(kNodeThreads)
+fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks,
+                 const GatherLeaf* __restrict__ leaves, const int32_t* __restrict__ rows,
+                 const GatherChan* __restrict__ chans, GatherDims g) {
+    // [row slots, zero chunk: g.half] x 2 halves, [windows: kNodeChans x kNodeWin]
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const uint32_t blk  = blockIdx.x % (uint32_t)g.nblocks;
+    const uint32_t grp  = blockIdx.x / (uint32_t)g.nblocks;
+    const int tau_first = g.tau_origin + (int)grp * (kNodeChunk * g.chunks);
+    const GatherBlock B = blocks[blk];
+    const int lo_c      = max(0, (-B.hmax - tau_first) / kNodeChunk);
+    const int hi_c      = min(g.chunks, (g.tile_len - tau_first + kNodeChunk - 1) / kNodeChunk);
+    if (tau_first + kNodeChunk * g.chunks <= -B.hmax || lo_c >= hi_c) return; // uniform
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    if (g.zero_slot >= 0 && threadIdx.x < kNodeChunk) {
+        lds[g.zero_slot + threadIdx.x]          = 0.0f;
+        lds[g.half + g.zero_slot + threadIdx.x] = 0.0f;
+    }
+    if (wave < 8) {
+        // Producer. Window of channel i for chunk c: element e is tile
+        // sample tau0 - xs - dmax + e, e < (grid length) + dmax, stored at
+        // window index kX0Dmax - dmax + e, so that grid position p (tile
+        // sample tau0 - xs + p) minus delay d sits at index p + kX0Dmax - d
+        // whatever the channel's dmax. Lane l loads e = l + 64 m.
+        constexpr int XM = (kNodeWin + 63) / 64;
+        float* xwin      = lds + 2 * g.half;
+        struct Chan {
+            int64_t xoff;
+            int xs, dmax, xmin;
+            uint32_t dmask;
+        } cs[2];
+        // Row k of channel s, in lane k: vbase = LDS float index of grid
+        // position 0 (quad-aligned), vq = its quads of grid positions
+        // [qlo, qhi) as qlo << 16 | qhi, vhc = its computed look-back.
+        int vbase[2], vq[2], vhc[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const GatherChan ch = chans[B.chan_first + min(wave + 8 * s, B.nchan - 1)];
+            cs[s]               = Chan{ch.xoff, ch.xs, ch.dmax, ch.xmin, ch.dmask};
+            const GatherLeaf lf = leaves[ch.leaf_first + min(lane, ch.nleaf - 1)];
+            const int plo       = ch.xs - lf.shmax;
+            vbase[s]            = lf.slot - plo;
+            vq[s]               = (plo & ~3) << 16 | ((plo + lf.len + 3) & ~3);
+            vhc[s]              = lf.hc;
+        }
+        float xa[2][XM], xb[2][XM]; // windows of two chunks in flight
+        // A fixed number of loads per chunk (channel records clamped, every
+        // lane loading), so waiting for one chunk leaves the next in flight.
+        auto xload = [&](float (&xv)[2][XM], int c) {
+            const int tau0 = tau_first + c * kNodeChunk;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const Chan& ch  = cs[s];
+                const int tw    = tau0 - ch.xs - ch.dmax;
+                const float* xr = g.x + ch.xoff;
+#pragma unroll
+                for (int m = 0; m < XM; ++m) xv[s][m] = xr[min(max(tw + lane + 64 * m, g.xlo), g.xhi)];
+            }
+        };
+        auto xstore = [&](const float (&xv)[2][XM]) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int i = wave + 8 * s;
+                if (i < B.nchan) {
+                    const Chan& ch = cs[s];
+                    const int wl   = kNodeChunk + ch.xs - ch.xmin + ch.dmax;
+                    float* xw      = xwin + i * kNodeWin + kX0Dmax - ch.dmax;
+#pragma unroll
+                    for (int m = 0; m < XM; ++m)
+                        if (lane + 64 * m < wl) xw[lane + 64 * m] = xv[s][m];
+                }
+            }
+        };
+        auto fold = [&](int c) {
+            const int tau0 = tau_first + c * kNodeChunk;
+            float* half    = lds + (c & 1) * g.half;
+#pragma unroll 1
+            for (int s = 0; s < 2; ++s) {
+                const int i = wave + 8 * s;
+                if (i >= B.nchan) break;
+                const Chan ch   = s ? cs[1] : cs[0];
+                const int lbase = s ? vbase[1] : vbase[0];
+                const int lq    = s ? vq[1] : vq[0];
+                const int lhc   = s ? vhc[1] : vhc[0];
+                // Lane l: grid positions 4l .. 4l + 3 (past the grid: nothing).
+                const int gl = kNodeChunk + ch.xs - ch.xmin;
+                if (4 * lane >= gl) continue;
+                float xr[kX0Dmax + 4]; // window indices 4l .. 4l + kX0Dmax + 3
+                const f4* xq = reinterpret_cast<const f4*>(xwin + i * kNodeWin) + lane;
+#pragma unroll
+                for (int q = 0; q < (kX0Dmax + 4) / 4; ++q) {
+                    const f4 v = xq[q];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) xr[4 * q + k] = v[k];
+                }
+                float acc[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc[k] = xr[kX0Dmax + k];
+                const int tq = tau0 - ch.xs; // tile sample of grid position 0
+                int li       = 0;
+                auto emit    = [&]() {
+                    const int base = __builtin_amdgcn_readlane(lbase, li);
+                    const int q    = __builtin_amdgcn_readlane(lq, li);
+                    const int hc   = __builtin_amdgcn_readlane(lhc, li);
+                    ++li;
+                    const int qlo = q >> 16, qhi = q & 0xffff;
+                    if (4 * lane >= qlo && 4 * lane < qhi) {
+                        f4 v;
+                        // Only quads at the row's ends of samples need zeros.
+                        if (tq + qlo >= -hc && tq + qhi <= g.tile_len) { // uniform
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) v[k] = acc[k];
+                        } else {
+                            const int t0 = tq + 4 * lane;
+#pragma unroll
+                            for (int k = 0; k < 4; ++k)
+                                v[k] = (t0 + k >= -hc && t0 + k < g.tile_len) ? acc[k] : 0.0f;
+                        }
+                        static_cast<f4*>(__builtin_assume_aligned(half + base, 16))[lane] = v;
+                    }
+                };
+                if (ch.dmask & 1U) emit();
+                // Every step taken (past dmax the sums are unused), so every
+                // xr index is a constant register.
+#pragma unroll
+                for (int d = 1; d <= kX0Dmax; ++d) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) acc[k] = acc[k] + xr[kX0Dmax + k - d];
+                    if ((ch.dmask >> d) & 1U) emit();
+                }
+            }
+        };
+        // Chunk c: its windows (loaded two chunks earlier) are stored to
+        // this wave's LDS windows (read back by this wave only), chunk c + 2
+        // is requested into the freed registers, then c is folded.
+        auto step = [&](float (&xv)[2][XM], int c) {
+            xstore(xv);
+            wave_sync();
+            if (c + 2 < hi_c) xload(xv, c + 2);
+            fold(c);
+            wave_sync(); // window reads done before the next store
+            lds_barrier();
+        };
+        xload(xa, lo_c);
+        if (lo_c + 1 < hi_c) xload(xb, lo_c + 1);
+        for (int c = lo_c; c < hi_c; c += 2) {
+            step(xa, c);
+            if (c + 1 < hi_c) step(xb, c + 1);
+        }
+    } else {
+        // Consumer: wave 8 + q sums rows 4 (q + 8 k) + (lane / 16): each
+        // quarter-wave one row, lane l the samples l % 16 + 16 s, s < 4, of
+        // the chunk (four LDS reads per address, each quarter-wave reading
+        // 16 consecutive floats). Row record in LDS (copied
+        // at block start): [K 16-bit slot offsets packed two per word, out
+        // lo, out hi, h, ring length]; carried rows also keep the ring
+        // position of tile sample 0.
+        constexpr int RW  = K / 2;            // packed offset words
+        constexpr int RS  = RW + 4;           // record words
+        constexpr int RSL = (RS + 3) & ~3;    // LDS record stride (whole quads)
+        int* srec         = reinterpret_cast<int*>(lds + 2 * g.half + kNodeChans * kNodeWin);
+        int* sq0          = srec + kNodeRows * RSL;
+        const int q       = wave - 8;
+        for (int i = threadIdx.x - 512; i < B.nrows * RS; i += 512) {
+            const int r = i / RS, f = i % RS;
+            // Global record: [out lo, out hi, h, ring length, offsets].
+            srec[r * RSL + f] = rows[(int64_t)(B.row_first + r) * RS + (f < RW ? f + 4 : f - RW)];
+        }
+        if constexpr (RING) {
+            const uint64_t t0abs = *g.gulp_abs_p + static_cast<uint64_t>(g.tile_t0);
+            for (int r = threadIdx.x - 512; r < B.nrows; r += 512)
+                sq0[r] = ring_of(t0abs, rows[(int64_t)(B.row_first + r) * RS + 3]);
+        }
+        const int sub = lane & 15;
+        for (int c = lo_c; c < hi_c; ++c) {
+            lds_barrier();
+            const float* half = lds + (c & 1) * g.half + sub;
+            const int t       = tau_first + c * kNodeChunk + sub; // tile sample of this lane's first
+            for (int k = 0; 4 * (q + 8 * k) < B.nrows; ++k) {
+                const int rr = 4 * (q + 8 * k) + (lane >> 4);
+                if (rr >= B.nrows) continue;
+                int rec[RSL];
+                {
+                    const int4* rp = reinterpret_cast<const int4*>(srec + rr * RSL);
+#pragma unroll
+                    for (int v = 0; v < RSL / 4; ++v) {
+                        const int4 x = rp[v];
+                        rec[4 * v] = x.x, rec[4 * v + 1] = x.y, rec[4 * v + 2] = x.z, rec[4 * v + 3] = x.w;
+                    }
+                }
+                float w[4][K];
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const float* p = half + ((uint32_t)rec[j / 2] >> (16 * (j & 1)) & 0xffffU);
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) w[s4][j] = p[16 * s4];
+                }
+                float v[4];
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4) v[s4] = tree_sum<K>(w[s4]);
+                const int64_t o = (int64_t)(uint32_t)rec[RW] | ((int64_t)rec[RW + 1] << 32);
+                if constexpr (RING) {
+                    // Ring row: tile samples [0, tile) only (h is 0).
+                    const int R  = rec[RW + 3];
+                    const int q0 = sq0[rr];
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4)
+                        if (t + 16 * s4 < g.tile_len)
+                            __builtin_nontemporal_store(v[s4], out + o + wrap_pos(q0 + t + 16 * s4, R));
+                } else {
+                    const int h = rec[RW + 2];
+                    float* dst  = out + o + t;
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4)
+                        if (t + 16 * s4 >= -h && t + 16 * s4 < g.tile_len)
+                            __builtin_nontemporal_store(v[s4], dst + 16 * s4);
+                }
+            }
+        }
+    }
+}
+
+template <int K, bool RING>
+void launch_node(const MergeLaunch& ml, float* out, const GatherDims& g, int tl, hipStream_t s) {
+    const unsigned ngrp   = ceil_div(tl - g.tau_origin, kNodeChunk * g.chunks);
+    const uint64_t blocks = static_cast<uint64_t>(ngrp) * ml.nblocks;
+    if (blocks * kNodeThreads >= (uint64_t{1} << 32))
+        throw std::invalid_argument("FDMT merge grid too large; reduce the time tile");
+    const size_t lds = (2 * static_cast<size_t>(g.half) + kNodeChans * kNodeWin +
+                        kNodeRows * (((K / 2 + 4 + 3) & ~3) + 1)) * sizeof(float);
+    fdmt_node_kernel<K, RING><<<static_cast<unsigned>(blocks), kNodeThreads, lds, s>>>(
+        out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), ml.chans.get(), g);
+}
+
 template <int K, int UMAX, bool X0, bool RING>
 void launch_gather_u(const MergeLaunch& ml, const float* in, float* out, const GatherDims& g,
                      int tl, hipStream_t s) {
@@ -634,6 +891,16 @@ int default_gather_chunks() {
         const char* v = std::getenv("FROST_AMD_GATHER_CHUNKS");
         const int c   = v ? std::atoi(v) : 16;
         return (c >= 1 && c <= 64) ? c : 16;
+    }();
+    return n;
+}
+
+// Chunks per node-form block (FROST_AMD_NODE_CHUNKS, default 32: 2048 samples).
+int node_chunks() {
+    static const int n = [] {
+        const char* v = std::getenv("FROST_AMD_NODE_CHUNKS");
+        const int c   = v ? std::atoi(v) : 32;
+        return (c >= 1 && c <= 256) ? c : 32;
     }();
     return n;
 }
@@ -676,6 +943,11 @@ void launch_gather(const MergeLaunch& ml, const float* in, float* out, int tl, c
     g.x          = x0;
     g.xlo        = xlo;
     g.xhi        = xhi;
+    if (ml.node) {
+        g.chunks = node_chunks();
+        if (ring) return launch_node<K, true>(ml, out, g, tl, s);
+        return launch_node<K, false>(ml, out, g, tl, s);
+    }
     if (ring) return launch_gather_r<K, true>(ml, in, out, g, tl, s);
     return launch_gather_r<K, false>(ml, in, out, g, tl, s);
 }
@@ -1705,13 +1977,22 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     // Tuning: FROST_AMD_FUSE (levels per launch, 1-4), FROST_AMD_MERGE=direct
     // (gather from global memory instead of LDS staging), FROST_AMD_GATHER_LDS
     // (LDS floats per staged block), FROST_AMD_INIT=fused (form level 0 in
-    // the first gather launch instead of storing it with the init kernel).
+    // the first gather launch instead of storing it with the init kernel:
+    // whole sub-bands per block; fused-runs: the row runs of the other
+    // launches), FROST_AMD_NODE_LDS (LDS floats of a sub-band block's rows).
     if (const char* fv = std::getenv("FROST_AMD_FUSE")) {
         const long f = std::atol(fv);
         if (f >= 1 && f <= 4) max_fuse_ = static_cast<uint32_t>(f);
     }
     if (const char* mv = std::getenv("FROST_AMD_MERGE")) staged_merge_ = std::string(mv) != "direct";
-    if (const char* iv = std::getenv("FROST_AMD_INIT")) x0_fused_ = std::string(iv) == "fused";
+    if (const char* iv = std::getenv("FROST_AMD_INIT")) {
+        x0_fused_ = std::string(iv) == "fused" || std::string(iv) == "fused-runs";
+        x0_node_  = std::string(iv) != "fused-runs";
+    }
+    if (const char* nv = std::getenv("FROST_AMD_NODE_LDS")) {
+        const long f = std::atol(nv);
+        if (f >= 2048 && f <= 16000) node_lds_ = static_cast<uint32_t>(f); // two halves fit 160 KB
+    }
     x0_fused_ = x0_fused_ && staged_merge_;
     if (!staged_merge_) max_fuse_ = std::min<uint32_t>(max_fuse_, 3);
     if (const char* gv = std::getenv("FROST_AMD_GATHER_LDS")) {
@@ -2322,9 +2603,13 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
     // channel's rows must fit one grid (shift range + chunk within kX0Grid,
     // delays within the window).
     const bool x0         = ml.from_input;
+    // Node form: a block's rows span whole sub-bands over short chunks.
+    const bool node       = x0 && x0_node_;
+    const int CL          = node ? kNodeChunk : kChunkG; // samples per staged chunk
     const auto& l0        = plan_.levels[0];
-    const std::size_t max_chans = 16; // four channel windows per producer wave
+    const std::size_t max_chans = node ? kNodeChans : 16; // four channel windows per producer wave
     if (x0) ml.max_leaves = static_cast<uint32_t>(max_chans);
+    ml.node = node;
     auto x0_ok = [&](const std::vector<Lf>& v) {
         std::vector<std::array<int32_t, 4>> ch; // channel, min shmin, max shmax, max delay
         for (const auto& x : v) {
@@ -2340,20 +2625,38 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
             if (!found) ch.push_back({c, x.shmin, x.shmax, d});
         }
         if (ch.size() > max_chans) return false;
-        for (const auto& e : ch)
-            if (e[2] - e[1] + kChunkG > kX0Grid || e[3] > kX0Dmax) return false;
+        for (const auto& e : ch) {
+            if (e[3] > kX0Dmax) return false;
+            if (node ? e[2] - e[1] + kNodeChunk > kNodeGrid : e[2] - e[1] + kChunkG > kX0Grid)
+                return false;
+        }
         return true;
     };
     auto need = [&](const std::vector<Lf>& v, bool& ok) {
         uint64_t n = 0;
         ok         = true;
         for (const auto& x : v) {
-            n += kChunkG + (x.shmax - x.shmin);
+            n += CL + (x.shmax - x.shmin) + (node ? 6 : 0); // node form: quad padding
             ok = ok && (x.shmax - x.shmin) <= kSpreadG;
         }
         if (x0) ok = ok && x0_ok(v);
         return n;
     };
+    // Record order: the launch's rows, except that the node form groups the
+    // rows of each sub-band (the rows whose lowest channel is the same; tree
+    // and staggered sub-bands interleave in the level's order). A record
+    // holds its row's output offset, so any order stores the same level.
+    std::vector<uint32_t> ord(ml.nrows);
+    for (uint32_t r = 0; r < ml.nrows; ++r) ord[r] = r;
+    if (node) {
+        std::vector<int32_t> key(ml.nrows, std::numeric_limits<int32_t>::max());
+        for (uint32_t r = 0; r < ml.nrows; ++r)
+            for (int k = 0; k < K; ++k) {
+                const int32_t c = rl[static_cast<std::size_t>(r) * K + k].first;
+                if (c != zero_row) key[r] = std::min(key[r], l0.chan_or_lo[c]);
+            }
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t p, uint32_t q) { return key[p] < key[q]; });
+    }
     std::vector<GatherChan> gc;
     // Sub-band (node) of each input row, its largest delay, and a union-find
     // over nodes that share a block (used for the dispatch order below).
@@ -2384,7 +2687,7 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
         GatherBlock b{};
         b.leaf_first = static_cast<int32_t>(gl.size());
         b.nleaves    = static_cast<int32_t>(cur.size());
-        if (x0 && cur.size() > static_cast<std::size_t>(kX0LeafMax))
+        if (x0 && !node && cur.size() > static_cast<std::size_t>(kX0LeafMax))
             throw std::logic_error("level-0 run holds more input rows than the kernel stages");
         b.row_first  = static_cast<int32_t>(first);
         b.nrows      = static_cast<int32_t>(end - first);
@@ -2398,25 +2701,43 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
             });
         std::vector<int32_t> slot(cur.size());
         int32_t off = 0;
+        // Node form: each channel's largest shift (its grid origin); a row's
+        // slot is padded to whole quads of grid positions, quad-aligned.
+        std::vector<int32_t> cxs;
+        if (node) {
+            cxs.assign(plan_.nchans_fdmt + 1, std::numeric_limits<int32_t>::min());
+            for (const auto& x : cur) {
+                int32_t& v = cxs[l0.chan_or_lo[x.c]];
+                v          = std::max(v, x.shmax);
+            }
+        }
         for (std::size_t i = 0; i < cur.size(); ++i) {
             const auto& x = cur[i];
             GatherLeaf f{};
             // Linear rows: offset of sample t0 - shmax; rings: the ring start.
             f.src   = carried_ ? base_a[x.c] : base_a[x.c] + hist_a[x.c] - x.shmax;
             f.slot  = off;
-            f.len   = kChunkG + (x.shmax - x.shmin);
+            f.len   = CL + (x.shmax - x.shmin);
             f.shmax = x.shmax;
             f.hc    = hist_a[x.c];
             f.delay = x0 ? l0.delay[x.c] : 0;
             f.ring  = carried_ && !x0 ? ring_a[x.c] : 0;
-            slot[i] = off;
-            off += f.len;
+            int32_t area = f.len;
+            if (node) {
+                const int32_t plo = cxs[l0.chan_or_lo[x.c]] - x.shmax; // grid position of element 0
+                const int32_t alo = plo & ~3, ahi = (plo + f.len + 3) & ~3;
+                f.slot            = off + (plo - alo);
+                area              = ahi - alo;
+            }
+            slot[i] = f.slot;
+            off += area;
             if (x0) {
                 const int32_t c = l0.chan_or_lo[x.c];
                 if (b.nchan == 0 || gc.back().xoff != static_cast<int64_t>(c) * xstride_) {
                     GatherChan g{};
                     g.xoff       = static_cast<int64_t>(c) * xstride_;
                     g.xs         = x.shmax;
+                    g.xmin       = x.shmin;
                     g.dmax       = f.delay;
                     g.leaf_first = static_cast<int32_t>(gl.size());
                     g.nleaf      = 0;
@@ -2425,6 +2746,7 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
                 }
                 auto& g = gc.back();
                 g.xs    = std::max(g.xs, x.shmax);
+                g.xmin  = std::min(g.xmin, x.shmin);
                 g.dmax  = std::max(g.dmax, f.delay);
                 g.dmask |= 1U << f.delay;
                 ++g.nleaf;
@@ -2434,7 +2756,7 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
         lds_max = std::max<uint32_t>(lds_max, static_cast<uint32_t>(off));
         for (uint32_t r = first; r < end; ++r)
             for (int k = 0; k < K; ++k) {
-                const auto [c, sh] = rl[static_cast<std::size_t>(r) * K + k];
+                const auto [c, sh] = rl[static_cast<std::size_t>(ord[r]) * K + k];
                 if (c == zero_row) {
                     rec[r * RS + 4 + k] = kZeroLeaf;
                     continue;
@@ -2446,15 +2768,16 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
         gb.push_back(b);
     };
     for (uint32_t r = 0; r < ml.nrows; ++r) {
-        const int32_t h    = orow[r].h;
-        const int64_t o    = orow[r].out;
+        const std::size_t sr = ord[r]; // the row whose record is r
+        const int32_t h    = orow[sr].h;
+        const int64_t o    = orow[sr].out;
         rec[r * RS + 0]    = static_cast<int32_t>(static_cast<uint32_t>(o & 0xffffffff));
         rec[r * RS + 1]    = static_cast<int32_t>(o >> 32);
         rec[r * RS + 2]    = h;
-        rec[r * RS + 3]    = orow[r].ring;
+        rec[r * RS + 3]    = orow[sr].ring;
         tent               = cur;
         for (int k = 0; k < K; ++k) {
-            const auto [c, sh32] = rl[static_cast<std::size_t>(r) * K + k];
+            const auto [c, sh32] = rl[sr * K + k];
             const int64_t sh     = sh32;
             if (sh < 0 || sh > std::numeric_limits<int32_t>::max() / 2 || h + sh > hist_a[c])
                 throw std::logic_error("FDMT gather shift out of range");
@@ -2470,14 +2793,15 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
         }
         bool ok          = true;
         const uint64_t n = need(tent, ok);
-        if (r > first && (!ok || n > gather_lds_ || (!x0 && tent.size() > max_leaves) ||
-                          r - first >= static_cast<uint32_t>(kGatherRowsMax))) {
+        if (r > first && (!ok || n > (node ? node_lds_ : gather_lds_) ||
+                          (!x0 && tent.size() > max_leaves) ||
+                          r - first >= static_cast<uint32_t>(node ? kNodeRows : kGatherRowsMax))) {
             close(r);
             first = r;
             hmax  = 0;
             cur.clear();
             for (int k = 0; k < K; ++k) {
-                const auto [c, sh] = rl[static_cast<std::size_t>(r) * K + k];
+                const auto [c, sh] = rl[sr * K + k];
                 if (c == zero_row) continue;
                 bool found         = false;
                 for (auto& x : cur)
@@ -2522,12 +2846,26 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
     if (zero_row >= 0) {
         // One chunk of zeros after the largest block's staged rows.
         ml.zero_slot = static_cast<int32_t>((lds_max + 63) & ~63U);
-        ml.lds_floats = static_cast<uint32_t>(ml.zero_slot) + kChunkG;
+        ml.lds_floats = static_cast<uint32_t>(ml.zero_slot) + CL;
         for (std::size_t r = 0; r < ml.nrows; ++r)
             for (int k = 0; k < K; ++k) {
                 int32_t& v = rec[r * RS + 4 + static_cast<std::size_t>(k)];
                 if (v == kZeroLeaf) v = ml.zero_slot;
             }
+    }
+    if (node) {
+        // Node form records: the K offsets as 16-bit halves, two per word.
+        const std::size_t RN = 4 + static_cast<std::size_t>(K) / 2;
+        std::vector<int32_t> packed(static_cast<std::size_t>(ml.nrows) * RN);
+        for (std::size_t r = 0; r < ml.nrows; ++r) {
+            for (std::size_t j = 0; j < 4; ++j) packed[r * RN + j] = rec[r * RS + j];
+            for (int k = 0; k < K; ++k) {
+                const int32_t v = rec[r * RS + 4 + static_cast<std::size_t>(k)];
+                if (v < 0 || v > 0xffff) throw std::logic_error("node-form LDS offset out of range");
+                packed[r * RN + 4 + k / 2] |= static_cast<int32_t>(static_cast<uint32_t>(v) << (16 * (k & 1)));
+            }
+        }
+        rec.swap(packed);
     }
     ml.blocks.resize(gb.size());
     ml.leaves.resize(gl.size());

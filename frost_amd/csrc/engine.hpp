@@ -97,7 +97,7 @@ struct GatherChan {
     int32_t leaf_first; // the rows: leaves[leaf_first .. + nleaf), delay ascending
     int32_t nleaf;
     uint32_t dmask;     // bit D set: a row with delay D is among them
-    int32_t pad;
+    int32_t xmin;       // smallest shmin of the channel's input rows (node form)
 };
 
 // One launch computes levels in_level+1 .. out_level (1 to 3 tree levels):
@@ -120,6 +120,7 @@ struct MergeLaunch {
     uint32_t max_leaves = 0; // input rows per block bound (16, 32, 48 or 64)
     int32_t zero_slot   = -1; // LDS floats offset of a chunk of zeros (zero-channel leaves), or -1
     bool from_input     = false; // in_level 0 is not stored: producers form it from x
+    bool node           = false; // from_input in the node form (whole sub-band per block)
     DeviceBuffer<GatherBlock> blocks;
     DeviceBuffer<GatherLeaf> leaves;
     DeviceBuffer<GatherChan> chans;
@@ -285,6 +286,8 @@ private:
     bool staged_merge_   = true;
     bool x0_fused_       = false; // level 0 formed inside the first gather launch
     uint32_t gather_lds_ = 8192; // LDS floats per staged row block (one half)
+    bool x0_node_        = true;  // fused level 0 in the node form when its blocks fit
+    uint32_t node_lds_   = 16000; // node form: LDS floats of one block's input rows
     // Staged program of a launch: per output row its offset (sample t0, or
     // ring start), computed look-back h, ring length (0: linear) and its K
     // descendants (input row, summed shift) in tree order.

@@ -138,7 +138,8 @@ def test_carried_equals_recomputed_at_production_shape(stream):
     256, gulps of 2^17: the carried form (tile 4096, so the highest levels'
     rings hold their look-back of up to ~19000 samples across tiles) emits
     the same candidate rows, drop counts and frontier as the recomputed
-    form, gulp by gulp."""
+    form, gulp by gulp; so does the carried form with level 0 formed from the
+    input inside the first launch (whole 16-channel sub-bands per block)."""
     say("test_carried_equals_recomputed_at_production_shape")
     nch, N = 4096, 1 << 17
     df = -300.0 / nch
@@ -147,29 +148,36 @@ def test_carried_equals_recomputed_at_production_shape(stream):
     bcfg = abi.boxcar_config(max_boxcar=256, detect_thresh=6.0)
     plan = search.make_fdmt_plan(dcfg)
     engines = {}
-    for st, tile in (("recompute", None), ("carried", 4096)):
+    for st, tile, init in (("recompute", None, "init"), ("carried", 4096, "init"),
+                           ("carried", 4096, "fused")):
         os.environ["FROST_AMD_FDMT_STATE"] = st
+        os.environ["FROST_AMD_INIT"] = init
         if tile:
             os.environ["FROST_AMD_TILE"] = str(tile)
         try:
-            engines[st] = search.FDMTBoxcar(plan, bcfg)
+            engines[st, init] = search.FDMTBoxcar(plan, bcfg)
         finally:
             os.environ.pop("FROST_AMD_FDMT_STATE", None)
             os.environ.pop("FROST_AMD_TILE", None)
-        say(f"  {st}: {engines[st].state()}")
-    assert engines["carried"].carried() and not engines["recompute"].carried()
+            os.environ.pop("FROST_AMD_INIT", None)
+        say(f"  {st} {init}: {engines[st, init].state()}")
+    ref = engines["recompute", "init"]
+    assert engines["carried", "init"].carried() and not ref.carried()
+    assert engines["carried", "fused"].level_sources()[0] == -2  # level 0 formed from the input
     rng = np.random.default_rng(2024)
     total = 0
     for k in range(3):
         wf = pulse_waterfall(rng, nch, N, [(300.0 + 600 * k, 20000 + 30000 * k, 6, 0.6),
                                            (1500.0, 100000, 2, 1.2)], dcfg)
-        a = engines["recompute"].execute(wf, stream)
-        b = engines["carried"].execute(wf, stream)
+        a = ref.execute(wf, stream)
         say(f"  gulp {k}: {len(a[0])} rows")
-        assert a == b, k
+        for key in (("carried", "init"), ("carried", "fused")):
+            assert engines[key].execute(wf, stream) == a, (k, key)
         total += len(a[0])
     assert total > 0
-    assert engines["recompute"].drain(stream) == engines["carried"].drain(stream)
+    d = ref.drain(stream)
+    for key in (("carried", "init"), ("carried", "fused")):
+        assert engines[key].drain(stream) == d, key
 
 
 def test_carried_reduced_large_beam(oracle, monkeypatch):

@@ -28,7 +28,8 @@ def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None
     result lists (per gulp + drain) and, if keep, the plane rows. `fuse` caps
     the FDMT levels summed per launch (default 4); merge="direct" gathers
     from global memory instead of staging through LDS; init="fused" forms
-    level 0 from the input inside the first launch instead of storing it;
+    level 0 from the input inside the first launch instead of storing it
+    (whole sub-bands per block; "fused-runs": in the other launches' runs);
     state="carried" keeps the stored levels in per-row rings across tiles
     and gulps instead of recomputing each tile's look-back."""
     env = {"FROST_AMD_TILE": tile, "FROST_AMD_SEARCH": search_path, "FROST_AMD_FUSE": fuse,
@@ -117,7 +118,8 @@ def test_engine_single_gulp_small(oracle, stream):
     (0, None, None, None, None), (1, None, None, None, None), (2, None, None, None, None),
     (2, "ring", None, None, None), (2, None, 1, None, None), (2, None, 2, None, None),
     (2, None, 3, None, None), (2, None, 3, "direct", None), (1, None, 1, "direct", None),
-    (2, None, None, None, "fused"), (2, None, 2, None, "fused"), (0, None, 1, None, "fused")])
+    (2, None, None, None, "fused"), (2, None, 2, None, "fused"), (0, None, 1, None, "fused"),
+    (2, None, None, None, "fused-runs"), (0, None, 1, None, "fused-runs")])
 def test_engine_gulps_and_planes(oracle, stream, depth, path, fuse, merge, init):
     say(f"test_engine_gulps_and_planes depth={depth} search={path} fuse={fuse} merge={merge} "
         f"init={init}")
@@ -422,3 +424,42 @@ def test_full_size_rows_exact_on_integer_input(oracle, stream, monkeypatch):
             assert got == want, (p, r, len(got), len(want))
         del rows
     say("  checked", plan.summary()["nplanes"], "planes")
+
+
+def test_fused_level0_planes_equal_init_at_production_shape(stream):
+    """4096 channels, 1200-1500 MHz, 64 us, DM 0-2000, 11 planes: level 0
+    formed inside the first launch (whole 16-channel sub-bands per block,
+    level-0 windows of up to ~240 grid positions per channel) stores every
+    plane row bit for bit as the init kernel + stored level 0 does, and
+    emits the same candidates, over two gulps of 2^15."""
+    say("test_fused_level0_planes_equal_init_at_production_shape")
+    nch, N = 4096, 1 << 15
+    df = -300.0 / nch
+    dcfg = abi.dedisp_config(nch, 64e-6, 1500.0 + df / 2, df, 2000.0, block_nsamps=N,
+                             subband_depth=2)
+    bcfg = abi.boxcar_config(max_boxcar=256, detect_thresh=6.0)
+    plan = search.make_fdmt_plan(dcfg)
+    eng = {}
+    for init in ("init", "fused"):
+        os.environ["FROST_AMD_INIT"] = init
+        os.environ["FROST_AMD_FDMT_STATE"] = "recompute"
+        try:
+            eng[init] = search.FDMTBoxcar(plan, bcfg)
+        finally:
+            os.environ.pop("FROST_AMD_INIT", None)
+            os.environ.pop("FROST_AMD_FDMT_STATE", None)
+        eng[init].keep_planes(True)
+    assert eng["fused"].level_sources()[0] == -2 and eng["init"].level_sources()[0] == -1
+    rng = np.random.default_rng(77)
+    for k in range(2):
+        wf = pulse_waterfall(rng, nch, N, [(250.0 + 700 * k, 9000 + 9000 * k, 5, 0.7),
+                                           (1800.0, 30000, 2, 1.2)], dcfg)
+        a = eng["init"].execute(wf, stream)
+        b = eng["fused"].execute(wf, stream)
+        say(f"  gulp {k}: {len(a[0])} rows")
+        assert a == b, k
+        for p in range(plan.summary()["nplanes"]):
+            ra, rb = eng["init"].plane_rows(p, N), eng["fused"].plane_rows(p, N)
+            bad = np.argwhere(ra.view(np.uint32) != rb.view(np.uint32))
+            assert bad.size == 0, (k, p, bad[:4])
+    assert eng["init"].drain(stream) == eng["fused"].drain(stream)
