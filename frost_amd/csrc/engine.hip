@@ -667,20 +667,32 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
             int xs, dmax, xmin;
             uint32_t dmask;
         } cs[2];
-        // Row k of channel s, in lane k: vbase = LDS float index of grid
-        // position 0 (quad-aligned), vq = its quads of grid positions
-        // [qlo, qhi) as qlo << 16 | qhi, vhc = its computed look-back.
-        int vbase[2], vq[2], vhc[2];
+        // Row k of channel s, in lane k: vpk = (LDS float index of grid
+        // position 0, quad-aligned, plus 256) | (qlo / 4) << 15 |
+        // (qhi / 4) << 22 for its quads of grid positions [qlo, qhi); vhc =
+        // its computed look-back. tqmin / tqmax: the range of the tile
+        // sample of grid position 0 in which no quad of any row needs zeros.
+        int vpk[2], vhc[2], tqmin[2], tqmax[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const GatherChan ch = chans[B.chan_first + min(wave + 8 * s, B.nchan - 1)];
             cs[s]               = Chan{ch.xoff, ch.xs, ch.dmax, ch.xmin, ch.dmask};
             const GatherLeaf lf = leaves[ch.leaf_first + min(lane, ch.nleaf - 1)];
             const int plo       = ch.xs - lf.shmax;
-            vbase[s]            = lf.slot - plo;
-            vq[s]               = (plo & ~3) << 16 | ((plo + lf.len + 3) & ~3);
-            vhc[s]              = lf.hc;
+            const int qlo = plo & ~3, qhi = (plo + lf.len + 3) & ~3;
+            vpk[s] = (lf.slot - plo + 256) | (qlo >> 2) << 15 | (qhi >> 2) << 22;
+            vhc[s] = lf.hc;
+            int lo = INT32_MIN, hi = INT32_MAX;
+            for (int k = 0; k < ch.nleaf; ++k) {
+                lo = max(lo, __builtin_amdgcn_readlane(-lf.hc - qlo, k));
+                hi = min(hi, __builtin_amdgcn_readlane(g.tile_len - qhi, k));
+            }
+            tqmin[s] = lo;
+            tqmax[s] = hi;
         }
+        // Quads of lanes outside a row's slot go to this scratch area.
+        f4* trash = reinterpret_cast<f4*>(lds + 2 * g.half + kNodeChans * kNodeWin +
+                                          kNodeRows * (((K / 2 + 4 + 3) & ~3) + 1));
         float xa[2][XM], xb[2][XM]; // windows of two chunks in flight
         // A fixed number of loads per chunk (channel records clamped, every
         // lane loading), so waiting for one chunk leaves the next in flight.
@@ -716,10 +728,11 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
             for (int s = 0; s < 2; ++s) {
                 const int i = wave + 8 * s;
                 if (i >= B.nchan) break;
-                const Chan ch   = s ? cs[1] : cs[0];
-                const int lbase = s ? vbase[1] : vbase[0];
-                const int lq    = s ? vq[1] : vq[0];
-                const int lhc   = s ? vhc[1] : vhc[0];
+                const Chan ch  = s ? cs[1] : cs[0];
+                const int lpk  = s ? vpk[1] : vpk[0];
+                const int lhc  = s ? vhc[1] : vhc[0];
+                const int tmin = s ? tqmin[1] : tqmin[0];
+                const int tmax = s ? tqmax[1] : tqmax[0];
                 // Lane l: grid positions 4l .. 4l + 3 (past the grid: nothing).
                 const int gl = kNodeChunk + ch.xs - ch.xmin;
                 if (4 * lane >= gl) continue;
@@ -734,28 +747,30 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
                 float acc[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) acc[k] = xr[kX0Dmax + k];
-                const int tq = tau0 - ch.xs; // tile sample of grid position 0
-                int li       = 0;
-                auto emit    = [&]() {
-                    const int base = __builtin_amdgcn_readlane(lbase, li);
-                    const int q    = __builtin_amdgcn_readlane(lq, li);
-                    const int hc   = __builtin_amdgcn_readlane(lhc, li);
-                    ++li;
-                    const int qlo = q >> 16, qhi = q & 0xffff;
-                    if (4 * lane >= qlo && 4 * lane < qhi) {
-                        f4 v;
+                const int tq      = tau0 - ch.xs; // tile sample of grid position 0
+                const bool inside = tq >= tmin && tq <= tmax; // no row needs zeros (uniform)
+                int li            = 0;
+                auto emit         = [&]() {
+                    const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane(lpk, li);
+                    const int base    = (int)(pk & 0x7fffU) - 256;
+                    const int qlo = (int)(pk >> 15 & 0x7fU) << 2, qhi = (int)(pk >> 22 & 0x7fU) << 2;
+                    const bool mine   = (uint32_t)(4 * lane - qlo) < (uint32_t)(qhi - qlo);
+                    f4* dst = mine ? static_cast<f4*>(__builtin_assume_aligned(half + base, 16)) + lane
+                                   : trash + lane;
+                    f4 v;
+                    if (inside) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) v[k] = acc[k];
+                    } else {
                         // Only quads at the row's ends of samples need zeros.
-                        if (tq + qlo >= -hc && tq + qhi <= g.tile_len) { // uniform
+                        const int hc = __builtin_amdgcn_readlane(lhc, li);
+                        const int t0 = tq + 4 * lane;
 #pragma unroll
-                            for (int k = 0; k < 4; ++k) v[k] = acc[k];
-                        } else {
-                            const int t0 = tq + 4 * lane;
-#pragma unroll
-                            for (int k = 0; k < 4; ++k)
-                                v[k] = (t0 + k >= -hc && t0 + k < g.tile_len) ? acc[k] : 0.0f;
-                        }
-                        static_cast<f4*>(__builtin_assume_aligned(half + base, 16))[lane] = v;
+                        for (int k = 0; k < 4; ++k)
+                            v[k] = (t0 + k >= -hc && t0 + k < g.tile_len) ? acc[k] : 0.0f;
                     }
+                    ++li;
+                    *dst = v;
                 };
                 if (ch.dmask & 1U) emit();
                 // Every step taken (past dmax the sums are unused), so every
@@ -848,10 +863,17 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
                 } else {
                     const int h = rec[RW + 2];
                     float* dst  = out + o + t;
+                    // Both branches use every sum, so the LDS reads all stay
+                    // ahead of the branch.
+                    if (t >= -h && t + 48 < g.tile_len) {
 #pragma unroll
-                    for (int s4 = 0; s4 < 4; ++s4)
-                        if (t + 16 * s4 >= -h && t + 16 * s4 < g.tile_len)
-                            __builtin_nontemporal_store(v[s4], dst + 16 * s4);
+                        for (int s4 = 0; s4 < 4; ++s4) __builtin_nontemporal_store(v[s4], dst + 16 * s4);
+                    } else {
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; ++s4)
+                            if (t + 16 * s4 >= -h && t + 16 * s4 < g.tile_len)
+                                __builtin_nontemporal_store(v[s4], dst + 16 * s4);
+                    }
                 }
             }
         }
@@ -865,7 +887,7 @@ void launch_node(const MergeLaunch& ml, float* out, const GatherDims& g, int tl,
     if (blocks * kNodeThreads >= (uint64_t{1} << 32))
         throw std::invalid_argument("FDMT merge grid too large; reduce the time tile");
     const size_t lds = (2 * static_cast<size_t>(g.half) + kNodeChans * kNodeWin +
-                        kNodeRows * (((K / 2 + 4 + 3) & ~3) + 1)) * sizeof(float);
+                        kNodeRows * (((K / 2 + 4 + 3) & ~3) + 1) + 256) * sizeof(float);
     fdmt_node_kernel<K, RING><<<static_cast<unsigned>(blocks), kNodeThreads, lds, s>>>(
         out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), ml.chans.get(), g);
 }
