@@ -11,8 +11,9 @@ seeded) and stays resident in HBM.
 Prints one JSON line: DM-trial*samples/s of K pipelined gulps (the next
 gulp's RFI measurement beside this gulp's FDMT and search; the gulp-at-a-time
 rate under "sequential"), p50/p99 gulp latency as a fraction of the gulp's
-real-time duration (gulps one at a time), the dominant FDMT gather kernel
-against the HBM roofline (live HIP-event timings) beside what plain read,
+real-time duration (gulps one at a time), the dominant FDMT launch kernel
+(and every other FDMT launch kind) against the HBM roofline (live HIP-event
+timings) beside what plain read,
 write and copy kernels reach on this device, and the CPU oracle timed on a
 bounded sample on this host. --copy-probe runs only the 1 GiB device copy
 the HBM traffic counters are calibrated against.
@@ -39,7 +40,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_SUMMARY = ROOT / "profiles" / "fdmt_gather_r16_4096ch_traffic.json"
+TRAFFIC_SUMMARY = ROOT / "profiles" / "fdmt_launch_traffic_4096ch_gulp1M_dm2000.json"
 
 
 def engine_source_digest() -> str:
@@ -49,17 +50,18 @@ def engine_source_digest() -> str:
     return hashlib.sha256((ROOT / "frost_amd" / "csrc" / "engine.hip").read_bytes()).hexdigest()
 
 
-def gather_traffic_from_profile():
-    """HBM bytes per radix-16 gather launch from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (calibrated against --copy-probe), and
-    whether it was measured on the kernel source built now."""
+def launch_traffic_from_profile(kernel):
+    """HBM bytes per launch of an FDMT kernel ("fdmt_node_kernel<16>",
+    "fdmt_gather_kernel<16>", ...) from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against --copy-probe), and whether it was
+    measured on the kernel source built now."""
     if not TRAFFIC_SUMMARY.exists():
         return None, "no committed traffic summary"
     try:
         d = json.loads(TRAFFIC_SUMMARY.read_text())
-        b = float(d["hbm_bytes_per_launch"])
+        b = float(d["kernels"][kernel]["hbm_bytes_per_launch"])
     except Exception:
-        return None, "unreadable traffic summary"
+        return None, f"no traffic summary for {kernel}"
     if d.get("engine_source_sha256") != engine_source_digest():
         return None, "the committed traffic summary was measured on other kernel source"
     return b, "measured on this kernel source"
@@ -742,17 +744,24 @@ def main() -> int:
     summ = plan.summary()
     rows = plan.level_rows().astype(np.float64)
     ndm = summ["ndm"]
-    # Dominant kernel: the FDMT gather launches that sum four tree levels
-    # (radix 16): levels src -> src + 4 in one pass.
+    # FDMT launches by kernel: the first launch forms level 0 from the input
+    # (the node form) unless level 0 is stored by the init kernel; the others
+    # gather 2^(l - src) rows of a stored level.
     src = beam.level_sources()
-    r16 = [l for l in range(len(src)) if src[l] >= 0 and l - src[l] == 4]
-    if not r16:  # other shapes: the deepest launches present
-        depth = max(l - src[l] for l in range(len(src)) if src[l] >= 0)
-        r16 = [l for l in range(len(src)) if src[l] >= 0 and l - src[l] == depth]
+
+    def kernel_of(l):
+        k = 1 << int(l - src[l])
+        return f"fdmt_node_kernel<{k}>" if src[l] == 0 and src[0] == -2 else f"fdmt_gather_kernel<{k}>"
+    kinds = {}
+    for l in range(len(src)):
+        if src[l] >= 0:
+            kinds.setdefault(kernel_of(l), []).append(l)
+    ksec = {k: 0.0 for k in kinds}
+    kcnt = {k: 0 for k in kinds}
     for i in range(a.warmup):
         beam.execute_device(gulp, W.gulp, stream)
         log(f"warmup {i}: {beam.candidate_count()} candidates")
-    lat, merge_s, merge_launches, ncand, fdmt_seconds = [], 0.0, 0, 0, 0.0
+    lat, ncand, fdmt_seconds = [], 0, 0.0
     stage = {"rfi": 0.0, "init": 0.0, "merge": 0.0, "search": 0.0}
     rfi_parts = {"bandpass": 0.0, "moments_and_mask": 0.0, "scans": 0.0, "flags": 0.0,
                  "apply": 0.0}
@@ -769,8 +778,9 @@ def main() -> int:
         nc = beam.candidate_count()
         ncand += nc
         sec, cnt, tile = beam.level_times()
-        merge_s += float(sec[r16].sum())
-        merge_launches += int(cnt[r16].sum())
+        for k, ls in kinds.items():
+            ksec[k] += float(sec[ls].sum())
+            kcnt[k] += int(cnt[ls].sum())
         fdmt_seconds += float(sec.sum())
         tm = beam.engine_timings()
         cs, rc = beam.host_stats()
@@ -812,7 +822,9 @@ def main() -> int:
     def min_bytes(a, b):
         tot = 0.0
         for tl in tiles:
-            tot += (lsamp[a] - rows[a] * (tile - tl)) + (lsamp[b] - rows[b] * (tile - tl))
+            # Level 0 formed from the input: the input channels' tile read once.
+            read = W.nchans * tl if a == 0 and src[0] == -2 else lsamp[a] - rows[a] * (tile - tl)
+            tot += read + (lsamp[b] - rows[b] * (tile - tl))
         return 4.0 * tot
 
     # Unfused per-level bytes of the same launches (each covered level read
@@ -822,17 +834,36 @@ def main() -> int:
         if a == 0 and src[0] == -2:
             n += W.nchans + rows[0]
         return 4.0 * W.gulp * float(n)
-    min_bytes_gulp = sum(min_bytes(int(src[l]), l) for l in r16)
-    alg_bytes_gulp = sum(unfused_bytes(int(src[l]), l) for l in r16)
-    achieved = min_bytes_gulp * a.steps / merge_s / 1e9
-    launches_per_gulp = merge_launches / a.steps
-    avg_launch_s = merge_s / merge_launches
-    traffic, traffic_note = gather_traffic_from_profile()
+    # Every launch kind against the roofline; the dominant kernel (the most
+    # kernel time per gulp) is the one reported as "roofline".
+    launches = {}
+    for k, ls in kinds.items():
+        mb = sum(min_bytes(int(src[l]), l) for l in ls)
+        ub = sum(unfused_bytes(int(src[l]), l) for l in ls)
+        tr, note = launch_traffic_from_profile(k)
+        gbs = mb * a.steps / ksec[k] / 1e9
+        launches[k] = {
+            "levels": [[int(src[l]), int(l)] for l in ls],
+            "ms_per_gulp": 1e3 * ksec[k] / a.steps,
+            "launches_per_gulp": kcnt[k] / a.steps,
+            "avg_launch_ms": 1e3 * ksec[k] / kcnt[k],
+            "bytes_per_launch": mb / (kcnt[k] / a.steps),
+            "achieved_gbs": gbs,
+            "frac": gbs / HBM_PEAK_GBS,
+            "unfused_equivalent_gbs": ub * a.steps / ksec[k] / 1e9,
+            "traffic": tr,
+            "traffic_source": note,
+        }
+    dom = max(launches, key=lambda k: launches[k]["ms_per_gulp"])
+    D = launches[dom]
+    achieved = D["achieved_gbs"]
+    avg_launch_s = D["avg_launch_ms"] * 1e-3
+    traffic, traffic_note = D["traffic"], D["traffic_source"]
     # Every FDMT kernel of the gulp (init and all tree launches): fused
     # minimum bytes over their summed kernel time.
     fdmt_min = 0.0
     for l in range(len(src)):
-        if src[l] == -1:  # init: the input channels read, level 0 written
+        if src[l] == -1 and l == 0:  # init: the input channels read, level 0 written
             fdmt_min += sum(4.0 * (W.nchans * tl + lsamp[0] - rows[0] * (tile - tl))
                             for tl in tiles)
         elif src[l] >= 0:
@@ -883,26 +914,29 @@ def main() -> int:
                                "on the host (the latency figures' steps)"},
         "rfi_transforms": rfi_tf,
         "roofline": {
-            "kernel": "fdmt_gather_kernel<16> (FDMT levels l-4 -> l in one pass)",
-            "levels": [[int(src[l]), int(l)] for l in r16],
+            "kernel": dom,
+            "levels": D["levels"],
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "bytes_per_launch": min_bytes_gulp / launches_per_gulp,
-            "avg_launch_ms": 1e3 * avg_launch_s,
-            "unfused_equivalent_gbs": alg_bytes_gulp * a.steps / merge_s / 1e9,
+            "bytes_per_launch": D["bytes_per_launch"],
+            "avg_launch_ms": D["avg_launch_ms"],
+            "unfused_equivalent_gbs": D["unfused_equivalent_gbs"],
             "measured_gbs": None if traffic is None else traffic / avg_launch_s / 1e9,
             "measured_frac": None if traffic is None else traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS,
             "traffic_source": traffic_note,
-            "note": "achieved = fused minimum bytes of each launch (every input row's tile "
-                    "plus look-back read once, every output row's written once, float32) / "
+            "launches": launches,
+            "note": "kernel = the FDMT launch kind with the most kernel time per gulp; "
+                    "achieved = fused minimum bytes of each launch (every input row's tile "
+                    "plus look-back read once, or the input channels' tile when level 0 is "
+                    "formed inside the launch; every output row's written once, float32) / "
                     "HIP-event launch time; unfused_equivalent_gbs counts the per-level bytes "
-                    "of the levels a launch sums (the work of separate merge passes); "
-                    "traffic is rocprof FETCH_SIZE + WRITE_SIZE per launch, scaled by a "
-                    "plain copy's counts",
+                    "of the levels a launch sums (the work of separate merge passes, level 0's "
+                    "write and read included); traffic is rocprof FETCH_SIZE + WRITE_SIZE per "
+                    "launch, scaled by a plain copy's counts",
             "hbm_probe": dict(probe, note="16-byte-access read, streamed-write and copy kernels "
                               "on this device: the practical ceiling beside the published peak"),
             "fdmt_all_kernels": {

@@ -1998,10 +1998,11 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
 
     // Tuning: FROST_AMD_FUSE (levels per launch, 1-4), FROST_AMD_MERGE=direct
     // (gather from global memory instead of LDS staging), FROST_AMD_GATHER_LDS
-    // (LDS floats per staged block), FROST_AMD_INIT=fused (form level 0 in
-    // the first gather launch instead of storing it with the init kernel:
-    // whole sub-bands per block; fused-runs: the row runs of the other
-    // launches), FROST_AMD_NODE_LDS (LDS floats of a sub-band block's rows).
+    // (LDS floats per staged block), FROST_AMD_INIT=init (store level 0 with
+    // the init kernel instead of forming it inside the first gather launch
+    // with whole sub-bands per block, the default; fused-runs: formed in the
+    // row runs of the other launches), FROST_AMD_NODE_LDS (LDS floats of a
+    // sub-band block's rows).
     if (const char* fv = std::getenv("FROST_AMD_FUSE")) {
         const long f = std::atol(fv);
         if (f >= 1 && f <= 4) max_fuse_ = static_cast<uint32_t>(f);
@@ -2253,6 +2254,8 @@ void Engine::plane_setup() {
     dmax0_ = 0;
     for (const int32_t d : plan_.levels[0].delay) dmax0_ = std::max(dmax0_, d);
     if (dmax0_ > kX0Dmax) x0_fused_ = false; // level-0 delays too long for the fused form
+    for (const auto& p : plan_.planes)
+        if (p.level == 0) x0_fused_ = false; // a searched level 0 must be stored
 }
 
 // Level-0 program: per channel [first row, rows, computed look-back, largest

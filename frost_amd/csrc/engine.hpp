@@ -284,7 +284,7 @@ private:
     uint64_t spill_cap_    = 0;
     uint32_t max_fuse_   = 4;
     bool staged_merge_   = true;
-    bool x0_fused_       = false; // level 0 formed inside the first gather launch
+    bool x0_fused_       = true;  // level 0 formed inside the first gather launch (when it can be)
     uint32_t gather_lds_ = 8192; // LDS floats per staged row block (one half)
     bool x0_node_        = true;  // fused level 0 in the node form when its blocks fit
     uint32_t node_lds_   = 16000; // node form: LDS floats of one block's input rows
