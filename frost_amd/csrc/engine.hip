@@ -692,7 +692,7 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
         }
         // Quads of lanes outside a row's slot go to this scratch area.
         f4* trash = reinterpret_cast<f4*>(lds + 2 * g.half + kNodeChans * kNodeWin +
-                                          kNodeRows * (((K / 2 + 4 + 3) & ~3) + 1));
+                                          kNodeRows * (((K + 4 + 3) & ~3) + 1));
         float xa[2][XM], xb[2][XM]; // windows of two chunks in flight
         // A fixed number of loads per chunk (channel records clamped, every
         // lane loading), so waiting for one chunk leaves the next in flight.
@@ -805,10 +805,10 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
         // quarter-wave one row, lane l the samples l % 16 + 16 s, s < 4, of
         // the chunk (four LDS reads per address, each quarter-wave reading
         // 16 consecutive floats). Row record in LDS (copied
-        // at block start): [K 16-bit slot offsets packed two per word, out
-        // lo, out hi, h, ring length]; carried rows also keep the ring
-        // position of tile sample 0.
-        constexpr int RW  = K / 2;            // packed offset words
+        // at block start): [K slot offsets, out lo, out hi, h, ring
+        // length]; carried rows also keep the ring position of tile
+        // sample 0.
+        constexpr int RW  = K;                // offset words
         constexpr int RS  = RW + 4;           // record words
         constexpr int RSL = (RS + 3) & ~3;    // LDS record stride (whole quads)
         int* srec         = reinterpret_cast<int*>(lds + 2 * g.half + kNodeChans * kNodeWin);
@@ -844,7 +844,7 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
                 float w[4][K];
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    const float* p = half + ((uint32_t)rec[j / 2] >> (16 * (j & 1)) & 0xffffU);
+                    const float* p = half + rec[j];
 #pragma unroll
                     for (int s4 = 0; s4 < 4; ++s4) w[s4][j] = p[16 * s4];
                 }
@@ -887,7 +887,7 @@ void launch_node(const MergeLaunch& ml, float* out, const GatherDims& g, int tl,
     if (blocks * kNodeThreads >= (uint64_t{1} << 32))
         throw std::invalid_argument("FDMT merge grid too large; reduce the time tile");
     const size_t lds = (2 * static_cast<size_t>(g.half) + kNodeChans * kNodeWin +
-                        kNodeRows * (((K / 2 + 4 + 3) & ~3) + 1) + 256) * sizeof(float);
+                        kNodeRows * (((K + 4 + 3) & ~3) + 1) + 256) * sizeof(float);
     fdmt_node_kernel<K, RING><<<static_cast<unsigned>(blocks), kNodeThreads, lds, s>>>(
         out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), ml.chans.get(), g);
 }
@@ -917,12 +917,12 @@ int default_gather_chunks() {
     return n;
 }
 
-// Chunks per node-form block (FROST_AMD_NODE_CHUNKS, default 32: 2048 samples).
+// Chunks per node-form block (FROST_AMD_NODE_CHUNKS, default 64: 4096 samples).
 int node_chunks() {
     static const int n = [] {
         const char* v = std::getenv("FROST_AMD_NODE_CHUNKS");
-        const int c   = v ? std::atoi(v) : 32;
-        return (c >= 1 && c <= 256) ? c : 32;
+        const int c   = v ? std::atoi(v) : 64;
+        return (c >= 1 && c <= 256) ? c : 64;
     }();
     return n;
 }
@@ -2014,7 +2014,7 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
     }
     if (const char* nv = std::getenv("FROST_AMD_NODE_LDS")) {
         const long f = std::atol(nv);
-        if (f >= 2048 && f <= 16000) node_lds_ = static_cast<uint32_t>(f); // two halves fit 160 KB
+        if (f >= 2048 && f <= 15000) node_lds_ = static_cast<uint32_t>(f); // two halves fit 160 KB
     }
     x0_fused_ = x0_fused_ && staged_merge_;
     if (!staged_merge_) max_fuse_ = std::min<uint32_t>(max_fuse_, 3);
@@ -2877,20 +2877,6 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
                 int32_t& v = rec[r * RS + 4 + static_cast<std::size_t>(k)];
                 if (v == kZeroLeaf) v = ml.zero_slot;
             }
-    }
-    if (node) {
-        // Node form records: the K offsets as 16-bit halves, two per word.
-        const std::size_t RN = 4 + static_cast<std::size_t>(K) / 2;
-        std::vector<int32_t> packed(static_cast<std::size_t>(ml.nrows) * RN);
-        for (std::size_t r = 0; r < ml.nrows; ++r) {
-            for (std::size_t j = 0; j < 4; ++j) packed[r * RN + j] = rec[r * RS + j];
-            for (int k = 0; k < K; ++k) {
-                const int32_t v = rec[r * RS + 4 + static_cast<std::size_t>(k)];
-                if (v < 0 || v > 0xffff) throw std::logic_error("node-form LDS offset out of range");
-                packed[r * RN + 4 + k / 2] |= static_cast<int32_t>(static_cast<uint32_t>(v) << (16 * (k & 1)));
-            }
-        }
-        rec.swap(packed);
     }
     ml.blocks.resize(gb.size());
     ml.leaves.resize(gl.size());

@@ -287,7 +287,7 @@ private:
     bool x0_fused_       = true;  // level 0 formed inside the first gather launch (when it can be)
     uint32_t gather_lds_ = 8192; // LDS floats per staged row block (one half)
     bool x0_node_        = true;  // fused level 0 in the node form when its blocks fit
-    uint32_t node_lds_   = 16000; // node form: LDS floats of one block's input rows
+    uint32_t node_lds_   = 15000; // node form: LDS floats of one block's input rows (one half)
     // Staged program of a launch: per output row its offset (sample t0, or
     // ring start), computed look-back h, ring length (0: linear) and its K
     // descendants (input row, summed shift) in tree order.

@@ -23,7 +23,8 @@ def stream():
 
 
 def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None,
-             search_path=None, fuse=None, merge=None, init=None, state=None):
+             search_path=None, fuse=None, merge=None, init=None, state=None, node_lds=None,
+             level0=None):
     """Runs the GPU engine and the oracle on the same gulps; returns both
     result lists (per gulp + drain) and, if keep, the plane rows. `fuse` caps
     the FDMT levels summed per launch (default 4); merge="direct" gathers
@@ -31,9 +32,12 @@ def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None
     level 0 from the input inside the first launch instead of storing it
     (whole sub-bands per block; "fused-runs": in the other launches' runs);
     state="carried" keeps the stored levels in per-row rings across tiles
-    and gulps instead of recomputing each tile's look-back."""
+    and gulps instead of recomputing each tile's look-back; node_lds caps
+    the LDS floats of a sub-band block's level-0 rows (smaller caps split a
+    sub-band's rows over several blocks)."""
     env = {"FROST_AMD_TILE": tile, "FROST_AMD_SEARCH": search_path, "FROST_AMD_FUSE": fuse,
-           "FROST_AMD_MERGE": merge, "FROST_AMD_INIT": init, "FROST_AMD_FDMT_STATE": state}
+           "FROST_AMD_MERGE": merge, "FROST_AMD_INIT": init, "FROST_AMD_FDMT_STATE": state,
+           "FROST_AMD_NODE_LDS": node_lds}
     for k, v in env.items():
         if v:
             os.environ[k] = str(v)
@@ -44,6 +48,8 @@ def run_pair(oracle, stream, dcfg, bcfg, gulps, masks=None, keep=True, tile=None
         for k in env:
             os.environ.pop(k, None)
     src = gpu.level_sources()
+    if level0 is not None:  # -2: formed inside the first launch, -1: stored by the init kernel
+        assert src[0] == level0, src
     stored = [0] + [l for l in range(1, len(src)) if src[l] >= 0]
     gaps = [b - a for a, b in zip(stored, stored[1:])]
     say(f"  stored levels {stored}, levels per launch {gaps}, carried {gpu.carried()}")
@@ -119,19 +125,43 @@ def test_engine_single_gulp_small(oracle, stream):
     (2, "ring", None, None, None), (2, None, 1, None, None), (2, None, 2, None, None),
     (2, None, 3, None, None), (2, None, 3, "direct", None), (1, None, 1, "direct", None),
     (2, None, None, None, "fused"), (2, None, 2, None, "fused"), (0, None, 1, None, "fused"),
-    (2, None, None, None, "fused-runs"), (0, None, 1, None, "fused-runs")])
+    (2, None, None, None, "fused-runs"), (0, None, 1, None, "fused-runs"),
+    (2, None, None, None, "init")])
 def test_engine_gulps_and_planes(oracle, stream, depth, path, fuse, merge, init):
     say(f"test_engine_gulps_and_planes depth={depth} search={path} fuse={fuse} merge={merge} "
         f"init={init}")
-    dcfg = abi.dedisp_config(64, 256e-6, 1500.0, -300 / 64, 500.0, block_nsamps=4096,
+    # Level 0 formed from the input needs every level-0 delay within 16:
+    # DM <= 120 at this shape (at DM 500 the engine stores level 0 itself).
+    fused = init is not None and init.startswith("fused")
+    dm = 120.0 if fused else 500.0
+    dcfg = abi.dedisp_config(64, 256e-6, 1500.0, -300 / 64, dm, block_nsamps=4096,
                              subband_depth=depth)
     bcfg = abi.boxcar_config(max_boxcar=64, detect_thresh=5.0)
     rng = np.random.default_rng(10 + depth)
-    gulps = [pulse_waterfall(rng, 64, 4096, [(100.0 * (k + 1), 1000 + 500 * k, 8, 1.5),
-                                             (320.0, 3900, 3, 3.0)], dcfg)
+    gulps = [pulse_waterfall(rng, 64, 4096, [(dm / 5 * (k + 1), 1000 + 500 * k, 8, 1.5),
+                                             (dm * 0.64, 3900, 3, 3.0)], dcfg)
              for k in range(3)]
     g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, search_path=path, fuse=fuse, merge=merge,
-                    init=init)
+                    init=init, level0=-2 if fused else -1 if init == "init" else None)
+    assert_same(g, o)
+
+
+@pytest.mark.parametrize("node_lds,state", [(None, None), (2048, None), (3000, "carried")])
+def test_engine_level0_in_first_launch(oracle, stream, node_lds, state):
+    """Level 0 formed inside the first launch (64 channels at DM <= 120 keep
+    every level-0 delay within 16): whole sub-bands per block, and with a
+    small LDS cap each sub-band's rows split over several blocks (each block
+    folding the channel windows its rows need), recomputed and carried,
+    ragged tiles."""
+    say(f"test_engine_level0_in_first_launch node_lds={node_lds} state={state}")
+    dcfg = abi.dedisp_config(64, 256e-6, 1500.0, -300 / 64, 120.0, block_nsamps=4096,
+                             subband_depth=2)
+    bcfg = abi.boxcar_config(max_boxcar=64, detect_thresh=5.0)
+    rng = np.random.default_rng(41)
+    gulps = [pulse_waterfall(rng, 64, 4096, [(150.0 * (k + 1), 900 + 700 * k, 6, 1.5)], dcfg)
+             for k in range(3)]
+    g, o = run_pair(oracle, stream, dcfg, bcfg, gulps, tile=1500 if state is None else 1024,
+                    node_lds=node_lds, state=state, level0=-2)
     assert_same(g, o)
 
 
