@@ -802,9 +802,9 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
         }
     } else {
         // Consumer: wave 8 + q sums rows 4 (q + 8 k) + (lane / 16): each
-        // quarter-wave one row, lane l the samples l % 16 + 16 s, s < 4, of
-        // the chunk (four LDS reads per address, each quarter-wave reading
-        // 16 consecutive floats). Row record in LDS (copied
+        // quarter-wave one row, lane l the sample pairs 2 (l % 16) + {0, 1}
+        // and 32 later of the chunk (four LDS reads per address; each
+        // quarter-wave stores 128 contiguous bytes per store). Row record in LDS (copied
         // at block start): [K slot offsets, out lo, out hi, h, ring
         // length]; carried rows also keep the ring position of tile
         // sample 0.
@@ -827,8 +827,8 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
         const int sub = lane & 15;
         for (int c = lo_c; c < hi_c; ++c) {
             lds_barrier();
-            const float* half = lds + (c & 1) * g.half + sub;
-            const int t       = tau_first + c * kNodeChunk + sub; // tile sample of this lane's first
+            const float* half = lds + (c & 1) * g.half + 2 * sub;
+            const int t       = tau_first + c * kNodeChunk + 2 * sub; // tile sample of this lane's first
             for (int k = 0; 4 * (q + 8 * k) < B.nrows; ++k) {
                 const int rr = 4 * (q + 8 * k) + (lane >> 4);
                 if (rr >= B.nrows) continue;
@@ -846,7 +846,7 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
                 for (int j = 0; j < K; ++j) {
                     const float* p = half + rec[j];
 #pragma unroll
-                    for (int s4 = 0; s4 < 4; ++s4) w[s4][j] = p[16 * s4];
+                    for (int s4 = 0; s4 < 4; ++s4) w[s4][j] = p[(s4 & 1) + 32 * (s4 >> 1)];
                 }
                 float v[4];
 #pragma unroll
@@ -857,22 +857,25 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
                     const int R  = rec[RW + 3];
                     const int q0 = sq0[rr];
 #pragma unroll
-                    for (int s4 = 0; s4 < 4; ++s4)
-                        if (t + 16 * s4 < g.tile_len)
-                            __builtin_nontemporal_store(v[s4], out + o + wrap_pos(q0 + t + 16 * s4, R));
+                    for (int s4 = 0; s4 < 4; ++s4) {
+                        const int ts = t + (s4 & 1) + 32 * (s4 >> 1);
+                        if (ts < g.tile_len) __builtin_nontemporal_store(v[s4], out + o + wrap_pos(q0 + ts, R));
+                    }
                 } else {
                     const int h = rec[RW + 2];
                     float* dst  = out + o + t;
                     // Both branches use every sum, so the LDS reads all stay
                     // ahead of the branch.
-                    if (t >= -h && t + 48 < g.tile_len) {
-#pragma unroll
-                        for (int s4 = 0; s4 < 4; ++s4) __builtin_nontemporal_store(v[s4], dst + 16 * s4);
+                    if (t >= -h && t + 33 < g.tile_len) {
+                        typedef float f2v __attribute__((ext_vector_type(2)));
+                        __builtin_nontemporal_store(f2v{v[0], v[1]}, reinterpret_cast<f2v*>(dst));
+                        __builtin_nontemporal_store(f2v{v[2], v[3]}, reinterpret_cast<f2v*>(dst + 32));
                     } else {
 #pragma unroll
-                        for (int s4 = 0; s4 < 4; ++s4)
-                            if (t + 16 * s4 >= -h && t + 16 * s4 < g.tile_len)
-                                __builtin_nontemporal_store(v[s4], dst + 16 * s4);
+                        for (int s4 = 0; s4 < 4; ++s4) {
+                            const int d4 = (s4 & 1) + 32 * (s4 >> 1);
+                            if (t + d4 >= -h && t + d4 < g.tile_len) __builtin_nontemporal_store(v[s4], dst + d4);
+                        }
                     }
                 }
             }
