@@ -690,9 +690,6 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
             tqmin[s] = lo;
             tqmax[s] = hi;
         }
-        // Quads of lanes outside a row's slot go to this scratch area.
-        f4* trash = reinterpret_cast<f4*>(lds + 2 * g.half + kNodeChans * kNodeWin +
-                                          kNodeRows * (((K + 4 + 3) & ~3) + 1));
         float xa[2][XM], xb[2][XM]; // windows of two chunks in flight
         // A fixed number of loads per chunk (channel records clamped, every
         // lane loading), so waiting for one chunk leaves the next in flight.
@@ -754,9 +751,9 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
                     const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane(lpk, li);
                     const int base    = (int)(pk & 0x7fffU) - 256;
                     const int qlo = (int)(pk >> 15 & 0x7fU) << 2, qhi = (int)(pk >> 22 & 0x7fU) << 2;
+                    // Only the lanes whose quad lies in the row's slot write
+                    // (masked lanes take no LDS bandwidth).
                     const bool mine   = (uint32_t)(4 * lane - qlo) < (uint32_t)(qhi - qlo);
-                    f4* dst = mine ? static_cast<f4*>(__builtin_assume_aligned(half + base, 16)) + lane
-                                   : trash + lane;
                     f4 v;
                     if (inside) {
 #pragma unroll
@@ -770,7 +767,7 @@ fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks
                             v[k] = (t0 + k >= -hc && t0 + k < g.tile_len) ? acc[k] : 0.0f;
                     }
                     ++li;
-                    *dst = v;
+                    if (mine) static_cast<f4*>(__builtin_assume_aligned(half + base, 16))[lane] = v;
                 };
                 if (ch.dmask & 1U) emit();
                 // Every step taken (past dmax the sums are unused), so every
@@ -890,7 +887,7 @@ void launch_node(const MergeLaunch& ml, float* out, const GatherDims& g, int tl,
     if (blocks * kNodeThreads >= (uint64_t{1} << 32))
         throw std::invalid_argument("FDMT merge grid too large; reduce the time tile");
     const size_t lds = (2 * static_cast<size_t>(g.half) + kNodeChans * kNodeWin +
-                        kNodeRows * (((K + 4 + 3) & ~3) + 1) + 256) * sizeof(float);
+                        kNodeRows * (((K + 4 + 3) & ~3) + 1)) * sizeof(float);
     fdmt_node_kernel<K, RING><<<static_cast<unsigned>(blocks), kNodeThreads, lds, s>>>(
         out, ml.blocks.get(), ml.leaves.get(), ml.rows.get(), ml.chans.get(), g);
 }
