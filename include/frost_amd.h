@@ -334,16 +334,19 @@ int fa_engine_plane_rows(fa_engine* engine, uint32_t p, float* out,
 int fa_engine_timings(const fa_engine* engine, double* t_init,
                       double* t_merge, double* t_search, double* t_total);
 
-/* Per FDMT level (0 = init kernel): summed kernel seconds and launches in the
- * last execute, the time-tile length and the number of levels. */
+/* Per FDMT level (0 = init kernel, when level 0 is stored): summed kernel
+ * seconds and launches in the last execute, the time-tile length and the
+ * number of levels. */
 int fa_engine_level_times(const fa_engine* engine, double* seconds,
                           uint32_t* launches, uint64_t cap, uint64_t* tile,
                           uint64_t* nlevels);
 
 /* Per FDMT level: the level its launch reads (-1: the input waterfall, for
- * level 0), or -2 when the level is summed inside a later launch and never
- * stored. A launch reading level a and writing level b sums 2^(b-a) rows of
- * level a per output row. No reference counterpart (diagnostics). */
+ * level 0 stored by the init kernel), or -2 when the level is summed inside
+ * a later launch and never stored (level 0 included: by default the first
+ * launch forms level 0 from the input itself). A launch reading level a and
+ * writing level b sums 2^(b-a) rows of level a per output row. No reference
+ * counterpart (diagnostics). */
 int fa_engine_level_sources(const fa_engine* engine, int32_t* source, uint64_t cap,
                             uint64_t* nlevels);
 
