@@ -124,7 +124,8 @@ def test_engine_single_gulp_small(oracle, stream):
     (0, None, None, None, None), (1, None, None, None, None), (2, None, None, None, None),
     (2, "ring", None, None, None), (2, None, 1, None, None), (2, None, 2, None, None),
     (2, None, 3, None, None), (2, None, 3, "direct", None), (1, None, 1, "direct", None),
-    (2, None, None, None, "fused"), (2, None, 2, None, "fused"), (0, None, 1, None, "fused"),
+    (2, None, None, None, "fused"), (2, None, 3, None, "fused"), (2, None, 2, None, "fused"),
+    (0, None, 1, None, "fused"),
     (2, None, None, None, "fused-runs"), (0, None, 1, None, "fused-runs"),
     (2, None, None, None, "init")])
 def test_engine_gulps_and_planes(oracle, stream, depth, path, fuse, merge, init):
