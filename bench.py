@@ -44,10 +44,16 @@ TRAFFIC_SUMMARY = ROOT / "profiles" / "fdmt_launch_traffic_4096ch_gulp1M_dm2000.
 
 
 def engine_source_digest() -> str:
-    """sha256 of the FDMT kernels' source, to tell whether a committed
-    traffic summary was measured on the kernels built now."""
+    """sha256 of the FDMT kernels' code (engine.hip with comments and blank
+    lines removed), to tell whether a committed traffic summary was measured
+    on the kernels built now."""
     import hashlib
-    return hashlib.sha256((ROOT / "frost_amd" / "csrc" / "engine.hip").read_bytes()).hexdigest()
+    import re
+    src = (ROOT / "frost_amd" / "csrc" / "engine.hip").read_text()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"//[^\n]*", "", src)
+    code = "\n".join(line.rstrip() for line in src.splitlines() if line.strip())
+    return hashlib.sha256(code.encode()).hexdigest()
 
 
 def launch_traffic_from_profile(kernel):
@@ -62,7 +68,7 @@ def launch_traffic_from_profile(kernel):
         b = float(d["kernels"][kernel]["hbm_bytes_per_launch"])
     except Exception:
         return None, f"no traffic summary for {kernel}"
-    if d.get("engine_source_sha256") != engine_source_digest():
+    if d.get("engine_code_sha256") != engine_source_digest():
         return None, "the committed traffic summary was measured on other kernel source"
     return b, "measured on this kernel source"
 

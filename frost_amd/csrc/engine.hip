@@ -617,15 +617,16 @@ fdmt_gather_kernel(const float* __restrict__ in, float* __restrict__ out,
 // block holds a whole sub-band's rows (up to kNodeChans channels, every
 // level-0 row they read and all output rows over them) over a run of short
 // chunks, so each channel window is loaded and folded once per chunk for the
-// whole sub-band. Waves 0-7 (producers) own two channels each: they load the
-// channel windows of chunk c + 1 while folding chunk c into its level-0 rows,
-// x(t) + x(t-1) + ... + x(t-D) in that order, four grid positions per lane
-// in registers, and write each row's slot in one half of a double-buffered
-// LDS area as aligned quads (zero outside the row's samples
-// [t0 - hc, t0 + T)). Waves 8-15 (consumers) sum each output row's K rows
-// from the other half in tree order, one sample per lane, and store it.
-// Producers only load and consumers only store, so neither waits on the
-// other's memory traffic; one barrier per chunk hands the halves over.
+// whole sub-band. Waves 0-7 (producers) own two channels each: they keep the
+// channel windows of chunks c + 1 and c + 2 in flight while folding chunk c
+// into its level-0 rows, x(t) + x(t-1) + ... + x(t-D) in that order, four
+// grid positions per lane in registers, and write each row's slot in one
+// half of a double-buffered LDS area as aligned quads (zero outside the
+// row's samples [t0 - hc, t0 + T)). Waves 8-15 (consumers) sum each output
+// row's K rows from the other half in tree order, one row per quarter-wave,
+// and store it. Producers only load and consumers only store, so neither
+// waits on the other's memory traffic; one barrier per chunk hands the
+// halves over.
 constexpr int kNodeChunk   = 64;   // samples per chunk: one per lane
 constexpr int kNodeChans   = 16;   // channels per block: two per producer wave
 constexpr int kNodeGrid    = 256;  // grid positions per channel (chunk + shift spread): four per lane
@@ -638,7 +639,8 @@ __global__ void __launch_bounds__(kNodeThreads)
 fdmt_node_kernel(float* __restrict__ out, const GatherBlock* __restrict__ blocks,
                  const GatherLeaf* __restrict__ leaves, const int32_t* __restrict__ rows,
                  const GatherChan* __restrict__ chans, GatherDims g) {
-    // [row slots, zero chunk: g.half] x 2 halves, [windows: kNodeChans x kNodeWin]
+    // [row slots, zero chunk: g.half] x 2 halves, [windows: kNodeChans x kNodeWin],
+    // [row records: kNodeRows x (K + 4, rounded to quads)], [ring positions: kNodeRows]
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const uint32_t blk  = blockIdx.x % (uint32_t)g.nblocks;
     const uint32_t grp  = blockIdx.x / (uint32_t)g.nblocks;
