@@ -2628,13 +2628,17 @@ void Engine::build_gather(MergeLaunch& ml, const std::vector<GatherRow>& orow,
     // Level-0 rows formed from the input: rows of one channel share a
     // window, so a run is bounded by its channel count (16) and each
     // channel's rows must fit one grid (shift range + chunk within kX0Grid,
-    // delays within the window).
+    // or kNodeGrid in the node form; delays within the window).
     const bool x0         = ml.from_input;
-    // Node form: a block's rows span whole sub-bands over short chunks.
+    // Node form: a block's rows span whole sub-bands over short chunks
+    // (blocks are cut only where a sub-band's rows exceed the LDS budget
+    // node_lds_ or kNodeRows).
     const bool node       = x0 && x0_node_;
     const int CL          = node ? kNodeChunk : kChunkG; // samples per staged chunk
     const auto& l0        = plan_.levels[0];
-    const std::size_t max_chans = node ? kNodeChans : 16; // four channel windows per producer wave
+    // Channels per block: four windows per producer wave (run form), two per
+    // producer wave (node form).
+    const std::size_t max_chans = node ? kNodeChans : 16;
     if (x0) ml.max_leaves = static_cast<uint32_t>(max_chans);
     ml.node = node;
     auto x0_ok = [&](const std::vector<Lf>& v) {
