@@ -123,3 +123,25 @@ def test_large_beam_fits_with_carried_levels():
     # The engine's own share: the float input with its short level-0 history,
     # and the rings.
     assert 4 * nch * gulp < e.fdmt_bytes < 4 * nch * gulp + 110 * 10**9
+
+
+def test_level0_formed_in_the_first_launch_is_not_stored(monkeypatch):
+    """4096 ch, 1200-1500 MHz, 64 us, DM 0-2000: every level-0 delay is
+    within 16, so the first FDMT launch forms level 0 from the input and the
+    FDMT holds less than with level 0 stored by the init kernel
+    (FROST_AMD_INIT=init); at 512 ch and 1 MHz channels near 1400 MHz the
+    delays are longer and both estimates are the same."""
+    from frost_amd import workloads
+    W = workloads.WORKLOADS["4096ch_64us_gulp1M_dm2000_11planes"]
+
+    def est(init, args):
+        if init:
+            monkeypatch.setenv("FROST_AMD_INIT", init)
+        else:
+            monkeypatch.delenv("FROST_AMD_INIT", raising=False)
+        return pipeline.estimate_memory(*args).fdmt_bytes
+
+    prod = (W.rfi_config(), W.layout(), W.baseline, W.dedisp_config(), W.boxcar_config())
+    assert est(None, prod) < est("init", prod)
+    wide = configs(gulp=65536)
+    assert est(None, wide) == est("init", wide)
