@@ -2127,6 +2127,18 @@ Engine::Engine(const FdmtPlan& plan, const fa_boxcar_config& cfg, const uint8_t*
         if (level_buf_[l] == 0) len0 = std::max(len0, level_len_[l]);
         if (level_buf_[l] == 1) len1 = std::max(len1, level_len_[l]);
     }
+    // The last level's search still reads its buffer when the next tile's
+    // first launch writes its output: when the two share a buffer (level 0
+    // formed inside the first launch and an odd count of stored levels),
+    // the two buffers swap roles every other tile so that they overlap.
+    // Owned buffers only; both then hold the largest level.
+    {
+        uint32_t first = 0;
+        while (first < plan_.nlevels && level_buf_[first] < 0) ++first;
+        const uint32_t L = plan_.nlevels - 1;
+        flip_ = !carried_ && !shared_ && first < L && level_buf_[first] == level_buf_[L];
+        if (flip_) len0 = len1 = std::max(len0, len1);
+    }
     len1 = std::max<uint64_t>(len1, 1);
     if (carried_) {
         x_.resize(nx * xstride_);
@@ -3101,7 +3113,8 @@ void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
     // Searches still reading each level buffer (their last event; carried:
     // each level's rings), and the last search of the call.
     std::vector<hipEvent_t> reading(plan_.nlevels, nullptr);
-    auto buf_id = [&](uint32_t l) { return carried_ ? static_cast<int>(l) : level_buf_[l]; };
+    int flip    = 0; // this tile's buffer swap (flip_: every other tile)
+    auto buf_id = [&](uint32_t l) { return carried_ ? static_cast<int>(l) : level_buf_[l] ^ flip; };
     hipEvent_t last_search = nullptr;
     const int gchunks      = gather_chunks();
     begin_gulp_kernel<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(counters_.get()),
@@ -3109,9 +3122,10 @@ void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
     const uint32_t L = plan_.nlevels - 1;
     for (uint64_t t0 = 0; t0 < N; t0 += tile_) {
         const int tl = static_cast<int>(std::min<uint64_t>(tile_, N - t0));
+        flip         = flip_ ? static_cast<int>((t0 / tile_) & 1) : 0;
         for (uint32_t l = 0; l <= L; ++l) {
             if (level_source_[l] == -2) continue; // computed inside a fused launch
-            float* out = buf_ptr_[level_buf_[l]];
+            float* out = buf_ptr_[level_buf_[l] ^ flip];
             if (hipEvent_t& r = reading[buf_id(l)]) {
                 FA_HIP(hipStreamWaitEvent(s, r, 0)); // searches of the old contents done
                 r = nullptr;
@@ -3135,7 +3149,7 @@ void Engine::enqueue_gulp(uint64_t N, hipStream_t s, bool captured) {
                 for (const auto& mlc : launches_)
                     if (mlc.out_level == l) ml = &mlc;
                 if (!ml) throw std::logic_error("FDMT level without a launch");
-                const float* in = ml->from_input ? nullptr : buf_ptr_[ml->in_buf];
+                const float* in = ml->from_input ? nullptr : buf_ptr_[ml->in_buf ^ flip];
                 // Input samples of this tile: columns hist_in_ + t0 + tau,
                 // tau in [xlo, xhi] inside the row.
                 const float* x0 = x_ptr_ + hist_in_ + t0;

@@ -270,7 +270,8 @@ private:
     std::vector<uint64_t> level_len_;          // floats per level buffer
     std::vector<MergeLaunch> launches_;
     std::vector<int32_t> level_source_; // level read to make l; -1 input; -2 not stored
-    std::vector<int> level_buf_;        // buffer holding a stored level
+    std::vector<int> level_buf_;        // buffer holding a stored level (tiles with flip_: swapped)
+    bool flip_ = false;                 // the two level buffers swap roles every other tile
     // Carried form: ring length per stored row (0 elsewhere), and the rings.
     bool carried_ = false;
     std::vector<uint32_t> carried_stored_;
